@@ -1,0 +1,861 @@
+// Host orchestration of the pressure projection and the C ABI of include/elq.h.
+//
+// One call of elq_project_device reproduces Simulator::solve_pressure
+// (simulator.cpp:96-112) on the GPU:
+//   1. mask + lexicographic scan        -> indices   (projection.cpp:14-28)
+//   2. brick compaction + level lists
+//   3. assembly                         -> A, b      (projection.cpp:30-83)
+//   4. IC(0)/MIC(0) factor (wavefront)
+//   5. PCG with Eigen's ConjugateGradient semantics (simulator.cpp:102-109),
+//      FP64, or FP32 inner solves under FP64 iterative refinement
+//   6. scatter + velocity update                    (projection.cpp:85-134)
+// All arithmetic runs in the kernels of elq_kernels.hip; the host only sizes
+// buffers, launches, and polls the device-side control block.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/elq.h"
+#include "elq_api_internal.hpp"
+#include "elq_kernels.hpp"
+
+using namespace elq;
+
+namespace {
+
+double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// ============================================================ context basics
+hipError_t DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+}
+
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+int elq_ctx::fail(int code, const std::string& msg) {
+    last_error = msg;
+    return code;
+}
+
+int elq_ctx::hip_fail(hipError_t e, const char* what) {
+    last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return ELQ_ERR_DEVICE;
+}
+
+void elq_ctx::prof_begin(const char* name) {
+    if (!profiling) return;
+    if (ev_used + 2 > events.size()) {
+        for (int q = 0; q < 256; ++q) {
+            hipEvent_t ev;
+            if (hipEventCreate(&ev) != hipSuccess) return;
+            events.push_back(ev);
+        }
+    }
+    (void)hipEventRecord(events[ev_used], stream);
+    ev_names.push_back(name);
+    ++ev_used;
+}
+
+void elq_ctx::prof_end() {
+    if (!profiling) return;
+    (void)hipEventRecord(events[ev_used], stream);
+    ++ev_used;
+}
+
+void elq_ctx::prof_collect() {
+    if (!profiling) return;
+    (void)hipStreamSynchronize(stream);
+    for (size_t q = 0; q + 1 < ev_used; q += 2) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, events[q], events[q + 1]);
+        auto& e = kt[ev_names[q / 2]];
+        e.first += ms;
+        e.second += 1;
+    }
+    ev_used = 0;
+    ev_names.clear();
+}
+
+// ============================================================ projection core
+namespace {
+
+struct Plan {
+    int ni, nj, nk;
+    int64_t C;
+    int nbx, nby, nbz, nlev;
+    int64_t NB;
+};
+
+Plan make_plan(int ni, int nj, int nk) {
+    Plan p;
+    p.ni = ni;
+    p.nj = nj;
+    p.nk = nk;
+    p.C = (int64_t)ni * nj * nk;
+    p.nbx = (ni + kBrick - 1) / kBrick;
+    p.nby = (nj + kBrick - 1) / kBrick;
+    p.nbz = (nk + kBrick - 1) / kBrick;
+    p.nlev = p.nbx + p.nby + p.nbz - 2;
+    p.NB = (int64_t)p.nbx * p.nby * p.nbz;
+    return p;
+}
+
+bool valid_dims(int ni, int nj, int nk) {
+    return ni >= 1 && nj >= 1 && nk >= 1 && ni <= 4092 && nj <= 4092 && nk <= 4092 &&
+           (int64_t)(ni + 1) * (nj + 1) * (nk + 1) < (int64_t)INT32_MAX;
+}
+
+#define HIPCHK(expr, what)                                  \
+    do {                                                    \
+        hipError_t _e = (expr);                             \
+        if (_e != hipSuccess) return ctx->hip_fail(_e, what); \
+    } while (0)
+
+}  // namespace
+
+size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged) {
+    const Plan p = make_plan(ni, nj, nk);
+    // unknown bricks are bounded by all bricks of the grid interior
+    const int64_t nbmax = p.NB;
+    const size_t cells = (size_t)nbmax * kBrickCells;
+    const size_t tsz = precision == ELQ_PRECISION_MIXED ? 4 : 8;
+    size_t b = 0;
+    b += (size_t)p.C * (1 + 4 + 4);                  // mask, indices, pressure
+    b += (size_t)p.NB * (1 + 4 + 4);                 // bflag, map, coord
+    b += (size_t)nbmax * (6 * 4 + 4 + 4 + 3 * 8);    // nbr, lvl_list, flags, partials
+    b += cells * (8 + 8 + 8);                        // diag64, b64, inv_e64
+    b += cells * tsz * (4 + 6);                      // coefficients + vectors
+    if (precision == ELQ_PRECISION_MIXED) b += cells * (4 + 8);  // inv_e32, x64
+    if (staged) b += sizeof(float) * ((size_t)p.C + 2 * ((size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk +
+                                                          (size_t)ni * nj * (nk + 1)));
+    return b;
+}
+
+namespace {
+
+// Phase A: mask, indices, brick compaction. Leaves N, nb, err in ctx.
+int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
+    const Plan p = make_plan(d.ni, d.nj, d.nk);
+    Workspace& W = ctx->ws;
+    const int64_t G = scan_blocks(std::max(p.C, p.NB));
+    HIPCHK(W.mask.ensure((size_t)p.C), "alloc mask");
+    HIPCHK(W.bflag.ensure((size_t)p.NB), "alloc bflag");
+    HIPCHK(W.map.ensure((size_t)p.NB * 4), "alloc map");
+    HIPCHK(W.coord.ensure((size_t)p.NB * 4), "alloc coord");
+    HIPCHK(W.scan_cnt.ensure((size_t)G * 4), "alloc scan");
+    HIPCHK(W.scan_off.ensure((size_t)G * 4), "alloc scan");
+    HIPCHK(W.lvl_cnt.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
+    HIPCHK(W.lvl_off.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
+    HIPCHK(W.lvl_fill.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
+    HIPCHK(W.info.ensure(sizeof(int64_t) * 2 + sizeof(int32_t) * 2), "alloc info");
+    int32_t* indices = indices_out;
+    if (!indices) {
+        HIPCHK(W.indices.ensure((size_t)p.C * 4), "alloc indices");
+        indices = (int32_t*)W.indices.p;
+    }
+    W.cur_indices = indices;
+    hipStream_t s = ctx->stream;
+    int64_t* totals = (int64_t*)W.info.p;
+    int32_t* err = (int32_t*)(totals + 2);
+    HIPCHK(hipMemsetAsync(W.info.p, 0, W.info.cap, s), "memset");
+    HIPCHK(hipMemsetAsync(W.lvl_cnt.p, 0, (size_t)(p.nlev + 1) * 4, s), "memset");
+    ctx->prof_begin("setup_mask_scan");
+    HIPCHK(launch_mask(d, (uint8_t*)W.mask.p, err, s), "k_mask");
+    HIPCHK(launch_scan((const uint8_t*)W.mask.p, p.C, kFluid, indices, ELQ_UNKNOWN_INDEX, (int32_t*)W.scan_cnt.p,
+                       (int32_t*)W.scan_off.p, &totals[0], s),
+           "scan indices");
+    HIPCHK(launch_brick_flags(d, (const uint8_t*)W.mask.p, p.nbx, p.nby, p.nbz, (uint8_t*)W.bflag.p, s),
+           "k_brick_flags");
+    HIPCHK(launch_scan((const uint8_t*)W.bflag.p, p.NB, 1, (int32_t*)W.map.p, -1, (int32_t*)W.scan_cnt.p,
+                       (int32_t*)W.scan_off.p, &totals[1], s),
+           "scan bricks");
+    HIPCHK(launch_brick_meta((const int32_t*)W.map.p, p.nbx, p.nby, p.nbz, (int32_t*)W.coord.p,
+                             (int32_t*)W.lvl_cnt.p, s),
+           "k_brick_meta");
+    HIPCHK(launch_level_scan((const int32_t*)W.lvl_cnt.p, p.nlev, (int32_t*)W.lvl_off.p, (int32_t*)W.lvl_fill.p, s),
+           "k_level_scan");
+    ctx->prof_end();
+    int64_t host_tot[2];
+    int32_t host_err = 0;
+    HIPCHK(hipMemcpyAsync(host_tot, totals, sizeof(host_tot), hipMemcpyDeviceToHost, s), "d2h");
+    HIPCHK(hipMemcpyAsync(&host_err, err, sizeof(host_err), hipMemcpyDeviceToHost, s), "d2h");
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    W.n_unknowns = host_tot[0];
+    W.nb = (int)host_tot[1];
+    W.setup_err = host_err;
+    W.plan_ni = d.ni;
+    W.plan_nj = d.nj;
+    W.plan_nk = d.nk;
+    return ELQ_OK;
+}
+
+Bricks make_bricks(elq_ctx* ctx, const Plan& p) {
+    Workspace& W = ctx->ws;
+    Bricks bk;
+    bk.nbx = p.nbx;
+    bk.nby = p.nby;
+    bk.nbz = p.nbz;
+    bk.nb = W.nb;
+    bk.nlev = p.nlev;
+    bk.map = (const int32_t*)W.map.p;
+    bk.coord = (const int32_t*)W.coord.p;
+    bk.nbr = (const int32_t*)W.nbr.p;
+    bk.lvl_list = (const int32_t*)W.lvl_list.p;
+    return bk;
+}
+
+// Phase B: lists, assembly (T storage). FP64 arithmetic for diag/b always.
+template <class T>
+int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
+    const Plan p = make_plan(d.ni, d.nj, d.nk);
+    Workspace& W = ctx->ws;
+    const int nb = W.nb;
+    const size_t cells = (size_t)nb * kBrickCells;
+    hipStream_t s = ctx->stream;
+    HIPCHK(W.nbr.ensure((size_t)nb * 6 * 4), "alloc nbr");
+    HIPCHK(W.lvl_list.ensure((size_t)nb * 4), "alloc lvl_list");
+    if ((size_t)nb * 4 > W.flags.cap) {
+        HIPCHK(W.flags.ensure((size_t)nb * 4), "alloc flags");
+        HIPCHK(hipMemsetAsync(W.flags.p, 0, W.flags.cap, s), "memset flags");
+    }
+    HIPCHK(W.diag64.ensure(cells * 8), "alloc");
+    HIPCHK(W.b64.ensure(cells * 8), "alloc");
+    HIPCHK(W.inv_e64.ensure(cells * 8), "alloc");
+    for (DevBuf* b : {&W.cdiag, &W.cax, &W.cay, &W.caz, &W.vx, &W.vr, &W.vp0, &W.vp1, &W.vq, &W.vy})
+        HIPCHK(b->ensure(cells * sizeof(T)), "alloc");
+    HIPCHK(W.inv_e32.ensure(cells * 4), "alloc");
+    HIPCHK(W.part_rr.ensure((size_t)nb * 8), "alloc");
+    HIPCHK(W.part_rz.ensure((size_t)nb * 8), "alloc");
+    HIPCHK(W.part_pq.ensure((size_t)nb * 8), "alloc");
+    HIPCHK(W.ctrl.ensure(sizeof(Ctrl)), "alloc");
+    HIPCHK(W.nfixed.ensure(sizeof(int32_t)), "alloc");
+    Bricks bk = make_bricks(ctx, p);
+    ctx->prof_begin("setup_lists_assemble");
+    HIPCHK(launch_brick_lists(bk.map, bk.coord, p.nbx, p.nby, p.nbz, nb, (const int32_t*)W.lvl_off.p,
+                              (int32_t*)W.lvl_fill.p, (int32_t*)W.lvl_list.p, (int32_t*)W.nbr.p, s),
+           "k_brick_lists");
+    HIPCHK(launch_assemble<T>(d, dt, (const uint8_t*)W.mask.p, bk, (double*)W.diag64.p, (double*)W.b64.p,
+                              (T*)W.cdiag.p, (T*)W.cax.p, (T*)W.cay.p, (T*)W.caz.p, s),
+           "k_assemble");
+    ctx->prof_end();
+    if (need_levels) {
+        W.lvl_host.resize((size_t)p.nlev + 1);
+        HIPCHK(hipMemcpyAsync(W.lvl_host.data(), W.lvl_off.p, (size_t)(p.nlev + 1) * 4, hipMemcpyDeviceToHost, s),
+               "d2h");
+        HIPCHK(hipStreamSynchronize(s), "sync");
+    }
+    return ELQ_OK;
+}
+
+void ctrl_host_init(Ctrl& c, const elq_opts& o, int mixed) {
+    std::memset(&c, 0, sizeof(c));
+    c.tol2 = o.tol * o.tol;
+    c.inner_tol2 = o.inner_tol * o.inner_tol;
+    c.max_iter = o.max_iter;
+    c.mixed = mixed;
+}
+
+int upload_ctrl(elq_ctx* ctx, const Ctrl& c) {
+    // the epoch/ticket words persist on the device across solves
+    HIPCHK(hipMemcpyAsync(ctx->ws.ctrl.p, &c, offsetof(Ctrl, epoch), hipMemcpyHostToDevice, ctx->stream), "h2d");
+    HIPCHK(hipMemsetAsync((char*)ctx->ws.ctrl.p + offsetof(Ctrl, ticket), 0, 2 * sizeof(uint32_t), ctx->stream),
+           "memset");
+    return ELQ_OK;
+}
+
+int read_ctrl(elq_ctx* ctx, Ctrl& c) {
+    HIPCHK(hipMemcpyAsync(&c, ctx->ws.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+template <class T>
+struct Solver {
+    elq_ctx* ctx;
+    Plan p;
+    Bricks bk;
+    Coef<T> c;
+    const elq_opts* o;
+    bool levels;
+
+    Ctrl* ctrl() { return (Ctrl*)ctx->ws.ctrl.p; }
+    double* prr() { return (double*)ctx->ws.part_rr.p; }
+    double* prz() { return (double*)ctx->ws.part_rz.p; }
+    double* ppq() { return (double*)ctx->ws.part_pq.p; }
+    T* vec(DevBuf& b) { return (T*)b.p; }
+
+    int scalar(int mode, const double* parts, const double* parts2 = nullptr) {
+        ctx->prof_begin("scalar");
+        HIPCHK(launch_scalar(ctrl(), mode, parts, bk.nb, parts2, ctx->stream), "k_scalar");
+        ctx->prof_end();
+        return ELQ_OK;
+    }
+
+    int fwd(int update, T* p_cur) {
+        Workspace& W = ctx->ws;
+        ctx->prof_begin(update ? "fwd_sweep_update" : "fwd_sweep");
+        if (!levels) {
+            HIPCHK(launch_fwd<T>(bk, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
+                                 (uint32_t*)W.flags.p, update, -1, 0, 0, ctx->timeout_ticks, ctx->stream),
+                   "k_fwd");
+        } else {
+            for (int l = 0; l < p.nlev; ++l) {
+                const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
+                HIPCHK(launch_fwd<T>(bk, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
+                                     (uint32_t*)W.flags.p, update, l, base, cnt, ctx->timeout_ticks, ctx->stream),
+                       "k_fwd");
+            }
+        }
+        ctx->prof_end();
+        return ELQ_OK;
+    }
+
+    int bwd() {
+        Workspace& W = ctx->ws;
+        ctx->prof_begin("bwd_sweep");
+        if (!levels) {
+            HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint32_t*)W.flags.p, -1, 0, 0,
+                                 ctx->timeout_ticks, ctx->stream),
+                   "k_bwd");
+        } else {
+            for (int l = p.nlev - 1; l >= 0; --l) {
+                const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
+                HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint32_t*)W.flags.p, l, base,
+                                     cnt, ctx->timeout_ticks, ctx->stream),
+                       "k_bwd");
+            }
+        }
+        ctx->prof_end();
+        return ELQ_OK;
+    }
+
+    int factor(double tau, double sigma, float* inv_e32) {
+        Workspace& W = ctx->ws;
+        int rc = scalar(SC_NOP, nullptr);
+        if (rc) return rc;
+        HIPCHK(hipMemsetAsync(W.nfixed.p, 0, sizeof(int32_t), ctx->stream), "memset");
+        ctx->prof_begin("factor");
+        if (!levels) {
+            HIPCHK(launch_factor<T>(bk, (const double*)W.diag64.p, c.ax, c.ay, c.az, tau, sigma, (double*)W.inv_e64.p,
+                                    inv_e32, ctrl(), (uint32_t*)W.flags.p, (int32_t*)W.nfixed.p, -1, 0, 0,
+                                    ctx->timeout_ticks, ctx->stream),
+                   "k_factor");
+        } else {
+            for (int l = 0; l < p.nlev; ++l) {
+                const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
+                HIPCHK(launch_factor<T>(bk, (const double*)W.diag64.p, c.ax, c.ay, c.az, tau, sigma,
+                                        (double*)W.inv_e64.p, inv_e32, ctrl(), (uint32_t*)W.flags.p,
+                                        (int32_t*)W.nfixed.p, l, base, cnt, ctx->timeout_ticks, ctx->stream),
+                       "k_factor");
+            }
+        }
+        ctx->prof_end();
+        return ELQ_OK;
+    }
+
+    // Krylov loop from an initial residual in r (x = 0, p buffers = 0), Eigen
+    // semantics. Polls the device every check_every iterations.
+    int krylov(Ctrl& hc) {
+        Workspace& W = ctx->ws;
+        int rc;
+        if ((rc = fwd(0, nullptr))) return rc;
+        if ((rc = scalar(SC_NOP, nullptr))) return rc;
+        if ((rc = bwd())) return rc;
+        if ((rc = scalar(SC_RZ_INIT, prz()))) return rc;
+        const int check = std::max(1, o->check_every);
+        T* P[2] = {vec(W.vp0), vec(W.vp1)};
+        // host-side bound in case the device never raises done
+        const int64_t cap = (int64_t)std::max(o->max_iter, 0) + 2 * (int64_t)check + 2;
+        for (int64_t k = 0;; ++k) {
+            if (k > cap) return ctx->fail(ELQ_ERR_DEVICE, "Krylov loop exceeded its host-side bound");
+            ctx->prof_begin("spmv");
+            HIPCHK(launch_spmv<T>(bk, c, vec(W.vy), P[k & 1], P[(k + 1) & 1], vec(W.vq), ppq(), ctrl(), ctx->stream),
+                   "k_spmv");
+            ctx->prof_end();
+            if ((rc = scalar(SC_ALPHA, ppq()))) return rc;
+            if ((rc = fwd(1, P[(k + 1) & 1]))) return rc;
+            if ((rc = scalar(SC_RR, prr()))) return rc;
+            if ((rc = bwd())) return rc;
+            if ((rc = scalar(SC_BETA, prz()))) return rc;
+            if ((k + 1) % check == 0) {
+                ctx->prof_collect();
+                if ((rc = read_ctrl(ctx, hc))) return rc;
+                if (hc.abort) return ctx->fail(ELQ_ERR_DEVICE, "wavefront wait exceeded its time limit");
+                if (hc.done) break;
+            }
+        }
+        return ELQ_OK;
+    }
+};
+
+template <class T>
+int solve_fp64(elq_ctx* ctx, Solver<T>& S, Ctrl& hc) {
+    Workspace& W = ctx->ws;
+    int rc;
+    ctx->prof_begin("init");
+    HIPCHK(launch_init<T>(S.bk, (const double*)W.b64.p, (T*)W.vr.p, (T*)W.vx.p, (T*)W.vp0.p, (T*)W.vp1.p, S.prr(),
+                          S.ctrl(), ctx->stream),
+           "k_init");
+    ctx->prof_end();
+    if ((rc = S.scalar(SC_INIT, S.prr()))) return rc;
+    if ((rc = S.krylov(hc))) return rc;
+    return ELQ_OK;
+}
+
+int solve_mixed(elq_ctx* ctx, Solver<float>& S, Ctrl& hc) {
+    Workspace& W = ctx->ws;
+    int rc;
+    const size_t cells = (size_t)S.bk.nb * kBrickCells;
+    HIPCHK(W.x64.ensure(cells * 8), "alloc x64");
+    HIPCHK(hipMemsetAsync(W.x64.p, 0, cells * 8, ctx->stream), "memset");
+    for (int outer = 0;; ++outer) {
+        if (outer > 200) return ctx->fail(ELQ_ERR_DEVICE, "refinement exceeded its host-side bound");
+        if (outer > 0) {
+            ctx->prof_begin("xupdate");
+            HIPCHK(launch_xupdate(S.bk, (double*)W.x64.p, (const float*)W.vx.p, S.ctrl(), ctx->stream), "k_xupdate");
+            ctx->prof_end();
+        }
+        ctx->prof_begin("resid64");
+        HIPCHK(launch_resid64<float>(S.bk, (const double*)W.diag64.p, S.c.ax, S.c.ay, S.c.az, (const double*)W.b64.p,
+                                     (const double*)W.x64.p, S.prr(), S.prz(), (float*)W.vr.p, (float*)W.vx.p,
+                                     (float*)W.vp0.p, (float*)W.vp1.p, S.ctrl(), ctx->stream),
+               "k_resid64");
+        ctx->prof_end();
+        if ((rc = S.scalar(SC_OUTER, S.prr(), S.prz()))) return rc;
+        ctx->prof_collect();
+        if ((rc = read_ctrl(ctx, hc))) return rc;
+        if (hc.outer_done) break;
+        if ((rc = S.krylov(hc))) return rc;
+    }
+    return ELQ_OK;
+}
+
+template <class T>
+int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_stats* st, float* pressure_out,
+                 int32_t* indices_out) {
+    const double t0 = now_ms();
+    Workspace& W = ctx->ws;
+    const Plan p = make_plan(d.ni, d.nj, d.nk);
+    const bool mixed = o.precision == ELQ_PRECISION_MIXED;
+    const bool levels = o.schedule == ELQ_SCHED_LEVELS;
+    int rc = setup_phase_a(ctx, d, indices_out);
+    if (rc) return rc;
+    if (st) {
+        st->n_unknowns = W.n_unknowns;
+        st->n_bricks = W.nb;
+    }
+    if (W.setup_err & kErrBoundary)
+        return ctx->fail(ELQ_ERR_BOUNDARY,
+                         "an unknown lies on the outer cell layer (the reference reads outside its arrays there)");
+    if (W.n_unknowns == 0) return ctx->fail(ELQ_ERR_NO_FLUID, "No fluid");
+    if ((rc = setup_phase_b<T>(ctx, d, dt, levels))) return rc;
+    Solver<T> S;
+    S.ctx = ctx;
+    S.p = p;
+    S.bk = make_bricks(ctx, p);
+    S.o = &o;
+    S.levels = levels;
+    S.c.diag = (const T*)W.cdiag.p;
+    S.c.ax = (const T*)W.cax.p;
+    S.c.ay = (const T*)W.cay.p;
+    S.c.az = (const T*)W.caz.p;
+    S.c.inv_e = mixed ? (const T*)W.inv_e32.p : (const T*)W.inv_e64.p;
+    Ctrl hc;
+    ctrl_host_init(hc, o, mixed);
+    if ((rc = upload_ctrl(ctx, hc))) return rc;
+    const double tau = o.precond == ELQ_PRECOND_MIC0 ? o.tau : 0.0;
+    if ((rc = S.factor(tau, o.sigma, mixed ? (float*)W.inv_e32.p : nullptr))) return rc;
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    const double t1 = now_ms();
+    if constexpr (std::is_same<T, float>::value) {
+        if ((rc = solve_mixed(ctx, S, hc))) return rc;
+    } else {
+        if ((rc = solve_fp64<T>(ctx, S, hc))) return rc;
+    }
+    const double t2 = now_ms();
+    const double* xsol = mixed ? (const double*)W.x64.p : (const double*)W.vx.p;
+    // true residual of the final iterate (FP64)
+    if (!mixed) {
+        ctx->prof_begin("resid64");
+        HIPCHK(launch_resid64<T>(S.bk, (const double*)W.diag64.p, S.c.ax, S.c.ay, S.c.az, (const double*)W.b64.p,
+                                 xsol, S.prr(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, ctx->stream),
+               "k_resid64");
+        ctx->prof_end();
+        if ((rc = S.scalar(SC_TRUE, S.prr()))) return rc;
+    }
+    float* pressure = pressure_out;
+    if (!pressure) {
+        HIPCHK(W.pressure.ensure((size_t)p.C * 4), "alloc pressure");
+        pressure = (float*)W.pressure.p;
+    }
+    ctx->prof_begin("scatter_update");
+    HIPCHK(launch_scatter(d, (const uint8_t*)W.mask.p, S.bk, xsol, pressure, ctx->stream), "k_scatter");
+    HIPCHK(launch_update(d, dt, pressure, ctx->stream), "k_update");
+    ctx->prof_end();
+    ctx->prof_collect();
+    if ((rc = read_ctrl(ctx, hc))) return rc;
+    if (hc.abort) return ctx->fail(ELQ_ERR_DEVICE, "wavefront wait exceeded its time limit");
+    const double t3 = now_ms();
+    if (st) {
+        st->iterations = hc.iter_total;
+        st->outer_iterations = mixed ? hc.outer : 1;
+        st->error = hc.last_err;
+        const double rr = mixed ? hc.rr64 : hc.true_rr;
+        st->true_rel_residual = hc.bnorm2 > 0 ? std::sqrt(rr / hc.bnorm2) : 0.0;
+        st->converged = hc.converged;
+        st->status = hc.status ? hc.status : (hc.converged ? ELQ_OK : ELQ_ERR_NOT_CONVERGED);
+        st->t_setup_ms = t1 - t0;
+        st->t_solve_ms = t2 - t1;
+        st->t_update_ms = t3 - t2;
+        st->t_total_ms = t3 - t0;
+    }
+    if (hc.status == ELQ_ERR_BREAKDOWN) return ctx->fail(ELQ_ERR_BREAKDOWN, "non-finite Krylov scalar");
+    if (!hc.converged) {
+        ctx->last_error = "max_iter reached before the tolerance";
+        return ELQ_ERR_NOT_CONVERGED;
+    }
+    return ELQ_OK;
+}
+
+bool check_opts(const elq_opts& o) {
+    return o.tol >= 0 && o.max_iter >= 0 && (o.precision == 0 || o.precision == 1) &&
+           (o.precond == 0 || o.precond == 1) && (o.schedule == 0 || o.schedule == 1) && o.sigma >= 0 &&
+           o.inner_tol > 0 && o.inner_tol < 1;
+}
+
+Dense dense_of(const elq_grid& g) {
+    Dense d;
+    d.ni = g.ni;
+    d.nj = g.nj;
+    d.nk = g.nk;
+    d.dx = g.dx;
+    d.phi = g.level_set;
+    d.u = g.u;
+    d.v = g.v;
+    d.w = g.w;
+    d.uw = g.u_weight;
+    d.vw = g.v_weight;
+    d.ww = g.w_weight;
+    return d;
+}
+
+struct Sizes {
+    size_t c, u, v, w;
+};
+Sizes sizes_of(int ni, int nj, int nk) {
+    return {(size_t)ni * nj * nk, (size_t)(ni + 1) * nj * nk, (size_t)ni * (nj + 1) * nk, (size_t)ni * nj * (nk + 1)};
+}
+
+// Uploads host grids into the context's staging buffers; returns device view.
+int stage_inputs(elq_ctx* ctx, const elq_grid& g, elq_grid& dg) {
+    Workspace& W = ctx->ws;
+    const Sizes z = sizes_of(g.ni, g.nj, g.nk);
+    HIPCHK(W.s_phi.ensure(z.c * 4), "alloc");
+    HIPCHK(W.s_u.ensure(z.u * 4), "alloc");
+    HIPCHK(W.s_v.ensure(z.v * 4), "alloc");
+    HIPCHK(W.s_w.ensure(z.w * 4), "alloc");
+    HIPCHK(W.s_uw.ensure(z.u * 4), "alloc");
+    HIPCHK(W.s_vw.ensure(z.v * 4), "alloc");
+    HIPCHK(W.s_ww.ensure(z.w * 4), "alloc");
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(W.s_phi.p, g.level_set, z.c * 4, hipMemcpyHostToDevice, s), "h2d");
+    HIPCHK(hipMemcpyAsync(W.s_u.p, g.u, z.u * 4, hipMemcpyHostToDevice, s), "h2d");
+    HIPCHK(hipMemcpyAsync(W.s_v.p, g.v, z.v * 4, hipMemcpyHostToDevice, s), "h2d");
+    HIPCHK(hipMemcpyAsync(W.s_w.p, g.w, z.w * 4, hipMemcpyHostToDevice, s), "h2d");
+    HIPCHK(hipMemcpyAsync(W.s_uw.p, g.u_weight, z.u * 4, hipMemcpyHostToDevice, s), "h2d");
+    HIPCHK(hipMemcpyAsync(W.s_vw.p, g.v_weight, z.v * 4, hipMemcpyHostToDevice, s), "h2d");
+    HIPCHK(hipMemcpyAsync(W.s_ww.p, g.w_weight, z.w * 4, hipMemcpyHostToDevice, s), "h2d");
+    dg = g;
+    dg.level_set = (const float*)W.s_phi.p;
+    dg.u = (float*)W.s_u.p;
+    dg.v = (float*)W.s_v.p;
+    dg.w = (float*)W.s_w.p;
+    dg.u_weight = (const float*)W.s_uw.p;
+    dg.v_weight = (const float*)W.s_vw.p;
+    dg.w_weight = (const float*)W.s_ww.p;
+    dg.pressure = nullptr;
+    dg.indices = nullptr;
+    return ELQ_OK;
+}
+
+int check_grid(elq_ctx* ctx, const elq_grid* g) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!g || !g->level_set || !g->u || !g->v || !g->w || !g->u_weight || !g->v_weight || !g->w_weight)
+        return ctx->fail(ELQ_ERR_INVALID_ARG, "null grid pointer");
+    if (!valid_dims(g->ni, g->nj, g->nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    if (!(g->dx > 0.0f) || !std::isfinite(g->dx)) return ctx->fail(ELQ_ERR_INVALID_ARG, "dx must be positive");
+    return ELQ_OK;
+}
+
+int run_project(elq_ctx* ctx, const elq_grid& dg, float dt, const elq_opts& o, elq_stats* st) {
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return ctx->fail(ELQ_ERR_INVALID_ARG, "dt must be positive");
+    if (!check_opts(o)) return ctx->fail(ELQ_ERR_INVALID_ARG, "invalid solver options");
+    if (ctx->budget) {
+        const size_t need = elq_footprint_bytes(dg.ni, dg.nj, dg.nk, o.precision, false);
+        if (need > ctx->budget) return ctx->fail(ELQ_ERR_ADMISSION, "projection exceeds the context's HBM budget");
+    }
+    const Dense d = dense_of(dg);
+    if (o.precision == ELQ_PRECISION_MIXED) return project_impl<float>(ctx, d, dt, o, st, dg.pressure, dg.indices);
+    return project_impl<double>(ctx, d, dt, o, st, dg.pressure, dg.indices);
+}
+
+}  // namespace
+
+// ============================================================ C ABI
+extern "C" {
+
+void elq_default_opts(elq_opts* o) {
+    if (!o) return;
+    o->tol = 1e-4;
+    o->max_iter = 1000;
+    o->precision = ELQ_PRECISION_FP64;
+    o->precond = ELQ_PRECOND_IC0;
+    o->tau = 0.97;
+    o->sigma = 0.25;
+    o->inner_tol = 1e-4;
+    o->schedule = ELQ_SCHED_WAVEFRONT;
+    o->check_every = 8;
+}
+
+int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
+    if (!out) return ELQ_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return ELQ_ERR_DEVICE;
+    if (hipSetDevice(device) != hipSuccess) return ELQ_ERR_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return ELQ_ERR_DEVICE;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return ELQ_ERR_DEVICE;
+    elq_ctx* ctx = new elq_ctx();
+    ctx->device = device;
+    ctx->budget = hbm_budget_bytes;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return ELQ_ERR_DEVICE;
+    }
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
+        rate_khz = 100000;
+    ctx->timeout_ticks = (uint64_t)rate_khz * 1000ull * 4ull;  // 4 s per wait
+    *out = ctx;
+    return ELQ_OK;
+}
+
+void elq_ctx_destroy(elq_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->ws.release_all();
+    for (auto ev : ctx->events) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* elq_ctx_last_error(const elq_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int elq_project_device(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    elq_opts o;
+    if (opts)
+        o = *opts;
+    else
+        elq_default_opts(&o);
+    if (st) std::memset(st, 0, sizeof(*st));
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    rc = run_project(ctx, *g, dt, o, st);
+    if (st && rc != ELQ_OK && st->status == 0) st->status = rc;
+    return rc;
+}
+
+int elq_project(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    elq_opts o;
+    if (opts)
+        o = *opts;
+    else
+        elq_default_opts(&o);
+    if (st) std::memset(st, 0, sizeof(*st));
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    elq_grid dg;
+    if ((rc = stage_inputs(ctx, *g, dg))) return rc;
+    if (g->pressure) {
+        if (ctx->ws.s_pressure.ensure(sizes_of(g->ni, g->nj, g->nk).c * 4) != hipSuccess)
+            return ctx->fail(ELQ_ERR_DEVICE, "alloc");
+        dg.pressure = (float*)ctx->ws.s_pressure.p;
+    }
+    if (g->indices) {
+        if (ctx->ws.s_indices.ensure(sizes_of(g->ni, g->nj, g->nk).c * 4) != hipSuccess)
+            return ctx->fail(ELQ_ERR_DEVICE, "alloc");
+        dg.indices = (int32_t*)ctx->ws.s_indices.p;
+    }
+    rc = run_project(ctx, dg, dt, o, st);
+    if (st && rc != ELQ_OK && st->status == 0) st->status = rc;
+    if (rc != ELQ_OK && rc != ELQ_ERR_NOT_CONVERGED) {
+        // indices are meaningful even on NO_FLUID (all INT_MAX); copy them back
+        if (rc == ELQ_ERR_NO_FLUID && g->indices) {
+            const Sizes z = sizes_of(g->ni, g->nj, g->nk);
+            if (hipMemcpy(g->indices, dg.indices, z.c * 4, hipMemcpyDeviceToHost) != hipSuccess)
+                return ctx->fail(ELQ_ERR_DEVICE, "d2h");
+        }
+        return rc;
+    }
+    const Sizes z = sizes_of(g->ni, g->nj, g->nk);
+    hipStream_t s = ctx->stream;
+    if (hipMemcpyAsync(g->u, dg.u, z.u * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(g->v, dg.v, z.v * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(g->w, dg.w, z.w * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        (g->pressure && hipMemcpyAsync(g->pressure, dg.pressure, z.c * 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        (g->indices && hipMemcpyAsync(g->indices, dg.indices, z.c * 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return ctx->fail(ELQ_ERR_DEVICE, "copy back");
+    return rc;
+}
+
+int elq_assign_indices(elq_ctx* ctx, const elq_grid* g, int32_t* indices, int64_t* n_unknowns) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!indices) return ctx->fail(ELQ_ERR_INVALID_ARG, "null indices");
+    (void)hipSetDevice(ctx->device);
+    elq_grid dg;
+    if ((rc = stage_inputs(ctx, *g, dg))) return rc;
+    if ((rc = setup_phase_a(ctx, dense_of(dg), nullptr))) return rc;
+    const Sizes z = sizes_of(g->ni, g->nj, g->nk);
+    if (hipMemcpy(indices, ctx->ws.cur_indices, z.c * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return ctx->fail(ELQ_ERR_DEVICE, "d2h");
+    if (n_unknowns) *n_unknowns = ctx->ws.n_unknowns;
+    if (ctx->ws.n_unknowns == 0) return ctx->fail(ELQ_ERR_NO_FLUID, "No fluid");
+    if (ctx->ws.setup_err & kErrBoundary) return ctx->fail(ELQ_ERR_BOUNDARY, "unknown on the outer cell layer");
+    return ELQ_OK;
+}
+
+int elq_build_matrix(elq_ctx* ctx, const elq_grid* g, float dt, double* diag, double* ax, double* ay, double* az,
+                     double* b) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!diag || !ax || !ay || !az || !b) return ctx->fail(ELQ_ERR_INVALID_ARG, "null output");
+    if (!(dt > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "dt must be positive");
+    (void)hipSetDevice(ctx->device);
+    elq_grid dg;
+    if ((rc = stage_inputs(ctx, *g, dg))) return rc;
+    const Dense d = dense_of(dg);
+    if ((rc = setup_phase_a(ctx, d, nullptr))) return rc;
+    Workspace& W = ctx->ws;
+    if (W.n_unknowns == 0) return ctx->fail(ELQ_ERR_NO_FLUID, "No fluid");
+    if (W.setup_err & kErrBoundary) return ctx->fail(ELQ_ERR_BOUNDARY, "unknown on the outer cell layer");
+    if ((rc = setup_phase_b<double>(ctx, d, dt, false))) return rc;
+    const size_t n = (size_t)W.n_unknowns;
+    HIPCHK(W.export_buf.ensure(n * 8 * 5), "alloc");
+    double* e = (double*)W.export_buf.p;
+    const Plan p = make_plan(d.ni, d.nj, d.nk);
+    Bricks bk = make_bricks(ctx, p);
+    HIPCHK(launch_export<double>(d, (const uint8_t*)W.mask.p, W.cur_indices, bk, (const double*)W.diag64.p,
+                                 (const double*)W.b64.p, (const double*)W.cax.p, (const double*)W.cay.p,
+                                 (const double*)W.caz.p, e, e + n, e + 2 * n, e + 3 * n, e + 4 * n, ctx->stream),
+           "k_export");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    double* outs[5] = {diag, ax, ay, az, b};
+    for (int q = 0; q < 5; ++q) HIPCHK(hipMemcpy(outs[q], e + q * n, n * 8, hipMemcpyDeviceToHost), "d2h");
+    return ELQ_OK;
+}
+
+int elq_update_velocity(elq_ctx* ctx, const elq_grid* g, float dt) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!g->pressure) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pressure");
+    (void)hipSetDevice(ctx->device);
+    elq_grid dg;
+    if ((rc = stage_inputs(ctx, *g, dg))) return rc;
+    const Sizes z = sizes_of(g->ni, g->nj, g->nk);
+    Workspace& W = ctx->ws;
+    HIPCHK(W.s_pressure.ensure(z.c * 4), "alloc");
+    HIPCHK(hipMemcpyAsync(W.s_pressure.p, g->pressure, z.c * 4, hipMemcpyHostToDevice, ctx->stream), "h2d");
+    HIPCHK(launch_update(dense_of(dg), dt, (const float*)W.s_pressure.p, ctx->stream), "k_update");
+    hipStream_t s = ctx->stream;
+    HIPCHK(hipMemcpyAsync(g->u, dg.u, z.u * 4, hipMemcpyDeviceToHost, s), "d2h");
+    HIPCHK(hipMemcpyAsync(g->v, dg.v, z.v * 4, hipMemcpyDeviceToHost, s), "d2h");
+    HIPCHK(hipMemcpyAsync(g->w, dg.w, z.w * 4, hipMemcpyDeviceToHost, s), "d2h");
+    HIPCHK(hipStreamSynchronize(s), "sync");
+    return ELQ_OK;
+}
+
+int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return ELQ_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(hipMalloc(out, bytes), "hipMalloc");
+    return ELQ_OK;
+}
+
+int elq_device_free(elq_ctx* ctx, void* ptr) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    HIPCHK(hipFree(ptr), "hipFree");
+    return ELQ_OK;
+}
+
+int elq_memcpy_h2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "h2d");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_memcpy_d2h(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+void elq_set_profiling(elq_ctx* ctx, int enable) {
+    if (ctx) ctx->profiling = enable != 0;
+}
+
+int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, double* ms, int64_t* launches, int cap) {
+    if (!ctx || !names || names_cap == 0) return 0;
+    std::string all;
+    int n = 0;
+    for (const auto& kv : ctx->kt) {
+        if (n >= cap) break;
+        if (!all.empty()) all += ",";
+        all += kv.first;
+        if (ms) ms[n] = kv.second.first;
+        if (launches) launches[n] = kv.second.second;
+        ++n;
+    }
+    std::strncpy(names, all.c_str(), names_cap - 1);
+    names[names_cap - 1] = '\0';
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// Context and workspace of the C ABI (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+// Device buffer that only grows (hipMalloc on growth, reused otherwise).
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes);
+    void release();
+};
+
+// Device buffers of one projection; sized for the largest problem seen.
+struct Workspace {
+    // dense
+    DevBuf mask, indices, pressure;
+    // scans and brick metadata
+    DevBuf scan_cnt, scan_off, bflag, map, coord, lvl_cnt, lvl_off, lvl_fill, info;
+    DevBuf nbr, lvl_list, flags;
+    // system
+    DevBuf diag64, b64, inv_e64, inv_e32, cdiag, cax, cay, caz;
+    // Krylov vectors (storage type of the solve)
+    DevBuf vx, vr, vp0, vp1, vq, vy, x64;
+    DevBuf part_rr, part_rz, part_pq, ctrl, nfixed;
+    // staging of host inputs
+    DevBuf s_phi, s_u, s_v, s_w, s_uw, s_vw, s_ww, s_pressure, s_indices;
+    DevBuf export_buf;
+
+    int64_t n_unknowns = 0;
+    int nb = 0;
+    int setup_err = 0;
+    int plan_ni = 0, plan_nj = 0, plan_nk = 0;
+    int32_t* cur_indices = nullptr;
+    std::vector<int32_t> lvl_host;
+
+    void release_all() {
+        for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
+                          &lvl_fill, &info, &nbr, &lvl_list, &flags, &diag64, &b64, &inv_e64, &inv_e32, &cdiag, &cax,
+                          &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz, &part_pq, &ctrl,
+                          &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
+                          &export_buf})
+            b->release();
+    }
+};
+
+struct elq_ctx {
+    int device = 0;
+    size_t budget = 0;
+    hipStream_t stream = nullptr;
+    uint64_t timeout_ticks = 0;
+    std::string last_error;
+    Workspace ws;
+
+    // per-kernel-class event timing
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    size_t ev_used = 0;
+    std::vector<const char*> ev_names;
+    std::map<std::string, std::pair<double, int64_t>> kt;
+    void prof_begin(const char* name);
+    void prof_end();
+    void prof_collect();
+
+    int fail(int code, const std::string& msg);
+    int hip_fail(hipError_t e, const char* what);
+};
+
+size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged);

@@ -1,0 +1,1023 @@
+// gfx950 kernels of the pressure projection (ghost-fluid Poisson assembly,
+// natural-order IC(0)/MIC(0) preconditioned CG, pressure-gradient update).
+//
+// Reference path (daichi-nama/eulerian-liquid):
+//   assign_indices                 src/project/3d/fluid/projection.cpp:14-28
+//   build_matrix                   src/project/3d/fluid/projection.cpp:30-83
+//   ConjugateGradient + IC         src/project/3d/eulerian_liquid/simulator.cpp:102-109
+//   build_grid_from_vector         src/project/3d/fluid/projection.cpp:85-91
+//   update_velocity_using_pressure src/project/3d/fluid/projection.cpp:93-134
+//
+// Every kernel is HBM-bandwidth or latency bound; none uses MFMA. Built with
+// -ffp-contract=off: where a result must match the reference bit for bit
+// (indices, A, b, the IC pivots, the velocity update) the float/double
+// operation sequence is the reference's; the Krylov kernels use explicit fma().
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+
+#include "elq_kernels.hpp"
+
+namespace elq {
+namespace {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+
+template <class T>
+__device__ __forceinline__ T shfl(T v, int src) {
+    return __shfl(v, src, 64);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum_int(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// level_set.h:68-80
+__device__ __forceinline__ float fraction_inside2(float l, float r) {
+    if (l < 0.0f && r < 0.0f) return 1.0f;
+    if (l < 0.0f && r >= 0.0f) return l / (l - r);
+    if (l >= 0.0f && r < 0.0f) return r / (r - l);
+    return 0.0f;
+}
+
+// std::max(theta, 0.01f) as <algorithm> defines it
+__device__ __forceinline__ float clamp_theta(float t) { return (t < 0.01f) ? 0.01f : t; }
+
+__device__ __forceinline__ size_t cell(int i, int j, int k, int ni, int nj) {
+    return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
+}
+
+// Wavefront scheduling. Bricks are taken in level order from a ticket counter,
+// so every brick a wave waits on was taken earlier by a running wave: no
+// deadlock whatever the residency. Each wait is bounded in wall-clock time;
+// on expiry the wave raises ctrl->abort and every other wait returns at once.
+__device__ __forceinline__ int take_ticket(Ctrl* c) {
+    int t = 0;
+    if (lane_id() == 0) t = (int)atomicAdd(&c->ticket, 1u);
+    return __shfl(t, 0, 64);
+}
+
+__device__ __forceinline__ void wait_bricks(const uint32_t* flags, int p0, int p1, int p2, uint32_t epoch,
+                                            Ctrl* c, uint64_t timeout) {
+    const int lane = lane_id();
+    const int p = lane == 0 ? p0 : lane == 1 ? p1 : lane == 2 ? p2 : -1;
+    if (p >= 0) {
+        const uint64_t t0 = wall_clock64();
+        while (__hip_atomic_load(&flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+            if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+            if (wall_clock64() - t0 > timeout) {
+                __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+__device__ __forceinline__ void signal_brick(uint32_t* flags, int b, uint32_t epoch) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    if (lane_id() == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct Lane {
+    int lane, lx, ly, lz;
+    __device__ Lane() {
+        lane = lane_id();
+        lx = lane & 3;
+        ly = (lane >> 2) & 3;
+        lz = lane >> 4;
+    }
+};
+
+// Neighbour of a brick cell across a brick face: (brick id, lane) or -1.
+// Directions: 0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z.
+__device__ __forceinline__ int nbr_id(const Bricks& bk, int d, int b) { return bk.nbr[(size_t)d * bk.nb + b]; }
+
+// ================================================================== setup
+// projection.cpp:8-12 (validity) and :17 (phi < 0); flags unknowns on the
+// outer cell layer, where the reference's build_matrix would index outside
+// its arrays.
+__global__ __launch_bounds__(256) void k_mask(Dense d, uint8_t* mask, int32_t* err) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.ni || j >= d.nj) return;
+    const int ni = d.ni, nj = d.nj, nk = d.nk;
+    const size_t c = cell(i, j, k, ni, nj);
+    const size_t ui = cell(i, j, k, ni + 1, nj);
+    const size_t vi = cell(i, j, k, ni, nj + 1);
+    const bool valid = d.uw[ui] > 0.0f || d.uw[ui + 1] > 0.0f || d.vw[vi] > 0.0f || d.vw[vi + ni] > 0.0f ||
+                       d.ww[c] > 0.0f || d.ww[c + (size_t)ni * nj] > 0.0f;
+    const bool fluid = valid && d.phi[c] < 0.0f;
+    mask[c] = (uint8_t)((valid ? kValid : 0) | (fluid ? kFluid : 0));
+    if (fluid && (i == 0 || j == 0 || k == 0 || i == ni - 1 || j == nj - 1 || k == nk - 1))
+        atomicOr(err, kErrBoundary);
+}
+
+// Lexicographic exclusive scan of a byte mask (three passes). Reproduces the
+// serial k->j->i numbering of assign_indices exactly.
+__global__ __launch_bounds__(kScanBlock) void k_scan_count(const uint8_t* f, int64_t n, uint8_t bit,
+                                                           int32_t* blk_cnt) {
+    __shared__ int ws[kScanBlock / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    int cnt = 0;
+#pragma unroll
+    for (int m = 0; m < kScanItems; ++m) {
+        const int64_t c = base + (int64_t)m * kScanBlock + threadIdx.x;
+        if (c < n && (f[c] & bit)) ++cnt;
+    }
+    cnt = wave_sum_int(cnt);
+    if (lane_id() == 0) ws[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int w = 0; w < kScanBlock / 64; ++w) s += ws[w];
+        blk_cnt[blockIdx.x] = s;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(const int32_t* blk_cnt, int64_t g, int32_t* blk_off,
+                                                       int64_t* total) {
+    __shared__ int64_t sh[1024];
+    const int t = threadIdx.x;
+    const int64_t per = (g + 1023) / 1024;
+    const int64_t lo = t * per, hi = (lo + per < g) ? lo + per : g;
+    int64_t s = 0;
+    for (int64_t q = lo; q < hi; ++q) s += blk_cnt[q];
+    sh[t] = s;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int64_t v = (t >= o) ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += v;
+        __syncthreads();
+    }
+    int64_t run = sh[t] - s;  // exclusive prefix of this thread's segment
+    for (int64_t q = lo; q < hi; ++q) {
+        blk_off[q] = (int32_t)run;
+        run += blk_cnt[q];
+    }
+    if (t == 1023) *total = sh[1023];
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_write(const uint8_t* f, int64_t n, uint8_t bit,
+                                                           const int32_t* blk_off, int32_t* out, int32_t fill) {
+    __shared__ int ws[kScanBlock / 64];
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * kScanChunk;
+    int run = blk_off[blockIdx.x];
+    for (int m = 0; m < kScanItems; ++m) {
+        const int64_t c = base + (int64_t)m * kScanBlock + threadIdx.x;
+        const bool fl = c < n && (f[c] & bit);
+        const unsigned long long bal = __ballot(fl);
+        const int pre = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) ws[wave] = __popcll(bal);
+        __syncthreads();
+        int woff = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kScanBlock / 64; ++w) {
+            woff += (w < wave) ? ws[w] : 0;
+            tot += ws[w];
+        }
+        if (c < n) out[c] = fl ? run + woff + pre : fill;
+        run += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_brick_flags(Dense d, const uint8_t* mask, int nbx, int nby, int nbz,
+                                                     uint8_t* bflag) {
+    const int64_t bi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (bi >= (int64_t)nbx * nby * nbz) return;
+    const int bx = (int)(bi % nbx);
+    const int by = (int)((bi / nbx) % nby);
+    const int bz = (int)(bi / ((int64_t)nbx * nby));
+    uint8_t any = 0;
+    for (int lz = 0; lz < kBrick; ++lz)
+        for (int ly = 0; ly < kBrick; ++ly)
+            for (int lx = 0; lx < kBrick; ++lx) {
+                const int i = bx * kBrick + lx, j = by * kBrick + ly, k = bz * kBrick + lz;
+                if (i < d.ni && j < d.nj && k < d.nk && (mask[cell(i, j, k, d.ni, d.nj)] & kFluid)) any = 1;
+            }
+    bflag[bi] = any;
+}
+
+__global__ __launch_bounds__(256) void k_brick_meta(const int32_t* map, int nbx, int nby, int nbz,
+                                                    int32_t* coord, int32_t* lvl_cnt) {
+    const int64_t bi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (bi >= (int64_t)nbx * nby * nbz) return;
+    const int id = map[bi];
+    if (id < 0) return;
+    const int bx = (int)(bi % nbx);
+    const int by = (int)((bi / nbx) % nby);
+    const int bz = (int)(bi / ((int64_t)nbx * nby));
+    coord[id] = bx | (by << 10) | (bz << 20);
+    atomicAdd(&lvl_cnt[bx + by + bz], 1);
+}
+
+__global__ void k_level_scan(const int32_t* lvl_cnt, int nlev, int32_t* lvl_off, int32_t* lvl_fill) {
+    if (threadIdx.x != 0) return;
+    int s = 0;
+    for (int l = 0; l < nlev; ++l) {
+        lvl_off[l] = s;
+        s += lvl_cnt[l];
+        lvl_fill[l] = 0;
+    }
+    lvl_off[nlev] = s;
+}
+
+__global__ __launch_bounds__(256) void k_brick_lists(const int32_t* map, const int32_t* coord, int nbx, int nby,
+                                                     int nbz, int nb, const int32_t* lvl_off, int32_t* lvl_fill,
+                                                     int32_t* lvl_list, int32_t* nbr) {
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= nb) return;
+    const int cc = coord[id];
+    const int bx = cc & 1023, by = (cc >> 10) & 1023, bz = (cc >> 20) & 1023;
+    auto look = [&](int x, int y, int z) -> int {
+        if (x < 0 || y < 0 || z < 0 || x >= nbx || y >= nby || z >= nbz) return -1;
+        return map[(size_t)x + (size_t)nbx * ((size_t)y + (size_t)nby * z)];
+    };
+    nbr[0 * (size_t)nb + id] = look(bx - 1, by, bz);
+    nbr[1 * (size_t)nb + id] = look(bx + 1, by, bz);
+    nbr[2 * (size_t)nb + id] = look(bx, by - 1, bz);
+    nbr[3 * (size_t)nb + id] = look(bx, by + 1, bz);
+    nbr[4 * (size_t)nb + id] = look(bx, by, bz - 1);
+    nbr[5 * (size_t)nb + id] = look(bx, by, bz + 1);
+    const int l = bx + by + bz;
+    const int pos = atomicAdd(&lvl_fill[l], 1);
+    lvl_list[lvl_off[l] + pos] = id;
+}
+
+// projection.cpp:30-83 for the cells of one brick per wave. Per unknown, the
+// neighbours are visited in the reference's order -x,+x,-y,+y,-z,+z; diag is a
+// double sum of float terms; b accumulates float products in double and is
+// scaled by (double)(dx/dt) with dx/dt a float division done on the host.
+template <class T>
+__global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const uint8_t* mask, Bricks bk,
+                                                  double* diag64, double* b64, T* diag, T* ax, T* ay, T* az) {
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= bk.nb) return;
+    const Lane L;
+    const int cc = bk.coord[b];
+    const int i = (cc & 1023) * kBrick + L.lx, j = ((cc >> 10) & 1023) * kBrick + L.ly,
+              k = ((cc >> 20) & 1023) * kBrick + L.lz;
+    const int ni = d.ni, nj = d.nj, nk = d.nk;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    double dg = 0.0, bb = 0.0;
+    float cpl[3] = {0.0f, 0.0f, 0.0f};
+    const bool inside = i < ni && j < nj && k < nk;
+    const size_t c = inside ? cell(i, j, k, ni, nj) : 0;
+    if (inside && (mask[c] & kFluid)) {
+        const float phic = d.phi[c];
+        const size_t ui = cell(i, j, k, ni + 1, nj), vi = cell(i, j, k, ni, nj + 1);
+        const size_t plane = (size_t)ni * nj;
+        const float vel[6] = {d.u[ui], d.u[ui + 1], d.v[vi], d.v[vi + ni], d.w[c], d.w[c + plane]};
+        const float wgt[6] = {d.uw[ui], d.uw[ui + 1], d.vw[vi], d.vw[vi + ni], d.ww[c], d.ww[c + plane]};
+        const int nx[6] = {i - 1, i + 1, i, i, i, i};
+        const int ny[6] = {j, j, j - 1, j + 1, j, j};
+        const int nz[6] = {k, k, k, k, k - 1, k + 1};
+#pragma unroll
+        for (int dir = 0; dir < 6; ++dir) {
+            const int x = nx[dir], y = ny[dir], z = nz[dir];
+            const bool in = x >= 0 && y >= 0 && z >= 0 && x < ni && y < nj && z < nk;
+            const size_t cn = in ? cell(x, y, z, ni, nj) : 0;
+            const uint8_t mn = in ? mask[cn] : 0;
+            if (mn & kValid) {
+                if (mn & kFluid) {
+                    dg += (double)wgt[dir];
+                    if (dir & 1) cpl[dir >> 1] = -wgt[dir];
+                } else {
+                    const float theta = clamp_theta(fraction_inside2(phic, d.phi[cn]));
+                    dg += (double)(wgt[dir] / theta);
+                }
+            }
+            if (dir & 1)
+                bb -= (double)(wgt[dir] * vel[dir]);
+            else
+                bb += (double)(wgt[dir] * vel[dir]);
+        }
+        bb = bb * (double)scale;
+    }
+    diag64[o] = dg;
+    b64[o] = bb;
+    diag[o] = (T)dg;
+    ax[o] = (T)cpl[0];
+    ay[o] = (T)cpl[1];
+    az[o] = (T)cpl[2];
+}
+
+// Natural-order IC(0)/MIC(0) pivots, one brick per wave, FP64 arithmetic in
+// the oracle's operation order (bit-identical pivots):
+//   t = d - a_xm^2/e_xm - a_ym^2/e_ym - a_zm^2/e_zm - tau*(...);
+//   t <= sigma*d -> t = d;   inv_e = 1/t.
+// Inside the brick the ten hyperplanes lx+ly+lz = h are swept in order; values
+// from the three predecessor bricks are read once their flags are raised.
+template <class T, bool LEVELS>
+__global__ __launch_bounds__(64) void k_factor(Bricks bk, const double* diag64, const T* ax, const T* ay,
+                                               const T* az, double tau, double sigma, double* inv_e64,
+                                               float* inv_e32, Ctrl* ctrl, uint32_t* flags, int32_t* n_fixed,
+                                               int level_base, uint64_t timeout) {
+    const Lane L;
+    const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
+    if (slot >= bk.nb) return;
+    const int b = bk.lvl_list[slot];
+    const uint32_t epoch = ctrl->epoch;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    const int nxm = nbr_id(bk, 0, b), nym = nbr_id(bk, 2, b), nzm = nbr_id(bk, 4, b);
+    const double dg = diag64[o];
+    const double axo = (double)ax[o], ayo = (double)ay[o], azo = (double)az[o];
+    auto ext = [&](const T* a, int nb_, int off) -> double {
+        return nb_ >= 0 ? (double)a[(size_t)nb_ * kBrickCells + L.lane + off] : 0.0;
+    };
+    // lower couplings a(c-x, c) = ax[c-x] etc., and for MIC the other upper
+    // couplings of each lower neighbour
+    const double sx_ax = shfl(axo, L.lane - 1), sx_ay = shfl(ayo, L.lane - 1), sx_az = shfl(azo, L.lane - 1);
+    const double sy_ax = shfl(axo, L.lane - 4), sy_ay = shfl(ayo, L.lane - 4), sy_az = shfl(azo, L.lane - 4);
+    const double sz_ax = shfl(axo, L.lane - 16), sz_ay = shfl(ayo, L.lane - 16), sz_az = shfl(azo, L.lane - 16);
+    const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
+    const double axm = ix ? sx_ax : ext(ax, nxm, 3);
+    const double ay_xm = ix ? sx_ay : ext(ay, nxm, 3);
+    const double az_xm = ix ? sx_az : ext(az, nxm, 3);
+    const double aym = iy ? sy_ay : ext(ay, nym, 12);
+    const double ax_ym = iy ? sy_ax : ext(ax, nym, 12);
+    const double az_ym = iy ? sy_az : ext(az, nym, 12);
+    const double azm = iz ? sz_az : ext(az, nzm, 48);
+    const double ax_zm = iz ? sz_ax : ext(ax, nzm, 48);
+    const double ay_zm = iz ? sz_ay : ext(ay, nzm, 48);
+    if (!LEVELS) wait_bricks(flags, nxm, nym, nzm, epoch, ctrl, timeout);
+    const double ex = (!ix && nxm >= 0) ? inv_e64[(size_t)nxm * kBrickCells + L.lane + 3] : 0.0;
+    const double ey = (!iy && nym >= 0) ? inv_e64[(size_t)nym * kBrickCells + L.lane + 12] : 0.0;
+    const double ez = (!iz && nzm >= 0) ? inv_e64[(size_t)nzm * kBrickCells + L.lane + 48] : 0.0;
+    const int h_own = L.lx + L.ly + L.lz;
+    double ie = 0.0;
+    int fixed = 0;
+#pragma unroll
+    for (int h = 0; h < 10; ++h) {
+        const double vx = shfl(ie, L.lane - 1), vy = shfl(ie, L.lane - 4), vz = shfl(ie, L.lane - 16);
+        if (h == h_own && dg > 0.0) {
+            const double exm = ix ? vx : ex, eym = iy ? vy : ey, ezm = iz ? vz : ez;
+            double t = dg;
+            t = t - axm * axm * exm;
+            t = t - aym * aym * eym;
+            t = t - azm * azm * ezm;
+            if (tau != 0.0) {
+                double m = 0.0;
+                m = m + axm * (ay_xm + az_xm) * exm;
+                m = m + aym * (ax_ym + az_ym) * eym;
+                m = m + azm * (ax_zm + ay_zm) * ezm;
+                t = t - tau * m;
+            }
+            if (!(t > sigma * dg)) {
+                t = dg;
+                fixed = 1;
+            }
+            ie = 1.0 / t;
+        }
+    }
+    inv_e64[o] = ie;
+    if (inv_e32) inv_e32[o] = (float)ie;
+    const int nf = wave_sum_int(fixed);
+    if (L.lane == 0 && nf) atomicAdd(n_fixed, nf);
+    if (!LEVELS) signal_brick(flags, b, epoch);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_export(Dense d, const uint8_t* mask, const int32_t* indices, Bricks bk,
+                                                const double* diag64, const double* b64, const T* ax, const T* ay,
+                                                const T* az, double* odiag, double* oax, double* oay, double* oaz,
+                                                double* ob) {
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= bk.nb) return;
+    const Lane L;
+    const int cc = bk.coord[b];
+    const int i = (cc & 1023) * kBrick + L.lx, j = ((cc >> 10) & 1023) * kBrick + L.ly,
+              k = ((cc >> 20) & 1023) * kBrick + L.lz;
+    if (i >= d.ni || j >= d.nj || k >= d.nk) return;
+    const size_t c = cell(i, j, k, d.ni, d.nj);
+    if (!(mask[c] & kFluid)) return;
+    const int n = indices[c];
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    odiag[n] = diag64[o];
+    oax[n] = (double)ax[o];
+    oay[n] = (double)ay[o];
+    oaz[n] = (double)az[o];
+    ob[n] = b64[o];
+}
+
+// ================================================================== Krylov
+// x0 = 0, r = b (Eigen: residual = rhs - mat*x with x = 0), |b|^2 partials.
+template <class T>
+__global__ __launch_bounds__(256) void k_init(Bricks bk, const double* b64, T* r, T* x, T* p0, T* p1,
+                                              double* part) {
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= bk.nb) return;
+    const size_t o = (size_t)b * kBrickCells + lane_id();
+    const double bv = b64[o];
+    r[o] = (T)bv;
+    x[o] = (T)0;
+    p0[o] = (T)0;
+    p1[o] = (T)0;
+    const double s = wave_sum(bv * bv);
+    if (lane_id() == 0) part[b] = s;
+}
+
+// Scalar recurrences of Eigen's conjugate_gradient (see the oracle's orc_pcg)
+// and of the mixed-precision refinement. One workgroup; partial sums are
+// added in a fixed order, so every scalar is deterministic.
+__global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, const double* parts, int n,
+                                                           const double* parts2) {
+    __shared__ double sh[kReduceThreads];
+    __shared__ double sh2[kReduceThreads];
+    const int t = threadIdx.x;
+    double s = 0.0, s2 = 0.0;
+    if (parts)
+        for (int q = t; q < n; q += kReduceThreads) s += parts[q];
+    if (parts2)
+        for (int q = t; q < n; q += kReduceThreads) s2 += parts2[q];
+    sh[t] = s;
+    sh2[t] = s2;
+    __syncthreads();
+    for (int o = kReduceThreads / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            sh[t] += sh[t + o];
+            sh2[t] += sh2[t + o];
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    s = sh[0];
+    s2 = sh2[0];
+    c->epoch += 1u;
+    c->ticket = 0u;
+    const bool was_done = c->done != 0;
+    switch (mode) {
+        case SC_INIT: {
+            c->bnorm2 = s;
+            c->rr = s;
+            c->threshold = fmax(c->tol2 * s, DBL_MIN);
+            c->iter = 0;
+            c->iter_total = 0;
+            c->beta = 0.0;
+            c->done = 0;
+            c->converged = 0;
+            c->status = 0;
+            if (s == 0.0) {
+                c->done = 1;
+                c->converged = 1;
+                c->last_err = 0.0;
+            } else if (s < c->threshold) {
+                c->done = 1;
+                c->converged = 1;
+                c->last_err = sqrt(c->rr / c->bnorm2);
+            }
+            break;
+        }
+        case SC_RZ_INIT:
+            if (!was_done) {
+                c->rz = s;
+                c->beta = 0.0;
+                if (!isfinite(s)) {
+                    c->done = 1;
+                    c->status = 3;
+                }
+            }
+            break;
+        case SC_ALPHA:
+            if (!was_done) {
+                c->pq = s;
+                c->alpha = c->rz / s;
+                if (!isfinite(c->alpha)) {
+                    c->done = 1;
+                    c->status = 3;
+                    c->last_err = sqrt(c->rr / c->bnorm2);
+                }
+            }
+            break;
+        case SC_RR:
+            if (!was_done) {
+                c->rr = s;
+                if (s < c->threshold) {
+                    c->done = 1;
+                    c->converged = 1;
+                    c->last_err = sqrt(s / c->bnorm2);
+                } else if (!isfinite(s)) {
+                    c->done = 1;
+                    c->status = 3;
+                }
+            }
+            break;
+        case SC_BETA:
+            if (!was_done) {
+                const double rz_new = s;
+                c->beta = rz_new / c->rz;
+                c->rz = rz_new;
+                c->iter += 1;
+                c->iter_total += 1;
+                if (!isfinite(c->beta)) {
+                    c->done = 1;
+                    c->status = 3;
+                } else if (c->iter_total >= c->max_iter) {
+                    c->done = 1;
+                    c->last_err = sqrt(c->rr / c->bnorm2);
+                }
+            }
+            break;
+        case SC_OUTER: {
+            if (c->outer == 0) c->bnorm2 = s;
+            c->rr64 = s;
+            const double thr = fmax(c->tol2 * c->bnorm2, DBL_MIN);
+            if (c->bnorm2 == 0.0) {
+                c->outer_done = 1;
+                c->converged = 1;
+                c->done = 1;
+                c->last_err = 0.0;
+            } else if (s < thr) {
+                c->outer_done = 1;
+                c->converged = 1;
+                c->done = 1;
+                c->last_err = sqrt(s / c->bnorm2);
+            } else if (c->iter_total >= c->max_iter || c->status != 0 || !isfinite(s)) {
+                c->outer_done = 1;
+                c->converged = 0;
+                c->done = 1;
+                c->last_err = sqrt(s / c->bnorm2);
+                if (!isfinite(s)) c->status = 3;
+            } else {
+                c->rr = s2;
+                c->threshold = fmax(c->inner_tol2 * s2, DBL_MIN);
+                c->iter = 0;
+                c->beta = 0.0;
+                c->done = 0;
+                c->converged = 0;
+                c->outer += 1;
+            }
+            break;
+        }
+        case SC_TRUE:
+            c->true_rr = s;
+            break;
+        default:
+            break;
+    }
+}
+
+// p = z + beta p_old (ping-pong buffers), q = A p, p.q partials. Cross-brick
+// neighbour values of p are rebuilt from z and p_old; in-brick ones come
+// through lane shuffles. Accumulation order: diag, -x, +x, -y, +y, -z, +z.
+template <class T>
+__global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
+                                              double* part, const Ctrl* ctrl) {
+    if (ctrl->done) return;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= bk.nb) return;
+    const Lane L;
+    const T beta = (T)ctrl->beta;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    const T pc = fma(beta, p_in[o], z[o]);
+    p_out[o] = pc;
+    const T axo = c.ax[o], ayo = c.ay[o], azo = c.az[o];
+    auto pext = [&](int d, int off) -> T {
+        const int nb_ = nbr_id(bk, d, b);
+        if (nb_ < 0) return (T)0;
+        const size_t e = (size_t)nb_ * kBrickCells + L.lane + off;
+        return fma(beta, p_in[e], z[e]);
+    };
+    auto aext = [&](const T* a, int d, int off) -> T {
+        const int nb_ = nbr_id(bk, d, b);
+        return nb_ >= 0 ? a[(size_t)nb_ * kBrickCells + L.lane + off] : (T)0;
+    };
+    const T sxm = shfl(pc, L.lane - 1), sxp = shfl(pc, L.lane + 1);
+    const T sym = shfl(pc, L.lane - 4), syp = shfl(pc, L.lane + 4);
+    const T szm = shfl(pc, L.lane - 16), szp = shfl(pc, L.lane + 16);
+    const T cxm = shfl(axo, L.lane - 1), cym = shfl(ayo, L.lane - 4), czm = shfl(azo, L.lane - 16);
+    const T p_xm = L.lx > 0 ? sxm : pext(0, 3);
+    const T p_xp = L.lx < 3 ? sxp : pext(1, -3);
+    const T p_ym = L.ly > 0 ? sym : pext(2, 12);
+    const T p_yp = L.ly < 3 ? syp : pext(3, -12);
+    const T p_zm = L.lz > 0 ? szm : pext(4, 48);
+    const T p_zp = L.lz < 3 ? szp : pext(5, -48);
+    const T axm = L.lx > 0 ? cxm : aext(c.ax, 0, 3);
+    const T aym = L.ly > 0 ? cym : aext(c.ay, 2, 12);
+    const T azm = L.lz > 0 ? czm : aext(c.az, 4, 48);
+    T acc = c.diag[o] * pc;
+    acc = fma(axm, p_xm, acc);
+    acc = fma(axo, p_xp, acc);
+    acc = fma(aym, p_ym, acc);
+    acc = fma(ayo, p_yp, acc);
+    acc = fma(azm, p_zm, acc);
+    acc = fma(azo, p_zp, acc);
+    q[o] = acc;
+    const double s = wave_sum((double)pc * (double)acc);
+    if (L.lane == 0) part[b] = s;
+}
+
+// Forward sweep of M^{-1} = (E+L^T)^{-1} E (E+L)^{-1}: y = (r - L y) / e, one
+// brick per wave, hyperplanes lx+ly+lz = 0..9 inside the brick. With UPDATE,
+// fused ahead of it: x += alpha p, r -= alpha q, |r|^2 partials.
+template <class T, bool UPDATE, bool LEVELS>
+__global__ __launch_bounds__(64) void k_fwd(Bricks bk, Coef<T> c, T* r, T* x, const T* p, const T* q, T* y,
+                                            double* part, Ctrl* ctrl, uint32_t* flags, int level_base,
+                                            uint64_t timeout) {
+    if (ctrl->done) return;
+    const Lane L;
+    const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
+    if (slot >= bk.nb) return;
+    const int b = bk.lvl_list[slot];
+    const uint32_t epoch = ctrl->epoch;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    T rv = r[o];
+    if (UPDATE) {
+        const T alpha = (T)ctrl->alpha;
+        const T xv = fma(alpha, p[o], x[o]);
+        rv = fma(-alpha, q[o], rv);
+        x[o] = xv;
+        r[o] = rv;
+        const double s = wave_sum((double)rv * (double)rv);
+        if (L.lane == 0) part[b] = s;
+    }
+    const T ie = c.inv_e[o];
+    const int nxm = nbr_id(bk, 0, b), nym = nbr_id(bk, 2, b), nzm = nbr_id(bk, 4, b);
+    const T sx = shfl(c.ax[o], L.lane - 1), sy = shfl(c.ay[o], L.lane - 4), sz = shfl(c.az[o], L.lane - 16);
+    const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
+    const T axm = ix ? sx : (nxm >= 0 ? c.ax[(size_t)nxm * kBrickCells + L.lane + 3] : (T)0);
+    const T aym = iy ? sy : (nym >= 0 ? c.ay[(size_t)nym * kBrickCells + L.lane + 12] : (T)0);
+    const T azm = iz ? sz : (nzm >= 0 ? c.az[(size_t)nzm * kBrickCells + L.lane + 48] : (T)0);
+    if (!LEVELS) wait_bricks(flags, nxm, nym, nzm, epoch, ctrl, timeout);
+    const T ex = (!ix && nxm >= 0) ? y[(size_t)nxm * kBrickCells + L.lane + 3] : (T)0;
+    const T ey = (!iy && nym >= 0) ? y[(size_t)nym * kBrickCells + L.lane + 12] : (T)0;
+    const T ez = (!iz && nzm >= 0) ? y[(size_t)nzm * kBrickCells + L.lane + 48] : (T)0;
+    const int h_own = L.lx + L.ly + L.lz;
+    T yv = (T)0;
+#pragma unroll
+    for (int h = 0; h < 10; ++h) {
+        const T vx = shfl(yv, L.lane - 1), vy = shfl(yv, L.lane - 4), vz = shfl(yv, L.lane - 16);
+        if (h == h_own) {
+            T t = fma(-axm, ix ? vx : ex, rv);
+            t = fma(-aym, iy ? vy : ey, t);
+            t = fma(-azm, iz ? vz : ez, t);
+            yv = t * ie;
+        }
+    }
+    y[o] = yv;
+    if (!LEVELS) signal_brick(flags, b, epoch);
+}
+
+// Backward sweep z = y - E^{-1} L^T z, in place over y, hyperplanes 9..0;
+// fused r.z partials.
+template <class T, bool LEVELS>
+__global__ __launch_bounds__(64) void k_bwd(Bricks bk, Coef<T> c, const T* r, T* y, double* part, Ctrl* ctrl,
+                                            uint32_t* flags, int level_base, uint64_t timeout) {
+    if (ctrl->done) return;
+    const Lane L;
+    const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
+    if (slot >= bk.nb) return;
+    const int b = LEVELS ? bk.lvl_list[slot] : bk.lvl_list[bk.nb - 1 - slot];
+    const uint32_t epoch = ctrl->epoch;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    const T yv = y[o];
+    const T ie = c.inv_e[o];
+    const T axo = c.ax[o], ayo = c.ay[o], azo = c.az[o];
+    const int nxp = nbr_id(bk, 1, b), nyp = nbr_id(bk, 3, b), nzp = nbr_id(bk, 5, b);
+    const bool ix = L.lx < 3, iy = L.ly < 3, iz = L.lz < 3;
+    if (!LEVELS) wait_bricks(flags, nxp, nyp, nzp, epoch, ctrl, timeout);
+    const T ex = (!ix && nxp >= 0) ? y[(size_t)nxp * kBrickCells + L.lane - 3] : (T)0;
+    const T ey = (!iy && nyp >= 0) ? y[(size_t)nyp * kBrickCells + L.lane - 12] : (T)0;
+    const T ez = (!iz && nzp >= 0) ? y[(size_t)nzp * kBrickCells + L.lane - 48] : (T)0;
+    const int h_own = L.lx + L.ly + L.lz;
+    T zv = (T)0;
+#pragma unroll
+    for (int h = 9; h >= 0; --h) {
+        const T vx = shfl(zv, L.lane + 1), vy = shfl(zv, L.lane + 4), vz = shfl(zv, L.lane + 16);
+        if (h == h_own) {
+            T t = axo * (ix ? vx : ex);
+            t = fma(ayo, iy ? vy : ey, t);
+            t = fma(azo, iz ? vz : ez, t);
+            zv = fma(-ie, t, yv);
+        }
+    }
+    y[o] = zv;
+    const double s = wave_sum((double)r[o] * (double)zv);
+    if (L.lane == 0) part[b] = s;
+    if (!LEVELS) signal_brick(flags, b, epoch);
+}
+
+// r64 = b - A x64 in FP64 (order as the oracle's spmv: diag, -x, +x, -y, +y,
+// -z, +z). Optionally hands float(r64) to the FP32 inner solve.
+template <class TC>
+__global__ __launch_bounds__(256) void k_resid64(Bricks bk, const double* diag64, const TC* ax, const TC* ay,
+                                                 const TC* az, const double* b64, const double* x64, double* part,
+                                                 double* part2, float* r32, float* x32, float* p0, float* p1,
+                                                 const Ctrl* ctrl) {
+    if (ctrl && ctrl->outer_done) return;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= bk.nb) return;
+    const Lane L;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    const double xc = x64[o];
+    const double axo = (double)ax[o], ayo = (double)ay[o], azo = (double)az[o];
+    auto xext = [&](int d, int off) -> double {
+        const int nb_ = nbr_id(bk, d, b);
+        return nb_ >= 0 ? x64[(size_t)nb_ * kBrickCells + L.lane + off] : 0.0;
+    };
+    auto aext = [&](const TC* a, int d, int off) -> double {
+        const int nb_ = nbr_id(bk, d, b);
+        return nb_ >= 0 ? (double)a[(size_t)nb_ * kBrickCells + L.lane + off] : 0.0;
+    };
+    const double sxm = shfl(xc, L.lane - 1), sxp = shfl(xc, L.lane + 1);
+    const double sym = shfl(xc, L.lane - 4), syp = shfl(xc, L.lane + 4);
+    const double szm = shfl(xc, L.lane - 16), szp = shfl(xc, L.lane + 16);
+    const double cxm = shfl(axo, L.lane - 1), cym = shfl(ayo, L.lane - 4), czm = shfl(azo, L.lane - 16);
+    const double x_xm = L.lx > 0 ? sxm : xext(0, 3);
+    const double x_xp = L.lx < 3 ? sxp : xext(1, -3);
+    const double x_ym = L.ly > 0 ? sym : xext(2, 12);
+    const double x_yp = L.ly < 3 ? syp : xext(3, -12);
+    const double x_zm = L.lz > 0 ? szm : xext(4, 48);
+    const double x_zp = L.lz < 3 ? szp : xext(5, -48);
+    const double axm = L.lx > 0 ? cxm : aext(ax, 0, 3);
+    const double aym = L.ly > 0 ? cym : aext(ay, 2, 12);
+    const double azm = L.lz > 0 ? czm : aext(az, 4, 48);
+    double acc = diag64[o] * xc;
+    acc = acc + axm * x_xm;
+    acc = acc + axo * x_xp;
+    acc = acc + aym * x_ym;
+    acc = acc + ayo * x_yp;
+    acc = acc + azm * x_zm;
+    acc = acc + azo * x_zp;
+    const double rv = b64[o] - acc;
+    const double s = wave_sum(rv * rv);
+    if (L.lane == 0) part[b] = s;
+    if (r32) {
+        const float rf = (float)rv;
+        r32[o] = rf;
+        x32[o] = 0.0f;
+        p0[o] = 0.0f;
+        p1[o] = 0.0f;
+        const double s2 = wave_sum((double)rf * (double)rf);
+        if (L.lane == 0) part2[b] = s2;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_xupdate(Bricks bk, double* x64, const float* x32, const Ctrl* ctrl) {
+    if (ctrl->outer_done) return;
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= bk.nb) return;
+    const size_t o = (size_t)b * kBrickCells + lane_id();
+    x64[o] = x64[o] + (double)x32[o];
+}
+
+// ================================================================== finish
+// projection.cpp:85-91 with background 0 (simulator.cpp:110).
+__global__ __launch_bounds__(256) void k_scatter(Dense d, const uint8_t* mask, Bricks bk, const double* x,
+                                                 float* pressure) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= d.ni || j >= d.nj) return;
+    const size_t c = cell(i, j, k, d.ni, d.nj);
+    float p = 0.0f;
+    if (mask[c] & kFluid) {
+        const int b = bk.map[(size_t)(i >> 2) + (size_t)bk.nbx * ((size_t)(j >> 2) + (size_t)bk.nby * (k >> 2))];
+        const int l = (i & 3) + 4 * (j & 3) + 16 * (k & 3);
+        p = (float)x[(size_t)b * kBrickCells + l];
+    }
+    pressure[c] = p;
+}
+
+// projection.cpp:93-134: one face component per launch (AXIS 0 u, 1 v, 2 w).
+// Loop ranges of the reference: normal index in [1, n_axis-1], tangential
+// indices in [1, n-2].
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_update(Dense d, float dt, const float* pr) {
+    const int fi = blockIdx.x * 64 + threadIdx.x;
+    const int fj = blockIdx.y * 4 + threadIdx.y;
+    const int fk = blockIdx.z;
+    const int ni = d.ni, nj = d.nj, nk = d.nk;
+    const int ei = AXIS == 0 ? ni : ni - 1;  // exclusive upper bounds
+    const int ej = AXIS == 1 ? nj : nj - 1;
+    const int ek = AXIS == 2 ? nk : nk - 1;
+    if (fi < 1 || fj < 1 || fk < 1 || fi >= ei || fj >= ej || fk >= ek) return;
+    const int di = AXIS == 0, dj = AXIS == 1, dk = AXIS == 2;
+    const size_t c1 = cell(fi, fj, fk, ni, nj);
+    const size_t c0 = cell(fi - di, fj - dj, fk - dk, ni, nj);
+    const float ph1 = d.phi[c1], ph0 = d.phi[c0];
+    if (ph1 < 0.0f || ph0 < 0.0f) {
+        float theta = 1.0f;
+        if (ph1 >= 0.0f || ph0 >= 0.0f) theta = fraction_inside2(ph0, ph1);
+        theta = clamp_theta(theta);
+        float* f;
+        if (AXIS == 0)
+            f = d.u + cell(fi, fj, fk, ni + 1, nj);
+        else if (AXIS == 1)
+            f = d.v + cell(fi, fj, fk, ni, nj + 1);
+        else
+            f = d.w + cell(fi, fj, fk, ni, nj);
+        const float num = dt * (pr[c1] - pr[c0]);
+        const float den = d.dx * theta;
+        *f = *f - num / den;
+    }
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+// ================================================================== launchers
+hipError_t launch_mask(const Dense& d, uint8_t* mask, int32_t* err, hipStream_t s) {
+    dim3 grid(cdiv(d.ni, 64), cdiv(d.nj, 4), d.nk), block(64, 4);
+    hipLaunchKernelGGL(k_mask, grid, block, 0, s, d, mask, err);
+    return hipGetLastError();
+}
+
+int64_t scan_blocks(int64_t n) { return (n + kScanChunk - 1) / kScanChunk; }
+
+hipError_t launch_scan(const uint8_t* flags, int64_t n, uint8_t bit, int32_t* out, int32_t fill, int32_t* blk_cnt,
+                       int32_t* blk_off, int64_t* total, hipStream_t s) {
+    const int64_t g = scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_count, dim3((unsigned)g), dim3(kScanBlock), 0, s, flags, n, bit, blk_cnt);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, blk_cnt, g, blk_off, total);
+    hipLaunchKernelGGL(k_scan_write, dim3((unsigned)g), dim3(kScanBlock), 0, s, flags, n, bit, blk_off, out, fill);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_flags(const Dense& d, const uint8_t* mask, int nbx, int nby, int nbz, uint8_t* bflag,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_brick_flags, dim3(cdiv((int64_t)nbx * nby * nbz, 256)), dim3(256), 0, s, d, mask, nbx,
+                       nby, nbz, bflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_meta(const int32_t* map, int nbx, int nby, int nbz, int32_t* coord, int32_t* lvl_cnt,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_brick_meta, dim3(cdiv((int64_t)nbx * nby * nbz, 256)), dim3(256), 0, s, map, nbx, nby,
+                       nbz, coord, lvl_cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_scan(const int32_t* lvl_cnt, int nlev, int32_t* lvl_off, int32_t* lvl_fill,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_level_scan, dim3(1), dim3(64), 0, s, lvl_cnt, nlev, lvl_off, lvl_fill);
+    return hipGetLastError();
+}
+
+hipError_t launch_brick_lists(const int32_t* map, const int32_t* coord, int nbx, int nby, int nbz, int nb,
+                              const int32_t* lvl_off, int32_t* lvl_fill, int32_t* lvl_list, int32_t* nbr,
+                              hipStream_t s) {
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_brick_lists, dim3(cdiv(nb, 256)), dim3(256), 0, s, map, coord, nbx, nby, nbz, nb,
+                       lvl_off, lvl_fill, lvl_list, nbr);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const Bricks& bk, double* diag64,
+                           double* b64, T* diag, T* ax, T* ay, T* az, hipStream_t s) {
+    const float scale = d.dx / dt;  // projection.cpp:82, float division
+    hipLaunchKernelGGL(k_assemble<T>, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, d, scale, mask, bk, diag64, b64,
+                       diag, ax, ay, az);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, const T* ay, const T* az, double tau,
+                         double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl, uint32_t* flags,
+                         int32_t* n_fixed, int level, int level_base, int level_count, uint64_t timeout_ticks,
+                         hipStream_t s) {
+    if (level < 0) {
+        hipLaunchKernelGGL((k_factor<T, false>), dim3(bk.nb), dim3(64), 0, s, bk, diag64, ax, ay, az, tau, sigma,
+                           inv_e64, inv_e32, ctrl, flags, n_fixed, 0, timeout_ticks);
+    } else if (level_count > 0) {
+        hipLaunchKernelGGL((k_factor<T, true>), dim3(level_count), dim3(64), 0, s, bk, diag64, ax, ay, az, tau,
+                           sigma, inv_e64, inv_e32, ctrl, flags, n_fixed, level_base, timeout_ticks);
+    }
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_export(const Dense& d, const uint8_t* mask, const int32_t* indices, const Bricks& bk,
+                         const double* diag64, const double* b64, const T* ax, const T* ay, const T* az,
+                         double* odiag, double* oax, double* oay, double* oaz, double* ob, hipStream_t s) {
+    hipLaunchKernelGGL(k_export<T>, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, d, mask, indices, bk, diag64, b64, ax,
+                       ay, az, odiag, oax, oay, oaz, ob);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T* p1, double* part, Ctrl* ctrl,
+                       hipStream_t s) {
+    (void)ctrl;
+    hipLaunchKernelGGL(k_init<T>, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, b64, r, x, p0, p1, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2, hipStream_t s) {
+    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(kReduceThreads), 0, s, ctrl, mode, parts, n, parts2);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q, double* part,
+                       Ctrl* ctrl, hipStream_t s) {
+    hipLaunchKernelGGL(k_spmv<T>, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, c, z, p_in, p_out, q, part,
+                       (const Ctrl*)ctrl);
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y, double* part,
+                      Ctrl* ctrl, uint32_t* flags, int update, int level, int level_base, int level_count,
+                      uint64_t timeout_ticks, hipStream_t s) {
+    if (level < 0) {
+        if (update)
+            hipLaunchKernelGGL((k_fwd<T, true, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
+                               ctrl, flags, 0, timeout_ticks);
+        else
+            hipLaunchKernelGGL((k_fwd<T, false, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
+                               ctrl, flags, 0, timeout_ticks);
+    } else if (level_count > 0) {
+        if (update)
+            hipLaunchKernelGGL((k_fwd<T, true, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y,
+                               part, ctrl, flags, level_base, timeout_ticks);
+        else
+            hipLaunchKernelGGL((k_fwd<T, false, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y,
+                               part, ctrl, flags, level_base, timeout_ticks);
+    }
+    return hipGetLastError();
+}
+
+template <class T>
+hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
+                      uint32_t* flags, int level, int level_base, int level_count, uint64_t timeout_ticks,
+                      hipStream_t s) {
+    if (level < 0) {
+        hipLaunchKernelGGL((k_bwd<T, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, y, part, ctrl, flags, 0,
+                           timeout_ticks);
+    } else if (level_count > 0) {
+        hipLaunchKernelGGL((k_bwd<T, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, flags,
+                           level_base, timeout_ticks);
+    }
+    return hipGetLastError();
+}
+
+template <class TC>
+hipError_t launch_resid64(const Bricks& bk, const double* diag64, const TC* ax, const TC* ay, const TC* az,
+                          const double* b64, const double* x64, double* part, double* part2, float* r32, float* x32,
+                          float* p0, float* p1, Ctrl* ctrl, hipStream_t s) {
+    hipLaunchKernelGGL(k_resid64<TC>, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, diag64, ax, ay, az, b64, x64,
+                       part, part2, r32, x32, p0, p1, (const Ctrl*)ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_xupdate(const Bricks& bk, double* x64, const float* x32, Ctrl* ctrl, hipStream_t s) {
+    hipLaunchKernelGGL(k_xupdate, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, x64, x32, (const Ctrl*)ctrl);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(const Dense& d, const uint8_t* mask, const Bricks& bk, const double* x, float* pressure,
+                          hipStream_t s) {
+    dim3 grid(cdiv(d.ni, 64), cdiv(d.nj, 4), d.nk), block(64, 4);
+    hipLaunchKernelGGL(k_scatter, grid, block, 0, s, d, mask, bk, x, pressure);
+    return hipGetLastError();
+}
+
+hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStream_t s) {
+    dim3 block(64, 4);
+    hipLaunchKernelGGL(k_update<0>, dim3(cdiv(d.ni + 1, 64), cdiv(d.nj, 4), d.nk), block, 0, s, d, dt, pressure);
+    hipLaunchKernelGGL(k_update<1>, dim3(cdiv(d.ni, 64), cdiv(d.nj + 1, 4), d.nk), block, 0, s, d, dt, pressure);
+    hipLaunchKernelGGL(k_update<2>, dim3(cdiv(d.ni, 64), cdiv(d.nj, 4), d.nk + 1), block, 0, s, d, dt, pressure);
+    return hipGetLastError();
+}
+
+// explicit instantiations
+#define ELQ_INST(T)                                                                                                 \
+    template hipError_t launch_assemble<T>(const Dense&, float, const uint8_t*, const Bricks&, double*, double*, T*, \
+                                           T*, T*, T*, hipStream_t);                                                \
+    template hipError_t launch_factor<T>(const Bricks&, const double*, const T*, const T*, const T*, double, double, \
+                                         double*, float*, Ctrl*, uint32_t*, int32_t*, int, int, int, uint64_t,      \
+                                         hipStream_t);                                                              \
+    template hipError_t launch_export<T>(const Dense&, const uint8_t*, const int32_t*, const Bricks&, const double*, \
+                                         const double*, const T*, const T*, const T*, double*, double*, double*,    \
+                                         double*, double*, hipStream_t);                                            \
+    template hipError_t launch_init<T>(const Bricks&, const double*, T*, T*, T*, T*, double*, Ctrl*, hipStream_t);  \
+    template hipError_t launch_spmv<T>(const Bricks&, const Coef<T>&, const T*, const T*, T*, T*, double*, Ctrl*,   \
+                                       hipStream_t);                                                                \
+    template hipError_t launch_fwd<T>(const Bricks&, const Coef<T>&, T*, T*, const T*, const T*, T*, double*, Ctrl*, \
+                                      uint32_t*, int, int, int, int, uint64_t, hipStream_t);                        \
+    template hipError_t launch_bwd<T>(const Bricks&, const Coef<T>&, const T*, T*, double*, Ctrl*, uint32_t*, int,  \
+                                      int, int, uint64_t, hipStream_t);                                             \
+    template hipError_t launch_resid64<T>(const Bricks&, const double*, const T*, const T*, const T*, const double*, \
+                                          const double*, double*, double*, float*, float*, float*, float*, Ctrl*,   \
+                                          hipStream_t);
+ELQ_INST(float)
+ELQ_INST(double)
+#undef ELQ_INST
+
+}  // namespace elq

@@ -1,0 +1,112 @@
+// Host-side launch wrappers for the gfx950 kernels of the pressure projection.
+// Every wrapper enqueues on the given stream and returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "elq_internal.hpp"
+
+namespace elq {
+
+// Dense inputs/outputs of one projection (device pointers, reference layout).
+struct Dense {
+    int ni, nj, nk;
+    float dx;
+    const float* phi;
+    float* u;
+    float* v;
+    float* w;
+    const float* uw;
+    const float* vw;
+    const float* ww;
+};
+
+// Brick metadata (device pointers).
+struct Bricks {
+    int nbx, nby, nbz;  // brick grid
+    int nb;             // active bricks
+    int nlev;           // brick levels (nbx + nby + nbz - 2)
+    const int32_t* map;      // nbx*nby*nbz -> active id or -1
+    const int32_t* coord;    // nb packed (bx | by << 10 | bz << 20)
+    const int32_t* nbr;      // 6*nb: [-x,+x,-y,+y,-z,+z][b]
+    const int32_t* lvl_list; // nb, sorted by bx+by+bz
+};
+
+// Coefficients of the stored system (device pointers), T = storage type.
+template <class T>
+struct Coef {
+    const T* diag;
+    const T* ax;  // coupling of a cell to its +x neighbour (0 if none)
+    const T* ay;
+    const T* az;
+    const T* inv_e;  // IC factor pivots 1/e
+};
+
+// ---------------------------------------------------------------- setup
+hipError_t launch_mask(const Dense& d, uint8_t* mask, int32_t* err, hipStream_t s);
+// Exclusive scan of (flags[i] & bit) != 0 in index order: out[i] = rank or fill.
+hipError_t launch_scan(const uint8_t* flags, int64_t n, uint8_t bit, int32_t* out, int32_t fill,
+                       int32_t* blk_cnt, int32_t* blk_off, int64_t* total, hipStream_t s);
+int64_t scan_blocks(int64_t n);
+hipError_t launch_brick_flags(const Dense& d, const uint8_t* mask, int nbx, int nby, int nbz,
+                              uint8_t* bflag, hipStream_t s);
+hipError_t launch_brick_meta(const int32_t* map, int nbx, int nby, int nbz, int32_t* coord,
+                             int32_t* lvl_cnt, hipStream_t s);
+hipError_t launch_level_scan(const int32_t* lvl_cnt, int nlev, int32_t* lvl_off, int32_t* lvl_fill,
+                             hipStream_t s);
+hipError_t launch_brick_lists(const int32_t* map, const int32_t* coord, int nbx, int nby, int nbz, int nb,
+                              const int32_t* lvl_off, int32_t* lvl_fill, int32_t* lvl_list,
+                              int32_t* nbr, hipStream_t s);
+template <class T>
+hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const Bricks& bk, double* diag64,
+                           double* b64, T* diag, T* ax, T* ay, T* az, hipStream_t s);
+// IC(0)/MIC(0) factor in FP64 arithmetic; writes inv_e64 and, if inv_e32 is
+// non-null, its float rounding. level < 0: wavefront launch over all bricks.
+template <class T>
+hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, const T* ay, const T* az,
+                         double tau, double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl,
+                         uint32_t* flags, int32_t* n_fixed, int level, int level_base, int level_count,
+                         uint64_t timeout_ticks, hipStream_t s);
+// Natural-order export of the assembled system (elq_build_matrix).
+template <class T>
+hipError_t launch_export(const Dense& d, const uint8_t* mask, const int32_t* indices, const Bricks& bk,
+                         const double* diag64, const double* b64, const T* ax, const T* ay, const T* az,
+                         double* odiag, double* oax, double* oay, double* oaz, double* ob, hipStream_t s);
+
+// ---------------------------------------------------------------- Krylov
+template <class T>
+hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T* p1, double* part,
+                       Ctrl* ctrl, hipStream_t s);
+hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2,
+                         hipStream_t s);
+template <class T>
+hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q,
+                       double* part, Ctrl* ctrl, hipStream_t s);
+// Forward sweep y = (E+L)^{-1} r, fused with x += alpha p, r -= alpha q and
+// |r|^2 partials when update != 0.
+template <class T>
+hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
+                      double* part, Ctrl* ctrl, uint32_t* flags, int update, int level, int level_base,
+                      int level_count, uint64_t timeout_ticks, hipStream_t s);
+// Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
+template <class T>
+hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
+                      uint32_t* flags, int level, int level_base, int level_count, uint64_t timeout_ticks,
+                      hipStream_t s);
+// r64 = b64 - A x64 in FP64 (couplings TC read exactly); partials |r64|^2 and,
+// when r32 != nullptr, |float(r64)|^2 with r32 = float(r64) and x32/p0/p1 zeroed.
+template <class TC>
+hipError_t launch_resid64(const Bricks& bk, const double* diag64, const TC* ax, const TC* ay, const TC* az,
+                          const double* b64, const double* x64, double* part, double* part2, float* r32,
+                          float* x32, float* p0, float* p1, Ctrl* ctrl, hipStream_t s);
+hipError_t launch_xupdate(const Bricks& bk, double* x64, const float* x32, Ctrl* ctrl, hipStream_t s);
+
+// ---------------------------------------------------------------- finish
+hipError_t launch_scatter(const Dense& d, const uint8_t* mask, const Bricks& bk, const double* x,
+                          float* pressure, hipStream_t s);
+hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStream_t s);
+
+}  // namespace elq

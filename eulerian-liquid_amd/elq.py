@@ -1,0 +1,428 @@
+"""Python host binding of libelq.so (include/elq.h) — ctypes + numpy.
+
+Mirrors the reference's projection interface
+(``include/nama/project/3d/fluid/projection.h:10-22`` and
+``Simulator::solve_pressure``, ``src/project/3d/eulerian_liquid/simulator.cpp:96-112``):
+
+* :func:`solve_pressure` — the whole projection; updates ``u, v, w`` in place,
+  returns ``(pressure, indices, stats)``; raises ``RuntimeError("No fluid")``
+  like ``projection.cpp:24-26``.
+* :meth:`Context.assign_indices`, :meth:`Context.build_matrix`,
+  :meth:`Context.update_velocity_using_pressure` — the free functions.
+
+Every numeric step runs on the GPU through the C ABI. Loading the library or
+creating a context fails loudly (``RuntimeError``) when ``libelq.so`` is
+missing or no gfx950 device is present; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libelq.so")
+UNKNOWN_INDEX = 2147483647
+
+OK, NO_FLUID, NOT_CONVERGED, BREAKDOWN, DEVICE, ADMISSION, INVALID_ARG, BOUNDARY, BUSY = range(9)
+STATUS_NAMES = {0: "OK", 1: "NO_FLUID", 2: "NOT_CONVERGED", 3: "BREAKDOWN", 4: "DEVICE", 5: "ADMISSION",
+                6: "INVALID_ARG", 7: "BOUNDARY", 8: "BUSY"}
+FP64, MIXED = 0, 1
+IC0, MIC0 = 0, 1
+WAVEFRONT, LEVELS = 0, 1
+
+
+class Grid(C.Structure):
+    _fields_ = [("ni", C.c_int32), ("nj", C.c_int32), ("nk", C.c_int32), ("dx", C.c_float),
+                ("level_set", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p), ("w", C.c_void_p),
+                ("u_weight", C.c_void_p), ("v_weight", C.c_void_p), ("w_weight", C.c_void_p),
+                ("pressure", C.c_void_p), ("indices", C.c_void_p)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("precision", C.c_int32),
+                ("precond", C.c_int32), ("tau", C.c_double), ("sigma", C.c_double),
+                ("inner_tol", C.c_double), ("schedule", C.c_int32), ("check_every", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_unknowns", C.c_int64), ("n_bricks", C.c_int64), ("iterations", C.c_int32),
+                ("outer_iterations", C.c_int32), ("error", C.c_double), ("true_rel_residual", C.c_double),
+                ("status", C.c_int32), ("converged", C.c_int32), ("t_setup_ms", C.c_double),
+                ("t_solve_ms", C.c_double), ("t_update_ms", C.c_double), ("t_total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_slots", C.c_int32), ("max_queue", C.c_int32),
+                ("opts", Opts)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("iterations", C.c_int32),
+                ("outer_iterations", C.c_int32), ("error", C.c_double), ("t_submit_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Loads libelq.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libelq.so is missing at {LIB_PATH}: build it (python -c 'import __graft_entry__ as g; g.build()')")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f32, f64, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_size_t
+    L.elq_default_opts.argtypes = [C.POINTER(Opts)]
+    L.elq_default_opts.restype = None
+    L.elq_ctx_create.argtypes = [C.c_int, sz, C.POINTER(vp)]
+    L.elq_ctx_destroy.argtypes = [vp]
+    L.elq_ctx_destroy.restype = None
+    L.elq_ctx_last_error.argtypes = [vp]
+    L.elq_ctx_last_error.restype = C.c_char_p
+    L.elq_project.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(Opts), C.POINTER(Stats)]
+    L.elq_project_device.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(Opts), C.POINTER(Stats)]
+    L.elq_assign_indices.argtypes = [vp, C.POINTER(Grid), vp, C.POINTER(i64)]
+    L.elq_build_matrix.argtypes = [vp, C.POINTER(Grid), f32, vp, vp, vp, vp, vp]
+    L.elq_update_velocity.argtypes = [vp, C.POINTER(Grid), f32]
+    L.elq_device_alloc.argtypes = [vp, sz, C.POINTER(vp)]
+    L.elq_device_free.argtypes = [vp, vp]
+    L.elq_memcpy_h2d.argtypes = [vp, vp, vp, sz]
+    L.elq_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+    L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]
+    L.elq_set_profiling.argtypes = [vp, C.c_int]
+    L.elq_set_profiling.restype = None
+    L.elq_server_create.argtypes = [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]
+    L.elq_server_destroy.argtypes = [vp]
+    L.elq_server_destroy.restype = None
+    L.elq_server_submit.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(i64)]
+    L.elq_server_step.argtypes = [vp]
+    L.elq_server_poll.argtypes = [vp, C.POINTER(Result), C.c_int]
+    L.elq_server_footprint.argtypes = [vp, i32, i32, i32]
+    L.elq_server_footprint.restype = sz
+    L.elq_scene_level_set.argtypes = [i32, i32, i32, f32, f32, i32, vp, vp]
+    L.elq_scene_weights.argtypes = [i32, i32, i32, vp, vp, vp]
+    L.elq_scene_add_gravity.argtypes = [i32, i32, i32, vp, f32]
+    L.elq_scene_valid_and_zero.argtypes = [C.POINTER(Grid)]
+    L.elq_scene_random_spheres.argtypes = [C.POINTER(C.c_uint64), i32, f64, f64, f64, f64, f64, f64, f64, vp]
+    L.elq_scene_random_velocity.argtypes = [C.POINTER(C.c_uint64), i32, i32, i32, f64, vp, vp, vp]
+    L.elq_uniform01.argtypes = [C.POINTER(C.c_uint64)]
+    L.elq_uniform01.restype = f64
+    _lib = L
+    return L
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    load().elq_default_opts(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise AttributeError(k)
+        setattr(o, k, v)
+    return o
+
+
+def _ptr(a):
+    return a.ctypes.data if a is not None else None
+
+
+# --------------------------------------------------------------------------- scenes (host set-up steps)
+@dataclass
+class Scene:
+    """Inputs of one projection in the reference layout (float32, i fastest)."""
+    ni: int
+    nj: int
+    nk: int
+    dx: float
+    dt: float
+    phi: np.ndarray
+    u: np.ndarray
+    v: np.ndarray
+    w: np.ndarray
+    uw: np.ndarray
+    vw: np.ndarray
+    ww: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    def copy(self) -> "Scene":
+        return Scene(self.ni, self.nj, self.nk, self.dx, self.dt, self.phi.copy(), self.u.copy(),
+                     self.v.copy(), self.w.copy(), self.uw.copy(), self.vw.copy(), self.ww.copy(),
+                     dict(self.meta))
+
+    def grid(self, pressure=None, indices=None) -> Grid:
+        g = Grid()
+        g.ni, g.nj, g.nk, g.dx = self.ni, self.nj, self.nk, self.dx
+        g.level_set, g.u, g.v, g.w = _ptr(self.phi), _ptr(self.u), _ptr(self.v), _ptr(self.w)
+        g.u_weight, g.v_weight, g.w_weight = _ptr(self.uw), _ptr(self.vw), _ptr(self.ww)
+        g.pressure, g.indices = _ptr(pressure), _ptr(indices)
+        return g
+
+    @property
+    def cells(self):
+        return self.ni * self.nj * self.nk
+
+
+def make_scene(ni, nj, nk, slab, spheres, dt=0.01, width=1.0) -> Scene:
+    """create_grid + emitters + init_scene + compute_weights; zero velocity."""
+    L = load()
+    dx = float(np.float32(np.float32(width) / np.float32(ni)))
+    phi = np.empty(ni * nj * nk, np.float32)
+    sph = np.ascontiguousarray(np.asarray(spheres, np.float32).reshape(-1))
+    rc = L.elq_scene_level_set(ni, nj, nk, dx, np.float32(slab if slab is not None else -1e38),
+                               len(sph) // 4, _ptr(sph) if len(sph) else None, _ptr(phi))
+    _check_rc(rc, None)
+    uw = np.empty((ni + 1) * nj * nk, np.float32)
+    vw = np.empty(ni * (nj + 1) * nk, np.float32)
+    ww = np.empty(ni * nj * (nk + 1), np.float32)
+    _check_rc(L.elq_scene_weights(ni, nj, nk, _ptr(uw), _ptr(vw), _ptr(ww)), None)
+    z = lambda n: np.zeros(n, np.float32)
+    return Scene(ni, nj, nk, dx, dt, phi, z(uw.size), z(vw.size), z(ww.size), uw, vw, ww,
+                 {"slab": slab, "spheres": [tuple(map(float, s)) for s in np.asarray(spheres).reshape(-1, 4)]})
+
+
+def add_gravity(s: Scene, dt: float):
+    _check_rc(load().elq_scene_add_gravity(s.ni, s.nj, s.nk, _ptr(s.v), np.float32(dt)), None)
+
+
+def valid_and_zero(s: Scene):
+    g = s.grid()
+    _check_rc(load().elq_scene_valid_and_zero(C.byref(g)), None)
+
+
+class Rng:
+    """SplitMix64 stream used by every synthetic workload."""
+
+    def __init__(self, seed: int):
+        self.state = C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF)
+
+    def uniform(self) -> float:
+        return load().elq_uniform01(C.byref(self.state))
+
+    def spheres(self, count, c_lo, c_hi, r_lo, r_hi, lengths=(1.0, 1.0, 1.0)):
+        out = np.zeros(4 * count, np.float32)
+        _check_rc(load().elq_scene_random_spheres(C.byref(self.state), count, c_lo, c_hi, r_lo, r_hi,
+                                                  lengths[0], lengths[1], lengths[2],
+                                                  _ptr(out) if count else None), None)
+        return out.reshape(-1, 4)
+
+    def velocity(self, s: Scene, amp: float):
+        _check_rc(load().elq_scene_random_velocity(C.byref(self.state), s.ni, s.nj, s.nk, amp, _ptr(s.u),
+                                                   _ptr(s.v), _ptr(s.w)), None)
+
+
+def scene_drop(n: int, dt: float = 0.01) -> Scene:
+    """Slab y < 0.4 plus a sphere of radius 0.15 at (0.5, 0.7, 0.5); zero
+    velocity, one gravity step, then find_valid_velocity and
+    set_invalid_velocity_zero (apps/eulerian_liquid_3d.cpp:115-118)."""
+    s = make_scene(n, n, n, 0.4, [(0.5, 0.7, 0.5, 0.15)], dt)
+    add_gravity(s, dt)
+    valid_and_zero(s)
+    s.meta["workload"] = f"drop{n}"
+    return s
+
+
+def scene_spheres(n: int, seed: int, n_spheres: int = 32, dt: float = 0.01, vel_amp: float = 0.5) -> Scene:
+    """Slab y < 0.3 plus seeded spheres (centres U[0.15,0.85]^3, radii
+    U[0.03,0.08]); face velocities U[-amp, amp]; validity + zeroing."""
+    rng = Rng(seed)
+    sph = rng.spheres(n_spheres, 0.15, 0.85, 0.03, 0.08)
+    s = make_scene(n, n, n, 0.3, sph, dt)
+    rng.velocity(s, vel_amp)
+    valid_and_zero(s)
+    s.meta["workload"] = f"spheres{n}_s{seed}_k{n_spheres}"
+    return s
+
+
+SERVE_DIMS = (32, 48, 64, 96, 128)
+
+
+def scene_request(seed: int, dt: float = 0.01) -> Scene:
+    """One serving request: dims per axis from {32,48,64,96,128}, slab height
+    U[0.2,0.45] of the y extent, 1-8 spheres (centres U[0.2,0.8] of each
+    extent, radii U[0.05,0.2]), face velocities U[-1,1]."""
+    rng = Rng(seed)
+    dims = [SERVE_DIMS[min(4, int(rng.uniform() * 5))] for _ in range(3)]
+    ni, nj, nk = dims
+    lengths = (1.0, nj / ni, nk / ni)
+    nsph = 1 + min(7, int(rng.uniform() * 8))
+    slab = np.float32(0.2 * lengths[1] + (0.45 - 0.2) * lengths[1] * rng.uniform())
+    sph = rng.spheres(nsph, 0.2, 0.8, 0.05, 0.2, lengths)
+    s = make_scene(ni, nj, nk, float(slab), sph, dt)
+    rng.velocity(s, 1.0)
+    valid_and_zero(s)
+    s.meta["workload"] = f"request_s{seed}_{ni}x{nj}x{nk}"
+    return s
+
+
+# --------------------------------------------------------------------------- context
+def _check_rc(rc, ctx):
+    if rc == OK:
+        return
+    msg = STATUS_NAMES.get(rc, str(rc))
+    if ctx is not None:
+        msg += ": " + load().elq_ctx_last_error(ctx).decode()
+    if rc == NO_FLUID:
+        raise RuntimeError("No fluid")
+    raise ElqError(rc, msg)
+
+
+class ElqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class Context:
+    """One device context (owns the library's device buffers)."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        L = load()
+        h = C.c_void_p()
+        rc = L.elq_ctx_create(device, hbm_budget, C.byref(h))
+        if rc != OK:
+            raise RuntimeError(f"elq_ctx_create failed ({STATUS_NAMES.get(rc, rc)}): no usable gfx950 device")
+        self.h = h
+        self.L = L
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.elq_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def last_error(self) -> str:
+        return self.L.elq_ctx_last_error(self.h).decode()
+
+    # ---- projection
+    def solve_pressure(self, s: Scene, opts: Opts | None = None, raise_on_error=True):
+        """Projects s in place (u, v, w updated); returns (pressure, indices, stats)."""
+        p = np.zeros(s.cells, np.float32)
+        idx = np.zeros(s.cells, np.int32)
+        st = Stats()
+        g = s.grid(p, idx)
+        rc = self.L.elq_project(self.h, C.byref(g), np.float32(s.dt), C.byref(opts) if opts else None, C.byref(st))
+        if raise_on_error and rc not in (OK, NOT_CONVERGED):
+            _check_rc(rc, self.h)
+        return p, idx, st
+
+    def project_device(self, g: Grid, dt: float, opts: Opts | None = None) -> tuple[int, Stats]:
+        st = Stats()
+        rc = self.L.elq_project_device(self.h, C.byref(g), np.float32(dt), C.byref(opts) if opts else None,
+                                       C.byref(st))
+        return rc, st
+
+    def assign_indices(self, s: Scene):
+        idx = np.zeros(s.cells, np.int32)
+        n = C.c_int64(0)
+        g = s.grid()
+        rc = self.L.elq_assign_indices(self.h, C.byref(g), _ptr(idx), C.byref(n))
+        _check_rc(rc, self.h)
+        return n.value, idx
+
+    def build_matrix(self, s: Scene, n: int):
+        out = [np.zeros(n, np.float64) for _ in range(5)]
+        g = s.grid()
+        rc = self.L.elq_build_matrix(self.h, C.byref(g), np.float32(s.dt), *[_ptr(a) for a in out])
+        _check_rc(rc, self.h)
+        return dict(zip(("diag", "ax", "ay", "az", "b"), out))
+
+    def update_velocity_using_pressure(self, s: Scene, pressure: np.ndarray):
+        pressure = np.ascontiguousarray(pressure, np.float32)
+        g = s.grid(pressure)
+        _check_rc(self.L.elq_update_velocity(self.h, C.byref(g), np.float32(s.dt)), self.h)
+
+    # ---- device residency
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        _check_rc(self.L.elq_device_alloc(self.h, nbytes, C.byref(p)), self.h)
+        return p.value
+
+    def free(self, ptr: int):
+        _check_rc(self.L.elq_device_free(self.h, ptr), self.h)
+
+    def h2d(self, dst: int, a: np.ndarray):
+        _check_rc(self.L.elq_memcpy_h2d(self.h, dst, _ptr(a), a.nbytes), self.h)
+
+    def d2h(self, a: np.ndarray, src: int):
+        _check_rc(self.L.elq_memcpy_d2h(self.h, _ptr(a), src, a.nbytes), self.h)
+
+    def set_profiling(self, on: bool):
+        self.L.elq_set_profiling(self.h, 1 if on else 0)
+
+    def kernel_times(self) -> dict:
+        names = C.create_string_buffer(4096)
+        ms = (C.c_double * 64)()
+        cnt = (C.c_int64 * 64)()
+        n = self.L.elq_last_kernel_times(self.h, names, 4096, ms, cnt, 64)
+        keys = names.value.decode().split(",") if n else []
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(keys)}
+
+
+class DeviceScene:
+    """A scene resident in device memory (inputs already in HBM)."""
+
+    FIELDS = ("phi", "u", "v", "w", "uw", "vw", "ww")
+
+    def __init__(self, ctx: Context, s: Scene):
+        self.ctx, self.s = ctx, s
+        self.ptr = {}
+        for f in self.FIELDS:
+            a = getattr(s, f)
+            self.ptr[f] = ctx.alloc(a.nbytes)
+            ctx.h2d(self.ptr[f], a)
+        self.ptr["pressure"] = ctx.alloc(s.cells * 4)
+        self.ptr["indices"] = ctx.alloc(s.cells * 4)
+
+    def reset_velocity(self):
+        for f in ("u", "v", "w"):
+            self.ctx.h2d(self.ptr[f], getattr(self.s, f))
+
+    def grid(self) -> Grid:
+        s, p = self.s, self.ptr
+        g = Grid()
+        g.ni, g.nj, g.nk, g.dx = s.ni, s.nj, s.nk, s.dx
+        g.level_set, g.u, g.v, g.w = p["phi"], p["u"], p["v"], p["w"]
+        g.u_weight, g.v_weight, g.w_weight = p["uw"], p["vw"], p["ww"]
+        g.pressure, g.indices = p["pressure"], p["indices"]
+        return g
+
+    def fetch(self, name: str, dtype=np.float32, count=None):
+        n = count if count is not None else (self.s.cells if name in ("pressure", "indices", "phi")
+                                             else getattr(self.s, name).size)
+        a = np.empty(n, dtype)
+        self.ctx.d2h(a, self.ptr[name])
+        return a
+
+    def close(self):
+        for p in self.ptr.values():
+            self.ctx.free(p)
+        self.ptr = {}
+
+
+def solve_pressure(s: Scene, opts: Opts | None = None, ctx: Context | None = None):
+    """Simulator::solve_pressure(dt) on the GPU (simulator.cpp:96-112)."""
+    own = ctx is None
+    ctx = ctx or Context()
+    try:
+        return ctx.solve_pressure(s, opts)
+    finally:
+        if own:
+            ctx.close()

@@ -1,0 +1,234 @@
+/*
+ * elq.h — C ABI of the MI355X (gfx950) pressure projection.
+ *
+ * Drop-in for the reference simulator's pressure projection
+ * (daichi-nama/eulerian-liquid):
+ *   Simulator::solve_pressure(float dt)          simulator.cpp:96-112 / simulator.h:39
+ *   fluid::assign_indices(...)                   projection.cpp:14-28 / projection.h:10-11
+ *   fluid::build_matrix(...)                     projection.cpp:30-83 / projection.h:13-16
+ *   fluid::build_grid_from_vector(...)           projection.cpp:85-91 / projection.h:18-19
+ *   fluid::update_velocity_using_pressure(...)   projection.cpp:93-134 / projection.h:21-22
+ *
+ * Plain C types only: pointers, sizes, POD structs. No exceptions cross this
+ * boundary; every entry point returns an elq_status. Grids use the
+ * reference's layout (include/nama/common/array.h:285): flat, index
+ * i + ni*(j + nj*k); face grids u (ni+1)*nj*nk, v ni*(nj+1)*nk,
+ * w ni*nj*(nk+1) (include/nama/common/staggered_grid.h:100-112).
+ *
+ * All computation of the projection runs on the GPU. A process without a
+ * usable gfx950 device gets ELQ_ERR_DEVICE from elq_ctx_create.
+ */
+#ifndef ELQ_H
+#define ELQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ELQ_ABI_VERSION 1
+#define ELQ_UNKNOWN_INDEX 2147483647 /* projection.cpp:6 (INT_MAX) */
+
+typedef enum {
+    ELQ_OK = 0,
+    ELQ_ERR_NO_FLUID = 1,       /* projection.cpp:24-26 throws "No fluid" */
+    ELQ_ERR_NOT_CONVERGED = 2,  /* max_iter reached; the reference ignores this
+                                   (simulator.cpp:105-109), the C++ shim does too */
+    ELQ_ERR_BREAKDOWN = 3,      /* non-finite scalar inside the Krylov loop */
+    ELQ_ERR_DEVICE = 4,         /* HIP error or no gfx950 device */
+    ELQ_ERR_ADMISSION = 5,      /* request exceeds the context's HBM budget */
+    ELQ_ERR_INVALID_ARG = 6,
+    ELQ_ERR_BOUNDARY = 7,       /* an unknown lies on the outer cell layer: the
+                                   reference reads outside its arrays there
+                                   (projection.cpp:54 via :8-12), undefined */
+    ELQ_ERR_BUSY = 8            /* serving queue full */
+} elq_status;
+
+typedef enum {
+    ELQ_PRECISION_FP64 = 0,  /* plain FP64 drop-in: FP64 storage and arithmetic */
+    ELQ_PRECISION_MIXED = 1  /* FP32 SpMV/preconditioner/vectors, FP64 dots and
+                                scalars, FP64 outer residual + refinement */
+} elq_precision;
+
+typedef enum {
+    ELQ_PRECOND_IC0 = 0, /* natural-order IC(0): the default */
+    ELQ_PRECOND_MIC0 = 1 /* natural-order modified IC(0), parameter tau */
+} elq_precond;
+
+typedef enum {
+    ELQ_SCHED_WAVEFRONT = 0, /* one launch per sweep; bricks taken in level order,
+                                predecessors awaited through device flags */
+    ELQ_SCHED_LEVELS = 1     /* one launch per brick level (no device waits) */
+} elq_schedule;
+
+/* Inputs/outputs of one projection. Host or device pointers depending on the
+ * entry point. pressure and indices are optional outputs (NULL to skip). */
+typedef struct {
+    int32_t ni, nj, nk;
+    float dx;                 /* Grid3::dx(), width / ni (simulator.cpp:56) */
+    const float* level_set;   /* ni*nj*nk, liquid phi (<0 inside) */
+    float* u;                 /* (ni+1)*nj*nk, in/out */
+    float* v;                 /* ni*(nj+1)*nk, in/out */
+    float* w;                 /* ni*nj*(nk+1), in/out */
+    const float* u_weight;    /* face weights, same shapes as u/v/w */
+    const float* v_weight;
+    const float* w_weight;
+    float* pressure;          /* ni*nj*nk, out (background 0), optional */
+    int32_t* indices;         /* ni*nj*nk, out (ELQ_UNKNOWN_INDEX), optional */
+} elq_grid;
+
+typedef struct {
+    double tol;        /* relative residual; reference 1e-4 (simulator.cpp:106) */
+    int32_t max_iter;  /* reference 1000 (simulator.cpp:105); for MIXED it bounds
+                          the total number of inner iterations */
+    int32_t precision; /* elq_precision */
+    int32_t precond;   /* elq_precond */
+    double tau;        /* MIC(0) parameter (ignored for IC0) */
+    double sigma;      /* pivot safeguard: pivot t <= sigma*diag -> diag */
+    double inner_tol;  /* MIXED: relative reduction per refinement step */
+    int32_t schedule;  /* elq_schedule */
+    int32_t check_every; /* PCG iterations between host convergence polls */
+} elq_opts;
+
+typedef struct {
+    int64_t n_unknowns;        /* N (mat_size of projection.cpp:14-28) */
+    int64_t n_bricks;          /* active 4x4x4 bricks */
+    int32_t iterations;        /* Eigen iterations(): PCG iterations (FP64) or
+                                  total inner iterations (MIXED) */
+    int32_t outer_iterations;  /* MIXED refinement steps (1 for FP64) */
+    double error;              /* Eigen error(): sqrt(|r|^2/|b|^2), recursive r */
+    double true_rel_residual;  /* |b - A p|_2 / |b|_2 recomputed in FP64 */
+    int32_t status;            /* elq_status of the solve */
+    int32_t converged;
+    double t_setup_ms;         /* indices, bricks, assembly, factorisation */
+    double t_solve_ms;         /* Krylov iterations */
+    double t_update_ms;        /* pressure scatter + velocity update */
+    double t_total_ms;
+} elq_stats;
+
+typedef struct elq_ctx elq_ctx;
+
+/* Fills *opts with the reference defaults (tol 1e-4, max_iter 1000, FP64,
+ * IC0, sigma 0.25, wavefront schedule). */
+void elq_default_opts(elq_opts* opts);
+
+/* Context bound to one HIP device; owns device buffers that persist across
+ * calls and grow on demand. hbm_budget_bytes = 0: no limit. */
+int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out);
+void elq_ctx_destroy(elq_ctx* ctx);
+/* Last error message of this context (static storage of the context). */
+const char* elq_ctx_last_error(const elq_ctx* ctx);
+
+/* Simulator::solve_pressure(dt) on host grids (simulator.cpp:96-112): copies
+ * inputs to HBM, projects, copies u/v/w (and pressure/indices) back. */
+int elq_project(elq_ctx* ctx, const elq_grid* host_grid, float dt, const elq_opts* opts,
+                elq_stats* stats);
+
+/* Same on grids already resident in device memory (pointers from
+ * elq_device_alloc or any hipMalloc of the same device). Runs on the
+ * context's stream and returns after completion. */
+int elq_project_device(elq_ctx* ctx, const elq_grid* device_grid, float dt, const elq_opts* opts,
+                       elq_stats* stats);
+
+/* projection.cpp:14-28 on host grids: writes indices (ELQ_UNKNOWN_INDEX for
+ * non-unknowns) and *n_unknowns. */
+int elq_assign_indices(elq_ctx* ctx, const elq_grid* host_grid, int32_t* indices,
+                       int64_t* n_unknowns);
+
+/* projection.cpp:30-83 on host grids, stencil form in the reference's unknown
+ * numbering: A(n,n) = diag[n]; A(n, +x neighbour) = ax[n] (0 when the
+ * neighbour is not an unknown), same for ay/az; A is symmetric. b as the
+ * reference builds it. Arrays have n_unknowns entries (from
+ * elq_assign_indices). */
+int elq_build_matrix(elq_ctx* ctx, const elq_grid* host_grid, float dt, double* diag, double* ax,
+                     double* ay, double* az, double* b);
+
+/* projection.cpp:93-134 on host grids: u/v/w updated in place from a given
+ * pressure grid (grid->pressure is the input here). */
+int elq_update_velocity(elq_ctx* ctx, const elq_grid* host_grid, float dt);
+
+/* Device memory helpers for callers without their own allocator. */
+int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out);
+int elq_device_free(elq_ctx* ctx, void* ptr);
+int elq_memcpy_h2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes);
+int elq_memcpy_d2h(elq_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/* Per-kernel timing of the last projection on this context, measured with
+ * HIP events on the context's stream. names: comma-separated kernel classes;
+ * ms[i]: total time, launches[i]: launch count. Returns the number of classes
+ * written (at most cap). */
+int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, double* ms,
+                          int64_t* launches, int cap);
+/* Enables (1) or disables (0) per-kernel event timing (adds event records). */
+void elq_set_profiling(elq_ctx* ctx, int enable);
+
+/* ------------------------------------------------------------------ serving
+ * Continuous batching of independent projections (uneven grid sizes) on one
+ * device. Requests are admitted into slots while their HBM footprint fits the
+ * budget; all admitted requests advance together, one batched PCG iteration
+ * per step, on device-resident grids and Krylov vectors. A request's result
+ * does not depend on which other requests share the batch.
+ */
+typedef struct elq_server elq_server;
+
+typedef struct {
+    size_t hbm_budget_bytes; /* admission limit for resident slots */
+    int32_t max_slots;       /* concurrently resident requests */
+    int32_t max_queue;       /* pending requests waiting for admission */
+    elq_opts opts;           /* solver options for every request */
+} elq_server_opts;
+
+typedef struct {
+    int64_t id;
+    int32_t status;           /* elq_status of the solve */
+    int32_t iterations;
+    int32_t outer_iterations;
+    double error;
+    double t_submit_ms;       /* host clock, ms since server start */
+    double t_admit_ms;
+    double t_done_ms;
+} elq_result;
+
+int elq_server_create(int device, const elq_server_opts* opts, elq_server** out);
+void elq_server_destroy(elq_server* srv);
+/* Submits a projection on host grids; the grid arrays must stay valid until
+ * the result is polled (u/v/w/pressure are written back at completion).
+ * Returns ELQ_ERR_ADMISSION if the request alone can never fit the budget. */
+int elq_server_submit(elq_server* srv, const elq_grid* host_grid, float dt, int64_t* id);
+/* Advances the batch by one step (admission, one batched iteration,
+ * completion). Returns the number of requests in flight after the step. */
+int elq_server_step(elq_server* srv);
+/* Pops up to cap finished results. */
+int elq_server_poll(elq_server* srv, elq_result* out, int cap);
+/* Device bytes a request of this size occupies while resident. */
+size_t elq_server_footprint(const elq_server* srv, int32_t ni, int32_t nj, int32_t nk);
+
+/* ------------------------------------------------------------------ scenes
+ * Host-side input generation for the synthetic workloads (the simulator's own
+ * set-up steps, not part of the projection): create_grid + emitters +
+ * init_scene + compute_weights (simulator.cpp:52-76,187-189;
+ * levelset_emitter.cpp:10-32; boundary.cpp:6-34), add_gravity
+ * (simulator.cpp:90-93), find_valid_velocity + set_invalid_velocity_zero
+ * (simulator.cpp:125-178). spheres: n_spheres*(cx,cy,cz,r). slab_height
+ * <= -1e30 disables the pool emitter. */
+int elq_scene_level_set(int32_t ni, int32_t nj, int32_t nk, float dx, float slab_height,
+                        int32_t n_spheres, const float* spheres, float* level_set);
+int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* u_weight, float* v_weight,
+                      float* w_weight);
+int elq_scene_add_gravity(int32_t ni, int32_t nj, int32_t nk, float* v, float dt);
+int elq_scene_valid_and_zero(const elq_grid* host_grid);
+/* SplitMix64-driven draws: spheres (centres U[c_lo,c_hi]*L_axis, radii
+ * U[r_lo,r_hi], drawn cx,cy,cz,r per sphere) and face velocities U[-amp,amp]
+ * (u, then v, then w, each in linear index order). state is in/out. */
+int elq_scene_random_spheres(uint64_t* state, int32_t count, double c_lo, double c_hi, double r_lo,
+                             double r_hi, double lx, double ly, double lz, float* spheres);
+int elq_scene_random_velocity(uint64_t* state, int32_t ni, int32_t nj, int32_t nk, double amp,
+                              float* u, float* v, float* w);
+double elq_uniform01(uint64_t* state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ELQ_H */

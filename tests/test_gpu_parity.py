@@ -1,0 +1,131 @@
+"""GPU parity of the projection against the CPU oracle, through the C ABI.
+
+Bit-exact: indices (projection.cpp:14-28), A and b (projection.cpp:30-83),
+the velocity update (projection.cpp:93-134) for a given pressure.
+Tolerance (solve): both sides converge to relative residual 1e-8 with the
+same preconditioner algebra; the pressure agrees to within
+max|p - p_ref| <= 2e-6 * max|p_ref| (FP64) and 2e-5 (mixed), the true FP64
+relative residual is <= 1e-8, and the velocities after the update agree to
+within the same relative bound on the face values.
+"""
+import numpy as np
+import pytest
+
+import elq
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def to_oracle(s: "elq.Scene") -> "O.Scene":
+    return O.Scene(s.ni, s.nj, s.nk, s.dx, s.dt, s.phi.copy(), s.u.copy(), s.v.copy(), s.w.copy(),
+                   s.uw.copy(), s.vw.copy(), s.ww.copy())
+
+
+SCENES = {
+    "drop16": lambda: elq.scene_drop(16),
+    "drop32": lambda: elq.scene_drop(32),
+    "spheres24": lambda: elq.scene_spheres(24, seed=7, n_spheres=6),
+    "aniso": lambda: elq.scene_request(3),
+    "odd": lambda: _odd(),
+}
+
+
+def _odd():
+    """Grid dims that are not multiples of the 4x4x4 brick, one gravity step."""
+    s = elq.make_scene(19, 13, 22, 0.45, [(0.5, 0.9, 0.4, 0.2)])
+    elq.add_gravity(s, s.dt)
+    elq.valid_and_zero(s)
+    return s
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_assign_indices_bit_exact(ctx, name):
+    s = SCENES[name]()
+    n, idx = ctx.assign_indices(s)
+    n_ref, idx_ref = O.assign_indices(to_oracle(s))
+    assert n == n_ref
+    np.testing.assert_array_equal(idx, idx_ref)
+
+
+@pytest.mark.parametrize("name", sorted(SCENES))
+def test_build_matrix_bit_exact(ctx, name):
+    s = SCENES[name]()
+    n, _ = ctx.assign_indices(s)
+    m = ctx.build_matrix(s, n)
+    os_ = to_oracle(s)
+    n_ref, idx_ref = O.assign_indices(os_)
+    ref = O.build_matrix(os_, idx_ref, n_ref)
+    for k in ("diag", "ax", "ay", "az", "b"):
+        np.testing.assert_array_equal(m[k].view(np.uint64), getattr(ref, k).view(np.uint64), err_msg=k)
+
+
+@pytest.mark.parametrize("name", ["drop32", "spheres24"])
+def test_update_velocity_bit_exact(ctx, name):
+    s = SCENES[name]()
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(s.cells).astype(np.float32)
+    g = s.copy()
+    ctx.update_velocity_using_pressure(g, p)
+    os_ = to_oracle(s)
+    O.update_velocity(os_, p)
+    for f in ("u", "v", "w"):
+        np.testing.assert_array_equal(getattr(g, f).view(np.uint32), getattr(os_, f).view(np.uint32), err_msg=f)
+
+
+def _check_solution(s, p, idx, st, tol_rel):
+    ref = O.solve_pressure(to_oracle(s), tol=1e-12, max_iter=10000)
+    np.testing.assert_array_equal(idx, ref.indices)
+    assert st.n_unknowns == ref.n
+    scale = np.abs(ref.pressure).max()
+    err = np.abs(p - ref.pressure).max() / scale
+    assert err <= tol_rel, (err, st.as_dict())
+    assert st.true_rel_residual <= 1e-8, st.as_dict()
+    return ref
+
+
+@pytest.mark.parametrize("schedule", [elq.WAVEFRONT, elq.LEVELS])
+@pytest.mark.parametrize("name", ["drop32", "spheres24", "aniso", "odd"])
+def test_projection_fp64(ctx, name, schedule):
+    s = SCENES[name]()
+    g = s.copy()
+    o = elq.default_opts(tol=1e-8, max_iter=5000, schedule=schedule)
+    p, idx, st = ctx.solve_pressure(g, o)
+    assert st.status == elq.OK and st.converged, st.as_dict()
+    ref = _check_solution(s, p, idx, st, 2e-6)
+    umax = max(np.abs(ref.u).max(), np.abs(ref.v).max(), np.abs(ref.w).max())
+    for f in ("u", "v", "w"):
+        assert np.abs(getattr(g, f) - getattr(ref, f)).max() <= 2e-6 * umax, f
+
+
+@pytest.mark.parametrize("name", ["drop32", "spheres24", "aniso"])
+def test_projection_mixed(ctx, name):
+    s = SCENES[name]()
+    g = s.copy()
+    o = elq.default_opts(tol=1e-8, max_iter=5000, precision=elq.MIXED)
+    p, idx, st = ctx.solve_pressure(g, o)
+    assert st.status == elq.OK and st.converged, st.as_dict()
+    _check_solution(s, p, idx, st, 2e-5)
+    assert st.outer_iterations >= 2
+
+
+def test_reference_defaults_iteration_semantics(ctx):
+    """tol 1e-4 / maxit 1000 (simulator.cpp:105-106): Eigen error() below tol."""
+    s = SCENES["drop32"]()
+    p, idx, st = ctx.solve_pressure(s.copy(), elq.default_opts())
+    assert st.converged and st.error < 1e-4
+    ref = O.solve_pressure(to_oracle(s), tol=1e-4, max_iter=1000)
+    assert abs(st.iterations - ref.iterations) <= 2, (st.iterations, ref.iterations)
+
+
+def test_no_fluid(ctx):
+    s = elq.make_scene(16, 16, 16, None, [])
+    with pytest.raises(RuntimeError, match="No fluid"):
+        ctx.solve_pressure(s, elq.default_opts())
+
+
+def test_zero_rhs_gives_zero_pressure(ctx):
+    s = elq.make_scene(16, 16, 16, 0.5, [])
+    p, idx, st = ctx.solve_pressure(s, elq.default_opts(tol=1e-8))
+    assert st.converged and st.iterations == 0
+    assert np.all(p == 0)

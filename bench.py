@@ -1,0 +1,216 @@
+"""Benchmark of the pressure projection on one MI355X.
+
+Workload (default): the 256^3 seeded scene — slab y < 0.3 plus 32 spheres
+(centres U[0.15,0.85]^3, radii U[0.03,0.08], SplitMix64 seed 1), face
+velocities U[-0.5,0.5], validity + zeroing; projected to relative residual
+1e-8 in mixed precision (FP32 SpMV/preconditioner/vectors, FP64 dots/scalars,
+FP64 iterative refinement), natural-order IC(0), wavefront-scheduled sweeps.
+One step = one full projection (indices, assembly, factorisation, PCG,
+pressure scatter, velocity update) on inputs already resident in HBM.
+
+Prints one JSON line: solves/s, the dominant kernel's roofline (HIP events on
+the library's stream, algorithmic bytes per launch), the FP64 GPU drop-in on
+the same inputs, and the CPU oracle timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "eulerian-liquid_amd"))
+
+import elq  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+# Algorithmic bytes per unknown per launch of each kernel class: s_m bytes per
+# stored coefficient (diag, ax, ay, az, inv_e), s_v bytes per Krylov element.
+#   spmv (+ p update, p.q):           4 s_m + 2 s_v + 3 s_v
+#   fwd sweep (+ x, r axpy, |r|^2):   4 s_m + 2 s_v + 6 s_v
+#   bwd sweep (+ r.z):                4 s_m + 3 s_v
+BYTES_PER_UNKNOWN = {
+    "spmv": lambda sm, sv: 4 * sm + 5 * sv,
+    "fwd_sweep_update": lambda sm, sv: 4 * sm + 8 * sv,
+    "fwd_sweep": lambda sm, sv: 4 * sm + 2 * sv,
+    "bwd_sweep": lambda sm, sv: 4 * sm + 3 * sv,
+}
+B_ITER = lambda sm, sv: 12 * sm + 16 * sv  # per PCG iteration and unknown
+B_OUTER = 64  # per refinement step and unknown (FP64 residual + x update)
+
+
+def workload(name: str, seed: int):
+    if name == "cfg3":
+        return elq.scene_spheres(256, seed=seed, n_spheres=32), "spheres256"
+    if name == "cfg2":
+        return elq.scene_drop(128), "drop128"
+    if name == "cfg1":
+        return elq.scene_drop(64), "drop64"
+    if name == "cfg5":
+        return elq.scene_spheres(512, seed=seed, n_spheres=128), "spheres512"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_steps(ctx, ds, opts, steps, warmup, profile):
+    stats = []
+    ctx.set_profiling(False)
+    for _ in range(warmup):
+        ds.reset_velocity()
+        rc, st = ctx.project_device(ds.grid(), ds.s.dt, opts)
+        if rc not in (elq.OK,):
+            raise SystemExit(f"projection failed: {elq.STATUS_NAMES.get(rc)} {ctx.last_error()}")
+    ctx.set_profiling(profile)
+    times, ktimes = [], {}
+    for _ in range(steps):
+        ds.reset_velocity()
+        t0 = time.perf_counter()
+        rc, st = ctx.project_device(ds.grid(), ds.s.dt, opts)
+        t1 = time.perf_counter()
+        if rc != elq.OK:
+            raise SystemExit(f"projection failed: {elq.STATUS_NAMES.get(rc)} {ctx.last_error()}")
+        times.append((t1 - t0) * 1e3)
+        stats.append(st.as_dict())
+        for k, (ms, n) in ctx.kernel_times().items():
+            a = ktimes.setdefault(k, [0.0, 0])
+            a[0] += ms
+            a[1] += n
+    ctx.set_profiling(False)
+    return times, stats, ktimes
+
+
+def roofline(ktimes, n_unknowns, sm, sv):
+    best = max((k for k in ktimes if k in BYTES_PER_UNKNOWN), key=lambda k: ktimes[k][0])
+    ms, launches = ktimes[best]
+    avg_s = ms / launches / 1e3
+    bytes_per_launch = BYTES_PER_UNKNOWN[best](sm, sv) * n_unknowns
+    achieved = bytes_per_launch / avg_s / 1e9
+    return best, {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                  "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                  "kernel": best, "avg_launch_us": round(avg_s * 1e6, 2),
+                  "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+
+def cpu_baseline(scene, fp64_iters):
+    """Oracle (FP64 restatement, single thread) on the same inputs: one timed
+    call with a small and one with a larger iteration cap give setup time and
+    time per PCG iteration; the projection time is extrapolated to the
+    iteration count the same algorithm needs on this workload."""
+    import oracle as O
+    os_ = O.Scene(scene.ni, scene.nj, scene.nk, scene.dx, scene.dt, scene.phi, scene.u.copy(), scene.v.copy(),
+                  scene.w.copy(), scene.uw, scene.vw, scene.ww)
+    m1, m2 = 4, 24
+    t = []
+    for m in (m1, m2):
+        t0 = time.perf_counter()
+        O.solve_pressure(os_, tol=1e-8, max_iter=m)
+        t.append(time.perf_counter() - t0)
+    per_iter = (t[1] - t[0]) / (m2 - m1)
+    setup = max(t[0] - m1 * per_iter, 0.0)
+    est = setup + per_iter * fp64_iters
+    return {"value": round(1.0 / est, 5), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"oracle projection on the full workload capped at {m1} and {m2} PCG iterations "
+                      f"({t[0]:.1f}s, {t[1]:.1f}s): setup {setup:.2f}s + {per_iter * 1e3:.1f} ms/iteration "
+                      f"x {fp64_iters} iterations (FP64 IC(0) count of the GPU drop-in)",
+            "est_s_per_solve": round(est, 2)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="cfg3")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--precond", default="ic0", choices=["ic0", "mic0"])
+    ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "levels"])
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 drop-in (profiling runs)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        raise SystemExit(2)
+
+    t_gen = time.perf_counter()
+    scene, wname = workload(a.workload, a.seed)
+    t_gen = time.perf_counter() - t_gen
+    print(f"# workload {wname}: {scene.ni}x{scene.nj}x{scene.nk}, generated in {t_gen:.1f}s", file=sys.stderr,
+          flush=True)
+    precond = elq.IC0 if a.precond == "ic0" else elq.MIC0
+    schedule = elq.WAVEFRONT if a.schedule == "wavefront" else elq.LEVELS
+    base = dict(tol=1e-8, max_iter=20000, precond=precond, schedule=schedule, check_every=16)
+    o_mixed = elq.default_opts(precision=elq.MIXED, **base)
+    o_fp64 = elq.default_opts(precision=elq.FP64, **base)
+
+    ctx = elq.Context(0)
+    ds = elq.DeviceScene(ctx, scene)
+    out = {}
+    if not a.fp64_only:
+        times, stats, kt = run_steps(ctx, ds, o_mixed, a.steps, a.warmup, True)
+        p_mixed = ds.fetch("pressure")
+        print(f"# mixed: {[round(t, 1) for t in times]} ms, iters {stats[-1]['iterations']} "
+              f"outer {stats[-1]['outer_iterations']}", file=sys.stderr, flush=True)
+        out["mixed"] = (times, stats, kt)
+    if not a.no_fp64 or a.fp64_only:
+        steps64 = a.steps if a.fp64_only else max(1, min(a.steps, 2))
+        times64, stats64, kt64 = run_steps(ctx, ds, o_fp64, steps64, 1 if a.fp64_only else 0, True)
+        p_fp64 = ds.fetch("pressure")
+        print(f"# fp64: {[round(t, 1) for t in times64]} ms, iters {stats64[-1]['iterations']}", file=sys.stderr,
+              flush=True)
+        out["fp64"] = (times64, stats64, kt64)
+
+    key = "fp64" if a.fp64_only else "mixed"
+    times, stats, kt = out[key]
+    st = stats[-1]
+    n = st["n_unknowns"]
+    sm = sv = 8 if key == "fp64" else 4
+    ms_step = float(np.mean(times))
+    iters, outer = st["iterations"], st["outer_iterations"]
+    setup_bytes = 92 * scene.cells + n * (9 * sm + 16)
+    total_bytes = iters * n * B_ITER(sm, sv) + (outer * n * B_OUTER if key == "mixed" else 0) + setup_bytes
+    kernel, roof = roofline(kt, n, sm, sv)
+    line = {
+        "metric": "PCG pressure solves/s & HBM GB/s at 256^3 (rel.res 1e-8); p99 batched latency",
+        "value": round(1000.0 / ms_step, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if key == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{wname} seed {a.seed} ({scene.ni}^3 slab+spheres, rel.res 1e-8)",
+                   "precision": key, "precond": a.precond, "schedule": a.schedule, "unknowns": n,
+                   "bricks": st["n_bricks"], "iterations": iters, "outer_iterations": outer,
+                   "true_rel_residual": st["true_rel_residual"]},
+        "hbm_gbs_whole_solve": round(total_bytes / (ms_step / 1e3) / 1e9, 1),
+        "hbm_frac_whole_solve": round(total_bytes / (ms_step / 1e3) / 1e9 / PEAK_HBM_GBS, 4),
+        "roofline": roof,
+        "kernel_ms_per_step": {k: round(v[0] / a.steps, 3) for k, v in sorted(kt.items())},
+    }
+    if "fp64" in out and key == "mixed":
+        t64, s64, _ = out["fp64"]
+        ms64 = float(np.mean(t64))
+        scale = float(np.abs(p_fp64).max())
+        line["fp64_drop_in"] = {"solves_per_s": round(1000.0 / ms64, 4), "ms_per_solve": round(ms64, 2),
+                                "iterations": s64[-1]["iterations"],
+                                "true_rel_residual": s64[-1]["true_rel_residual"]}
+        line["speedup_mixed_vs_fp64"] = round(ms64 / ms_step, 3)
+        line["max_abs_p_minus_p_fp64"] = float(np.abs(p_mixed - p_fp64).max())
+        line["max_rel_p_minus_p_fp64"] = float(np.abs(p_mixed - p_fp64).max()) / scale if scale else 0.0
+    ds.close()
+    ctx.close()
+    if not a.no_cpu:
+        fp64_iters = out["fp64"][1][-1]["iterations"] if "fp64" in out else iters
+        line["cpu_baseline"] = cpu_baseline(scene, fp64_iters)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

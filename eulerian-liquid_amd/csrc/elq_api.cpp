@@ -238,9 +238,14 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     hipStream_t s = ctx->stream;
     HIPCHK(W.nbr.ensure((size_t)nb * 6 * 4), "alloc nbr");
     HIPCHK(W.lvl_list.ensure((size_t)nb * 4), "alloc lvl_list");
-    if ((size_t)nb * 4 > W.flags.cap) {
-        HIPCHK(W.flags.ensure((size_t)nb * 4), "alloc flags");
-        HIPCHK(hipMemsetAsync(W.flags.p, 0, W.flags.cap, s), "memset flags");
+    // tagged exchange words: zeroed on (re)allocation; tags (epochs) never 0
+    if ((size_t)nb * kXchWords * 8 > W.xch.cap) {
+        HIPCHK(W.xch.ensure((size_t)nb * kXchWords * 8), "alloc xch");
+        HIPCHK(hipMemsetAsync(W.xch.p, 0, W.xch.cap, s), "memset xch");
+    }
+    if (!W.stage.p) {
+        HIPCHK(W.stage.ensure((size_t)kRedBlocks * 4 * 8), "alloc stage");
+        HIPCHK(hipMemsetAsync(W.stage.p, 0, W.stage.cap, s), "memset stage");
     }
     HIPCHK(W.diag64.ensure(cells * 8), "alloc");
     HIPCHK(W.b64.ensure(cells * 8), "alloc");
@@ -251,7 +256,10 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     HIPCHK(W.part_rr.ensure((size_t)nb * 8), "alloc");
     HIPCHK(W.part_rz.ensure((size_t)nb * 8), "alloc");
     HIPCHK(W.part_pq.ensure((size_t)nb * 8), "alloc");
-    HIPCHK(W.ctrl.ensure(sizeof(Ctrl)), "alloc");
+    if (!W.ctrl.p) {
+        HIPCHK(W.ctrl.ensure(sizeof(Ctrl)), "alloc");
+        HIPCHK(hipMemsetAsync(W.ctrl.p, 0, sizeof(Ctrl), s), "memset ctrl");
+    }
     HIPCHK(W.nfixed.ensure(sizeof(int32_t)), "alloc");
     Bricks bk = make_bricks(ctx, p);
     ctx->prof_begin("setup_lists_assemble");
@@ -310,7 +318,9 @@ struct Solver {
 
     int scalar(int mode, const double* parts, const double* parts2 = nullptr) {
         ctx->prof_begin("scalar");
-        HIPCHK(launch_scalar(ctrl(), mode, parts, bk.nb, parts2, ctx->stream), "k_scalar");
+        HIPCHK(launch_scalar(ctrl(), mode, parts, bk.nb, parts2, (uint64_t*)ctx->ws.stage.p, ctx->timeout_ticks,
+                             ctx->stream),
+               "k_scalar");
         ctx->prof_end();
         return ELQ_OK;
     }
@@ -320,13 +330,13 @@ struct Solver {
         ctx->prof_begin(update ? "fwd_sweep_update" : "fwd_sweep");
         if (!levels) {
             HIPCHK(launch_fwd<T>(bk, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
-                                 (uint32_t*)W.flags.p, update, -1, 0, 0, ctx->timeout_ticks, ctx->stream),
+                                 (uint64_t*)W.xch.p, update, -1, 0, 0, ctx->timeout_ticks, ctx->stream),
                    "k_fwd");
         } else {
             for (int l = 0; l < p.nlev; ++l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
                 HIPCHK(launch_fwd<T>(bk, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
-                                     (uint32_t*)W.flags.p, update, l, base, cnt, ctx->timeout_ticks, ctx->stream),
+                                     (uint64_t*)W.xch.p, update, l, base, cnt, ctx->timeout_ticks, ctx->stream),
                        "k_fwd");
             }
         }
@@ -338,13 +348,13 @@ struct Solver {
         Workspace& W = ctx->ws;
         ctx->prof_begin("bwd_sweep");
         if (!levels) {
-            HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint32_t*)W.flags.p, -1, 0, 0,
+            HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, -1, 0, 0,
                                  ctx->timeout_ticks, ctx->stream),
                    "k_bwd");
         } else {
             for (int l = p.nlev - 1; l >= 0; --l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
-                HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint32_t*)W.flags.p, l, base,
+                HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, l, base,
                                      cnt, ctx->timeout_ticks, ctx->stream),
                        "k_bwd");
             }
@@ -361,14 +371,14 @@ struct Solver {
         ctx->prof_begin("factor");
         if (!levels) {
             HIPCHK(launch_factor<T>(bk, (const double*)W.diag64.p, c.ax, c.ay, c.az, tau, sigma, (double*)W.inv_e64.p,
-                                    inv_e32, ctrl(), (uint32_t*)W.flags.p, (int32_t*)W.nfixed.p, -1, 0, 0,
+                                    inv_e32, ctrl(), (uint64_t*)W.xch.p, (int32_t*)W.nfixed.p, -1, 0, 0,
                                     ctx->timeout_ticks, ctx->stream),
                    "k_factor");
         } else {
             for (int l = 0; l < p.nlev; ++l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
                 HIPCHK(launch_factor<T>(bk, (const double*)W.diag64.p, c.ax, c.ay, c.az, tau, sigma,
-                                        (double*)W.inv_e64.p, inv_e32, ctrl(), (uint32_t*)W.flags.p,
+                                        (double*)W.inv_e64.p, inv_e32, ctrl(), (uint64_t*)W.xch.p,
                                         (int32_t*)W.nfixed.p, l, base, cnt, ctx->timeout_ticks, ctx->stream),
                        "k_factor");
             }
@@ -377,36 +387,56 @@ struct Solver {
         return ELQ_OK;
     }
 
-    // Krylov loop from an initial residual in r (x = 0, p buffers = 0), Eigen
-    // semantics. Polls the device every check_every iterations.
-    int krylov(Ctrl& hc) {
-        Workspace& W = ctx->ws;
+    int64_t kiter = 0;  // iterations launched (selects the p ping-pong buffer)
+
+    // z = M^{-1} r and rz; mode SC_RZ_INIT (beta = 0) or SC_RZ_REPLACE.
+    int precondition(int rz_mode) {
         int rc;
         if ((rc = fwd(0, nullptr))) return rc;
         if ((rc = scalar(SC_NOP, nullptr))) return rc;
         if ((rc = bwd())) return rc;
-        if ((rc = scalar(SC_RZ_INIT, prz()))) return rc;
-        const int check = std::max(1, o->check_every);
+        return scalar(rz_mode, prz());
+    }
+
+    // n PCG iterations (Eigen's loop body), no host synchronisation.
+    int iterations(int n) {
+        Workspace& W = ctx->ws;
+        int rc;
         T* P[2] = {vec(W.vp0), vec(W.vp1)};
-        // host-side bound in case the device never raises done
-        const int64_t cap = (int64_t)std::max(o->max_iter, 0) + 2 * (int64_t)check + 2;
-        for (int64_t k = 0;; ++k) {
-            if (k > cap) return ctx->fail(ELQ_ERR_DEVICE, "Krylov loop exceeded its host-side bound");
+        for (int q = 0; q < n; ++q, ++kiter) {
             ctx->prof_begin("spmv");
-            HIPCHK(launch_spmv<T>(bk, c, vec(W.vy), P[k & 1], P[(k + 1) & 1], vec(W.vq), ppq(), ctrl(), ctx->stream),
+            HIPCHK(launch_spmv<T>(bk, c, vec(W.vy), P[kiter & 1], P[(kiter + 1) & 1], vec(W.vq), ppq(), ctrl(),
+                                  ctx->stream),
                    "k_spmv");
             ctx->prof_end();
             if ((rc = scalar(SC_ALPHA, ppq()))) return rc;
-            if ((rc = fwd(1, P[(k + 1) & 1]))) return rc;
+            if ((rc = fwd(1, P[(kiter + 1) & 1]))) return rc;
             if ((rc = scalar(SC_RR, prr()))) return rc;
             if ((rc = bwd())) return rc;
             if ((rc = scalar(SC_BETA, prz()))) return rc;
-            if ((k + 1) % check == 0) {
-                ctx->prof_collect();
-                if ((rc = read_ctrl(ctx, hc))) return rc;
-                if (hc.abort) return ctx->fail(ELQ_ERR_DEVICE, "wavefront wait exceeded its time limit");
-                if (hc.done) break;
-            }
+        }
+        return ELQ_OK;
+    }
+
+    int poll(Ctrl& hc) {
+        int rc;
+        ctx->prof_collect();
+        if ((rc = read_ctrl(ctx, hc))) return rc;
+        if (hc.abort) return ctx->fail(ELQ_ERR_DEVICE, "wavefront wait exceeded its time limit");
+        return ELQ_OK;
+    }
+
+    // FP64 Krylov loop with Eigen semantics, polling every check_every iterations.
+    int krylov(Ctrl& hc) {
+        int rc;
+        if ((rc = precondition(SC_RZ_INIT))) return rc;
+        const int check = std::max(1, o->check_every);
+        const int64_t cap = (int64_t)std::max(o->max_iter, 0) / check + 3;
+        for (int64_t chunk = 0;; ++chunk) {
+            if (chunk > cap) return ctx->fail(ELQ_ERR_DEVICE, "Krylov loop exceeded its host-side bound");
+            if ((rc = iterations(check))) return rc;
+            if ((rc = poll(hc))) return rc;
+            if (hc.done) break;
         }
         return ELQ_OK;
     }
@@ -426,15 +456,20 @@ int solve_fp64(elq_ctx* ctx, Solver<T>& S, Ctrl& hc) {
     return ELQ_OK;
 }
 
+// Mixed precision: FP32 PCG (FP32 storage of A, M and the Krylov vectors; FP64
+// dot products and scalars) whose recursive residual is replaced by the FP64
+// residual r = b - A x (x accumulated in FP64) whenever it has dropped by the
+// factor opts.inner_tol since the last replacement, and whenever it meets the
+// stopping test. The stopping test is always decided on the FP64 residual.
 int solve_mixed(elq_ctx* ctx, Solver<float>& S, Ctrl& hc) {
+    const double kReplaceDrop2 = S.o->inner_tol * S.o->inner_tol;
     Workspace& W = ctx->ws;
     int rc;
     const size_t cells = (size_t)S.bk.nb * kBrickCells;
     HIPCHK(W.x64.ensure(cells * 8), "alloc x64");
     HIPCHK(hipMemsetAsync(W.x64.p, 0, cells * 8, ctx->stream), "memset");
-    for (int outer = 0;; ++outer) {
-        if (outer > 200) return ctx->fail(ELQ_ERR_DEVICE, "refinement exceeded its host-side bound");
-        if (outer > 0) {
+    auto replace = [&](bool first) -> int {
+        if (!first) {
             ctx->prof_begin("xupdate");
             HIPCHK(launch_xupdate(S.bk, (double*)W.x64.p, (const float*)W.vx.p, S.ctrl(), ctx->stream), "k_xupdate");
             ctx->prof_end();
@@ -442,14 +477,28 @@ int solve_mixed(elq_ctx* ctx, Solver<float>& S, Ctrl& hc) {
         ctx->prof_begin("resid64");
         HIPCHK(launch_resid64<float>(S.bk, (const double*)W.diag64.p, S.c.ax, S.c.ay, S.c.az, (const double*)W.b64.p,
                                      (const double*)W.x64.p, S.prr(), S.prz(), (float*)W.vr.p, (float*)W.vx.p,
-                                     (float*)W.vp0.p, (float*)W.vp1.p, S.ctrl(), ctx->stream),
+                                     first ? (float*)W.vp0.p : nullptr, first ? (float*)W.vp1.p : nullptr, S.ctrl(),
+                                     ctx->stream),
                "k_resid64");
         ctx->prof_end();
-        if ((rc = S.scalar(SC_OUTER, S.prr(), S.prz()))) return rc;
-        ctx->prof_collect();
-        if ((rc = read_ctrl(ctx, hc))) return rc;
-        if (hc.outer_done) break;
-        if ((rc = S.krylov(hc))) return rc;
+        return S.scalar(SC_REPLACE, S.prr(), S.prz());
+    };
+    if ((rc = replace(true))) return rc;
+    if ((rc = S.poll(hc))) return rc;
+    if (hc.outer_done) return ELQ_OK;
+    if ((rc = S.precondition(SC_RZ_INIT))) return rc;
+    const int check = std::max(1, S.o->check_every);
+    const int64_t cap = (int64_t)std::max(S.o->max_iter, 0) / check + 64;
+    for (int64_t chunk = 0;; ++chunk) {
+        if (chunk > cap) return ctx->fail(ELQ_ERR_DEVICE, "mixed Krylov loop exceeded its host-side bound");
+        if ((rc = S.iterations(check))) return rc;
+        if ((rc = S.poll(hc))) return rc;
+        if (hc.done || hc.rr < kReplaceDrop2 * hc.rr_ref) {
+            if ((rc = replace(false))) return rc;
+            if ((rc = S.poll(hc))) return rc;
+            if (hc.outer_done) break;
+            if ((rc = S.precondition(SC_RZ_REPLACE))) return rc;
+        }
     }
     return ELQ_OK;
 }
@@ -636,7 +685,7 @@ void elq_default_opts(elq_opts* o) {
     o->precond = ELQ_PRECOND_IC0;
     o->tau = 0.97;
     o->sigma = 0.25;
-    o->inner_tol = 1e-4;
+    o->inner_tol = 0.1;
     o->schedule = ELQ_SCHED_WAVEFRONT;
     o->check_every = 8;
 }

@@ -23,7 +23,7 @@ struct Workspace {
     DevBuf mask, indices, pressure;
     // scans and brick metadata
     DevBuf scan_cnt, scan_off, bflag, map, coord, lvl_cnt, lvl_off, lvl_fill, info;
-    DevBuf nbr, lvl_list, flags;
+    DevBuf nbr, lvl_list, xch, stage;
     // system
     DevBuf diag64, b64, inv_e64, inv_e32, cdiag, cax, cay, caz;
     // Krylov vectors (storage type of the solve)
@@ -42,7 +42,7 @@ struct Workspace {
 
     void release_all() {
         for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
-                          &lvl_fill, &info, &nbr, &lvl_list, &flags, &diag64, &b64, &inv_e64, &inv_e32, &cdiag, &cax,
+                          &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &diag64, &b64, &inv_e64, &inv_e32, &cdiag, &cax,
                           &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz, &part_pq, &ctrl,
                           &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
                           &export_buf})

@@ -24,7 +24,10 @@ constexpr int kBrickCells = 64;
 constexpr int kScanBlock = 256;               // threads per scan workgroup
 constexpr int kScanItems = 16;                // cells per thread per scan block
 constexpr int kScanChunk = kScanBlock * kScanItems;
-constexpr int kReduceThreads = 1024;          // scalar kernel width
+constexpr int kReduceThreads = 256;           // scalar kernel workgroup width
+constexpr int kRedBlocks = 128;               // max workgroups of the scalar kernel
+constexpr int kRedPerBlock = 2048;            // partials per scalar workgroup (target)
+constexpr int kXchWords = 192;                // face-exchange words per brick (6 x 16 x 2)
 
 // Dense-cell mask bits
 constexpr uint8_t kValid = 1;   // projection.cpp:8-12
@@ -41,8 +44,9 @@ enum ScalarMode : int {
     SC_ALPHA = 3,     // alpha = rz / p.q
     SC_RR = 4,        // |r|^2 and convergence test
     SC_BETA = 5,      // beta = rz_new / rz, iteration++
-    SC_OUTER = 6,     // mixed: FP64 residual norm, outer convergence, inner threshold
+    SC_REPLACE = 6,   // mixed: FP64 residual replaces the FP32 recursive one; true convergence test
     SC_TRUE = 7,      // true residual norm of the final iterate
+    SC_RZ_REPLACE = 8 // mixed: rz and beta after a residual replacement
 };
 
 // Device-resident solver state. One per solve (slot).
@@ -59,19 +63,22 @@ struct alignas(64) Ctrl {
     double rr64;        // mixed: FP64 residual norm^2 of the current outer step
     double true_rr;     // |b - A x|^2 of the final iterate
     double last_err;    // sqrt(rr / bnorm2) at stop
+    double rz_div;      // rz the next beta divides by (mixed residual replacement)
+    double rr_ref;      // |r|^2 right after the last residual replacement
     int32_t iter;       // Krylov iterations of the current (inner) solve
     int32_t iter_total; // summed over refinement steps
     int32_t max_iter;
     int32_t done;       // Krylov loop finished (converged, max_iter or breakdown)
     int32_t converged;
     int32_t outer_done;
-    int32_t outer;      // refinement steps taken
+    int32_t outer;      // residual replacements (mixed)
     int32_t status;     // elq_status
     int32_t mixed;
     uint32_t epoch;     // wavefront flag generation
     uint32_t ticket;    // wavefront work counter
     uint32_t abort;     // set by a wait that exceeded its time limit
-    int32_t pad[4];
+    uint32_t red_epoch; // tag generation of the scalar kernel's partial sums
+    int32_t pad[3];
 };
 
 struct SetupInfo {

@@ -65,27 +65,58 @@ __device__ __forceinline__ int take_ticket(Ctrl* c) {
     return __shfl(t, 0, 64);
 }
 
-__device__ __forceinline__ void wait_bricks(const uint32_t* flags, int p0, int p1, int p2, uint32_t epoch,
-                                            Ctrl* c, uint64_t timeout) {
-    const int lane = lane_id();
-    const int p = lane == 0 ? p0 : lane == 1 ? p1 : lane == 2 ? p2 : -1;
-    if (p >= 0) {
-        const uint64_t t0 = wall_clock64();
-        while (__hip_atomic_load(&flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-            if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-            if (wall_clock64() - t0 > timeout) {
-                __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+// Face exchange between the bricks of one sweep. A brick publishes the values
+// of its boundary cells as 64-bit words whose upper half is the sweep's epoch
+// (one word per float, two per double), written and polled with relaxed
+// device-scope atomics: a reader sees either an old tag or the whole value, so
+// no cache-wide acquire/release fence (L2 write-back/invalidate) is needed.
+// Word slots: brick b, face f (0 -x .. 5 +z), index idx in [0,16).
+__device__ __forceinline__ size_t xslot(int b, int face, int idx) {
+    return (size_t)b * kXchWords + (size_t)(face * 16 + idx) * 2;
 }
 
-__device__ __forceinline__ void signal_brick(uint32_t* flags, int b, uint32_t epoch) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    if (lane_id() == 0) __hip_atomic_store(&flags[b], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void xput_word(uint64_t* w, uint32_t lo, uint32_t e) {
+    __hip_atomic_store(w, ((uint64_t)e << 32) | lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void xput(uint64_t* x, size_t s, float v, uint32_t e) { xput_word(x + s, __float_as_uint(v), e); }
+
+__device__ __forceinline__ void xput(uint64_t* x, size_t s, double v, uint32_t e) {
+    const uint64_t bits = (uint64_t)__double_as_longlong(v);
+    xput_word(x + s, (uint32_t)bits, e);
+    xput_word(x + s + 1, (uint32_t)(bits >> 32), e);
+}
+
+__device__ __forceinline__ uint32_t xwait(const uint64_t* w, uint32_t e, Ctrl* c, uint64_t timeout) {
+    uint64_t v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((uint32_t)(v >> 32) == e) return (uint32_t)v;
+    const uint64_t t0 = wall_clock64();
+    for (;;) {
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((uint32_t)(v >> 32) == e) break;
+        if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (wall_clock64() - t0 > timeout) {
+            __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+    }
+    return (uint32_t)v;
+}
+
+template <class T>
+__device__ __forceinline__ T xget(const uint64_t* x, size_t s, uint32_t e, Ctrl* c, uint64_t timeout);
+
+template <>
+__device__ __forceinline__ float xget<float>(const uint64_t* x, size_t s, uint32_t e, Ctrl* c, uint64_t timeout) {
+    return __uint_as_float(xwait(x + s, e, c, timeout));
+}
+
+template <>
+__device__ __forceinline__ double xget<double>(const uint64_t* x, size_t s, uint32_t e, Ctrl* c, uint64_t timeout) {
+    const uint32_t lo = xwait(x + s, e, c, timeout);
+    const uint32_t hi = xwait(x + s + 1, e, c, timeout);
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
 struct Lane {
@@ -319,12 +350,12 @@ __global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const ui
 // the oracle's operation order (bit-identical pivots):
 //   t = d - a_xm^2/e_xm - a_ym^2/e_ym - a_zm^2/e_zm - tau*(...);
 //   t <= sigma*d -> t = d;   inv_e = 1/t.
-// Inside the brick the ten hyperplanes lx+ly+lz = h are swept in order; values
-// from the three predecessor bricks are read once their flags are raised.
+// Inside the brick the ten hyperplanes lx+ly+lz = h are swept in order; pivots
+// of the three predecessor bricks arrive through the tagged face exchange.
 template <class T, bool LEVELS>
 __global__ __launch_bounds__(64) void k_factor(Bricks bk, const double* diag64, const T* ax, const T* ay,
                                                const T* az, double tau, double sigma, double* inv_e64,
-                                               float* inv_e32, Ctrl* ctrl, uint32_t* flags, int32_t* n_fixed,
+                                               float* inv_e32, Ctrl* ctrl, uint64_t* xch, int32_t* n_fixed,
                                                int level_base, uint64_t timeout) {
     const Lane L;
     const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
@@ -353,10 +384,16 @@ __global__ __launch_bounds__(64) void k_factor(Bricks bk, const double* diag64, 
     const double azm = iz ? sz_az : ext(az, nzm, 48);
     const double ax_zm = iz ? sz_ax : ext(ax, nzm, 48);
     const double ay_zm = iz ? sz_ay : ext(ay, nzm, 48);
-    if (!LEVELS) wait_bricks(flags, nxm, nym, nzm, epoch, ctrl, timeout);
-    const double ex = (!ix && nxm >= 0) ? inv_e64[(size_t)nxm * kBrickCells + L.lane + 3] : 0.0;
-    const double ey = (!iy && nym >= 0) ? inv_e64[(size_t)nym * kBrickCells + L.lane + 12] : 0.0;
-    const double ez = (!iz && nzm >= 0) ? inv_e64[(size_t)nzm * kBrickCells + L.lane + 48] : 0.0;
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    if (LEVELS) {
+        if (!ix && nxm >= 0) ex = inv_e64[(size_t)nxm * kBrickCells + L.lane + 3];
+        if (!iy && nym >= 0) ey = inv_e64[(size_t)nym * kBrickCells + L.lane + 12];
+        if (!iz && nzm >= 0) ez = inv_e64[(size_t)nzm * kBrickCells + L.lane + 48];
+    } else {
+        if (!ix && nxm >= 0) ex = xget<double>(xch, xslot(nxm, 1, L.ly + 4 * L.lz), epoch, ctrl, timeout);
+        if (!iy && nym >= 0) ey = xget<double>(xch, xslot(nym, 3, L.lx + 4 * L.lz), epoch, ctrl, timeout);
+        if (!iz && nzm >= 0) ez = xget<double>(xch, xslot(nzm, 5, L.lx + 4 * L.ly), epoch, ctrl, timeout);
+    }
     const int h_own = L.lx + L.ly + L.lz;
     double ie = 0.0;
     int fixed = 0;
@@ -383,11 +420,15 @@ __global__ __launch_bounds__(64) void k_factor(Bricks bk, const double* diag64, 
             ie = 1.0 / t;
         }
     }
+    if (!LEVELS) {
+        if (L.lx == 3) xput(xch, xslot(b, 1, L.ly + 4 * L.lz), ie, epoch);
+        if (L.ly == 3) xput(xch, xslot(b, 3, L.lx + 4 * L.lz), ie, epoch);
+        if (L.lz == 3) xput(xch, xslot(b, 5, L.lx + 4 * L.ly), ie, epoch);
+    }
     inv_e64[o] = ie;
     if (inv_e32) inv_e32[o] = (float)ie;
     const int nf = wave_sum_int(fixed);
     if (L.lane == 0 && nf) atomicAdd(n_fixed, nf);
-    if (!LEVELS) signal_brick(flags, b, epoch);
 }
 
 template <class T>
@@ -431,31 +472,11 @@ __global__ __launch_bounds__(256) void k_init(Bricks bk, const double* b64, T* r
 }
 
 // Scalar recurrences of Eigen's conjugate_gradient (see the oracle's orc_pcg)
-// and of the mixed-precision refinement. One workgroup; partial sums are
-// added in a fixed order, so every scalar is deterministic.
-__global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, const double* parts, int n,
-                                                           const double* parts2) {
-    __shared__ double sh[kReduceThreads];
-    __shared__ double sh2[kReduceThreads];
-    const int t = threadIdx.x;
-    double s = 0.0, s2 = 0.0;
-    if (parts)
-        for (int q = t; q < n; q += kReduceThreads) s += parts[q];
-    if (parts2)
-        for (int q = t; q < n; q += kReduceThreads) s2 += parts2[q];
-    sh[t] = s;
-    sh2[t] = s2;
-    __syncthreads();
-    for (int o = kReduceThreads / 2; o > 0; o >>= 1) {
-        if (t < o) {
-            sh[t] += sh[t + o];
-            sh2[t] += sh2[t + o];
-        }
-        __syncthreads();
-    }
-    if (t != 0) return;
-    s = sh[0];
-    s2 = sh2[0];
+// and of the mixed-precision residual replacement. Stage 1: workgroup g sums
+// the partials of a fixed contiguous segment; stage 2: workgroup 0 gathers the
+// segment sums (tagged words, no cache-wide fences) and adds them in segment
+// order. The result is deterministic for a given number of partials.
+__device__ void scalar_update(Ctrl* c, int mode, double s, double s2) {
     c->epoch += 1u;
     c->ticket = 0u;
     const bool was_done = c->done != 0;
@@ -508,6 +529,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
                 if (s < c->threshold) {
                     c->done = 1;
                     c->converged = 1;
+                    c->rz_div = c->rz;
                     c->last_err = sqrt(s / c->bnorm2);
                 } else if (!isfinite(s)) {
                     c->done = 1;
@@ -518,6 +540,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
         case SC_BETA:
             if (!was_done) {
                 const double rz_new = s;
+                c->rz_div = c->rz;
                 c->beta = rz_new / c->rz;
                 c->rz = rz_new;
                 c->iter += 1;
@@ -531,10 +554,11 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
                 }
             }
             break;
-        case SC_OUTER: {
-            if (c->outer == 0) c->bnorm2 = s;
+        case SC_REPLACE: {
+            if (c->outer == 0) c->bnorm2 = s;  // x64 = 0: r64 = b exactly
             c->rr64 = s;
             const double thr = fmax(c->tol2 * c->bnorm2, DBL_MIN);
+            c->outer += 1;
             if (c->bnorm2 == 0.0) {
                 c->outer_done = 1;
                 c->converged = 1;
@@ -553,21 +577,80 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
                 if (!isfinite(s)) c->status = 3;
             } else {
                 c->rr = s2;
-                c->threshold = fmax(c->inner_tol2 * s2, DBL_MIN);
-                c->iter = 0;
-                c->beta = 0.0;
+                c->rr_ref = s2;
+                c->threshold = thr;
                 c->done = 0;
                 c->converged = 0;
-                c->outer += 1;
             }
             break;
         }
+        case SC_RZ_REPLACE:
+            if (!was_done) {
+                c->beta = s / c->rz_div;
+                c->rz = s;
+                if (!isfinite(c->beta)) {
+                    c->done = 1;
+                    c->status = 3;
+                }
+            }
+            break;
         case SC_TRUE:
             c->true_rr = s;
             break;
         default:
             break;
     }
+}
+
+__global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, const double* parts, int n,
+                                                           const double* parts2, int nblk, uint64_t* stage,
+                                                           uint64_t timeout) {
+    __shared__ double sh[kReduceThreads];
+    __shared__ double sh2[kReduceThreads];
+    __shared__ double gs[kRedBlocks], gs2[kRedBlocks];
+    const int t = threadIdx.x, g = blockIdx.x;
+    const uint32_t tag = c->red_epoch + 1u;
+    const int seg = (n + nblk - 1) / nblk;
+    const int lo = g * seg, hi = min(n, lo + seg);
+    double s = 0.0, s2 = 0.0;
+    if (parts)
+        for (int q = lo + t; q < hi; q += kReduceThreads) s += parts[q];
+    if (parts2)
+        for (int q = lo + t; q < hi; q += kReduceThreads) s2 += parts2[q];
+    sh[t] = s;
+    sh2[t] = s2;
+    __syncthreads();
+    for (int o = kReduceThreads / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            sh[t] += sh[t + o];
+            sh2[t] += sh2[t + o];
+        }
+        __syncthreads();
+    }
+    if (g != 0) {
+        if (t == 0) {
+            xput(stage, (size_t)g * 4, sh[0], tag);
+            xput(stage, (size_t)g * 4 + 2, sh2[0], tag);
+        }
+        return;
+    }
+    if (t == 0) {
+        gs[0] = sh[0];
+        gs2[0] = sh2[0];
+    }
+    if (t > 0 && t < nblk) {
+        gs[t] = xget<double>(stage, (size_t)t * 4, tag, c, timeout);
+        gs2[t] = xget<double>(stage, (size_t)t * 4 + 2, tag, c, timeout);
+    }
+    __syncthreads();
+    if (t != 0) return;
+    double S = 0.0, S2 = 0.0;
+    for (int q = 0; q < nblk; ++q) {
+        S += gs[q];
+        S2 += gs2[q];
+    }
+    c->red_epoch = tag;
+    scalar_update(c, mode, S, S2);
 }
 
 // p = z + beta p_old (ping-pong buffers), q = A p, p.q partials. Cross-brick
@@ -625,7 +708,7 @@ __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, 
 // fused ahead of it: x += alpha p, r -= alpha q, |r|^2 partials.
 template <class T, bool UPDATE, bool LEVELS>
 __global__ __launch_bounds__(64) void k_fwd(Bricks bk, Coef<T> c, T* r, T* x, const T* p, const T* q, T* y,
-                                            double* part, Ctrl* ctrl, uint32_t* flags, int level_base,
+                                            double* part, Ctrl* ctrl, uint64_t* xch, int level_base,
                                             uint64_t timeout) {
     if (ctrl->done) return;
     const Lane L;
@@ -651,10 +734,16 @@ __global__ __launch_bounds__(64) void k_fwd(Bricks bk, Coef<T> c, T* r, T* x, co
     const T axm = ix ? sx : (nxm >= 0 ? c.ax[(size_t)nxm * kBrickCells + L.lane + 3] : (T)0);
     const T aym = iy ? sy : (nym >= 0 ? c.ay[(size_t)nym * kBrickCells + L.lane + 12] : (T)0);
     const T azm = iz ? sz : (nzm >= 0 ? c.az[(size_t)nzm * kBrickCells + L.lane + 48] : (T)0);
-    if (!LEVELS) wait_bricks(flags, nxm, nym, nzm, epoch, ctrl, timeout);
-    const T ex = (!ix && nxm >= 0) ? y[(size_t)nxm * kBrickCells + L.lane + 3] : (T)0;
-    const T ey = (!iy && nym >= 0) ? y[(size_t)nym * kBrickCells + L.lane + 12] : (T)0;
-    const T ez = (!iz && nzm >= 0) ? y[(size_t)nzm * kBrickCells + L.lane + 48] : (T)0;
+    T ex = (T)0, ey = (T)0, ez = (T)0;
+    if (LEVELS) {
+        if (!ix && nxm >= 0) ex = y[(size_t)nxm * kBrickCells + L.lane + 3];
+        if (!iy && nym >= 0) ey = y[(size_t)nym * kBrickCells + L.lane + 12];
+        if (!iz && nzm >= 0) ez = y[(size_t)nzm * kBrickCells + L.lane + 48];
+    } else {
+        if (!ix && nxm >= 0) ex = xget<T>(xch, xslot(nxm, 1, L.ly + 4 * L.lz), epoch, ctrl, timeout);
+        if (!iy && nym >= 0) ey = xget<T>(xch, xslot(nym, 3, L.lx + 4 * L.lz), epoch, ctrl, timeout);
+        if (!iz && nzm >= 0) ez = xget<T>(xch, xslot(nzm, 5, L.lx + 4 * L.ly), epoch, ctrl, timeout);
+    }
     const int h_own = L.lx + L.ly + L.lz;
     T yv = (T)0;
 #pragma unroll
@@ -667,15 +756,19 @@ __global__ __launch_bounds__(64) void k_fwd(Bricks bk, Coef<T> c, T* r, T* x, co
             yv = t * ie;
         }
     }
+    if (!LEVELS) {
+        if (L.lx == 3) xput(xch, xslot(b, 1, L.ly + 4 * L.lz), yv, epoch);
+        if (L.ly == 3) xput(xch, xslot(b, 3, L.lx + 4 * L.lz), yv, epoch);
+        if (L.lz == 3) xput(xch, xslot(b, 5, L.lx + 4 * L.ly), yv, epoch);
+    }
     y[o] = yv;
-    if (!LEVELS) signal_brick(flags, b, epoch);
 }
 
 // Backward sweep z = y - E^{-1} L^T z, in place over y, hyperplanes 9..0;
 // fused r.z partials.
 template <class T, bool LEVELS>
 __global__ __launch_bounds__(64) void k_bwd(Bricks bk, Coef<T> c, const T* r, T* y, double* part, Ctrl* ctrl,
-                                            uint32_t* flags, int level_base, uint64_t timeout) {
+                                            uint64_t* xch, int level_base, uint64_t timeout) {
     if (ctrl->done) return;
     const Lane L;
     const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
@@ -686,12 +779,19 @@ __global__ __launch_bounds__(64) void k_bwd(Bricks bk, Coef<T> c, const T* r, T*
     const T yv = y[o];
     const T ie = c.inv_e[o];
     const T axo = c.ax[o], ayo = c.ay[o], azo = c.az[o];
+    const T rv = r[o];
     const int nxp = nbr_id(bk, 1, b), nyp = nbr_id(bk, 3, b), nzp = nbr_id(bk, 5, b);
     const bool ix = L.lx < 3, iy = L.ly < 3, iz = L.lz < 3;
-    if (!LEVELS) wait_bricks(flags, nxp, nyp, nzp, epoch, ctrl, timeout);
-    const T ex = (!ix && nxp >= 0) ? y[(size_t)nxp * kBrickCells + L.lane - 3] : (T)0;
-    const T ey = (!iy && nyp >= 0) ? y[(size_t)nyp * kBrickCells + L.lane - 12] : (T)0;
-    const T ez = (!iz && nzp >= 0) ? y[(size_t)nzp * kBrickCells + L.lane - 48] : (T)0;
+    T ex = (T)0, ey = (T)0, ez = (T)0;
+    if (LEVELS) {
+        if (!ix && nxp >= 0) ex = y[(size_t)nxp * kBrickCells + L.lane - 3];
+        if (!iy && nyp >= 0) ey = y[(size_t)nyp * kBrickCells + L.lane - 12];
+        if (!iz && nzp >= 0) ez = y[(size_t)nzp * kBrickCells + L.lane - 48];
+    } else {
+        if (!ix && nxp >= 0) ex = xget<T>(xch, xslot(nxp, 0, L.ly + 4 * L.lz), epoch, ctrl, timeout);
+        if (!iy && nyp >= 0) ey = xget<T>(xch, xslot(nyp, 2, L.lx + 4 * L.lz), epoch, ctrl, timeout);
+        if (!iz && nzp >= 0) ez = xget<T>(xch, xslot(nzp, 4, L.lx + 4 * L.ly), epoch, ctrl, timeout);
+    }
     const int h_own = L.lx + L.ly + L.lz;
     T zv = (T)0;
 #pragma unroll
@@ -704,10 +804,14 @@ __global__ __launch_bounds__(64) void k_bwd(Bricks bk, Coef<T> c, const T* r, T*
             zv = fma(-ie, t, yv);
         }
     }
+    if (!LEVELS) {
+        if (L.lx == 0) xput(xch, xslot(b, 0, L.ly + 4 * L.lz), zv, epoch);
+        if (L.ly == 0) xput(xch, xslot(b, 2, L.lx + 4 * L.lz), zv, epoch);
+        if (L.lz == 0) xput(xch, xslot(b, 4, L.lx + 4 * L.ly), zv, epoch);
+    }
     y[o] = zv;
-    const double s = wave_sum((double)r[o] * (double)zv);
+    const double s = wave_sum((double)rv * (double)zv);
     if (L.lane == 0) part[b] = s;
-    if (!LEVELS) signal_brick(flags, b, epoch);
 }
 
 // r64 = b - A x64 in FP64 (order as the oracle's spmv: diag, -x, +x, -y, +y,
@@ -758,11 +862,13 @@ __global__ __launch_bounds__(256) void k_resid64(Bricks bk, const double* diag64
     if (r32) {
         const float rf = (float)rv;
         r32[o] = rf;
-        x32[o] = 0.0f;
-        p0[o] = 0.0f;
-        p1[o] = 0.0f;
         const double s2 = wave_sum((double)rf * (double)rf);
         if (L.lane == 0) part2[b] = s2;
+    }
+    if (x32) x32[o] = 0.0f;
+    if (p0) {
+        p0[o] = 0.0f;
+        p1[o] = 0.0f;
     }
 }
 
@@ -888,15 +994,15 @@ hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const 
 
 template <class T>
 hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, const T* ay, const T* az, double tau,
-                         double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl, uint32_t* flags,
+                         double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl, uint64_t* xch,
                          int32_t* n_fixed, int level, int level_base, int level_count, uint64_t timeout_ticks,
                          hipStream_t s) {
     if (level < 0) {
         hipLaunchKernelGGL((k_factor<T, false>), dim3(bk.nb), dim3(64), 0, s, bk, diag64, ax, ay, az, tau, sigma,
-                           inv_e64, inv_e32, ctrl, flags, n_fixed, 0, timeout_ticks);
+                           inv_e64, inv_e32, ctrl, xch, n_fixed, 0, timeout_ticks);
     } else if (level_count > 0) {
         hipLaunchKernelGGL((k_factor<T, true>), dim3(level_count), dim3(64), 0, s, bk, diag64, ax, ay, az, tau,
-                           sigma, inv_e64, inv_e32, ctrl, flags, n_fixed, level_base, timeout_ticks);
+                           sigma, inv_e64, inv_e32, ctrl, xch, n_fixed, level_base, timeout_ticks);
     }
     return hipGetLastError();
 }
@@ -918,8 +1024,16 @@ hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T
     return hipGetLastError();
 }
 
-hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2, hipStream_t s) {
-    hipLaunchKernelGGL(k_scalar, dim3(1), dim3(kReduceThreads), 0, s, ctrl, mode, parts, n, parts2);
+int scalar_blocks(int n) {
+    const int g = (n + kRedPerBlock - 1) / kRedPerBlock;
+    return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
+}
+
+hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2, uint64_t* stage,
+                         uint64_t timeout_ticks, hipStream_t s) {
+    const int nblk = scalar_blocks(n);
+    hipLaunchKernelGGL(k_scalar, dim3(nblk), dim3(kReduceThreads), 0, s, ctrl, mode, parts, n, parts2, nblk, stage,
+                       timeout_ticks);
     return hipGetLastError();
 }
 
@@ -933,35 +1047,35 @@ hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* 
 
 template <class T>
 hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y, double* part,
-                      Ctrl* ctrl, uint32_t* flags, int update, int level, int level_base, int level_count,
+                      Ctrl* ctrl, uint64_t* xch, int update, int level, int level_base, int level_count,
                       uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
         if (update)
             hipLaunchKernelGGL((k_fwd<T, true, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
-                               ctrl, flags, 0, timeout_ticks);
+                               ctrl, xch, 0, timeout_ticks);
         else
             hipLaunchKernelGGL((k_fwd<T, false, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
-                               ctrl, flags, 0, timeout_ticks);
+                               ctrl, xch, 0, timeout_ticks);
     } else if (level_count > 0) {
         if (update)
             hipLaunchKernelGGL((k_fwd<T, true, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y,
-                               part, ctrl, flags, level_base, timeout_ticks);
+                               part, ctrl, xch, level_base, timeout_ticks);
         else
             hipLaunchKernelGGL((k_fwd<T, false, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y,
-                               part, ctrl, flags, level_base, timeout_ticks);
+                               part, ctrl, xch, level_base, timeout_ticks);
     }
     return hipGetLastError();
 }
 
 template <class T>
 hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
-                      uint32_t* flags, int level, int level_base, int level_count, uint64_t timeout_ticks,
+                      uint64_t* xch, int level, int level_base, int level_count, uint64_t timeout_ticks,
                       hipStream_t s) {
     if (level < 0) {
-        hipLaunchKernelGGL((k_bwd<T, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, y, part, ctrl, flags, 0,
+        hipLaunchKernelGGL((k_bwd<T, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch, 0,
                            timeout_ticks);
     } else if (level_count > 0) {
-        hipLaunchKernelGGL((k_bwd<T, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, flags,
+        hipLaunchKernelGGL((k_bwd<T, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
                            level_base, timeout_ticks);
     }
     return hipGetLastError();
@@ -1001,7 +1115,7 @@ hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStr
     template hipError_t launch_assemble<T>(const Dense&, float, const uint8_t*, const Bricks&, double*, double*, T*, \
                                            T*, T*, T*, hipStream_t);                                                \
     template hipError_t launch_factor<T>(const Bricks&, const double*, const T*, const T*, const T*, double, double, \
-                                         double*, float*, Ctrl*, uint32_t*, int32_t*, int, int, int, uint64_t,      \
+                                         double*, float*, Ctrl*, uint64_t*, int32_t*, int, int, int, uint64_t,      \
                                          hipStream_t);                                                              \
     template hipError_t launch_export<T>(const Dense&, const uint8_t*, const int32_t*, const Bricks&, const double*, \
                                          const double*, const T*, const T*, const T*, double*, double*, double*,    \
@@ -1010,8 +1124,8 @@ hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStr
     template hipError_t launch_spmv<T>(const Bricks&, const Coef<T>&, const T*, const T*, T*, T*, double*, Ctrl*,   \
                                        hipStream_t);                                                                \
     template hipError_t launch_fwd<T>(const Bricks&, const Coef<T>&, T*, T*, const T*, const T*, T*, double*, Ctrl*, \
-                                      uint32_t*, int, int, int, int, uint64_t, hipStream_t);                        \
-    template hipError_t launch_bwd<T>(const Bricks&, const Coef<T>&, const T*, T*, double*, Ctrl*, uint32_t*, int,  \
+                                      uint64_t*, int, int, int, int, uint64_t, hipStream_t);                        \
+    template hipError_t launch_bwd<T>(const Bricks&, const Coef<T>&, const T*, T*, double*, Ctrl*, uint64_t*, int,  \
                                       int, int, uint64_t, hipStream_t);                                             \
     template hipError_t launch_resid64<T>(const Bricks&, const double*, const T*, const T*, const T*, const double*, \
                                           const double*, double*, double*, float*, float*, float*, float*, Ctrl*,   \

@@ -68,7 +68,7 @@ hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const 
 template <class T>
 hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, const T* ay, const T* az,
                          double tau, double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl,
-                         uint32_t* flags, int32_t* n_fixed, int level, int level_base, int level_count,
+                         uint64_t* xch, int32_t* n_fixed, int level, int level_base, int level_count,
                          uint64_t timeout_ticks, hipStream_t s);
 // Natural-order export of the assembled system (elq_build_matrix).
 template <class T>
@@ -80,8 +80,9 @@ hipError_t launch_export(const Dense& d, const uint8_t* mask, const int32_t* ind
 template <class T>
 hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T* p1, double* part,
                        Ctrl* ctrl, hipStream_t s);
-hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2,
-                         hipStream_t s);
+// stage: 4*kRedBlocks tagged words (zero-initialised once, never reset).
+hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2, uint64_t* stage,
+                         uint64_t timeout_ticks, hipStream_t s);
 template <class T>
 hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q,
                        double* part, Ctrl* ctrl, hipStream_t s);
@@ -89,15 +90,16 @@ hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* 
 // |r|^2 partials when update != 0.
 template <class T>
 hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
-                      double* part, Ctrl* ctrl, uint32_t* flags, int update, int level, int level_base,
+                      double* part, Ctrl* ctrl, uint64_t* xch, int update, int level, int level_base,
                       int level_count, uint64_t timeout_ticks, hipStream_t s);
 // Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
 template <class T>
 hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
-                      uint32_t* flags, int level, int level_base, int level_count, uint64_t timeout_ticks,
+                      uint64_t* xch, int level, int level_base, int level_count, uint64_t timeout_ticks,
                       hipStream_t s);
 // r64 = b64 - A x64 in FP64 (couplings TC read exactly); partials |r64|^2 and,
-// when r32 != nullptr, |float(r64)|^2 with r32 = float(r64) and x32/p0/p1 zeroed.
+// when r32 != nullptr, |float(r64)|^2 with r32 = float(r64); x32 and p0/p1 are
+// zeroed when non-null.
 template <class TC>
 hipError_t launch_resid64(const Bricks& bk, const double* diag64, const TC* ax, const TC* ay, const TC* az,
                           const double* b64, const double* x64, double* part, double* part2, float* r32,

@@ -59,7 +59,7 @@ typedef enum {
 
 typedef enum {
     ELQ_SCHED_WAVEFRONT = 0, /* one launch per sweep; bricks taken in level order,
-                                predecessors awaited through device flags */
+                                predecessor faces arrive as epoch-tagged words */
     ELQ_SCHED_LEVELS = 1     /* one launch per brick level (no device waits) */
 } elq_schedule;
 
@@ -87,7 +87,9 @@ typedef struct {
     int32_t precond;   /* elq_precond */
     double tau;        /* MIC(0) parameter (ignored for IC0) */
     double sigma;      /* pivot safeguard: pivot t <= sigma*diag -> diag */
-    double inner_tol;  /* MIXED: relative reduction per refinement step */
+    double inner_tol;  /* MIXED: the FP32 recursive residual is replaced by the FP64
+                          residual b - A x each time it has dropped by this factor
+                          (default 0.1) and whenever it meets the stopping test */
     int32_t schedule;  /* elq_schedule */
     int32_t check_every; /* PCG iterations between host convergence polls */
 } elq_opts;
@@ -97,7 +99,8 @@ typedef struct {
     int64_t n_bricks;          /* active 4x4x4 bricks */
     int32_t iterations;        /* Eigen iterations(): PCG iterations (FP64) or
                                   total inner iterations (MIXED) */
-    int32_t outer_iterations;  /* MIXED refinement steps (1 for FP64) */
+    int32_t outer_iterations;  /* MIXED: FP64 residual evaluations (residual
+                                  replacements incl. the first); 1 for FP64 */
     double error;              /* Eigen error(): sqrt(|r|^2/|b|^2), recursive r */
     double true_rel_residual;  /* |b - A p|_2 / |b|_2 recomputed in FP64 */
     int32_t status;            /* elq_status of the solve */

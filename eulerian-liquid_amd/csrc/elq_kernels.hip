@@ -16,6 +16,8 @@
 
 #include <cfloat>
 #include <climits>
+#include <tuple>
+#include <utility>
 
 #include "elq_kernels.hpp"
 
@@ -55,16 +57,9 @@ __device__ __forceinline__ size_t cell(int i, int j, int k, int ni, int nj) {
     return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
 }
 
-// Wavefront scheduling. Bricks are taken in level order from a ticket counter,
-// so every brick a wave waits on was taken earlier by a running wave: no
-// deadlock whatever the residency. Each wait is bounded in wall-clock time;
-// on expiry the wave raises ctrl->abort and every other wait returns at once.
-__device__ __forceinline__ int take_ticket(Ctrl* c) {
-    int t = 0;
-    if (lane_id() == 0) t = (int)atomicAdd(&c->ticket, 1u);
-    return __shfl(t, 0, 64);
-}
-
+// Wavefront scheduling: persistent waves walk the level-ordered brick list
+// (see k_fwd_wave); each wait is bounded in wall-clock time and on expiry the
+// wave raises ctrl->abort, after which every other wait returns at once.
 // Face exchange between the bricks of one sweep. A brick publishes the values
 // of its boundary cells as 64-bit words whose upper half is the sweep's epoch
 // (one word per float, two per double), written and polled with relaxed
@@ -353,15 +348,11 @@ __global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const ui
 // Inside the brick the ten hyperplanes lx+ly+lz = h are swept in order; pivots
 // of the three predecessor bricks arrive through the tagged face exchange.
 template <class T, bool LEVELS>
-__global__ __launch_bounds__(64) void k_factor(Bricks bk, const double* diag64, const T* ax, const T* ay,
-                                               const T* az, double tau, double sigma, double* inv_e64,
-                                               float* inv_e32, Ctrl* ctrl, uint64_t* xch, int32_t* n_fixed,
-                                               int level_base, uint64_t timeout) {
+__device__ __forceinline__ void factor_brick(int b, uint32_t epoch, const Bricks& bk, const double* diag64,
+                                             const T* ax, const T* ay, const T* az, double tau, double sigma,
+                                             double* inv_e64, float* inv_e32, Ctrl* ctrl, uint64_t* xch,
+                                             int32_t* n_fixed, uint64_t timeout) {
     const Lane L;
-    const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
-    if (slot >= bk.nb) return;
-    const int b = bk.lvl_list[slot];
-    const uint32_t epoch = ctrl->epoch;
     const size_t o = (size_t)b * kBrickCells + L.lane;
     const int nxm = nbr_id(bk, 0, b), nym = nbr_id(bk, 2, b), nzm = nbr_id(bk, 4, b);
     const double dg = diag64[o];
@@ -429,6 +420,30 @@ __global__ __launch_bounds__(64) void k_factor(Bricks bk, const double* diag64, 
     if (inv_e32) inv_e32[o] = (float)ie;
     const int nf = wave_sum_int(fixed);
     if (L.lane == 0 && nf) atomicAdd(n_fixed, nf);
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_factor_level(Bricks bk, const double* diag64, const T* ax, const T* ay,
+                                                     const T* az, double tau, double sigma, double* inv_e64,
+                                                     float* inv_e32, Ctrl* ctrl, uint64_t* xch, int32_t* n_fixed,
+                                                     int level_base, uint64_t timeout) {
+    factor_brick<T, true>(bk.lvl_list[level_base + (int)blockIdx.x], 0u, bk, diag64, ax, ay, az, tau, sigma,
+                          inv_e64, inv_e32, ctrl, xch, n_fixed, timeout);
+}
+
+// Persistent wavefront: wave w of W takes positions w, w+W, w+2W, ... of the
+// level-ordered brick list. Launched cooperatively (all waves resident), so a
+// brick's predecessors, which sit at smaller positions, always make progress.
+template <class T>
+__global__ __launch_bounds__(256) void k_factor_wave(Bricks bk, const double* diag64, const T* ax, const T* ay,
+                                                     const T* az, double tau, double sigma, double* inv_e64,
+                                                     float* inv_e32, Ctrl* ctrl, uint64_t* xch, int32_t* n_fixed,
+                                                     uint64_t timeout) {
+    const int W = (int)gridDim.x * 4;
+    const uint32_t epoch = ctrl->epoch;
+    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < bk.nb; pos += W)
+        factor_brick<T, false>(bk.lvl_list[pos], epoch, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl,
+                               xch, n_fixed, timeout);
 }
 
 template <class T>
@@ -707,15 +722,10 @@ __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, 
 // brick per wave, hyperplanes lx+ly+lz = 0..9 inside the brick. With UPDATE,
 // fused ahead of it: x += alpha p, r -= alpha q, |r|^2 partials.
 template <class T, bool UPDATE, bool LEVELS>
-__global__ __launch_bounds__(64) void k_fwd(Bricks bk, Coef<T> c, T* r, T* x, const T* p, const T* q, T* y,
-                                            double* part, Ctrl* ctrl, uint64_t* xch, int level_base,
-                                            uint64_t timeout) {
-    if (ctrl->done) return;
+__device__ __forceinline__ void fwd_brick(int b, uint32_t epoch, const Bricks& bk, const Coef<T>& c, T* r, T* x,
+                                          const T* p, const T* q, T* y, double* part, Ctrl* ctrl, uint64_t* xch,
+                                          uint64_t timeout) {
     const Lane L;
-    const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
-    if (slot >= bk.nb) return;
-    const int b = bk.lvl_list[slot];
-    const uint32_t epoch = ctrl->epoch;
     const size_t o = (size_t)b * kBrickCells + L.lane;
     T rv = r[o];
     if (UPDATE) {
@@ -764,17 +774,31 @@ __global__ __launch_bounds__(64) void k_fwd(Bricks bk, Coef<T> c, T* r, T* x, co
     y[o] = yv;
 }
 
+template <class T, bool UPDATE>
+__global__ __launch_bounds__(64) void k_fwd_level(Bricks bk, Coef<T> c, T* r, T* x, const T* p, const T* q, T* y,
+                                                  double* part, Ctrl* ctrl, uint64_t* xch, int level_base,
+                                                  uint64_t timeout) {
+    if (ctrl->done) return;
+    fwd_brick<T, UPDATE, true>(bk.lvl_list[level_base + (int)blockIdx.x], 0u, bk, c, r, x, p, q, y, part, ctrl, xch,
+                               timeout);
+}
+
+template <class T, bool UPDATE>
+__global__ __launch_bounds__(256) void k_fwd_wave(Bricks bk, Coef<T> c, T* r, T* x, const T* p, const T* q, T* y,
+                                                  double* part, Ctrl* ctrl, uint64_t* xch, uint64_t timeout) {
+    if (ctrl->done) return;
+    const int W = (int)gridDim.x * 4;
+    const uint32_t epoch = ctrl->epoch;
+    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < bk.nb; pos += W)
+        fwd_brick<T, UPDATE, false>(bk.lvl_list[pos], epoch, bk, c, r, x, p, q, y, part, ctrl, xch, timeout);
+}
+
 // Backward sweep z = y - E^{-1} L^T z, in place over y, hyperplanes 9..0;
 // fused r.z partials.
 template <class T, bool LEVELS>
-__global__ __launch_bounds__(64) void k_bwd(Bricks bk, Coef<T> c, const T* r, T* y, double* part, Ctrl* ctrl,
-                                            uint64_t* xch, int level_base, uint64_t timeout) {
-    if (ctrl->done) return;
+__device__ __forceinline__ void bwd_brick(int b, uint32_t epoch, const Bricks& bk, const Coef<T>& c, const T* r,
+                                          T* y, double* part, Ctrl* ctrl, uint64_t* xch, uint64_t timeout) {
     const Lane L;
-    const int slot = LEVELS ? level_base + (int)blockIdx.x : take_ticket(ctrl);
-    if (slot >= bk.nb) return;
-    const int b = LEVELS ? bk.lvl_list[slot] : bk.lvl_list[bk.nb - 1 - slot];
-    const uint32_t epoch = ctrl->epoch;
     const size_t o = (size_t)b * kBrickCells + L.lane;
     const T yv = y[o];
     const T ie = c.inv_e[o];
@@ -812,6 +836,23 @@ __global__ __launch_bounds__(64) void k_bwd(Bricks bk, Coef<T> c, const T* r, T*
     y[o] = zv;
     const double s = wave_sum((double)rv * (double)zv);
     if (L.lane == 0) part[b] = s;
+}
+
+template <class T>
+__global__ __launch_bounds__(64) void k_bwd_level(Bricks bk, Coef<T> c, const T* r, T* y, double* part, Ctrl* ctrl,
+                                                  uint64_t* xch, int level_base, uint64_t timeout) {
+    if (ctrl->done) return;
+    bwd_brick<T, true>(bk.lvl_list[level_base + (int)blockIdx.x], 0u, bk, c, r, y, part, ctrl, xch, timeout);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bwd_wave(Bricks bk, Coef<T> c, const T* r, T* y, double* part, Ctrl* ctrl,
+                                                  uint64_t* xch, uint64_t timeout) {
+    if (ctrl->done) return;
+    const int W = (int)gridDim.x * 4;
+    const uint32_t epoch = ctrl->epoch;
+    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < bk.nb; pos += W)
+        bwd_brick<T, false>(bk.lvl_list[bk.nb - 1 - pos], epoch, bk, c, r, y, part, ctrl, xch, timeout);
 }
 
 // r64 = b - A x64 in FP64 (order as the oracle's spmv: diag, -x, +x, -y, +y,
@@ -934,6 +975,39 @@ __global__ __launch_bounds__(256) void k_update(Dense d, float dt, const float* 
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
+constexpr int kWaveBlocksPerCU = 2;  // persistent sweeps: 2 workgroups (8 waves) per CU
+
+int device_cus() {
+    static int cus[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1;
+    if (!cus[dev] && hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus[dev] = 1;
+    return cus[dev] > 0 ? cus[dev] : 1;
+}
+
+// Cooperative launch of a persistent sweep kernel (256 threads = 4 waves per
+// workgroup): grid = min(bricks / 4, resident workgroups), so every wave is
+// resident for the whole sweep.
+template <typename... KArgs, typename... Args, size_t... I>
+hipError_t coop_impl(void (*kern)(KArgs...), int nb, hipStream_t s, std::index_sequence<I...>, Args&&... args) {
+    std::tuple<KArgs...> t(std::forward<Args>(args)...);
+    void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 256, 0) != hipSuccess || occ < 1) occ = 1;
+    const int per_cu = occ < kWaveBlocksPerCU ? occ : kWaveBlocksPerCU;
+    int g = (nb + 3) / 4;
+    if (g > per_cu * device_cus()) g = per_cu * device_cus();
+    if (g < 1) g = 1;
+    return hipLaunchCooperativeKernel((const void*)kern, dim3(g), dim3(256), ptrs, 0, s);
+}
+
+template <typename... KArgs, typename... Args>
+hipError_t coop(void (*kern)(KArgs...), int nb, hipStream_t s, Args&&... args) {
+    static_assert(sizeof...(KArgs) == sizeof...(Args), "argument count");
+    return coop_impl(kern, nb, s, std::index_sequence_for<KArgs...>{}, std::forward<Args>(args)...);
+}
+
 }  // namespace
 
 // ================================================================== launchers
@@ -994,16 +1068,16 @@ hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const 
 
 template <class T>
 hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, const T* ay, const T* az, double tau,
-                         double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl, uint64_t* xch,
-                         int32_t* n_fixed, int level, int level_base, int level_count, uint64_t timeout_ticks,
-                         hipStream_t s) {
+                         double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl, uint64_t* xch, int32_t* n_fixed,
+                         int level, int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
-        hipLaunchKernelGGL((k_factor<T, false>), dim3(bk.nb), dim3(64), 0, s, bk, diag64, ax, ay, az, tau, sigma,
-                           inv_e64, inv_e32, ctrl, xch, n_fixed, 0, timeout_ticks);
-    } else if (level_count > 0) {
-        hipLaunchKernelGGL((k_factor<T, true>), dim3(level_count), dim3(64), 0, s, bk, diag64, ax, ay, az, tau,
-                           sigma, inv_e64, inv_e32, ctrl, xch, n_fixed, level_base, timeout_ticks);
+        if (bk.nb == 0) return hipSuccess;
+        return coop(k_factor_wave<T>, bk.nb, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
+                    n_fixed, timeout_ticks);
     }
+    if (level_count > 0)
+        hipLaunchKernelGGL((k_factor_level<T>), dim3(level_count), dim3(64), 0, s, bk, diag64, ax, ay, az, tau, sigma,
+                           inv_e64, inv_e32, ctrl, xch, n_fixed, level_base, timeout_ticks);
     return hipGetLastError();
 }
 
@@ -1050,34 +1124,31 @@ hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p
                       Ctrl* ctrl, uint64_t* xch, int update, int level, int level_base, int level_count,
                       uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
+        if (bk.nb == 0) return hipSuccess;
+        if (update) return coop(k_fwd_wave<T, true>, bk.nb, s, bk, c, r, x, p, q, y, part, ctrl, xch, timeout_ticks);
+        return coop(k_fwd_wave<T, false>, bk.nb, s, bk, c, r, x, p, q, y, part, ctrl, xch, timeout_ticks);
+    }
+    if (level_count > 0) {
         if (update)
-            hipLaunchKernelGGL((k_fwd<T, true, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
-                               ctrl, xch, 0, timeout_ticks);
+            hipLaunchKernelGGL((k_fwd_level<T, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
+                               ctrl, xch, level_base, timeout_ticks);
         else
-            hipLaunchKernelGGL((k_fwd<T, false, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
-                               ctrl, xch, 0, timeout_ticks);
-    } else if (level_count > 0) {
-        if (update)
-            hipLaunchKernelGGL((k_fwd<T, true, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y,
-                               part, ctrl, xch, level_base, timeout_ticks);
-        else
-            hipLaunchKernelGGL((k_fwd<T, false, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y,
-                               part, ctrl, xch, level_base, timeout_ticks);
+            hipLaunchKernelGGL((k_fwd_level<T, false>), dim3(level_count), dim3(64), 0, s, bk, c, r, x, p, q, y, part,
+                               ctrl, xch, level_base, timeout_ticks);
     }
     return hipGetLastError();
 }
 
 template <class T>
-hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
-                      uint64_t* xch, int level, int level_base, int level_count, uint64_t timeout_ticks,
-                      hipStream_t s) {
+hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl, uint64_t* xch,
+                      int level, int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
-        hipLaunchKernelGGL((k_bwd<T, false>), dim3(bk.nb), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch, 0,
-                           timeout_ticks);
-    } else if (level_count > 0) {
-        hipLaunchKernelGGL((k_bwd<T, true>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
-                           level_base, timeout_ticks);
+        if (bk.nb == 0) return hipSuccess;
+        return coop(k_bwd_wave<T>, bk.nb, s, bk, c, r, y, part, ctrl, xch, timeout_ticks);
     }
+    if (level_count > 0)
+        hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
+                           level_base, timeout_ticks);
     return hipGetLastError();
 }
 

@@ -1,0 +1,168 @@
+// C++ host side of the projection over the C ABI (see simulator.hpp).
+#include "simulator.hpp"
+
+#include <cfloat>
+#include <string>
+
+namespace elq_host {
+
+namespace {
+
+elq_grid make_grid(int ni, int nj, int nk, float dx, const Grid3f& level_set, StaggeredGrid3f& velocity,
+                   const StaggeredGrid3f& weight) {
+    elq_grid g{};
+    g.ni = ni;
+    g.nj = nj;
+    g.nk = nk;
+    g.dx = dx;
+    g.level_set = level_set.data.data();
+    g.u = velocity.u().data.data();
+    g.v = velocity.v().data.data();
+    g.w = velocity.w().data.data();
+    g.u_weight = weight.u().data.data();
+    g.v_weight = weight.v().data.data();
+    g.w_weight = weight.w().data.data();
+    return g;
+}
+
+[[noreturn]] void raise(elq_ctx* ctx, int rc) {
+    if (rc == ELQ_ERR_NO_FLUID) throw std::runtime_error("No fluid");  // projection.cpp:25
+    throw std::runtime_error(std::string("elq: ") + elq_ctx_last_error(ctx));
+}
+
+}  // namespace
+
+Device::Device(int device, size_t hbm_budget) {
+    const int rc = elq_ctx_create(device, hbm_budget, &m_ctx);
+    if (rc != ELQ_OK) throw std::runtime_error("elq: no usable gfx950 device");
+}
+
+Device::~Device() { elq_ctx_destroy(m_ctx); }
+
+namespace fluid {
+
+int assign_indices(Device& dev, Grid3i& indices, const Grid3f& level_set, const StaggeredGrid3f& weight, float dx) {
+    StaggeredGrid3f zero;
+    zero.create(level_set.ni, level_set.nj, level_set.nk);
+    elq_grid g = make_grid(level_set.ni, level_set.nj, level_set.nk, dx, level_set, zero, weight);
+    indices.create(level_set.ni, level_set.nj, level_set.nk);
+    int64_t n = 0;
+    const int rc = elq_assign_indices(dev.ctx(), &g, indices.data.data(), &n);
+    if (rc != ELQ_OK) raise(dev.ctx(), rc);
+    return (int)n;
+}
+
+void build_matrix(Device& dev, StencilMatrix& A, std::vector<double>& b, int mat_size, const Grid3f& level_set,
+                  const StaggeredGrid3f& velocity, const StaggeredGrid3f& weight, float dx, float dt) {
+    StaggeredGrid3f vel = velocity;
+    elq_grid g = make_grid(level_set.ni, level_set.nj, level_set.nk, dx, level_set, vel, weight);
+    A.diag.assign(mat_size, 0.0);
+    A.ax.assign(mat_size, 0.0);
+    A.ay.assign(mat_size, 0.0);
+    A.az.assign(mat_size, 0.0);
+    b.assign(mat_size, 0.0);
+    const int rc = elq_build_matrix(dev.ctx(), &g, dt, A.diag.data(), A.ax.data(), A.ay.data(), A.az.data(), b.data());
+    if (rc != ELQ_OK) raise(dev.ctx(), rc);
+}
+
+void to_csr(const StencilMatrix& A, const Grid3i& indices, std::vector<int32_t>& row_ptr, std::vector<int32_t>& col,
+            std::vector<double>& val) {
+    const int n = (int)A.diag.size();
+    const int ni = indices.ni, nj = indices.nj, nk = indices.nk;
+    std::vector<int32_t> nb(6 * (size_t)n, -1);  // -x,+x,-y,+y,-z,+z
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const int32_t c = indices.data[(size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * k)];
+                if (c == ELQ_UNKNOWN_INDEX) continue;
+                auto at = [&](int x, int y, int z) -> int32_t {
+                    if (x < 0 || y < 0 || z < 0 || x >= ni || y >= nj || z >= nk) return -1;
+                    const int32_t v = indices.data[(size_t)x + (size_t)ni * ((size_t)y + (size_t)nj * z)];
+                    return v == ELQ_UNKNOWN_INDEX ? -1 : v;
+                };
+                const int32_t q[6] = {at(i - 1, j, k), at(i + 1, j, k), at(i, j - 1, k),
+                                      at(i, j + 1, k), at(i, j, k - 1), at(i, j, k + 1)};
+                for (int d = 0; d < 6; ++d) nb[6 * (size_t)c + d] = q[d];
+            }
+    row_ptr.assign(n + 1, 0);
+    col.clear();
+    val.clear();
+    for (int r = 0; r < n; ++r) {
+        row_ptr[r] = (int32_t)col.size();
+        const int32_t* q = &nb[6 * (size_t)r];
+        // lexicographic numbering: -z < -y < -x < self < +x < +y < +z
+        if (q[4] >= 0) { col.push_back(q[4]); val.push_back(A.az[q[4]]); }
+        if (q[2] >= 0) { col.push_back(q[2]); val.push_back(A.ay[q[2]]); }
+        if (q[0] >= 0) { col.push_back(q[0]); val.push_back(A.ax[q[0]]); }
+        col.push_back(r);
+        val.push_back(A.diag[r]);
+        if (q[1] >= 0) { col.push_back(q[1]); val.push_back(A.ax[r]); }
+        if (q[3] >= 0) { col.push_back(q[3]); val.push_back(A.ay[r]); }
+        if (q[5] >= 0) { col.push_back(q[5]); val.push_back(A.az[r]); }
+    }
+    row_ptr[n] = (int32_t)col.size();
+}
+
+void build_grid_from_vector(Grid3f& grid, const std::vector<double>& vector, const Grid3i& indices,
+                            float background_value) {
+    grid.create(indices.ni, indices.nj, indices.nk);
+    for (size_t c = 0; c < grid.data.size(); ++c)
+        grid.data[c] = indices.data[c] == ELQ_UNKNOWN_INDEX ? background_value : (float)vector[indices.data[c]];
+}
+
+void update_velocity_using_pressure(Device& dev, StaggeredGrid3f& velocity, const Grid3f& pressure,
+                                    const Grid3f& level_set, const StaggeredGrid3f& weight, float dx, float dt) {
+    elq_grid g = make_grid(level_set.ni, level_set.nj, level_set.nk, dx, level_set, velocity, weight);
+    g.pressure = const_cast<float*>(pressure.data.data());
+    const int rc = elq_update_velocity(dev.ctx(), &g, dt);
+    if (rc != ELQ_OK) raise(dev.ctx(), rc);
+}
+
+}  // namespace fluid
+
+void Simulator::create_grid(float width, int ni, int nj, int nk) {
+    m_ni = ni;
+    m_nj = nj;
+    m_nk = nk;
+    m_dx = width / ni;
+    m_velocity.create(ni, nj, nk);
+    m_velocity_weight.create(ni, nj, nk);
+    m_level_set.create(ni, nj, nk);
+    m_indices.create(ni, nj, nk);
+    m_pressure.create(ni, nj, nk);
+}
+
+void Simulator::init_scene(float pool_height, const std::vector<std::array<float, 4>>& spheres) {
+    std::vector<float> flat;
+    for (const auto& s : spheres) flat.insert(flat.end(), s.begin(), s.end());
+    const int rc = elq_scene_level_set(m_ni, m_nj, m_nk, m_dx, pool_height, (int32_t)spheres.size(),
+                                       flat.empty() ? nullptr : flat.data(), m_level_set.data.data());
+    if (rc != ELQ_OK) throw std::runtime_error("elq: invalid scene");
+}
+
+void Simulator::compute_weights() {
+    const int rc = elq_scene_weights(m_ni, m_nj, m_nk, m_velocity_weight.u().data.data(),
+                                     m_velocity_weight.v().data.data(), m_velocity_weight.w().data.data());
+    if (rc != ELQ_OK) throw std::runtime_error("elq: invalid grid");
+}
+
+void Simulator::add_gravity(float dt) {
+    const int rc = elq_scene_add_gravity(m_ni, m_nj, m_nk, m_velocity.v().data.data(), dt);
+    if (rc != ELQ_OK) throw std::runtime_error("elq: invalid grid");
+}
+
+void Simulator::find_valid_velocity_and_zero_invalid() {
+    elq_grid g = make_grid(m_ni, m_nj, m_nk, m_dx, m_level_set, m_velocity, m_velocity_weight);
+    const int rc = elq_scene_valid_and_zero(&g);
+    if (rc != ELQ_OK) throw std::runtime_error("elq: invalid grid");
+}
+
+void Simulator::solve_pressure(float dt) {
+    elq_grid g = make_grid(m_ni, m_nj, m_nk, m_dx, m_level_set, m_velocity, m_velocity_weight);
+    g.pressure = m_pressure.data.data();
+    g.indices = m_indices.data.data();
+    const int rc = elq_project(m_dev.ctx(), &g, dt, &m_opts, &m_stats);
+    if (rc != ELQ_OK && rc != ELQ_ERR_NOT_CONVERGED) raise(m_dev.ctx(), rc);
+}
+
+}  // namespace elq_host

@@ -1,0 +1,135 @@
+// C++ host side of the projection: the reference simulator's projection step
+// over the C ABI of include/elq.h.
+//
+// Mirrors the interface a caller of the reference uses:
+//   namespace nama::_3d::fluid       (include/nama/project/3d/fluid/projection.h:10-22)
+//     assign_indices, build_matrix, build_grid_from_vector,
+//     update_velocity_using_pressure
+//   class Simulator                  (include/nama/project/3d/eulerian_liquid/simulator.h:16-73)
+//     create_grid, init_scene, compute_weights, add_gravity,
+//     find_valid_velocity, set_invalid_velocity_zero, solve_pressure
+// Grids are flat std::vector<float>/<int> in the reference layout
+// (i + ni*(j + nj*k)); staggered grids are three such vectors (u, v, w).
+// Errors follow the reference: solve_pressure and assign_indices throw
+// std::runtime_error("No fluid"); non-convergence is not an error
+// (simulator.cpp:105-109 never checks it) but is reported in last_stats().
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/elq.h"
+
+namespace elq_host {
+
+struct Grid3f {
+    int ni = 0, nj = 0, nk = 0;
+    std::vector<float> data;
+    void create(int a, int b, int c, float value = 0.0f) {
+        ni = a;
+        nj = b;
+        nk = c;
+        data.assign((size_t)a * b * c, value);
+    }
+    float& operator()(int i, int j, int k) { return data[(size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * k)]; }
+    float operator()(int i, int j, int k) const {
+        return data[(size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * k)];
+    }
+};
+
+struct Grid3i {
+    int ni = 0, nj = 0, nk = 0;
+    std::vector<int32_t> data;
+    void create(int a, int b, int c, int32_t value = 0) {
+        ni = a;
+        nj = b;
+        nk = c;
+        data.assign((size_t)a * b * c, value);
+    }
+};
+
+// staggered_grid.h:100-112: u (ni+1)nj nk, v ni(nj+1)nk, w ni nj(nk+1)
+struct StaggeredGrid3f {
+    std::array<Grid3f, 3> c;
+    void create(int ni, int nj, int nk, float value = 0.0f) {
+        c[0].create(ni + 1, nj, nk, value);
+        c[1].create(ni, nj + 1, nk, value);
+        c[2].create(ni, nj, nk + 1, value);
+    }
+    Grid3f& u() { return c[0]; }
+    Grid3f& v() { return c[1]; }
+    Grid3f& w() { return c[2]; }
+    const Grid3f& u() const { return c[0]; }
+    const Grid3f& v() const { return c[1]; }
+    const Grid3f& w() const { return c[2]; }
+};
+
+// Owns one device context (device buffers persist across frames).
+class Device {
+   public:
+    explicit Device(int device = 0, size_t hbm_budget = 0);
+    ~Device();
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+    elq_ctx* ctx() const { return m_ctx; }
+
+   private:
+    elq_ctx* m_ctx = nullptr;
+};
+
+namespace fluid {
+// projection.cpp:14-28
+int assign_indices(Device& dev, Grid3i& indices, const Grid3f& level_set, const StaggeredGrid3f& weight, float dx);
+// projection.cpp:30-83; A in stencil form (A(n,n) = diag[n], A(n,+x) = ax[n], ...)
+struct StencilMatrix {
+    std::vector<double> diag, ax, ay, az;
+};
+void build_matrix(Device& dev, StencilMatrix& A, std::vector<double>& b, int mat_size, const Grid3f& level_set,
+                  const StaggeredGrid3f& velocity, const StaggeredGrid3f& weight, float dx, float dt);
+// CSR with Eigen::SparseMatrix<double, RowMajor> semantics (columns sorted,
+// explicit zeros kept), built from the stencil form and the indices grid.
+void to_csr(const StencilMatrix& A, const Grid3i& indices, std::vector<int32_t>& row_ptr, std::vector<int32_t>& col,
+            std::vector<double>& val);
+// projection.cpp:85-91
+void build_grid_from_vector(Grid3f& grid, const std::vector<double>& vector, const Grid3i& indices,
+                            float background_value);
+// projection.cpp:93-134
+void update_velocity_using_pressure(Device& dev, StaggeredGrid3f& velocity, const Grid3f& pressure,
+                                    const Grid3f& level_set, const StaggeredGrid3f& weight, float dx, float dt);
+}  // namespace fluid
+
+class Simulator {
+   public:
+    explicit Simulator(Device& dev) : m_dev(dev) { elq_default_opts(&m_opts); }
+
+    // simulator.cpp:52-70 (nodal solid grid is generated inside compute_weights)
+    void create_grid(float width, int ni, int nj, int nk);
+    // simulator.cpp:72-76: liquid = FLT_MAX, then min with the emitters
+    void init_scene(float pool_height, const std::vector<std::array<float, 4>>& spheres);
+    // simulator.cpp:187-189 with the room() solid of levelset_emitter.cpp:18-32
+    void compute_weights();
+    void add_gravity(float dt);                    // simulator.cpp:90-93
+    void find_valid_velocity_and_zero_invalid();   // simulator.cpp:125-158 + 169-178
+    void solve_pressure(float dt);                 // simulator.cpp:96-112
+
+    elq_opts& opts() { return m_opts; }
+    const elq_stats& last_stats() const { return m_stats; }
+
+    float m_dx = 0.0f;
+    int m_ni = 0, m_nj = 0, m_nk = 0;
+    StaggeredGrid3f m_velocity;
+    StaggeredGrid3f m_velocity_weight;
+    Grid3f m_level_set;
+    Grid3i m_indices;
+    Grid3f m_pressure;
+
+   private:
+    Device& m_dev;
+    elq_opts m_opts;
+    elq_stats m_stats{};
+};
+
+}  // namespace elq_host

@@ -171,7 +171,16 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     HIPCHK(W.lvl_cnt.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
     HIPCHK(W.lvl_off.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
     HIPCHK(W.lvl_fill.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
-    HIPCHK(W.info.ensure(sizeof(int64_t) * 2 + sizeof(int32_t) * 2), "alloc info");
+    const int64_t NP = (int64_t)p.nby * p.nbz;
+    const int pnlev = p.nby + p.nbz - 1;
+    HIPCHK(W.pflag.ensure((size_t)NP), "alloc pencils");
+    HIPCHK(W.pmap.ensure((size_t)NP * 4), "alloc pencils");
+    HIPCHK(W.pcoord.ensure((size_t)NP * 4), "alloc pencils");
+    HIPCHK(W.plvl_list.ensure((size_t)NP * 4), "alloc pencils");
+    HIPCHK(W.plvl_cnt.ensure((size_t)(pnlev + 1) * 4), "alloc pencils");
+    HIPCHK(W.plvl_off.ensure((size_t)(pnlev + 1) * 4), "alloc pencils");
+    HIPCHK(W.plvl_fill.ensure((size_t)(pnlev + 1) * 4), "alloc pencils");
+    HIPCHK(W.info.ensure(sizeof(int64_t) * 4 + sizeof(int32_t) * 2), "alloc info");
     int32_t* indices = indices_out;
     if (!indices) {
         HIPCHK(W.indices.ensure((size_t)p.C * 4), "alloc indices");
@@ -180,9 +189,10 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     W.cur_indices = indices;
     hipStream_t s = ctx->stream;
     int64_t* totals = (int64_t*)W.info.p;
-    int32_t* err = (int32_t*)(totals + 2);
+    int32_t* err = (int32_t*)(totals + 4);
     HIPCHK(hipMemsetAsync(W.info.p, 0, W.info.cap, s), "memset");
     HIPCHK(hipMemsetAsync(W.lvl_cnt.p, 0, (size_t)(p.nlev + 1) * 4, s), "memset");
+    HIPCHK(hipMemsetAsync(W.plvl_cnt.p, 0, (size_t)(pnlev + 1) * 4, s), "memset");
     ctx->prof_begin("setup_mask_scan");
     HIPCHK(launch_mask(d, (uint8_t*)W.mask.p, err, s), "k_mask");
     HIPCHK(launch_scan((const uint8_t*)W.mask.p, p.C, kFluid, indices, ELQ_UNKNOWN_INDEX, (int32_t*)W.scan_cnt.p,
@@ -198,14 +208,24 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
            "k_brick_meta");
     HIPCHK(launch_level_scan((const int32_t*)W.lvl_cnt.p, p.nlev, (int32_t*)W.lvl_off.p, (int32_t*)W.lvl_fill.p, s),
            "k_level_scan");
+    // x-pencils of the wavefront sweeps
+    HIPCHK(launch_pencil_flags((const int32_t*)W.map.p, p.nbx, p.nby, p.nbz, (uint8_t*)W.pflag.p, s), "k_pencil_flags");
+    HIPCHK(launch_scan((const uint8_t*)W.pflag.p, NP, 1, (int32_t*)W.pmap.p, -1, (int32_t*)W.scan_cnt.p,
+                       (int32_t*)W.scan_off.p, &totals[2], s),
+           "scan pencils");
+    HIPCHK(launch_pencil_meta((const int32_t*)W.pmap.p, p.nby, p.nbz, (int32_t*)W.pcoord.p, (int32_t*)W.plvl_cnt.p, s),
+           "k_pencil_meta");
+    HIPCHK(launch_level_scan((const int32_t*)W.plvl_cnt.p, pnlev, (int32_t*)W.plvl_off.p, (int32_t*)W.plvl_fill.p, s),
+           "k_level_scan");
     ctx->prof_end();
-    int64_t host_tot[2];
+    int64_t host_tot[3];
     int32_t host_err = 0;
     HIPCHK(hipMemcpyAsync(host_tot, totals, sizeof(host_tot), hipMemcpyDeviceToHost, s), "d2h");
     HIPCHK(hipMemcpyAsync(&host_err, err, sizeof(host_err), hipMemcpyDeviceToHost, s), "d2h");
     HIPCHK(hipStreamSynchronize(s), "sync");
     W.n_unknowns = host_tot[0];
     W.nb = (int)host_tot[1];
+    W.np = (int)host_tot[2];
     W.setup_err = host_err;
     W.plan_ni = d.ni;
     W.plan_nj = d.nj;
@@ -226,6 +246,16 @@ Bricks make_bricks(elq_ctx* ctx, const Plan& p) {
     bk.nbr = (const int32_t*)W.nbr.p;
     bk.lvl_list = (const int32_t*)W.lvl_list.p;
     return bk;
+}
+
+Pencils make_pencils(elq_ctx* ctx, const Plan& p) {
+    Workspace& W = ctx->ws;
+    Pencils pc;
+    pc.np = W.np;
+    pc.nlev = p.nby + p.nbz - 1;
+    pc.coord = (const int32_t*)W.pcoord.p;
+    pc.lvl_list = (const int32_t*)W.plvl_list.p;
+    return pc;
 }
 
 // Phase B: lists, assembly (T storage). FP64 arithmetic for diag/b always.
@@ -266,6 +296,9 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     HIPCHK(launch_brick_lists(bk.map, bk.coord, p.nbx, p.nby, p.nbz, nb, (const int32_t*)W.lvl_off.p,
                               (int32_t*)W.lvl_fill.p, (int32_t*)W.lvl_list.p, (int32_t*)W.nbr.p, s),
            "k_brick_lists");
+    HIPCHK(launch_pencil_lists((const int32_t*)W.pcoord.p, W.np, (const int32_t*)W.plvl_off.p, (int32_t*)W.plvl_fill.p,
+                               (int32_t*)W.plvl_list.p, s),
+           "k_pencil_lists");
     HIPCHK(launch_assemble<T>(d, dt, (const uint8_t*)W.mask.p, bk, (double*)W.diag64.p, (double*)W.b64.p,
                               (T*)W.cdiag.p, (T*)W.cax.p, (T*)W.cay.p, (T*)W.caz.p, s),
            "k_assemble");
@@ -306,6 +339,7 @@ struct Solver {
     elq_ctx* ctx;
     Plan p;
     Bricks bk;
+    Pencils pc;
     Coef<T> c;
     const elq_opts* o;
     bool levels;
@@ -329,13 +363,13 @@ struct Solver {
         Workspace& W = ctx->ws;
         ctx->prof_begin(update ? "fwd_sweep_update" : "fwd_sweep");
         if (!levels) {
-            HIPCHK(launch_fwd<T>(bk, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
+            HIPCHK(launch_fwd<T>(bk, pc, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
                                  (uint64_t*)W.xch.p, update, -1, 0, 0, ctx->timeout_ticks, ctx->stream),
                    "k_fwd");
         } else {
             for (int l = 0; l < p.nlev; ++l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
-                HIPCHK(launch_fwd<T>(bk, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
+                HIPCHK(launch_fwd<T>(bk, pc, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
                                      (uint64_t*)W.xch.p, update, l, base, cnt, ctx->timeout_ticks, ctx->stream),
                        "k_fwd");
             }
@@ -348,13 +382,13 @@ struct Solver {
         Workspace& W = ctx->ws;
         ctx->prof_begin("bwd_sweep");
         if (!levels) {
-            HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, -1, 0, 0,
+            HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, -1, 0, 0,
                                  ctx->timeout_ticks, ctx->stream),
                    "k_bwd");
         } else {
             for (int l = p.nlev - 1; l >= 0; --l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
-                HIPCHK(launch_bwd<T>(bk, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, l, base,
+                HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, l, base,
                                      cnt, ctx->timeout_ticks, ctx->stream),
                        "k_bwd");
             }
@@ -526,6 +560,7 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     S.ctx = ctx;
     S.p = p;
     S.bk = make_bricks(ctx, p);
+    S.pc = make_pencils(ctx, p);
     S.o = &o;
     S.levels = levels;
     S.c.diag = (const T*)W.cdiag.p;

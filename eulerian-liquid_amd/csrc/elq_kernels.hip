@@ -114,6 +114,62 @@ __device__ __forceinline__ double xget<double>(const uint64_t* x, size_t s, uint
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+// Gathers up to N tagged values at once (slot < 0: none, value 0). All loads
+// of a round are issued before any tag is tested, so faces that are already
+// published cost a single memory round trip.
+template <class T, int N>
+__device__ __forceinline__ void xgather(const uint64_t* x, const long (&slot)[N], uint32_t e, T (&out)[N], Ctrl* c,
+                                        uint64_t timeout) {
+    constexpr int W = sizeof(T) / 4;
+    uint32_t lo[N][W];
+    bool ok[N];
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        ok[q] = slot[q] < 0;
+#pragma unroll
+        for (int h = 0; h < W; ++h) lo[q][h] = 0u;
+    }
+    uint64_t t0 = 0;
+    for (int round = 0;; ++round) {
+        bool all = true;
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            if (ok[q]) continue;
+            uint64_t v[W];
+#pragma unroll
+            for (int h = 0; h < W; ++h)
+                v[h] = __hip_atomic_load(x + slot[q] + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bool good = true;
+#pragma unroll
+            for (int h = 0; h < W; ++h) {
+                good = good && (uint32_t)(v[h] >> 32) == e;
+                lo[q][h] = (uint32_t)v[h];
+            }
+            ok[q] = good;
+            all = all && good;
+        }
+        if (all) break;
+        if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        if (round == 0) {
+            t0 = wall_clock64();
+        } else if (wall_clock64() - t0 > timeout) {
+            __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+#pragma unroll
+    for (int q = 0; q < N; ++q) {
+        if (slot[q] < 0) {
+            out[q] = (T)0;
+        } else if constexpr (W == 1) {
+            out[q] = __uint_as_float(lo[q][0]);
+        } else {
+            out[q] = __longlong_as_double((long long)(((uint64_t)lo[q][1] << 32) | lo[q][0]));
+        }
+    }
+}
+
 struct Lane {
     int lane, lx, ly, lz;
     __device__ Lane() {
@@ -381,9 +437,14 @@ __device__ __forceinline__ void factor_brick(int b, uint32_t epoch, const Bricks
         if (!iy && nym >= 0) ey = inv_e64[(size_t)nym * kBrickCells + L.lane + 12];
         if (!iz && nzm >= 0) ez = inv_e64[(size_t)nzm * kBrickCells + L.lane + 48];
     } else {
-        if (!ix && nxm >= 0) ex = xget<double>(xch, xslot(nxm, 1, L.ly + 4 * L.lz), epoch, ctrl, timeout);
-        if (!iy && nym >= 0) ey = xget<double>(xch, xslot(nym, 3, L.lx + 4 * L.lz), epoch, ctrl, timeout);
-        if (!iz && nzm >= 0) ez = xget<double>(xch, xslot(nzm, 5, L.lx + 4 * L.ly), epoch, ctrl, timeout);
+        const long sl[3] = {(!ix && nxm >= 0) ? (long)xslot(nxm, 1, L.ly + 4 * L.lz) : -1L,
+                            (!iy && nym >= 0) ? (long)xslot(nym, 3, L.lx + 4 * L.lz) : -1L,
+                            (!iz && nzm >= 0) ? (long)xslot(nzm, 5, L.lx + 4 * L.ly) : -1L};
+        double got[3];
+        xgather<double, 3>(xch, sl, epoch, got, ctrl, timeout);
+        ex = got[0];
+        ey = got[1];
+        ez = got[2];
     }
     const int h_own = L.lx + L.ly + L.lz;
     double ie = 0.0;
@@ -783,16 +844,6 @@ __global__ __launch_bounds__(64) void k_fwd_level(Bricks bk, Coef<T> c, T* r, T*
                                timeout);
 }
 
-template <class T, bool UPDATE>
-__global__ __launch_bounds__(256) void k_fwd_wave(Bricks bk, Coef<T> c, T* r, T* x, const T* p, const T* q, T* y,
-                                                  double* part, Ctrl* ctrl, uint64_t* xch, uint64_t timeout) {
-    if (ctrl->done) return;
-    const int W = (int)gridDim.x * 4;
-    const uint32_t epoch = ctrl->epoch;
-    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < bk.nb; pos += W)
-        fwd_brick<T, UPDATE, false>(bk.lvl_list[pos], epoch, bk, c, r, x, p, q, y, part, ctrl, xch, timeout);
-}
-
 // Backward sweep z = y - E^{-1} L^T z, in place over y, hyperplanes 9..0;
 // fused r.z partials.
 template <class T, bool LEVELS>
@@ -845,14 +896,395 @@ __global__ __launch_bounds__(64) void k_bwd_level(Bricks bk, Coef<T> c, const T*
     bwd_brick<T, true>(bk.lvl_list[level_base + (int)blockIdx.x], 0u, bk, c, r, y, part, ctrl, xch, timeout);
 }
 
+// Walks the active bricks of one pencil in x order (REVERSE: descending),
+// 64 map entries per chunk held one per lane, together with the map rows of
+// the two neighbouring pencils the sweep waits on (-y/-z forward, +y/+z
+// backward), so neighbour brick ids cost no memory round trip per brick.
+template <bool REVERSE>
+struct PencilIter {
+    const int32_t* row;
+    const int32_t* row_y;  // nullptr when the neighbour pencil is outside the grid
+    const int32_t* row_z;
+    int nbx, base;
+    int mv, my, mz;
+    unsigned long long mask;
+    __device__ void load_chunk() {
+        const int bx = base + lane_id();
+        const bool in = bx >= 0 && bx < nbx;
+        mv = in ? row[bx] : -1;
+        my = (in && row_y) ? row_y[bx] : -1;
+        mz = (in && row_z) ? row_z[bx] : -1;
+        mask = __ballot(mv >= 0);
+    }
+    __device__ PencilIter(const int32_t* r, const int32_t* ry, const int32_t* rz, int n)
+        : row(r), row_y(ry), row_z(rz), nbx(n) {
+        base = REVERSE ? ((n - 1) / 64) * 64 : 0;
+        load_chunk();
+    }
+    // brick id, its x index and the ids of its y/z neighbours (-1 if none);
+    // once the pencil is exhausted it returns false and leaves the outputs
+    // unchanged
+    __device__ bool next(int& bx, int& b, int& ny, int& nz) {
+        while (mask == 0ull) {
+            if (REVERSE) {
+                if (base == 0) return false;
+                base -= 64;
+            } else {
+                base += 64;
+                if (base >= nbx) return false;
+            }
+            load_chunk();
+        }
+        const int l = REVERSE ? 63 - __builtin_clzll(mask) : __builtin_ctzll(mask);
+        mask &= ~(1ull << l);
+        bx = base + l;
+        b = __builtin_amdgcn_readlane(mv, l);
+        ny = __builtin_amdgcn_readlane(my, l);
+        nz = __builtin_amdgcn_readlane(mz, l);
+        return true;
+    }
+};
+
+__device__ __forceinline__ const int32_t* map_row(const Bricks& bk, int by, int bz) {
+    if (by < 0 || bz < 0 || by >= bk.nby || bz >= bk.nbz) return nullptr;
+    return bk.map + (size_t)bk.nbx * ((size_t)by + (size_t)bk.nby * bz);
+}
+
+// Tagged words of a whole brick (pencil sweeps): word (lane * W + h) of
+// brick b's slot, W = words per value. Consumers read the face cells they
+// need from it; every lane stores and loads unconditionally (branch-free
+// memory operations keep the compiler's wait counters exact).
 template <class T>
-__global__ __launch_bounds__(256) void k_bwd_wave(Bricks bk, Coef<T> c, const T* r, T* y, double* part, Ctrl* ctrl,
-                                                  uint64_t* xch, uint64_t timeout) {
+__device__ __forceinline__ size_t bslot(int b, int lane) {
+    return (size_t)b * kXchWords + (size_t)lane * (sizeof(T) / 4);
+}
+
+// Two tagged values per lane (the y and z neighbour cells), requested with
+// unconditional loads (unused requests point at a valid dummy word and
+// count as present). issue() starts a round; finish() completes it and
+// re-polls, all words at once, until every requested tag matches.
+template <class T>
+struct Polls2 {
+    static constexpr int W = sizeof(T) / 4;
+    size_t slot[2];
+    bool want[2];
+    uint64_t v[2][W];
+    __device__ __forceinline__ void load(const uint64_t* x) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int h = 0; h < W; ++h)
+                v[q][h] = __hip_atomic_load(x + slot[q] + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ bool ready(uint32_t e) const {
+        bool ok = true;
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int h = 0; h < W; ++h) ok = ok && (!want[q] || (uint32_t)(v[q][h] >> 32) == e);
+        return ok;
+    }
+    __device__ __forceinline__ void issue(const uint64_t* x) { load(x); }
+    __device__ __forceinline__ void finish(const uint64_t* x, uint32_t e, T (&out)[2], Ctrl* c, uint64_t timeout) {
+        if (!__all(ready(e))) {
+            const uint64_t t0 = wall_clock64();
+            for (int round = 1;; ++round) {
+                __builtin_amdgcn_s_sleep(1);
+                load(x);
+                if (__all(ready(e))) break;
+                if ((round & 63) == 0) {
+                    if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                    if (wall_clock64() - t0 > timeout) {
+                        __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            T val;
+            if constexpr (W == 1) {
+                val = __uint_as_float((uint32_t)v[q][0]);
+            } else {
+                val = __longlong_as_double((long long)(((uint64_t)(uint32_t)v[q][1] << 32) | (uint32_t)v[q][0]));
+            }
+            out[q] = want[q] ? val : (T)0;
+        }
+    }
+};
+
+template <class T>
+__device__ __forceinline__ void bput(uint64_t* x, int b, T v, uint32_t e, const Lane& L) {
+    xput(x, bslot<T>(b, L.lane), v, e);
+}
+
+// Wavefront sweeps over x-pencils. Wave w of W walks the pencils at positions
+// w, w+W, ... of the pencil list (sorted by by+bz) and inside a pencil its
+// bricks in x order (reverse order for the backward sweep). The x neighbour
+// of a brick is the wave's previous brick: its face values come from the
+// registers through a lane shuffle. y and z neighbours belong to other pencils
+// and arrive as epoch-tagged words. Brick k+1's loads and first tag polls are
+// issued (raw, nothing consumed) before brick k is computed, so a brick whose
+// neighbours are ready costs little more than its own arithmetic. Every brick
+// a wave waits for belongs to a pencil of lower (backward: higher) level,
+// i.e. an earlier position of the list, and all waves are resident
+// (cooperative launch), so every wait ends.
+template <class T, bool UPDATE>
+struct FwdIn {
+    int b, bx;
+    T rv, xv, pv, qv, ie, axo, ayo, azo, aym_e, azm_e;
+    Polls2<T> poll;
+};
+
+// Issues the loads of one brick (nothing is consumed here). Every load is
+// unconditional; requests a lane does not need read a valid dummy address.
+template <class T, bool UPDATE>
+__device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, int nym, int nzm, const Coef<T>& c,
+                                          const T* r, const T* x, const T* p, const T* q, const Lane& L) {
+    d.b = b;
+    d.bx = bx;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    d.rv = r[o];
+    if (UPDATE) {
+        d.xv = x[o];
+        d.pv = p[o];
+        d.qv = q[o];
+    }
+    d.ie = c.inv_e[o];
+    d.axo = c.ax[o];
+    d.ayo = c.ay[o];
+    d.azo = c.az[o];
+    const bool wy = L.ly == 0 && nym >= 0, wz = L.lz == 0 && nzm >= 0;
+    const int ly_lane = wy ? L.lane + 12 : L.lane, lz_lane = wz ? L.lane + 48 : L.lane;
+    const int yb = wy ? nym : b, zb = wz ? nzm : b;
+    const T ay_e = c.ay[(size_t)yb * kBrickCells + ly_lane];
+    const T az_e = c.az[(size_t)zb * kBrickCells + lz_lane];
+    d.aym_e = wy ? ay_e : (T)0;
+    d.azm_e = wz ? az_e : (T)0;
+    d.poll.want[0] = wy;
+    d.poll.want[1] = wz;
+    d.poll.slot[0] = bslot<T>(yb, ly_lane);
+    d.poll.slot[1] = bslot<T>(zb, lz_lane);
+}
+
+template <class T>
+struct FwdCarry {
+    T prev_y, prev_ax;
+    int prev_bx;
+};
+
+// Computes one brick of the forward sweep from its fetched inputs and the
+// x-neighbour values carried from the wave's previous brick.
+template <class T, bool UPDATE>
+__device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const T (&got)[2], FwdCarry<T>& cy, T alpha,
+                                            T* r, T* x, T* y, double* part, uint64_t* xch, uint32_t epoch,
+                                            const Lane& L) {
+    const int h_own = L.lx + L.ly + L.lz;
+    const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
+    const size_t o = (size_t)cur.b * kBrickCells + L.lane;
+    T rv = cur.rv;
+    if (UPDATE) {
+        rv = fma(-alpha, cur.qv, rv);
+        x[o] = fma(alpha, cur.pv, cur.xv);
+        r[o] = rv;
+    }
+    const bool adj = cy.prev_bx == cur.bx - 1;
+    const T sx = shfl(cur.axo, L.lane - 1), sxp = shfl(cy.prev_ax, L.lane + 3);
+    const T sy = shfl(cur.ayo, L.lane - 4), sz = shfl(cur.azo, L.lane - 16);
+    const T pyx = shfl(cy.prev_y, L.lane + 3);
+    const T axm = ix ? sx : (adj ? sxp : (T)0);
+    const T aym = iy ? sy : cur.aym_e;
+    const T azm = iz ? sz : cur.azm_e;
+    const T ex = adj ? pyx : (T)0;
+    T yv = (T)0;
+#pragma unroll
+    for (int h = 0; h < 10; ++h) {
+        const T vx = shfl(yv, L.lane - 1), vy = shfl(yv, L.lane - 4), vz = shfl(yv, L.lane - 16);
+        if (h == h_own) {
+            T t = fma(-axm, ix ? vx : ex, rv);
+            t = fma(-aym, iy ? vy : got[0], t);
+            t = fma(-azm, iz ? vz : got[1], t);
+            yv = t * cur.ie;
+        }
+    }
+    bput<T>(xch, cur.b, yv, epoch, L);
+    y[o] = yv;
+    if (UPDATE) {
+        const double s = wave_sum((double)rv * (double)rv);
+        if (L.lane == 0) part[cur.b] = s;
+    }
+    cy.prev_y = yv;
+    cy.prev_ax = cur.axo;
+    cy.prev_bx = cur.bx;
+}
+
+// Pencil loop, unrolled by two over buffers A and B so that the prefetched
+// registers are never copied (a copy would wait for the loads). Per brick:
+// finish the neighbour polls of the current brick, issue the first poll of
+// the next one (its round trip overlaps the current computation), compute.
+// Past the pencil's last brick the fetch repeats the current brick (valid
+// addresses, results unused), which keeps the loop free of memory branches.
+template <class T, bool UPDATE>
+__global__ __launch_bounds__(256) void k_fwd_pencil(Bricks bk, Pencils pc, Coef<T> c, T* r, T* x, const T* p,
+                                                    const T* q, T* y, double* part, Ctrl* ctrl, uint64_t* xch,
+                                                    uint64_t timeout) {
     if (ctrl->done) return;
+    const Lane L;
     const int W = (int)gridDim.x * 4;
     const uint32_t epoch = ctrl->epoch;
-    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < bk.nb; pos += W)
-        bwd_brick<T, false>(bk.lvl_list[bk.nb - 1 - pos], epoch, bk, c, r, y, part, ctrl, xch, timeout);
+    const T alpha = UPDATE ? (T)ctrl->alpha : (T)0;
+    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < pc.np; pos += W) {
+        const int cc = __builtin_amdgcn_readfirstlane(pc.coord[pc.lvl_list[pos]]);
+        const int by = cc & 1023, bz = (cc >> 10) & 1023;
+        PencilIter<false> it(map_row(bk, by, bz), map_row(bk, by - 1, bz), map_row(bk, by, bz - 1), bk.nbx);
+        int bx, b, ny, nz;
+        if (!it.next(bx, b, ny, nz)) continue;
+        FwdIn<T, UPDATE> A, B;
+        FwdCarry<T> cy{(T)0, (T)0, -2};
+        fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
+        A.poll.issue(xch);
+        T got[2];
+        for (;;) {
+            bool more = it.next(bx, b, ny, nz);
+            fwd_fetch<T, UPDATE>(B, b, bx, ny, nz, c, r, x, p, q, L);
+            A.poll.finish(xch, epoch, got, ctrl, timeout);
+            B.poll.issue(xch);
+            fwd_compute<T, UPDATE>(A, got, cy, alpha, r, x, y, part, xch, epoch, L);
+            if (!more) break;
+            more = it.next(bx, b, ny, nz);
+            fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
+            B.poll.finish(xch, epoch, got, ctrl, timeout);
+            A.poll.issue(xch);
+            fwd_compute<T, UPDATE>(B, got, cy, alpha, r, x, y, part, xch, epoch, L);
+            if (!more) break;
+        }
+    }
+}
+
+template <class T>
+struct BwdIn {
+    int b, bx;
+    T yv, ie, axo, ayo, azo, rv;
+    Polls2<T> poll;
+};
+
+template <class T>
+__device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, int nzp, const Coef<T>& c,
+                                          const T* r, const T* y, const Lane& L) {
+    d.b = b;
+    d.bx = bx;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    d.yv = y[o];
+    d.ie = c.inv_e[o];
+    d.axo = c.ax[o];
+    d.ayo = c.ay[o];
+    d.azo = c.az[o];
+    d.rv = r[o];
+    const bool wy = L.ly == 3 && nyp >= 0, wz = L.lz == 3 && nzp >= 0;
+    d.poll.want[0] = wy;
+    d.poll.want[1] = wz;
+    d.poll.slot[0] = bslot<T>(wy ? nyp : b, wy ? L.lane - 12 : L.lane);
+    d.poll.slot[1] = bslot<T>(wz ? nzp : b, wz ? L.lane - 48 : L.lane);
+}
+
+template <class T>
+struct BwdCarry {
+    T prev_z;
+    int prev_bx;
+};
+
+template <class T>
+__device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, const T (&got)[2], BwdCarry<T>& cy, T* y,
+                                            double* part, uint64_t* xch, uint32_t epoch, const Lane& L) {
+    const int h_own = L.lx + L.ly + L.lz;
+    const bool ix = L.lx < 3, iy = L.ly < 3, iz = L.lz < 3;
+    const bool adj = cy.prev_bx == cur.bx + 1;
+    const T pzx = shfl(cy.prev_z, L.lane - 3);
+    const T ex = adj ? pzx : (T)0;
+    T zv = (T)0;
+#pragma unroll
+    for (int h = 9; h >= 0; --h) {
+        const T vx = shfl(zv, L.lane + 1), vy = shfl(zv, L.lane + 4), vz = shfl(zv, L.lane + 16);
+        if (h == h_own) {
+            T t = cur.axo * (ix ? vx : ex);
+            t = fma(cur.ayo, iy ? vy : got[0], t);
+            t = fma(cur.azo, iz ? vz : got[1], t);
+            zv = fma(-cur.ie, t, cur.yv);
+        }
+    }
+    bput<T>(xch, cur.b, zv, epoch, L);
+    y[(size_t)cur.b * kBrickCells + L.lane] = zv;
+    const double s = wave_sum((double)cur.rv * (double)zv);
+    if (L.lane == 0) part[cur.b] = s;
+    cy.prev_z = zv;
+    cy.prev_bx = cur.bx;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<T> c, const T* r, T* y, double* part,
+                                                    Ctrl* ctrl, uint64_t* xch, uint64_t timeout) {
+    if (ctrl->done) return;
+    const Lane L;
+    const int W = (int)gridDim.x * 4;
+    const uint32_t epoch = ctrl->epoch;
+    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < pc.np; pos += W) {
+        const int cc = __builtin_amdgcn_readfirstlane(pc.coord[pc.lvl_list[pc.np - 1 - pos]]);
+        const int by = cc & 1023, bz = (cc >> 10) & 1023;
+        PencilIter<true> it(map_row(bk, by, bz), map_row(bk, by + 1, bz), map_row(bk, by, bz + 1), bk.nbx);
+        int bx, b, ny, nz;
+        if (!it.next(bx, b, ny, nz)) continue;
+        BwdIn<T> A, B;
+        BwdCarry<T> cy{(T)0, -2};
+        bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
+        A.poll.issue(xch);
+        T got[2];
+        for (;;) {
+            bool more = it.next(bx, b, ny, nz);
+            bwd_fetch<T>(B, b, bx, ny, nz, c, r, y, L);
+            A.poll.finish(xch, epoch, got, ctrl, timeout);
+            B.poll.issue(xch);
+            bwd_compute<T>(A, got, cy, y, part, xch, epoch, L);
+            if (!more) break;
+            more = it.next(bx, b, ny, nz);
+            bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
+            B.poll.finish(xch, epoch, got, ctrl, timeout);
+            A.poll.issue(xch);
+            bwd_compute<T>(B, got, cy, y, part, xch, epoch, L);
+            if (!more) break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- pencil setup
+__global__ __launch_bounds__(256) void k_pencil_flags(const int32_t* bmap, int nbx, int nby, int nbz, uint8_t* pflag) {
+    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pi >= (int64_t)nby * nbz) return;
+    const int32_t* row = bmap + (size_t)nbx * (size_t)pi;
+    uint8_t any = 0;
+    for (int bx = 0; bx < nbx; ++bx) any |= row[bx] >= 0 ? 1 : 0;
+    pflag[pi] = any;
+}
+
+__global__ __launch_bounds__(256) void k_pencil_meta(const int32_t* pmap, int nby, int nbz, int32_t* pcoord,
+                                                     int32_t* lvl_cnt) {
+    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (pi >= (int64_t)nby * nbz) return;
+    const int id = pmap[pi];
+    if (id < 0) return;
+    const int by = (int)(pi % nby), bz = (int)(pi / nby);
+    pcoord[id] = by | (bz << 10);
+    atomicAdd(&lvl_cnt[by + bz], 1);
+}
+
+__global__ __launch_bounds__(256) void k_pencil_lists(const int32_t* pcoord, int np, const int32_t* lvl_off,
+                                                      int32_t* lvl_fill, int32_t* lvl_list) {
+    const int id = blockIdx.x * 256 + threadIdx.x;
+    if (id >= np) return;
+    const int cc = pcoord[id];
+    const int l = (cc & 1023) + ((cc >> 10) & 1023);
+    const int pos = atomicAdd(&lvl_fill[l], 1);
+    lvl_list[lvl_off[l] + pos] = id;
 }
 
 // r64 = b - A x64 in FP64 (order as the oracle's spmv: diag, -x, +x, -y, +y,
@@ -975,7 +1407,8 @@ __global__ __launch_bounds__(256) void k_update(Dense d, float dt, const float* 
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
-constexpr int kWaveBlocksPerCU = 2;  // persistent sweeps: 2 workgroups (8 waves) per CU
+constexpr int kWaveBlocksPerCU = 2;    // persistent factor sweep: 2 workgroups (8 waves) per CU
+constexpr int kPencilBlocksPerCU = 4;  // pencil sweeps: up to 4 workgroups (16 waves) per CU
 
 int device_cus() {
     static int cus[64] = {0};
@@ -990,22 +1423,24 @@ int device_cus() {
 // workgroup): grid = min(bricks / 4, resident workgroups), so every wave is
 // resident for the whole sweep.
 template <typename... KArgs, typename... Args, size_t... I>
-hipError_t coop_impl(void (*kern)(KArgs...), int nb, hipStream_t s, std::index_sequence<I...>, Args&&... args) {
+hipError_t coop_impl(void (*kern)(KArgs...), int nb, int blocks_per_cu, hipStream_t s, std::index_sequence<I...>,
+                     Args&&... args) {
     std::tuple<KArgs...> t(std::forward<Args>(args)...);
     void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 256, 0) != hipSuccess || occ < 1) occ = 1;
-    const int per_cu = occ < kWaveBlocksPerCU ? occ : kWaveBlocksPerCU;
+    const int per_cu = occ < blocks_per_cu ? occ : blocks_per_cu;
     int g = (nb + 3) / 4;
     if (g > per_cu * device_cus()) g = per_cu * device_cus();
     if (g < 1) g = 1;
     return hipLaunchCooperativeKernel((const void*)kern, dim3(g), dim3(256), ptrs, 0, s);
 }
 
+// nb: work items (one per wave, four waves per 256-thread workgroup).
 template <typename... KArgs, typename... Args>
-hipError_t coop(void (*kern)(KArgs...), int nb, hipStream_t s, Args&&... args) {
+hipError_t coop(void (*kern)(KArgs...), int nb, int blocks_per_cu, hipStream_t s, Args&&... args) {
     static_assert(sizeof...(KArgs) == sizeof...(Args), "argument count");
-    return coop_impl(kern, nb, s, std::index_sequence_for<KArgs...>{}, std::forward<Args>(args)...);
+    return coop_impl(kern, nb, blocks_per_cu, s, std::index_sequence_for<KArgs...>{}, std::forward<Args>(args)...);
 }
 
 }  // namespace
@@ -1072,7 +1507,7 @@ hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, co
                          int level, int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
         if (bk.nb == 0) return hipSuccess;
-        return coop(k_factor_wave<T>, bk.nb, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
+        return coop(k_factor_wave<T>, bk.nb, kWaveBlocksPerCU, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
                     n_fixed, timeout_ticks);
     }
     if (level_count > 0)
@@ -1120,13 +1555,16 @@ hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* 
 }
 
 template <class T>
-hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y, double* part,
-                      Ctrl* ctrl, uint64_t* xch, int update, int level, int level_base, int level_count,
+hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
+                      double* part, Ctrl* ctrl, uint64_t* xch, int update, int level, int level_base, int level_count,
                       uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
-        if (bk.nb == 0) return hipSuccess;
-        if (update) return coop(k_fwd_wave<T, true>, bk.nb, s, bk, c, r, x, p, q, y, part, ctrl, xch, timeout_ticks);
-        return coop(k_fwd_wave<T, false>, bk.nb, s, bk, c, r, x, p, q, y, part, ctrl, xch, timeout_ticks);
+        if (pc.np == 0) return hipSuccess;
+        if (update)
+            return coop(k_fwd_pencil<T, true>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, part, ctrl,
+                        xch, timeout_ticks);
+        return coop(k_fwd_pencil<T, false>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, part, ctrl, xch,
+                    timeout_ticks);
     }
     if (level_count > 0) {
         if (update)
@@ -1140,15 +1578,36 @@ hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p
 }
 
 template <class T>
-hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl, uint64_t* xch,
-                      int level, int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
+hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part,
+                      Ctrl* ctrl, uint64_t* xch, int level, int level_base, int level_count, uint64_t timeout_ticks,
+                      hipStream_t s) {
     if (level < 0) {
-        if (bk.nb == 0) return hipSuccess;
-        return coop(k_bwd_wave<T>, bk.nb, s, bk, c, r, y, part, ctrl, xch, timeout_ticks);
+        if (pc.np == 0) return hipSuccess;
+        return coop(k_bwd_pencil<T>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, y, part, ctrl, xch, timeout_ticks);
     }
     if (level_count > 0)
         hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
                            level_base, timeout_ticks);
+    return hipGetLastError();
+}
+
+hipError_t launch_pencil_flags(const int32_t* bmap, int nbx, int nby, int nbz, uint8_t* pflag, hipStream_t s) {
+    hipLaunchKernelGGL(k_pencil_flags, dim3(cdiv((int64_t)nby * nbz, 256)), dim3(256), 0, s, bmap, nbx, nby, nbz,
+                       pflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_pencil_meta(const int32_t* pmap, int nby, int nbz, int32_t* pcoord, int32_t* lvl_cnt,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_pencil_meta, dim3(cdiv((int64_t)nby * nbz, 256)), dim3(256), 0, s, pmap, nby, nbz, pcoord,
+                       lvl_cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_pencil_lists(const int32_t* pcoord, int np, const int32_t* lvl_off, int32_t* lvl_fill,
+                               int32_t* lvl_list, hipStream_t s) {
+    if (np == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pencil_lists, dim3(cdiv(np, 256)), dim3(256), 0, s, pcoord, np, lvl_off, lvl_fill, lvl_list);
     return hipGetLastError();
 }
 
@@ -1194,9 +1653,11 @@ hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStr
     template hipError_t launch_init<T>(const Bricks&, const double*, T*, T*, T*, T*, double*, Ctrl*, hipStream_t);  \
     template hipError_t launch_spmv<T>(const Bricks&, const Coef<T>&, const T*, const T*, T*, T*, double*, Ctrl*,   \
                                        hipStream_t);                                                                \
-    template hipError_t launch_fwd<T>(const Bricks&, const Coef<T>&, T*, T*, const T*, const T*, T*, double*, Ctrl*, \
+    template hipError_t launch_fwd<T>(const Bricks&, const Pencils&, const Coef<T>&, T*, T*, const T*, const T*, T*, \
+                                      double*, Ctrl*,                                                               \
                                       uint64_t*, int, int, int, int, uint64_t, hipStream_t);                        \
-    template hipError_t launch_bwd<T>(const Bricks&, const Coef<T>&, const T*, T*, double*, Ctrl*, uint64_t*, int,  \
+    template hipError_t launch_bwd<T>(const Bricks&, const Pencils&, const Coef<T>&, const T*, T*, double*, Ctrl*,   \
+                                      uint64_t*, int,                                                               \
                                       int, int, uint64_t, hipStream_t);                                             \
     template hipError_t launch_resid64<T>(const Bricks&, const double*, const T*, const T*, const T*, const double*, \
                                           const double*, double*, double*, float*, float*, float*, float*, Ctrl*,   \

@@ -35,6 +35,16 @@ struct Bricks {
     const int32_t* lvl_list; // nb, sorted by bx+by+bz
 };
 
+// x-pencils: rows of bricks at fixed (by, bz) that hold at least one active
+// brick. In the wavefront schedule one wave walks a pencil in x, so the x
+// dependency of the triangular sweeps stays in registers.
+struct Pencils {
+    int np;                   // pencils with at least one active brick
+    int nlev;                 // pencil levels (nby + nbz - 1)
+    const int32_t* coord;     // np packed (by | bz << 10)
+    const int32_t* lvl_list;  // np, sorted by by + bz
+};
+
 // Coefficients of the stored system (device pointers), T = storage type.
 template <class T>
 struct Coef {
@@ -60,6 +70,13 @@ hipError_t launch_level_scan(const int32_t* lvl_cnt, int nlev, int32_t* lvl_off,
 hipError_t launch_brick_lists(const int32_t* map, const int32_t* coord, int nbx, int nby, int nbz, int nb,
                               const int32_t* lvl_off, int32_t* lvl_fill, int32_t* lvl_list,
                               int32_t* nbr, hipStream_t s);
+// Pencil metadata from the brick map: pflag over (by, bz), then (after a scan
+// into pmap) coordinates, level histogram and the level-ordered list.
+hipError_t launch_pencil_flags(const int32_t* bmap, int nbx, int nby, int nbz, uint8_t* pflag, hipStream_t s);
+hipError_t launch_pencil_meta(const int32_t* pmap, int nby, int nbz, int32_t* pcoord, int32_t* lvl_cnt,
+                              hipStream_t s);
+hipError_t launch_pencil_lists(const int32_t* pcoord, int np, const int32_t* lvl_off, int32_t* lvl_fill,
+                               int32_t* lvl_list, hipStream_t s);
 template <class T>
 hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const Bricks& bk, double* diag64,
                            double* b64, T* diag, T* ax, T* ay, T* az, hipStream_t s);
@@ -87,14 +104,15 @@ template <class T>
 hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q,
                        double* part, Ctrl* ctrl, hipStream_t s);
 // Forward sweep y = (E+L)^{-1} r, fused with x += alpha p, r -= alpha q and
-// |r|^2 partials when update != 0.
+// |r|^2 partials when update != 0. level < 0: one persistent launch over the
+// pencils (wavefront); else the bricks of one brick level.
 template <class T>
-hipError_t launch_fwd(const Bricks& bk, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
+hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
                       double* part, Ctrl* ctrl, uint64_t* xch, int update, int level, int level_base,
                       int level_count, uint64_t timeout_ticks, hipStream_t s);
 // Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
 template <class T>
-hipError_t launch_bwd(const Bricks& bk, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
+hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
                       uint64_t* xch, int level, int level_base, int level_count, uint64_t timeout_ticks,
                       hipStream_t s);
 // r64 = b64 - A x64 in FP64 (couplings TC read exactly); partials |r64|^2 and,

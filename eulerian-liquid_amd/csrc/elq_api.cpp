@@ -266,7 +266,7 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     const int nb = W.nb;
     const size_t cells = (size_t)nb * kBrickCells;
     hipStream_t s = ctx->stream;
-    HIPCHK(W.nbr.ensure((size_t)nb * 6 * 4), "alloc nbr");
+    HIPCHK(W.nbr.ensure((size_t)nb * 8 * 4), "alloc nbr");
     HIPCHK(W.lvl_list.ensure((size_t)nb * 4), "alloc lvl_list");
     // tagged exchange words: zeroed on (re)allocation; tags (epochs) never 0
     if ((size_t)nb * kXchWords * 8 > W.xch.cap) {

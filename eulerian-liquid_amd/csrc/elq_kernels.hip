@@ -182,7 +182,7 @@ struct Lane {
 
 // Neighbour of a brick cell across a brick face: (brick id, lane) or -1.
 // Directions: 0 -x, 1 +x, 2 -y, 3 +y, 4 -z, 5 +z.
-__device__ __forceinline__ int nbr_id(const Bricks& bk, int d, int b) { return bk.nbr[(size_t)d * bk.nb + b]; }
+__device__ __forceinline__ int nbr_id(const Bricks& bk, int d, int b) { return bk.nbr[(size_t)b * 8 + d]; }
 
 // ================================================================== setup
 // projection.cpp:8-12 (validity) and :17 (phi < 0); flags unknowns on the
@@ -328,12 +328,15 @@ __global__ __launch_bounds__(256) void k_brick_lists(const int32_t* map, const i
         if (x < 0 || y < 0 || z < 0 || x >= nbx || y >= nby || z >= nbz) return -1;
         return map[(size_t)x + (size_t)nbx * ((size_t)y + (size_t)nby * z)];
     };
-    nbr[0 * (size_t)nb + id] = look(bx - 1, by, bz);
-    nbr[1 * (size_t)nb + id] = look(bx + 1, by, bz);
-    nbr[2 * (size_t)nb + id] = look(bx, by - 1, bz);
-    nbr[3 * (size_t)nb + id] = look(bx, by + 1, bz);
-    nbr[4 * (size_t)nb + id] = look(bx, by, bz - 1);
-    nbr[5 * (size_t)nb + id] = look(bx, by, bz + 1);
+    int32_t* nb8 = nbr + (size_t)id * 8;
+    nb8[0] = look(bx - 1, by, bz);
+    nb8[1] = look(bx + 1, by, bz);
+    nb8[2] = look(bx, by - 1, bz);
+    nb8[3] = look(bx, by + 1, bz);
+    nb8[4] = look(bx, by, bz - 1);
+    nb8[5] = look(bx, by, bz + 1);
+    nb8[6] = -1;
+    nb8[7] = -1;
     const int l = bx + by + bz;
     const int pos = atomicAdd(&lvl_fill[l], 1);
     lvl_list[lvl_off[l] + pos] = id;
@@ -729,45 +732,77 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
     scalar_update(c, mode, S, S2);
 }
 
+// The six neighbour brick ids of brick b (wave-uniform): lanes 0..5 load
+// them in one access, readlane broadcasts them.
+struct Nbr6 {
+    int n[6];
+};
+__device__ __forceinline__ Nbr6 load_nbr6(const Bricks& bk, int b) {
+    const int lane = lane_id();
+    const int v = lane < 6 ? bk.nbr[(size_t)b * 8 + lane] : -1;
+    Nbr6 r;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) r.n[d] = __builtin_amdgcn_readlane(v, d);
+    return r;
+}
+
+// Offset of a lane's neighbour cell across face d of brick b, or the lane's
+// own offset when there is none (so loads stay unconditional); *has tells.
+__device__ __forceinline__ size_t halo_offset(const Nbr6& nb, int d, size_t own, const int (&lc)[3], int lane,
+                                              bool& has) {
+    const int axis = d >> 1, up = d & 1;
+    const bool face = up ? lc[axis] == 3 : lc[axis] == 0;
+    const int shift = axis == 0 ? 3 : axis == 1 ? 12 : 48;  // lane offset to the opposite face
+    has = face && nb.n[d] >= 0;
+    return has ? (size_t)nb.n[d] * kBrickCells + (up ? lane - shift : lane + shift) : own;
+}
+
 // p = z + beta p_old (ping-pong buffers), q = A p, p.q partials. Cross-brick
 // neighbour values of p are rebuilt from z and p_old; in-brick ones come
-// through lane shuffles. Accumulation order: diag, -x, +x, -y, +y, -z, +z.
+// through lane shuffles. All loads are issued unconditionally and at once.
+// Accumulation order: diag, -x, +x, -y, +y, -z, +z.
 template <class T>
 __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
                                               double* part, const Ctrl* ctrl) {
     if (ctrl->done) return;
-    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6));
     if (b >= bk.nb) return;
     const Lane L;
+    const Nbr6 nb = load_nbr6(bk, b);
     const T beta = (T)ctrl->beta;
     const size_t o = (size_t)b * kBrickCells + L.lane;
-    const T pc = fma(beta, p_in[o], z[o]);
+    const int lc[3] = {L.lx, L.ly, L.lz};
+    bool has[6];
+    size_t e[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) e[d] = halo_offset(nb, d, o, lc, L.lane, has[d]);
+    const T zc = z[o], pi = p_in[o], dg = c.diag[o], axo = c.ax[o], ayo = c.ay[o], azo = c.az[o];
+    T ze[6], pe[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) {
+        ze[d] = z[e[d]];
+        pe[d] = p_in[e[d]];
+    }
+    const T axe = c.ax[e[0]], aye = c.ay[e[2]], aze = c.az[e[4]];
+    const T pc = fma(beta, pi, zc);
     p_out[o] = pc;
-    const T axo = c.ax[o], ayo = c.ay[o], azo = c.az[o];
-    auto pext = [&](int d, int off) -> T {
-        const int nb_ = nbr_id(bk, d, b);
-        if (nb_ < 0) return (T)0;
-        const size_t e = (size_t)nb_ * kBrickCells + L.lane + off;
-        return fma(beta, p_in[e], z[e]);
-    };
-    auto aext = [&](const T* a, int d, int off) -> T {
-        const int nb_ = nbr_id(bk, d, b);
-        return nb_ >= 0 ? a[(size_t)nb_ * kBrickCells + L.lane + off] : (T)0;
-    };
+    T pn[6];
+#pragma unroll
+    for (int d = 0; d < 6; ++d) pn[d] = has[d] ? fma(beta, pe[d], ze[d]) : (T)0;
     const T sxm = shfl(pc, L.lane - 1), sxp = shfl(pc, L.lane + 1);
     const T sym = shfl(pc, L.lane - 4), syp = shfl(pc, L.lane + 4);
     const T szm = shfl(pc, L.lane - 16), szp = shfl(pc, L.lane + 16);
     const T cxm = shfl(axo, L.lane - 1), cym = shfl(ayo, L.lane - 4), czm = shfl(azo, L.lane - 16);
-    const T p_xm = L.lx > 0 ? sxm : pext(0, 3);
-    const T p_xp = L.lx < 3 ? sxp : pext(1, -3);
-    const T p_ym = L.ly > 0 ? sym : pext(2, 12);
-    const T p_yp = L.ly < 3 ? syp : pext(3, -12);
-    const T p_zm = L.lz > 0 ? szm : pext(4, 48);
-    const T p_zp = L.lz < 3 ? szp : pext(5, -48);
-    const T axm = L.lx > 0 ? cxm : aext(c.ax, 0, 3);
-    const T aym = L.ly > 0 ? cym : aext(c.ay, 2, 12);
-    const T azm = L.lz > 0 ? czm : aext(c.az, 4, 48);
-    T acc = c.diag[o] * pc;
+    const T p_xm = L.lx > 0 ? sxm : pn[0];
+    const T p_xp = L.lx < 3 ? sxp : pn[1];
+    const T p_ym = L.ly > 0 ? sym : pn[2];
+    const T p_yp = L.ly < 3 ? syp : pn[3];
+    const T p_zm = L.lz > 0 ? szm : pn[4];
+    const T p_zp = L.lz < 3 ? szp : pn[5];
+    const T axm = L.lx > 0 ? cxm : (has[0] ? axe : (T)0);
+    const T aym = L.ly > 0 ? cym : (has[2] ? aye : (T)0);
+    const T azm = L.lz > 0 ? czm : (has[4] ? aze : (T)0);
+    T acc = dg * pc;
     acc = fma(axm, p_xm, acc);
     acc = fma(axo, p_xp, acc);
     acc = fma(aym, p_ym, acc);
@@ -1072,15 +1107,22 @@ template <class T>
 struct FwdCarry {
     T prev_y, prev_ax;
     int prev_bx;
+    T rr_val;     // r of the last computed brick (its |r|^2 partial is pending)
+    int rr_b;     // that brick, -1 if none
 };
 
-// Computes one brick of the forward sweep from its fetched inputs and the
-// x-neighbour values carried from the wave's previous brick.
+// Part of a forward brick that needs no neighbour face: in-brick lower
+// couplings and the x-neighbour values carried from the previous brick.
 template <class T, bool UPDATE>
-__device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const T (&got)[2], FwdCarry<T>& cy, T alpha,
-                                            T* r, T* x, T* y, double* part, uint64_t* xch, uint32_t epoch,
-                                            const Lane& L) {
-    const int h_own = L.lx + L.ly + L.lz;
+struct FwdPre {
+    T axm, aym, azm, ex, rv;
+    bool adj;
+};
+
+template <class T, bool UPDATE>
+__device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur, const FwdCarry<T>& cy, T alpha,
+                                                     T* r, T* x, const Lane& L) {
+    FwdPre<T, UPDATE> pr;
     const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
     const size_t o = (size_t)cur.b * kBrickCells + L.lane;
     T rv = cur.rv;
@@ -1089,40 +1131,63 @@ __device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const T
         x[o] = fma(alpha, cur.pv, cur.xv);
         r[o] = rv;
     }
-    const bool adj = cy.prev_bx == cur.bx - 1;
+    pr.rv = rv;
+    pr.adj = cy.prev_bx == cur.bx - 1;
     const T sx = shfl(cur.axo, L.lane - 1), sxp = shfl(cy.prev_ax, L.lane + 3);
     const T sy = shfl(cur.ayo, L.lane - 4), sz = shfl(cur.azo, L.lane - 16);
     const T pyx = shfl(cy.prev_y, L.lane + 3);
-    const T axm = ix ? sx : (adj ? sxp : (T)0);
-    const T aym = iy ? sy : cur.aym_e;
-    const T azm = iz ? sz : cur.azm_e;
-    const T ex = adj ? pyx : (T)0;
+    pr.axm = ix ? sx : (pr.adj ? sxp : (T)0);
+    pr.aym = iy ? sy : cur.aym_e;
+    pr.azm = iz ? sz : cur.azm_e;
+    pr.ex = pr.adj ? pyx : (T)0;
+    return pr;
+}
+
+// Forward brick: the ten hyperplanes, then publication. The |r|^2 partial of
+// the brick is left pending in the carry (see flush_rr).
+template <class T, bool UPDATE>
+__device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdPre<T, UPDATE>& pr,
+                                            const T (&got)[2], FwdCarry<T>& cy, T* y, uint64_t* xch, uint32_t epoch,
+                                            const Lane& L) {
+    const int h_own = L.lx + L.ly + L.lz;
+    const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
     T yv = (T)0;
 #pragma unroll
     for (int h = 0; h < 10; ++h) {
         const T vx = shfl(yv, L.lane - 1), vy = shfl(yv, L.lane - 4), vz = shfl(yv, L.lane - 16);
         if (h == h_own) {
-            T t = fma(-axm, ix ? vx : ex, rv);
-            t = fma(-aym, iy ? vy : got[0], t);
-            t = fma(-azm, iz ? vz : got[1], t);
+            T t = fma(-pr.axm, ix ? vx : pr.ex, pr.rv);
+            t = fma(-pr.aym, iy ? vy : got[0], t);
+            t = fma(-pr.azm, iz ? vz : got[1], t);
             yv = t * cur.ie;
         }
     }
     bput<T>(xch, cur.b, yv, epoch, L);
-    y[o] = yv;
-    if (UPDATE) {
-        const double s = wave_sum((double)rv * (double)rv);
-        if (L.lane == 0) part[cur.b] = s;
-    }
+    y[(size_t)cur.b * kBrickCells + L.lane] = yv;
     cy.prev_y = yv;
     cy.prev_ax = cur.axo;
     cy.prev_bx = cur.bx;
+    if (UPDATE) {
+        cy.rr_val = pr.rv;
+        cy.rr_b = cur.b;
+    }
+}
+
+template <class T, bool UPDATE>
+__device__ __forceinline__ void flush_rr(FwdCarry<T>& cy, double* part, const Lane& L) {
+    if (UPDATE && cy.rr_b >= 0) {
+        const double s = wave_sum((double)cy.rr_val * (double)cy.rr_val);
+        if (L.lane == 0) part[cy.rr_b] = s;
+        cy.rr_b = -1;
+    }
 }
 
 // Pencil loop, unrolled by two over buffers A and B so that the prefetched
 // registers are never copied (a copy would wait for the loads). Per brick:
-// finish the neighbour polls of the current brick, issue the first poll of
-// the next one (its round trip overlaps the current computation), compute.
+// everything that needs no neighbour face (coupling shuffles, x carry, the
+// previous brick's |r|^2 partial) is done first, then the neighbour polls of
+// the brick are finished, the next brick's first poll is issued (its round
+// trip overlaps this computation) and the hyperplanes run.
 // Past the pencil's last brick the fetch repeats the current brick (valid
 // addresses, results unused), which keeps the loop free of memory branches.
 template <class T, bool UPDATE>
@@ -1141,24 +1206,33 @@ __global__ __launch_bounds__(256) void k_fwd_pencil(Bricks bk, Pencils pc, Coef<
         int bx, b, ny, nz;
         if (!it.next(bx, b, ny, nz)) continue;
         FwdIn<T, UPDATE> A, B;
-        FwdCarry<T> cy{(T)0, (T)0, -2};
+        FwdCarry<T> cy{(T)0, (T)0, -2, (T)0, -1};
         fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
         A.poll.issue(xch);
         T got[2];
         for (;;) {
             bool more = it.next(bx, b, ny, nz);
             fwd_fetch<T, UPDATE>(B, b, bx, ny, nz, c, r, x, p, q, L);
-            A.poll.finish(xch, epoch, got, ctrl, timeout);
-            B.poll.issue(xch);
-            fwd_compute<T, UPDATE>(A, got, cy, alpha, r, x, y, part, xch, epoch, L);
+            {
+                const FwdPre<T, UPDATE> pr = fwd_pre<T, UPDATE>(A, cy, alpha, r, x, L);
+                flush_rr<T, UPDATE>(cy, part, L);
+                A.poll.finish(xch, epoch, got, ctrl, timeout);
+                B.poll.issue(xch);
+                fwd_compute<T, UPDATE>(A, pr, got, cy, y, xch, epoch, L);
+            }
             if (!more) break;
             more = it.next(bx, b, ny, nz);
             fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
-            B.poll.finish(xch, epoch, got, ctrl, timeout);
-            A.poll.issue(xch);
-            fwd_compute<T, UPDATE>(B, got, cy, alpha, r, x, y, part, xch, epoch, L);
+            {
+                const FwdPre<T, UPDATE> pr = fwd_pre<T, UPDATE>(B, cy, alpha, r, x, L);
+                flush_rr<T, UPDATE>(cy, part, L);
+                B.poll.finish(xch, epoch, got, ctrl, timeout);
+                A.poll.issue(xch);
+                fwd_compute<T, UPDATE>(B, pr, got, cy, y, xch, epoch, L);
+            }
             if (!more) break;
         }
+        flush_rr<T, UPDATE>(cy, part, L);
     }
 }
 
@@ -1192,16 +1266,15 @@ template <class T>
 struct BwdCarry {
     T prev_z;
     int prev_bx;
+    T rz_r, rz_z;  // r and z of the last computed brick (r.z partial pending)
+    int rz_b;
 };
 
 template <class T>
-__device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, const T (&got)[2], BwdCarry<T>& cy, T* y,
-                                            double* part, uint64_t* xch, uint32_t epoch, const Lane& L) {
+__device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, T ex, const T (&got)[2], BwdCarry<T>& cy, T* y,
+                                            uint64_t* xch, uint32_t epoch, const Lane& L) {
     const int h_own = L.lx + L.ly + L.lz;
     const bool ix = L.lx < 3, iy = L.ly < 3, iz = L.lz < 3;
-    const bool adj = cy.prev_bx == cur.bx + 1;
-    const T pzx = shfl(cy.prev_z, L.lane - 3);
-    const T ex = adj ? pzx : (T)0;
     T zv = (T)0;
 #pragma unroll
     for (int h = 9; h >= 0; --h) {
@@ -1215,10 +1288,20 @@ __device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, const T (&got)[
     }
     bput<T>(xch, cur.b, zv, epoch, L);
     y[(size_t)cur.b * kBrickCells + L.lane] = zv;
-    const double s = wave_sum((double)cur.rv * (double)zv);
-    if (L.lane == 0) part[cur.b] = s;
     cy.prev_z = zv;
     cy.prev_bx = cur.bx;
+    cy.rz_r = cur.rv;
+    cy.rz_z = zv;
+    cy.rz_b = cur.b;
+}
+
+template <class T>
+__device__ __forceinline__ void flush_rz(BwdCarry<T>& cy, double* part, const Lane& L) {
+    if (cy.rz_b >= 0) {
+        const double s = wave_sum((double)cy.rz_r * (double)cy.rz_z);
+        if (L.lane == 0) part[cy.rz_b] = s;
+        cy.rz_b = -1;
+    }
 }
 
 template <class T>
@@ -1235,24 +1318,35 @@ __global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<
         int bx, b, ny, nz;
         if (!it.next(bx, b, ny, nz)) continue;
         BwdIn<T> A, B;
-        BwdCarry<T> cy{(T)0, -2};
+        BwdCarry<T> cy{(T)0, -2, (T)0, (T)0, -1};
         bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
         A.poll.issue(xch);
         T got[2];
         for (;;) {
             bool more = it.next(bx, b, ny, nz);
             bwd_fetch<T>(B, b, bx, ny, nz, c, r, y, L);
-            A.poll.finish(xch, epoch, got, ctrl, timeout);
-            B.poll.issue(xch);
-            bwd_compute<T>(A, got, cy, y, part, xch, epoch, L);
+            {
+                const T pzx = shfl(cy.prev_z, L.lane - 3);
+                const T ex = cy.prev_bx == A.bx + 1 ? pzx : (T)0;
+                flush_rz<T>(cy, part, L);
+                A.poll.finish(xch, epoch, got, ctrl, timeout);
+                B.poll.issue(xch);
+                bwd_compute<T>(A, ex, got, cy, y, xch, epoch, L);
+            }
             if (!more) break;
             more = it.next(bx, b, ny, nz);
             bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
-            B.poll.finish(xch, epoch, got, ctrl, timeout);
-            A.poll.issue(xch);
-            bwd_compute<T>(B, got, cy, y, part, xch, epoch, L);
+            {
+                const T pzx = shfl(cy.prev_z, L.lane - 3);
+                const T ex = cy.prev_bx == B.bx + 1 ? pzx : (T)0;
+                flush_rz<T>(cy, part, L);
+                B.poll.finish(xch, epoch, got, ctrl, timeout);
+                A.poll.issue(xch);
+                bwd_compute<T>(B, ex, got, cy, y, xch, epoch, L);
+            }
             if (!more) break;
         }
+        flush_rz<T>(cy, part, L);
     }
 }
 

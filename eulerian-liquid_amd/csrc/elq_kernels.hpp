@@ -31,7 +31,7 @@ struct Bricks {
     int nlev;           // brick levels (nbx + nby + nbz - 2)
     const int32_t* map;      // nbx*nby*nbz -> active id or -1
     const int32_t* coord;    // nb packed (bx | by << 10 | bz << 20)
-    const int32_t* nbr;      // 6*nb: [-x,+x,-y,+y,-z,+z][b]
+    const int32_t* nbr;      // 8*nb: [b][-x,+x,-y,+y,-z,+z,pad,pad]
     const int32_t* lvl_list; // nb, sorted by bx+by+bz
 };
 

@@ -124,6 +124,26 @@ def cpu_baseline(scene, fp64_iters):
             "est_s_per_solve": round(est, 2)}
 
 
+def copy_calibration(nbytes, reps):
+    """Copies nbytes device-to-device with the library's streaming kernel
+    (reads nbytes, writes nbytes) so that per-dispatch FETCH_SIZE/WRITE_SIZE
+    counters can be scaled against a kernel of known traffic."""
+    ctx = elq.Context(0)
+    src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+    ctx.h2d(src, np.ones(nbytes // 4, np.float32))
+    ts = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        ctx.d2d(dst, src, nbytes)
+        ts.append(time.perf_counter() - t0)
+    ctx.free(src)
+    ctx.free(dst)
+    ctx.close()
+    t = min(ts[1:])
+    print(json.dumps({"copy_bytes": nbytes, "dispatches": reps + 1, "best_s": round(t, 6),
+                      "copy_gbs_incl_sync": round(2 * nbytes / t / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,10 +156,15 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 drop-in (profiling runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the library's plain copy kernel on MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
         raise SystemExit(2)
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration << 20, max(a.steps, 1))
+        return
 
     t_gen = time.perf_counter()
     scene, wname = workload(a.workload, a.seed)

@@ -921,6 +921,15 @@ int elq_memcpy_d2h(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return ELQ_OK;
 }
 
+int elq_memcpy_d2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || (bytes & 3) || (((uintptr_t)dst | (uintptr_t)src) & 3)) return ELQ_ERR_INVALID_ARG;
+    if (bytes == 0) return ELQ_OK;
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(launch_copy(dst, src, bytes, ctx->stream), "k_copy");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
 void elq_set_profiling(elq_ctx* ctx, int enable) {
     if (ctx) ctx->profiling = enable != 0;
 }

@@ -14,6 +14,7 @@
 // operation sequence is the reference's; the Krylov kernels use explicit fma().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <tuple>
@@ -30,6 +31,46 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 template <class T>
 __device__ __forceinline__ T shfl(T v, int src) {
     return __shfl(v, src, 64);
+}
+
+// Neighbour moves inside a 4x4x4 brick (lane = lx + 4 ly + 16 lz) on the
+// VALU: DPP row shifts for x and y (rows of 16 lanes), permlane swaps for z.
+// Each returns the value held by the named neighbour lane; a lane without
+// that neighbour inside the brick receives an unspecified value.
+//   xm: lane-1 (lx > 0)   ym: lane-4 (ly > 0)   zm: lane-16 (lz > 0)
+//   xp: lane+1 (lx < 3)   yp: lane+4 (ly < 3)   zp: lane+16 (lz < 3)
+//   q3: lane 4q+3 of the lane's quad   q0: lane 4q of the lane's quad
+enum class Mv { xm, xp, ym, yp, zm, zp, q3, q0 };
+
+template <Mv M>
+__device__ __forceinline__ uint32_t move_u32(uint32_t v) {
+    if constexpr (M == Mv::xm) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x111, 0xF, 0xF, true);  // row_shr:1
+    if constexpr (M == Mv::xp) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x101, 0xF, 0xF, true);  // row_shl:1
+    if constexpr (M == Mv::ym) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x114, 0xF, 0xF, true);  // row_shr:4
+    if constexpr (M == Mv::yp) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x104, 0xF, 0xF, true);  // row_shl:4
+    if constexpr (M == Mv::q3) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xFF, 0xF, 0xF, true);   // quad_perm 3333
+    if constexpr (M == Mv::q0) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x00, 0xF, 0xF, true);   // quad_perm 0000
+    if constexpr (M == Mv::zm || M == Mv::zp) {
+        // rows r0..r3 of v. a = ([r0 r0 r2 r2], [r1 r1 r3 r3]);
+        // b = ([r0 r0 r1 r1], [r2 r2 r3 r3]).
+        const auto a = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        const auto b = __builtin_amdgcn_permlane32_swap(a[0], a[1], false, false);
+        const int row = (int)(threadIdx.x & 63) >> 4;
+        if constexpr (M == Mv::zm) return row == 2 ? b[0] : a[0];  // [-  r0 r1 r2]
+        else return row == 1 ? b[1] : a[1];                         // [r1 r2 r3 -]
+    }
+    return v;
+}
+
+template <Mv M, class T>
+__device__ __forceinline__ T move(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __uint_as_float(move_u32<M>(__float_as_uint(v)));
+    } else {
+        const uint64_t u = (uint64_t)__double_as_longlong(v);
+        const uint32_t lo = move_u32<M>((uint32_t)u), hi = move_u32<M>((uint32_t)(u >> 32));
+        return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    }
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -789,10 +830,10 @@ __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, 
     T pn[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) pn[d] = has[d] ? fma(beta, pe[d], ze[d]) : (T)0;
-    const T sxm = shfl(pc, L.lane - 1), sxp = shfl(pc, L.lane + 1);
-    const T sym = shfl(pc, L.lane - 4), syp = shfl(pc, L.lane + 4);
-    const T szm = shfl(pc, L.lane - 16), szp = shfl(pc, L.lane + 16);
-    const T cxm = shfl(axo, L.lane - 1), cym = shfl(ayo, L.lane - 4), czm = shfl(azo, L.lane - 16);
+    const T sxm = move<Mv::xm>(pc), sxp = move<Mv::xp>(pc);
+    const T sym = move<Mv::ym>(pc), syp = move<Mv::yp>(pc);
+    const T szm = move<Mv::zm>(pc), szp = move<Mv::zp>(pc);
+    const T cxm = move<Mv::xm>(axo), cym = move<Mv::ym>(ayo), czm = move<Mv::zm>(azo);
     const T p_xm = L.lx > 0 ? sxm : pn[0];
     const T p_xp = L.lx < 3 ? sxp : pn[1];
     const T p_ym = L.ly > 0 ? sym : pn[2];
@@ -1133,9 +1174,9 @@ __device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur
     }
     pr.rv = rv;
     pr.adj = cy.prev_bx == cur.bx - 1;
-    const T sx = shfl(cur.axo, L.lane - 1), sxp = shfl(cy.prev_ax, L.lane + 3);
-    const T sy = shfl(cur.ayo, L.lane - 4), sz = shfl(cur.azo, L.lane - 16);
-    const T pyx = shfl(cy.prev_y, L.lane + 3);
+    const T sx = move<Mv::xm>(cur.axo), sxp = move<Mv::q3>(cy.prev_ax);
+    const T sy = move<Mv::ym>(cur.ayo), sz = move<Mv::zm>(cur.azo);
+    const T pyx = move<Mv::q3>(cy.prev_y);
     pr.axm = ix ? sx : (pr.adj ? sxp : (T)0);
     pr.aym = iy ? sy : cur.aym_e;
     pr.azm = iz ? sz : cur.azm_e;
@@ -1154,7 +1195,7 @@ __device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const F
     T yv = (T)0;
 #pragma unroll
     for (int h = 0; h < 10; ++h) {
-        const T vx = shfl(yv, L.lane - 1), vy = shfl(yv, L.lane - 4), vz = shfl(yv, L.lane - 16);
+        const T vx = move<Mv::xm>(yv), vy = move<Mv::ym>(yv), vz = move<Mv::zm>(yv);
         if (h == h_own) {
             T t = fma(-pr.axm, ix ? vx : pr.ex, pr.rv);
             t = fma(-pr.aym, iy ? vy : got[0], t);
@@ -1278,7 +1319,7 @@ __device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, T ex, const T (
     T zv = (T)0;
 #pragma unroll
     for (int h = 9; h >= 0; --h) {
-        const T vx = shfl(zv, L.lane + 1), vy = shfl(zv, L.lane + 4), vz = shfl(zv, L.lane + 16);
+        const T vx = move<Mv::xp>(zv), vy = move<Mv::yp>(zv), vz = move<Mv::zp>(zv);
         if (h == h_own) {
             T t = cur.axo * (ix ? vx : ex);
             t = fma(cur.ayo, iy ? vy : got[0], t);
@@ -1326,7 +1367,7 @@ __global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<
             bool more = it.next(bx, b, ny, nz);
             bwd_fetch<T>(B, b, bx, ny, nz, c, r, y, L);
             {
-                const T pzx = shfl(cy.prev_z, L.lane - 3);
+                const T pzx = move<Mv::q0>(cy.prev_z);
                 const T ex = cy.prev_bx == A.bx + 1 ? pzx : (T)0;
                 flush_rz<T>(cy, part, L);
                 A.poll.finish(xch, epoch, got, ctrl, timeout);
@@ -1337,7 +1378,7 @@ __global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<
             more = it.next(bx, b, ny, nz);
             bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
             {
-                const T pzx = shfl(cy.prev_z, L.lane - 3);
+                const T pzx = move<Mv::q0>(cy.prev_z);
                 const T ex = cy.prev_bx == B.bx + 1 ? pzx : (T)0;
                 flush_rz<T>(cy, part, L);
                 B.poll.finish(xch, epoch, got, ctrl, timeout);
@@ -1716,6 +1757,31 @@ hipError_t launch_resid64(const Bricks& bk, const double* diag64, const TC* ax, 
 
 hipError_t launch_xupdate(const Bricks& bk, double* x64, const float* x32, Ctrl* ctrl, hipStream_t s) {
     hipLaunchKernelGGL(k_xupdate, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, x64, x32, (const Ctrl*)ctrl);
+    return hipGetLastError();
+}
+
+// Plain streaming copy: 16-byte words when both pointers are 16-byte aligned,
+// 4-byte words otherwise. Its HBM counters per byte copied calibrate the
+// traffic counters of the solver kernels.
+template <typename W>
+__global__ __launch_bounds__(256) void k_copy(const W* __restrict__ src, W* __restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    const bool wide = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+    const size_t n16 = wide ? bytes / 16 : 0;
+    if (n16) {
+        const unsigned blocks = (unsigned)std::min<size_t>((n16 + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_copy<uint4>, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n16);
+    }
+    const size_t n4 = (bytes - n16 * 16) / 4;
+    if (n4) {
+        const unsigned blocks = (unsigned)std::min<size_t>((n4 + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_copy<uint32_t>, dim3(blocks), dim3(256), 0, s, (const uint32_t*)src + n16 * 4,
+                           (uint32_t*)dst + n16 * 4, n4);
+    }
     return hipGetLastError();
 }
 

@@ -129,4 +129,8 @@ hipError_t launch_scatter(const Dense& d, const uint8_t* mask, const Bricks& bk,
                           float* pressure, hipStream_t s);
 hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStream_t s);
 
+// ---------------------------------------------------------------- utility
+// Device-to-device copy of bytes (a multiple of 4) with a plain streaming kernel.
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 }  // namespace elq

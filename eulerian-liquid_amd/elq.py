@@ -96,6 +96,7 @@ def load():
     L.elq_device_free.argtypes = [vp, vp]
     L.elq_memcpy_h2d.argtypes = [vp, vp, vp, sz]
     L.elq_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+    L.elq_memcpy_d2d.argtypes = [vp, vp, vp, sz]
     L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]
     L.elq_set_profiling.argtypes = [vp, C.c_int]
     L.elq_set_profiling.restype = None
@@ -363,6 +364,9 @@ class Context:
 
     def d2h(self, a: np.ndarray, src: int):
         _check_rc(self.L.elq_memcpy_d2h(self.h, _ptr(a), src, a.nbytes), self.h)
+
+    def d2d(self, dst: int, src: int, nbytes: int):
+        _check_rc(self.L.elq_memcpy_d2d(self.h, dst, src, nbytes), self.h)
 
     def set_profiling(self, on: bool):
         self.L.elq_set_profiling(self.h, 1 if on else 0)

@@ -157,6 +157,10 @@ int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out);
 int elq_device_free(elq_ctx* ctx, void* ptr);
 int elq_memcpy_h2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes);
 int elq_memcpy_d2h(elq_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy with the library's own streaming kernel; bytes and
+ * both pointers must be multiples of 4. No reference counterpart: a plain
+ * copy of known size, whose hardware counters calibrate traffic figures. */
+int elq_memcpy_d2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes);
 
 /* Per-kernel timing of the last projection on this context, measured with
  * HIP events on the context's stream. names: comma-separated kernel classes;

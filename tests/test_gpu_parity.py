@@ -129,3 +129,39 @@ def test_zero_rhs_gives_zero_pressure(ctx):
     p, idx, st = ctx.solve_pressure(s, elq.default_opts(tol=1e-8))
     assert st.converged and st.iterations == 0
     assert np.all(p == 0)
+
+
+@pytest.mark.parametrize("precision", [elq.FP64, elq.MIXED])
+def test_projection_mic0(ctx, precision):
+    """Modified IC(0) (tau = 0.97): same solution, oracle restates the same factor."""
+    s = SCENES["spheres24"]()
+    g = s.copy()
+    o = elq.default_opts(tol=1e-8, max_iter=5000, precision=precision, precond=elq.MIC0, tau=0.97)
+    p, idx, st = ctx.solve_pressure(g, o)
+    assert st.converged, st.as_dict()
+    ref = O.solve_pressure(to_oracle(s), tol=1e-12, max_iter=10000, tau=0.97)
+    np.testing.assert_array_equal(idx, ref.indices)
+    err = np.abs(p - ref.pressure).max() / np.abs(ref.pressure).max()
+    assert err <= (2e-6 if precision == elq.FP64 else 2e-5), err
+    it_ref = O.solve_pressure(to_oracle(s), tol=1e-8, max_iter=10000, tau=0.97).iterations
+    if precision == elq.FP64:
+        assert abs(st.iterations - it_ref) <= max(2, it_ref // 20), (st.iterations, it_ref)
+
+
+@pytest.mark.parametrize("nbytes,off", [(4, 0), (1 << 20, 0), ((1 << 20) + 12, 0), (4096 + 8, 4)])
+def test_device_copy(ctx, nbytes, off):
+    rng = np.random.default_rng(nbytes)
+    a = rng.integers(0, 2**32, size=(nbytes + off) // 4 + 4, dtype=np.uint32)
+    src, dst = ctx.alloc(a.nbytes), ctx.alloc(a.nbytes)
+    try:
+        ctx.h2d(src, a)
+        ctx.h2d(dst, np.zeros_like(a))
+        ctx.d2d(dst + off, src + off, nbytes)
+        out = np.empty_like(a)
+        ctx.d2h(out, dst)
+        w0, w1 = off // 4, (off + nbytes) // 4
+        np.testing.assert_array_equal(out[w0:w1], a[w0:w1])
+        assert not out[:w0].any() and not out[w1:].any()
+    finally:
+        ctx.free(src)
+        ctx.free(dst)

@@ -54,50 +54,66 @@ def workload(name: str, seed: int):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_steps(ctx, ds, opts, steps, warmup, profile):
-    """Warm-up steps run without per-kernel events (their times are reported
-    as the unprofiled reference); the timed steps record HIP events around
-    every kernel class on the library's stream."""
-    stats = []
+def project(ctx, ds, opts):
+    ds.reset_velocity()
+    t0 = time.perf_counter()
+    rc, st = ctx.project_device(ds.grid(), ds.s.dt, opts)
+    dt_ms = (time.perf_counter() - t0) * 1e3
+    if rc != elq.OK:
+        raise SystemExit(f"projection failed: {elq.STATUS_NAMES.get(rc)} {ctx.last_error()}")
+    return dt_ms, st
+
+
+def run_steps(ctx, ds, opts, steps, warmup):
+    """One untimed step with HIP events around every kernel class (the
+    per-class breakdown; it is the first warm-up step when warmup >= 1), the
+    remaining warm-up steps, then the timed steps with events around the
+    dominant kernel class only (its average launch duration gives the
+    roofline; the other classes run without events)."""
+    ctx.set_profiling(True)
+    project(ctx, ds, opts)
+    breakdown = ctx.kernel_times()
     ctx.set_profiling(False)
-    plain = []
-    for _ in range(warmup):
-        ds.reset_velocity()
-        t0 = time.perf_counter()
-        rc, st = ctx.project_device(ds.grid(), ds.s.dt, opts)
-        plain.append((time.perf_counter() - t0) * 1e3)
-        if rc not in (elq.OK,):
-            raise SystemExit(f"projection failed: {elq.STATUS_NAMES.get(rc)} {ctx.last_error()}")
-    ctx.set_profiling(profile)
-    times, ktimes = [], {}
+    dominant = max((k for k in breakdown if k in BYTES_PER_UNKNOWN), key=lambda k: breakdown[k][0])
+    for _ in range(max(warmup - 1, 0)):
+        project(ctx, ds, opts)
+    ctx.set_profiling(True, only=dominant)
+    times, stats, dom = [], [], [0.0, 0]
     for _ in range(steps):
-        ds.reset_velocity()
-        t0 = time.perf_counter()
-        rc, st = ctx.project_device(ds.grid(), ds.s.dt, opts)
-        t1 = time.perf_counter()
-        if rc != elq.OK:
-            raise SystemExit(f"projection failed: {elq.STATUS_NAMES.get(rc)} {ctx.last_error()}")
-        times.append((t1 - t0) * 1e3)
+        dt_ms, st = project(ctx, ds, opts)
+        times.append(dt_ms)
         stats.append(st.as_dict())
-        for k, (ms, n) in ctx.kernel_times().items():
-            a = ktimes.setdefault(k, [0.0, 0])
-            a[0] += ms
-            a[1] += n
+        ms, n = ctx.kernel_times().get(dominant, (0.0, 0))
+        dom[0] += ms
+        dom[1] += n
     ctx.set_profiling(False)
-    run_steps.plain = plain
-    return times, stats, ktimes
+    return times, stats, breakdown, (dominant, dom[0], dom[1])
 
 
-def roofline(ktimes, n_unknowns, sm, sv):
-    best = max((k for k in ktimes if k in BYTES_PER_UNKNOWN), key=lambda k: ktimes[k][0])
-    ms, launches = ktimes[best]
+def roofline(dom, n_unknowns, sm, sv):
+    kernel, ms, launches = dom
     avg_s = ms / launches / 1e3
-    bytes_per_launch = BYTES_PER_UNKNOWN[best](sm, sv) * n_unknowns
+    bytes_per_launch = BYTES_PER_UNKNOWN[kernel](sm, sv) * n_unknowns
     achieved = bytes_per_launch / avg_s / 1e9
-    return best, {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                  "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                  "kernel": best, "avg_launch_us": round(avg_s * 1e6, 2),
-                  "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None, "kernel": kernel,
+            "avg_launch_us": round(avg_s * 1e6, 2), "launches_timed": launches,
+            "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+
+
+def profiled_traffic(wname, precision, kernel):
+    """HBM bytes per launch of a kernel class on this workload, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (counters scaled by a
+    plain copy kernel of known size); None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"pcg_{wname}_{precision}_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch "):
+                v = json.loads(line.split(" ", 1)[1]).get(kernel)
+                return v, os.path.relpath(path, ROOT)
+    return None, None
 
 
 def cpu_baseline(scene, fp64_iters):
@@ -108,7 +124,7 @@ def cpu_baseline(scene, fp64_iters):
     import oracle as O
     os_ = O.Scene(scene.ni, scene.nj, scene.nk, scene.dx, scene.dt, scene.phi, scene.u.copy(), scene.v.copy(),
                   scene.w.copy(), scene.uw, scene.vw, scene.ww)
-    m1, m2 = 4, 24
+    m1, m2 = 8, 200  # about 15 s of CPU work at 256^3
     t = []
     for m in (m1, m2):
         t0 = time.perf_counter()
@@ -181,22 +197,21 @@ def main():
     ds = elq.DeviceScene(ctx, scene)
     out = {}
     if not a.fp64_only:
-        times, stats, kt = run_steps(ctx, ds, o_mixed, a.steps, a.warmup, True)
-        plain_mixed = list(run_steps.plain)
+        res = run_steps(ctx, ds, o_mixed, a.steps, a.warmup)
         p_mixed = ds.fetch("pressure")
-        print(f"# mixed: {[round(t, 1) for t in times]} ms, iters {stats[-1]['iterations']} "
-              f"outer {stats[-1]['outer_iterations']}", file=sys.stderr, flush=True)
-        out["mixed"] = (times, stats, kt)
+        print(f"# mixed: {[round(t, 1) for t in res[0]]} ms, iters {res[1][-1]['iterations']} "
+              f"outer {res[1][-1]['outer_iterations']}", file=sys.stderr, flush=True)
+        out["mixed"] = res
     if not a.no_fp64 or a.fp64_only:
         steps64 = a.steps if a.fp64_only else max(1, min(a.steps, 2))
-        times64, stats64, kt64 = run_steps(ctx, ds, o_fp64, steps64, 1 if a.fp64_only else 0, True)
+        res = run_steps(ctx, ds, o_fp64, steps64, a.warmup if a.fp64_only else 1)
         p_fp64 = ds.fetch("pressure")
-        print(f"# fp64: {[round(t, 1) for t in times64]} ms, iters {stats64[-1]['iterations']}", file=sys.stderr,
+        print(f"# fp64: {[round(t, 1) for t in res[0]]} ms, iters {res[1][-1]['iterations']}", file=sys.stderr,
               flush=True)
-        out["fp64"] = (times64, stats64, kt64)
+        out["fp64"] = res
 
     key = "fp64" if a.fp64_only else "mixed"
-    times, stats, kt = out[key]
+    times, stats, kt, dom = out[key]
     st = stats[-1]
     n = st["n_unknowns"]
     sm = sv = 8 if key == "fp64" else 4
@@ -204,7 +219,10 @@ def main():
     iters, outer = st["iterations"], st["outer_iterations"]
     setup_bytes = 92 * scene.cells + n * (9 * sm + 16)
     total_bytes = iters * n * B_ITER(sm, sv) + (outer * n * B_OUTER if key == "mixed" else 0) + setup_bytes
-    kernel, roof = roofline(kt, n, sm, sv)
+    roof = roofline(dom, n, sm, sv)
+    roof["traffic"], src = profiled_traffic(wname, key, roof["kernel"])
+    if src:
+        roof["traffic_source"] = src
     line = {
         "metric": "PCG pressure solves/s & HBM GB/s at 256^3 (rel.res 1e-8); p99 batched latency",
         "value": round(1000.0 / ms_step, 4),
@@ -225,11 +243,10 @@ def main():
         "hbm_gbs_whole_solve": round(total_bytes / (ms_step / 1e3) / 1e9, 1),
         "hbm_frac_whole_solve": round(total_bytes / (ms_step / 1e3) / 1e9 / PEAK_HBM_GBS, 4),
         "roofline": roof,
-        "kernel_ms_per_step": {k: round(v[0] / a.steps, 3) for k, v in sorted(kt.items())},
-        "ms_per_step_without_events": [round(t, 2) for t in (plain_mixed if key == "mixed" else run_steps.plain)],
+        "kernel_ms_per_step": {k: round(v[0], 3) for k, v in sorted(kt.items())},
     }
     if "fp64" in out and key == "mixed":
-        t64, s64, _ = out["fp64"]
+        t64, s64 = out["fp64"][0], out["fp64"][1]
         ms64 = float(np.mean(t64))
         scale = float(np.abs(p_fp64).max())
         line["fp64_drop_in"] = {"solves_per_s": round(1000.0 / ms64, 4), "ms_per_solve": round(ms64, 2),

@@ -67,7 +67,9 @@ int elq_ctx::hip_fail(hipError_t e, const char* what) {
 }
 
 void elq_ctx::prof_begin(const char* name) {
+    prof_open = false;
     if (!profiling) return;
+    if (!prof_filter.empty() && prof_filter.find("," + std::string(name) + ",") == std::string::npos) return;
     if (ev_used + 2 > events.size()) {
         for (int q = 0; q < 256; ++q) {
             hipEvent_t ev;
@@ -78,12 +80,14 @@ void elq_ctx::prof_begin(const char* name) {
     (void)hipEventRecord(events[ev_used], stream);
     ev_names.push_back(name);
     ++ev_used;
+    prof_open = true;
 }
 
 void elq_ctx::prof_end() {
-    if (!profiling) return;
+    if (!profiling || !prof_open) return;
     (void)hipEventRecord(events[ev_used], stream);
     ++ev_used;
+    prof_open = false;
 }
 
 void elq_ctx::prof_collect() {
@@ -274,7 +278,7 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
         HIPCHK(hipMemsetAsync(W.xch.p, 0, W.xch.cap, s), "memset xch");
     }
     if (!W.stage.p) {
-        HIPCHK(W.stage.ensure((size_t)kRedBlocks * 4 * 8), "alloc stage");
+        HIPCHK(W.stage.ensure((size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8), "alloc stage");
         HIPCHK(hipMemsetAsync(W.stage.p, 0, W.stage.cap, s), "memset stage");
     }
     HIPCHK(W.diag64.ensure(cells * 8), "alloc");
@@ -285,7 +289,6 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     HIPCHK(W.inv_e32.ensure(cells * 4), "alloc");
     HIPCHK(W.part_rr.ensure((size_t)nb * 8), "alloc");
     HIPCHK(W.part_rz.ensure((size_t)nb * 8), "alloc");
-    HIPCHK(W.part_pq.ensure((size_t)nb * 8), "alloc");
     if (!W.ctrl.p) {
         HIPCHK(W.ctrl.ensure(sizeof(Ctrl)), "alloc");
         HIPCHK(hipMemsetAsync(W.ctrl.p, 0, sizeof(Ctrl), s), "memset ctrl");
@@ -347,7 +350,7 @@ struct Solver {
     Ctrl* ctrl() { return (Ctrl*)ctx->ws.ctrl.p; }
     double* prr() { return (double*)ctx->ws.part_rr.p; }
     double* prz() { return (double*)ctx->ws.part_rz.p; }
-    double* ppq() { return (double*)ctx->ws.part_pq.p; }
+    uint64_t* stage() { return (uint64_t*)ctx->ws.stage.p; }
     T* vec(DevBuf& b) { return (T*)b.p; }
 
     int scalar(int mode, const double* parts, const double* parts2 = nullptr) {
@@ -364,37 +367,41 @@ struct Solver {
         ctx->prof_begin(update ? "fwd_sweep_update" : "fwd_sweep");
         if (!levels) {
             HIPCHK(launch_fwd<T>(bk, pc, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
-                                 (uint64_t*)W.xch.p, update, -1, 0, 0, ctx->timeout_ticks, ctx->stream),
+                                 (uint64_t*)W.xch.p, stage(), update, -1, 0, 0, ctx->timeout_ticks, ctx->stream),
                    "k_fwd");
         } else {
             for (int l = 0; l < p.nlev; ++l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
                 HIPCHK(launch_fwd<T>(bk, pc, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
-                                     (uint64_t*)W.xch.p, update, l, base, cnt, ctx->timeout_ticks, ctx->stream),
+                                     (uint64_t*)W.xch.p, stage(), update, l, base, cnt, ctx->timeout_ticks,
+                                     ctx->stream),
                        "k_fwd");
             }
         }
         ctx->prof_end();
-        return ELQ_OK;
+        // the level launches leave |r|^2 as per-brick partials
+        return levels ? scalar(update ? SC_RR : SC_NOP, update ? prr() : nullptr) : ELQ_OK;
     }
 
-    int bwd() {
+    // Backward sweep; the r.z reduction then applies mode (SC_BETA,
+    // SC_RZ_INIT or SC_RZ_REPLACE).
+    int bwd(int mode) {
         Workspace& W = ctx->ws;
         ctx->prof_begin("bwd_sweep");
         if (!levels) {
-            HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, -1, 0, 0,
-                                 ctx->timeout_ticks, ctx->stream),
+            HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(), mode,
+                                 -1, 0, 0, ctx->timeout_ticks, ctx->stream),
                    "k_bwd");
         } else {
             for (int l = p.nlev - 1; l >= 0; --l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
-                HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, l, base,
-                                     cnt, ctx->timeout_ticks, ctx->stream),
+                HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(),
+                                     mode, l, base, cnt, ctx->timeout_ticks, ctx->stream),
                        "k_bwd");
             }
         }
         ctx->prof_end();
-        return ELQ_OK;
+        return levels ? scalar(mode, prz()) : ELQ_OK;
     }
 
     int factor(double tau, double sigma, float* inv_e32) {
@@ -427,9 +434,7 @@ struct Solver {
     int precondition(int rz_mode) {
         int rc;
         if ((rc = fwd(0, nullptr))) return rc;
-        if ((rc = scalar(SC_NOP, nullptr))) return rc;
-        if ((rc = bwd())) return rc;
-        return scalar(rz_mode, prz());
+        return bwd(rz_mode);
     }
 
     // n PCG iterations (Eigen's loop body), no host synchronisation.
@@ -439,15 +444,12 @@ struct Solver {
         T* P[2] = {vec(W.vp0), vec(W.vp1)};
         for (int q = 0; q < n; ++q, ++kiter) {
             ctx->prof_begin("spmv");
-            HIPCHK(launch_spmv<T>(bk, c, vec(W.vy), P[kiter & 1], P[(kiter + 1) & 1], vec(W.vq), ppq(), ctrl(),
-                                  ctx->stream),
+            HIPCHK(launch_spmv<T>(bk, c, vec(W.vy), P[kiter & 1], P[(kiter + 1) & 1], vec(W.vq), ctrl(), stage(),
+                                  ctx->timeout_ticks, ctx->stream),
                    "k_spmv");
             ctx->prof_end();
-            if ((rc = scalar(SC_ALPHA, ppq()))) return rc;
             if ((rc = fwd(1, P[(kiter + 1) & 1]))) return rc;
-            if ((rc = scalar(SC_RR, prr()))) return rc;
-            if ((rc = bwd())) return rc;
-            if ((rc = scalar(SC_BETA, prz()))) return rc;
+            if ((rc = bwd(SC_BETA))) return rc;
         }
         return ELQ_OK;
     }
@@ -932,6 +934,12 @@ int elq_memcpy_d2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 void elq_set_profiling(elq_ctx* ctx, int enable) {
     if (ctx) ctx->profiling = enable != 0;
+}
+
+void elq_set_profiling_filter(elq_ctx* ctx, const char* classes) {
+    if (!ctx) return;
+    ctx->prof_filter.clear();
+    if (classes && *classes) ctx->prof_filter = "," + std::string(classes) + ",";
 }
 
 int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, double* ms, int64_t* launches, int cap) {

@@ -29,7 +29,7 @@ struct Workspace {
     DevBuf diag64, b64, inv_e64, inv_e32, cdiag, cax, cay, caz;
     // Krylov vectors (storage type of the solve)
     DevBuf vx, vr, vp0, vp1, vq, vy, x64;
-    DevBuf part_rr, part_rz, part_pq, ctrl, nfixed;
+    DevBuf part_rr, part_rz, ctrl, nfixed;
     // staging of host inputs
     DevBuf s_phi, s_u, s_v, s_w, s_uw, s_vw, s_ww, s_pressure, s_indices;
     DevBuf export_buf;
@@ -46,7 +46,7 @@ struct Workspace {
         for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
                           &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &pcoord, &plvl_cnt,
                           &plvl_off, &plvl_fill, &plvl_list, &diag64, &b64, &inv_e64, &inv_e32, &cdiag, &cax,
-                          &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz, &part_pq, &ctrl,
+                          &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz, &ctrl,
                           &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
                           &export_buf})
             b->release();
@@ -63,6 +63,8 @@ struct elq_ctx {
 
     // per-kernel-class event timing
     bool profiling = false;
+    std::string prof_filter;  // ",a,b," : only these classes are timed; empty: all
+    bool prof_open = false;   // the current prof_begin recorded an event
     std::vector<hipEvent_t> events;
     size_t ev_used = 0;
     std::vector<const char*> ev_names;

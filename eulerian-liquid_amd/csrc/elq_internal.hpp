@@ -26,6 +26,7 @@ constexpr int kScanItems = 16;                // cells per thread per scan block
 constexpr int kScanChunk = kScanBlock * kScanItems;
 constexpr int kReduceThreads = 256;           // scalar kernel workgroup width
 constexpr int kRedBlocks = 128;               // max workgroups of the scalar kernel
+constexpr int kTailMaxGroups = 4096;          // max workgroups of a kernel ending in a grid tail
 constexpr int kRedPerBlock = 2048;            // partials per scalar workgroup (target)
 constexpr int kXchWords = 192;                // face-exchange words per brick (6 x 16 x 2)
 

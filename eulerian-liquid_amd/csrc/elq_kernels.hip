@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 #include <cfloat>
 #include <climits>
 #include <tuple>
@@ -537,7 +539,7 @@ __global__ __launch_bounds__(64) void k_factor_level(Bricks bk, const double* di
 }
 
 // Persistent wavefront: wave w of W takes positions w, w+W, w+2W, ... of the
-// level-ordered brick list. Launched cooperatively (all waves resident), so a
+// level-ordered brick list. All waves are resident (see persistent()), so a
 // brick's predecessors, which sit at smaller positions, always make progress.
 template <class T>
 __global__ __launch_bounds__(256) void k_factor_wave(Bricks bk, const double* diag64, const T* ax, const T* ay,
@@ -773,6 +775,48 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
     scalar_update(c, mode, S, S2);
 }
 
+// Reduction tail of a persistent kernel (workgroup 0 waits only for
+// workgroups dispatched after it, which need nothing from it, so its waits end). Each workgroup adds its four
+// wave sums in wave order and publishes the result as a tagged word pair in
+// stage[2g]; workgroup 0 gathers the G sums (thread t takes g = t, t+256, ...
+// in that order, then a fixed tree) and applies the scalar recurrence. The
+// result depends only on the grid size, not on timing. Every thread of every
+// workgroup must call this; ctrl must not be read after it.
+__device__ void grid_tail(Ctrl* c, int mode, double wave_total, uint32_t tag, uint64_t* stage, uint64_t timeout) {
+    __shared__ double sw[4];
+    __shared__ double st[256];
+    const int t = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+    if ((t & 63) == 0) sw[t >> 6] = wave_total;
+    __syncthreads();
+    if (g != 0) {
+        if (t == 0) xput(stage, (size_t)g * 2, ((sw[0] + sw[1]) + sw[2]) + sw[3], tag);
+        return;
+    }
+    double s = t == 0 ? ((sw[0] + sw[1]) + sw[2]) + sw[3] : 0.0;
+    for (int k0 = 0; k0 * 256 < G; k0 += 4) {
+        long slot[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int gi = t + 256 * (k0 + k);
+            slot[k] = (gi > 0 && gi < G) ? (long)gi * 2 : -1;
+        }
+        double v[4];
+        xgather<double, 4>(stage, slot, tag, v, c, timeout);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s += v[k];
+    }
+    st[t] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (t < o) st[t] += st[t + o];
+        __syncthreads();
+    }
+    if (t == 0) {
+        c->red_epoch = tag;
+        scalar_update(c, mode, st[0], 0.0);
+    }
+}
+
 // The six neighbour brick ids of brick b (wave-uniform): lanes 0..5 load
 // them in one access, readlane broadcasts them.
 struct Nbr6 {
@@ -803,14 +847,9 @@ __device__ __forceinline__ size_t halo_offset(const Nbr6& nb, int d, size_t own,
 // through lane shuffles. All loads are issued unconditionally and at once.
 // Accumulation order: diag, -x, +x, -y, +y, -z, +z.
 template <class T>
-__global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
-                                              double* part, const Ctrl* ctrl) {
-    if (ctrl->done) return;
-    const int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6));
-    if (b >= bk.nb) return;
-    const Lane L;
+__device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in,
+                                             T* p_out, T* q, T beta, const Lane& L) {
     const Nbr6 nb = load_nbr6(bk, b);
-    const T beta = (T)ctrl->beta;
     const size_t o = (size_t)b * kBrickCells + L.lane;
     const int lc[3] = {L.lx, L.ly, L.lz};
     bool has[6];
@@ -851,8 +890,23 @@ __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, 
     acc = fma(azm, p_zm, acc);
     acc = fma(azo, p_zp, acc);
     q[o] = acc;
-    const double s = wave_sum((double)pc * (double)acc);
-    if (L.lane == 0) part[b] = s;
+    return (double)pc * (double)acc;
+}
+
+// Persistent: wave w of W takes bricks w, w+W, ...; its p.q terms accumulate
+// per lane in that order; the grid tail computes alpha.
+template <class T>
+__global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
+                                              Ctrl* ctrl, uint64_t* stage, uint64_t timeout) {
+    if (ctrl->done) return;
+    const uint32_t tag = ctrl->red_epoch + 1u;
+    const T beta = (T)ctrl->beta;
+    const Lane L;
+    const int W = (int)gridDim.x * 4;
+    double acc = 0.0;
+    for (int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6)); b < bk.nb; b += W)
+        acc += spmv_brick<T>(b, bk, c, z, p_in, p_out, q, beta, L);
+    grid_tail(ctrl, SC_ALPHA, wave_sum(acc), tag, stage, timeout);
 }
 
 // Forward sweep of M^{-1} = (E+L^T)^{-1} E (E+L)^{-1}: y = (r - L y) / e, one
@@ -1105,7 +1159,7 @@ __device__ __forceinline__ void bput(uint64_t* x, int b, T v, uint32_t e, const 
 // neighbours are ready costs little more than its own arithmetic. Every brick
 // a wave waits for belongs to a pencil of lower (backward: higher) level,
 // i.e. an earlier position of the list, and all waves are resident
-// (cooperative launch), so every wait ends.
+// (see persistent()), so every wait ends.
 template <class T, bool UPDATE>
 struct FwdIn {
     int b, bx;
@@ -1148,12 +1202,12 @@ template <class T>
 struct FwdCarry {
     T prev_y, prev_ax;
     int prev_bx;
-    T rr_val;     // r of the last computed brick (its |r|^2 partial is pending)
-    int rr_b;     // that brick, -1 if none
+    double acc;  // this lane's |r|^2 terms, in the wave's brick order
 };
 
 // Part of a forward brick that needs no neighbour face: in-brick lower
-// couplings and the x-neighbour values carried from the previous brick.
+// couplings, the x-neighbour values carried from the previous brick and, with
+// UPDATE, the x and r updates and the brick's |r|^2 terms.
 template <class T, bool UPDATE>
 struct FwdPre {
     T axm, aym, azm, ex, rv;
@@ -1161,7 +1215,7 @@ struct FwdPre {
 };
 
 template <class T, bool UPDATE>
-__device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur, const FwdCarry<T>& cy, T alpha,
+__device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur, FwdCarry<T>& cy, T alpha,
                                                      T* r, T* x, const Lane& L) {
     FwdPre<T, UPDATE> pr;
     const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
@@ -1171,6 +1225,7 @@ __device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur
         rv = fma(-alpha, cur.qv, rv);
         x[o] = fma(alpha, cur.pv, cur.xv);
         r[o] = rv;
+        cy.acc += (double)rv * (double)rv;
     }
     pr.rv = rv;
     pr.adj = cy.prev_bx == cur.bx - 1;
@@ -1184,8 +1239,7 @@ __device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur
     return pr;
 }
 
-// Forward brick: the ten hyperplanes, then publication. The |r|^2 partial of
-// the brick is left pending in the carry (see flush_rr).
+// Forward brick: the ten hyperplanes, then publication.
 template <class T, bool UPDATE>
 __device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdPre<T, UPDATE>& pr,
                                             const T (&got)[2], FwdCarry<T>& cy, T* y, uint64_t* xch, uint32_t epoch,
@@ -1208,38 +1262,29 @@ __device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const F
     cy.prev_y = yv;
     cy.prev_ax = cur.axo;
     cy.prev_bx = cur.bx;
-    if (UPDATE) {
-        cy.rr_val = pr.rv;
-        cy.rr_b = cur.b;
-    }
-}
-
-template <class T, bool UPDATE>
-__device__ __forceinline__ void flush_rr(FwdCarry<T>& cy, double* part, const Lane& L) {
-    if (UPDATE && cy.rr_b >= 0) {
-        const double s = wave_sum((double)cy.rr_val * (double)cy.rr_val);
-        if (L.lane == 0) part[cy.rr_b] = s;
-        cy.rr_b = -1;
-    }
 }
 
 // Pencil loop, unrolled by two over buffers A and B so that the prefetched
 // registers are never copied (a copy would wait for the loads). Per brick:
-// everything that needs no neighbour face (coupling shuffles, x carry, the
-// previous brick's |r|^2 partial) is done first, then the neighbour polls of
+// everything that needs no neighbour face (coupling moves, x carry, x and r
+// updates) is done first, then the neighbour polls of
 // the brick are finished, the next brick's first poll is issued (its round
 // trip overlaps this computation) and the hyperplanes run.
 // Past the pencil's last brick the fetch repeats the current brick (valid
 // addresses, results unused), which keeps the loop free of memory branches.
+// The wave's |r|^2 terms go to the grid tail (mode SC_RR with UPDATE; without
+// UPDATE the tail only advances the epochs, SC_NOP).
 template <class T, bool UPDATE>
 __global__ __launch_bounds__(256) void k_fwd_pencil(Bricks bk, Pencils pc, Coef<T> c, T* r, T* x, const T* p,
-                                                    const T* q, T* y, double* part, Ctrl* ctrl, uint64_t* xch,
+                                                    const T* q, T* y, Ctrl* ctrl, uint64_t* xch, uint64_t* stage,
                                                     uint64_t timeout) {
     if (ctrl->done) return;
     const Lane L;
     const int W = (int)gridDim.x * 4;
     const uint32_t epoch = ctrl->epoch;
+    const uint32_t tag = ctrl->red_epoch + 1u;
     const T alpha = UPDATE ? (T)ctrl->alpha : (T)0;
+    double acc = 0.0;
     for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < pc.np; pos += W) {
         const int cc = __builtin_amdgcn_readfirstlane(pc.coord[pc.lvl_list[pos]]);
         const int by = cc & 1023, bz = (cc >> 10) & 1023;
@@ -1247,7 +1292,7 @@ __global__ __launch_bounds__(256) void k_fwd_pencil(Bricks bk, Pencils pc, Coef<
         int bx, b, ny, nz;
         if (!it.next(bx, b, ny, nz)) continue;
         FwdIn<T, UPDATE> A, B;
-        FwdCarry<T> cy{(T)0, (T)0, -2, (T)0, -1};
+        FwdCarry<T> cy{(T)0, (T)0, -2, acc};
         fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
         A.poll.issue(xch);
         T got[2];
@@ -1256,7 +1301,6 @@ __global__ __launch_bounds__(256) void k_fwd_pencil(Bricks bk, Pencils pc, Coef<
             fwd_fetch<T, UPDATE>(B, b, bx, ny, nz, c, r, x, p, q, L);
             {
                 const FwdPre<T, UPDATE> pr = fwd_pre<T, UPDATE>(A, cy, alpha, r, x, L);
-                flush_rr<T, UPDATE>(cy, part, L);
                 A.poll.finish(xch, epoch, got, ctrl, timeout);
                 B.poll.issue(xch);
                 fwd_compute<T, UPDATE>(A, pr, got, cy, y, xch, epoch, L);
@@ -1266,15 +1310,15 @@ __global__ __launch_bounds__(256) void k_fwd_pencil(Bricks bk, Pencils pc, Coef<
             fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
             {
                 const FwdPre<T, UPDATE> pr = fwd_pre<T, UPDATE>(B, cy, alpha, r, x, L);
-                flush_rr<T, UPDATE>(cy, part, L);
                 B.poll.finish(xch, epoch, got, ctrl, timeout);
                 A.poll.issue(xch);
                 fwd_compute<T, UPDATE>(B, pr, got, cy, y, xch, epoch, L);
             }
             if (!more) break;
         }
-        flush_rr<T, UPDATE>(cy, part, L);
+        acc = cy.acc;
     }
+    grid_tail(ctrl, UPDATE ? SC_RR : SC_NOP, UPDATE ? wave_sum(acc) : 0.0, tag, stage, timeout);
 }
 
 template <class T>
@@ -1307,8 +1351,7 @@ template <class T>
 struct BwdCarry {
     T prev_z;
     int prev_bx;
-    T rz_r, rz_z;  // r and z of the last computed brick (r.z partial pending)
-    int rz_b;
+    double acc;  // this lane's r.z terms, in the wave's brick order
 };
 
 template <class T>
@@ -1331,27 +1374,20 @@ __device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, T ex, const T (
     y[(size_t)cur.b * kBrickCells + L.lane] = zv;
     cy.prev_z = zv;
     cy.prev_bx = cur.bx;
-    cy.rz_r = cur.rv;
-    cy.rz_z = zv;
-    cy.rz_b = cur.b;
+    cy.acc += (double)cur.rv * (double)zv;
 }
 
+// The wave's r.z terms go to the grid tail, which applies tail_mode
+// (SC_BETA, SC_RZ_INIT or SC_RZ_REPLACE).
 template <class T>
-__device__ __forceinline__ void flush_rz(BwdCarry<T>& cy, double* part, const Lane& L) {
-    if (cy.rz_b >= 0) {
-        const double s = wave_sum((double)cy.rz_r * (double)cy.rz_z);
-        if (L.lane == 0) part[cy.rz_b] = s;
-        cy.rz_b = -1;
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<T> c, const T* r, T* y, double* part,
-                                                    Ctrl* ctrl, uint64_t* xch, uint64_t timeout) {
+__global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<T> c, const T* r, T* y, Ctrl* ctrl,
+                                                    uint64_t* xch, uint64_t* stage, int tail_mode, uint64_t timeout) {
     if (ctrl->done) return;
     const Lane L;
     const int W = (int)gridDim.x * 4;
     const uint32_t epoch = ctrl->epoch;
+    const uint32_t tag = ctrl->red_epoch + 1u;
+    double acc = 0.0;
     for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < pc.np; pos += W) {
         const int cc = __builtin_amdgcn_readfirstlane(pc.coord[pc.lvl_list[pc.np - 1 - pos]]);
         const int by = cc & 1023, bz = (cc >> 10) & 1023;
@@ -1359,7 +1395,7 @@ __global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<
         int bx, b, ny, nz;
         if (!it.next(bx, b, ny, nz)) continue;
         BwdIn<T> A, B;
-        BwdCarry<T> cy{(T)0, -2, (T)0, (T)0, -1};
+        BwdCarry<T> cy{(T)0, -2, acc};
         bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
         A.poll.issue(xch);
         T got[2];
@@ -1369,7 +1405,6 @@ __global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<
             {
                 const T pzx = move<Mv::q0>(cy.prev_z);
                 const T ex = cy.prev_bx == A.bx + 1 ? pzx : (T)0;
-                flush_rz<T>(cy, part, L);
                 A.poll.finish(xch, epoch, got, ctrl, timeout);
                 B.poll.issue(xch);
                 bwd_compute<T>(A, ex, got, cy, y, xch, epoch, L);
@@ -1380,15 +1415,15 @@ __global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<
             {
                 const T pzx = move<Mv::q0>(cy.prev_z);
                 const T ex = cy.prev_bx == B.bx + 1 ? pzx : (T)0;
-                flush_rz<T>(cy, part, L);
                 B.poll.finish(xch, epoch, got, ctrl, timeout);
                 A.poll.issue(xch);
                 bwd_compute<T>(B, ex, got, cy, y, xch, epoch, L);
             }
             if (!more) break;
         }
-        flush_rz<T>(cy, part, L);
+        acc = cy.acc;
     }
+    grid_tail(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout);
 }
 
 // ---------------------------------------------------------------- pencil setup
@@ -1544,6 +1579,7 @@ inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b);
 
 constexpr int kWaveBlocksPerCU = 2;    // persistent factor sweep: 2 workgroups (8 waves) per CU
 constexpr int kPencilBlocksPerCU = 4;  // pencil sweeps: up to 4 workgroups (16 waves) per CU
+constexpr int kSpmvBlocksPerCU = 4;    // persistent SpMV: up to 4 workgroups (16 waves) per CU
 
 int device_cus() {
     static int cus[64] = {0};
@@ -1554,28 +1590,46 @@ int device_cus() {
     return cus[dev] > 0 ? cus[dev] : 1;
 }
 
-// Cooperative launch of a persistent sweep kernel (256 threads = 4 waves per
-// workgroup): grid = min(bricks / 4, resident workgroups), so every wave is
-// resident for the whole sweep.
-template <typename... KArgs, typename... Args, size_t... I>
-hipError_t coop_impl(void (*kern)(KArgs...), int nb, int blocks_per_cu, hipStream_t s, std::index_sequence<I...>,
-                     Args&&... args) {
-    std::tuple<KArgs...> t(std::forward<Args>(args)...);
-    void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
+// Resident workgroups per CU of a kernel at 256 threads (cached per kernel).
+int occupancy(const void* kern) {
+    static std::mutex mu;
+    static std::vector<std::pair<const void*, int>> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    for (const auto& kv : cache)
+        if (kv.first == kern) return kv.second;
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)kern, 256, 0) != hipSuccess || occ < 1) occ = 1;
-    const int per_cu = occ < blocks_per_cu ? occ : blocks_per_cu;
-    int g = (nb + 3) / 4;
-    if (g > per_cu * device_cus()) g = per_cu * device_cus();
-    if (g < 1) g = 1;
-    return hipLaunchCooperativeKernel((const void*)kern, dim3(g), dim3(256), ptrs, 0, s);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, 0) != hipSuccess || occ < 1) occ = 1;
+    cache.emplace_back(kern, occ);
+    return occ;
 }
 
-// nb: work items (one per wave, four waves per 256-thread workgroup).
+// Launch of a persistent kernel (256 threads = 4 waves per workgroup) whose
+// workgroups wait on each other: grid = min(items / 4, resident workgroups),
+// so on an otherwise idle device every workgroup is resident for the whole
+// launch. A plain launch is used: a cooperative launch gives the same
+// residency but costs about 20 us of extra dispatch latency per kernel here.
+// If another workload keeps some workgroups from starting, the bounded
+// waits end at the time limit and the launch reports the abort.
+template <typename... KArgs, typename... Args, size_t... I>
+hipError_t persistent_impl(void (*kern)(KArgs...), int items, int blocks_per_cu, hipStream_t s,
+                           std::index_sequence<I...>, Args&&... args) {
+    std::tuple<KArgs...> t(std::forward<Args>(args)...);
+    void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
+    const int occ = occupancy((const void*)kern);
+    const int per_cu = occ < blocks_per_cu ? occ : blocks_per_cu;
+    int g = (items + 3) / 4;
+    if (g > per_cu * device_cus()) g = per_cu * device_cus();
+    if (g > kTailMaxGroups) g = kTailMaxGroups;
+    if (g < 1) g = 1;
+    return hipLaunchKernel((const void*)kern, dim3(g), dim3(256), ptrs, 0, s);
+}
+
+// items: work items (one per wave, four waves per 256-thread workgroup).
 template <typename... KArgs, typename... Args>
-hipError_t coop(void (*kern)(KArgs...), int nb, int blocks_per_cu, hipStream_t s, Args&&... args) {
+hipError_t persistent(void (*kern)(KArgs...), int items, int blocks_per_cu, hipStream_t s, Args&&... args) {
     static_assert(sizeof...(KArgs) == sizeof...(Args), "argument count");
-    return coop_impl(kern, nb, blocks_per_cu, s, std::index_sequence_for<KArgs...>{}, std::forward<Args>(args)...);
+    return persistent_impl(kern, items, blocks_per_cu, s, std::index_sequence_for<KArgs...>{},
+                           std::forward<Args>(args)...);
 }
 
 }  // namespace
@@ -1642,7 +1696,7 @@ hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, co
                          int level, int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
         if (bk.nb == 0) return hipSuccess;
-        return coop(k_factor_wave<T>, bk.nb, kWaveBlocksPerCU, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
+        return persistent(k_factor_wave<T>, bk.nb, kWaveBlocksPerCU, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
                     n_fixed, timeout_ticks);
     }
     if (level_count > 0)
@@ -1682,23 +1736,20 @@ hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const
 }
 
 template <class T>
-hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q, double* part,
-                       Ctrl* ctrl, hipStream_t s) {
-    hipLaunchKernelGGL(k_spmv<T>, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, c, z, p_in, p_out, q, part,
-                       (const Ctrl*)ctrl);
-    return hipGetLastError();
+hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q, Ctrl* ctrl,
+                       uint64_t* stage, uint64_t timeout_ticks, hipStream_t s) {
+    return persistent(k_spmv<T>, bk.nb, kSpmvBlocksPerCU, s, bk, c, z, p_in, p_out, q, ctrl, stage, timeout_ticks);
 }
 
 template <class T>
 hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
-                      double* part, Ctrl* ctrl, uint64_t* xch, int update, int level, int level_base, int level_count,
-                      uint64_t timeout_ticks, hipStream_t s) {
+                      double* part, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int update, int level,
+                      int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
     if (level < 0) {
-        if (pc.np == 0) return hipSuccess;
         if (update)
-            return coop(k_fwd_pencil<T, true>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, part, ctrl,
-                        xch, timeout_ticks);
-        return coop(k_fwd_pencil<T, false>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, part, ctrl, xch,
+            return persistent(k_fwd_pencil<T, true>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, ctrl, xch,
+                        stage, timeout_ticks);
+        return persistent(k_fwd_pencil<T, false>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
                     timeout_ticks);
     }
     if (level_count > 0) {
@@ -1714,12 +1765,11 @@ hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* 
 
 template <class T>
 hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part,
-                      Ctrl* ctrl, uint64_t* xch, int level, int level_base, int level_count, uint64_t timeout_ticks,
-                      hipStream_t s) {
-    if (level < 0) {
-        if (pc.np == 0) return hipSuccess;
-        return coop(k_bwd_pencil<T>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, y, part, ctrl, xch, timeout_ticks);
-    }
+                      Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode, int level, int level_base,
+                      int level_count, uint64_t timeout_ticks, hipStream_t s) {
+    if (level < 0)
+        return persistent(k_bwd_pencil<T>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, y, ctrl, xch, stage, tail_mode,
+                    timeout_ticks);
     if (level_count > 0)
         hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
                            level_base, timeout_ticks);
@@ -1811,14 +1861,13 @@ hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStr
                                          const double*, const T*, const T*, const T*, double*, double*, double*,    \
                                          double*, double*, hipStream_t);                                            \
     template hipError_t launch_init<T>(const Bricks&, const double*, T*, T*, T*, T*, double*, Ctrl*, hipStream_t);  \
-    template hipError_t launch_spmv<T>(const Bricks&, const Coef<T>&, const T*, const T*, T*, T*, double*, Ctrl*,   \
-                                       hipStream_t);                                                                \
+    template hipError_t launch_spmv<T>(const Bricks&, const Coef<T>&, const T*, const T*, T*, T*, Ctrl*, uint64_t*, \
+                                       uint64_t, hipStream_t);                                                      \
     template hipError_t launch_fwd<T>(const Bricks&, const Pencils&, const Coef<T>&, T*, T*, const T*, const T*, T*, \
-                                      double*, Ctrl*,                                                               \
-                                      uint64_t*, int, int, int, int, uint64_t, hipStream_t);                        \
+                                      double*, Ctrl*, uint64_t*, uint64_t*, int, int, int, int, uint64_t,           \
+                                      hipStream_t);                                                                 \
     template hipError_t launch_bwd<T>(const Bricks&, const Pencils&, const Coef<T>&, const T*, T*, double*, Ctrl*,   \
-                                      uint64_t*, int,                                                               \
-                                      int, int, uint64_t, hipStream_t);                                             \
+                                      uint64_t*, uint64_t*, int, int, int, int, uint64_t, hipStream_t);             \
     template hipError_t launch_resid64<T>(const Bricks&, const double*, const T*, const T*, const T*, const double*, \
                                           const double*, double*, double*, float*, float*, float*, float*, Ctrl*,   \
                                           hipStream_t);

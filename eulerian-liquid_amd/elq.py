@@ -97,6 +97,8 @@ def load():
     L.elq_memcpy_h2d.argtypes = [vp, vp, vp, sz]
     L.elq_memcpy_d2h.argtypes = [vp, vp, vp, sz]
     L.elq_memcpy_d2d.argtypes = [vp, vp, vp, sz]
+    L.elq_set_profiling_filter.argtypes = [vp, C.c_char_p]
+    L.elq_set_profiling_filter.restype = None
     L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]
     L.elq_set_profiling.argtypes = [vp, C.c_int]
     L.elq_set_profiling.restype = None
@@ -368,7 +370,9 @@ class Context:
     def d2d(self, dst: int, src: int, nbytes: int):
         _check_rc(self.L.elq_memcpy_d2d(self.h, dst, src, nbytes), self.h)
 
-    def set_profiling(self, on: bool):
+    def set_profiling(self, on: bool, only: str = ""):
+        """Per-kernel event timing; only: comma-separated kernel classes to time (all if empty)."""
+        self.L.elq_set_profiling_filter(self.h, only.encode())
         self.L.elq_set_profiling(self.h, 1 if on else 0)
 
     def kernel_times(self) -> dict:

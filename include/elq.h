@@ -170,6 +170,9 @@ int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, dou
                           int64_t* launches, int cap);
 /* Enables (1) or disables (0) per-kernel event timing (adds event records). */
 void elq_set_profiling(elq_ctx* ctx, int enable);
+/* Restricts event timing to the named kernel classes (comma-separated, e.g.
+ * "fwd_sweep_update"); NULL or "" times every class. */
+void elq_set_profiling_filter(elq_ctx* ctx, const char* classes);
 
 /* ------------------------------------------------------------------ serving
  * Continuous batching of independent projections (uneven grid sizes) on one

@@ -160,6 +160,70 @@ def copy_calibration(nbytes, reps):
                       "copy_gbs_incl_sync": round(2 * nbytes / t / 1e9, 1)}), flush=True)
 
 
+def serve_run(scenes, arrivals, slots, opts):
+    """Submits fresh copies of the scenes at the given arrival times (s from
+    the start) to a server with `slots` slots; returns per-request results."""
+    work = [sc.copy() for sc in scenes]
+    n = len(work)
+    got = []
+    with elq.Server(0, max_slots=slots, max_queue=n, opts=opts) as srv:
+        t0 = time.perf_counter()
+        i = 0
+        while len(got) < n:
+            now = time.perf_counter() - t0
+            while i < n and arrivals[i] <= now:
+                srv.submit(work[i])
+                i += 1
+            nxt = (arrivals[i] - now) if i < n else 1.0
+            if srv.in_flight() == 0 and i < n:
+                time.sleep(max(nxt, 0.0))
+            else:
+                srv.wait(int(max(nxt, 0.0) * 1e3) + 1)
+            got += srv.poll()
+    for r in got:
+        r.pop("scene")
+        r.pop("pressure")
+    return got
+
+
+def serve_summary(got):
+    lat = np.array([r["t_done_ms"] - r["t_submit_ms"] for r in got])
+    span_s = (max(r["t_done_ms"] for r in got) - min(r["t_submit_ms"] for r in got)) / 1e3
+    return {"solves_per_s": round(len(got) / span_s, 2), "p50_ms": round(float(np.percentile(lat, 50)), 2),
+            "p99_ms": round(float(np.percentile(lat, 99)), 2), "max_ms": round(float(lat.max()), 2),
+            "all_ok": all(r["status"] == elq.OK for r in got),
+            "max_true_rel_residual": max(r["true_rel_residual"] for r in got),
+            "mean_iterations": round(float(np.mean([r["iterations"] for r in got])), 1)}
+
+
+def serving(n_req, slots, seed, base):
+    """Serving stream: n_req independent requests, dims per axis from
+    {32,48,64,96,128}, 1-8 spheres (radii 0.05-0.2), slab, face velocities
+    U[-1,1] (request seeds seed*100000 + i). Runs: saturation (all submitted
+    at t=0) with `slots` slots; Poisson arrivals at 0.8x that saturation rate;
+    the plain FP64 drop-in one request at a time (1 slot) at saturation."""
+    t = time.perf_counter()
+    scenes = [elq.scene_request(seed * 100000 + i) for i in range(n_req)]
+    cells = int(sum(sc.cells for sc in scenes))
+    print(f"# serving: {n_req} requests generated in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+    o_mixed = elq.default_opts(precision=elq.MIXED, **base)
+    o_fp64 = elq.default_opts(precision=elq.FP64, **base)
+    serve_run(scenes[:min(8, n_req)], [0.0] * min(8, n_req), slots, o_mixed)  # warm-up
+    sat = serve_summary(serve_run(scenes, [0.0] * n_req, slots, o_mixed))
+    lam = 0.8 * sat["solves_per_s"]
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req)).tolist()
+    poisson = serve_summary(serve_run(scenes, arrivals, slots, o_mixed))
+    print(f"# serving: saturation {sat}, poisson {poisson}", file=sys.stderr, flush=True)
+    seq64 = serve_summary(serve_run(scenes, [0.0] * n_req, 1, o_fp64))
+    print(f"# serving: fp64 one slot {seq64}", file=sys.stderr, flush=True)
+    return {"workload": f"{n_req} requests, dims per axis from (32,48,64,96,128), {cells} cells in all, "
+                        f"mixed precision to rel.res 1e-8, {slots} slots",
+            "poisson_rate_per_s": round(lam, 2), "poisson": poisson, "saturation": sat,
+            "fp64_drop_in_one_slot_saturation": seq64,
+            "speedup_vs_fp64_one_slot": round(sat["solves_per_s"] / seq64["solves_per_s"], 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +236,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 drop-in (profiling runs)")
+    ap.add_argument("--serve", type=int, default=512, metavar="N",
+                    help="serving stream of N requests after the single-solve benchmark (0: skip)")
+    ap.add_argument("--slots", type=int, default=4, help="serving slots")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel on MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -257,6 +324,8 @@ def main():
         line["max_rel_p_minus_p_fp64"] = float(np.abs(p_mixed - p_fp64).max()) / scale if scale else 0.0
     ds.close()
     ctx.close()
+    if a.serve > 0:
+        line["serving"] = serving(a.serve, a.slots, a.seed, base)
     if not a.no_cpu:
         fp64_iters = out["fp64"][1][-1]["iterations"] if "fp64" in out else iters
         line["cpu_baseline"] = cpu_baseline(scene, fp64_iters)

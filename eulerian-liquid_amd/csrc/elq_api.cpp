@@ -141,22 +141,45 @@ bool valid_dims(int ni, int nj, int nk) {
 
 }  // namespace
 
-size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged) {
+size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, int64_t nb_bound) {
+    // Mirrors the allocations of a projection on a fresh context (bytes
+    // requested from the allocator). nb_bound bounds the bricks that hold
+    // unknowns (< 0: every brick of the grid).
     const Plan p = make_plan(ni, nj, nk);
-    // unknown bricks are bounded by all bricks of the grid interior
-    const int64_t nbmax = p.NB;
-    const size_t cells = (size_t)nbmax * kBrickCells;
+    const int64_t nb = nb_bound < 0 ? p.NB : std::min<int64_t>(nb_bound, p.NB);
+    const size_t cells = (size_t)nb * kBrickCells;
     const size_t tsz = precision == ELQ_PRECISION_MIXED ? 4 : 8;
+    const size_t C = (size_t)p.C, NB = (size_t)p.NB, NP = (size_t)p.nby * p.nbz;
+    const size_t G = (size_t)scan_blocks(std::max<int64_t>(p.C, p.NB));
+    const size_t pnlev = (size_t)p.nby + p.nbz - 1;
     size_t b = 0;
-    b += (size_t)p.C * (1 + 4 + 4);                  // mask, indices, pressure
-    b += (size_t)p.NB * (1 + 4 + 4);                 // bflag, map, coord
-    b += (size_t)nbmax * (6 * 4 + 4 + 4 + 3 * 8);    // nbr, lvl_list, flags, partials
-    b += cells * (8 + 8 + 8);                        // diag64, b64, inv_e64
-    b += cells * tsz * (4 + 6);                      // coefficients + vectors
-    if (precision == ELQ_PRECISION_MIXED) b += cells * (4 + 8);  // inv_e32, x64
-    if (staged) b += sizeof(float) * ((size_t)p.C + 2 * ((size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk +
-                                                          (size_t)ni * nj * (nk + 1)));
+    b += C * (1 + 4 + 4);                                        // mask, indices, pressure
+    b += NB * (1 + 4 + 4) + 2 * G * 4 + 3 * ((size_t)p.nlev + 1) * 4 + 64;  // bricks, scans, levels, info
+    b += NP * (1 + 4 + 4 + 4) + 3 * (pnlev + 1) * 4;             // pencils
+    b += (size_t)nb * (8 * 4 + 4 + (size_t)kXchWords * 8 + 2 * 8);  // nbr, lvl_list, xch, partials
+    b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
+    b += cells * (3 * 8 + 10 * tsz + 4);                         // diag64, b64, inv_e64, coefficients, vectors, inv_e32
+    if (precision == ELQ_PRECISION_MIXED) b += cells * 8;        // x64
+    if (staged) b += 4 * (C + 2 * ((size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + (size_t)ni * nj * (nk + 1)));
     return b;
+}
+
+int64_t liquid_bricks(const float* phi, int ni, int nj, int nk) {
+    const Plan p = make_plan(ni, nj, nk);
+    std::vector<uint8_t> hit((size_t)p.NB, 0);
+    int64_t n = 0;
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j) {
+            const float* row = phi + (size_t)ni * ((size_t)j + (size_t)nj * k);
+            const size_t base = (size_t)p.nbx * ((size_t)(j / kBrick) + (size_t)p.nby * (k / kBrick));
+            for (int i = 0; i < ni; ++i)
+                if (row[i] < 0.0f) {
+                    uint8_t& h = hit[base + (size_t)(i / kBrick)];
+                    n += h == 0;
+                    h = 1;
+                }
+        }
+    return n;
 }
 
 namespace {
@@ -367,14 +390,15 @@ struct Solver {
         ctx->prof_begin(update ? "fwd_sweep_update" : "fwd_sweep");
         if (!levels) {
             HIPCHK(launch_fwd<T>(bk, pc, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
-                                 (uint64_t*)W.xch.p, stage(), update, -1, 0, 0, ctx->timeout_ticks, ctx->stream),
+                                 (uint64_t*)W.xch.p, stage(), update, -1, 0, 0, ctx->timeout_ticks,
+                                 ctx->max_groups, ctx->stream),
                    "k_fwd");
         } else {
             for (int l = 0; l < p.nlev; ++l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
                 HIPCHK(launch_fwd<T>(bk, pc, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
                                      (uint64_t*)W.xch.p, stage(), update, l, base, cnt, ctx->timeout_ticks,
-                                     ctx->stream),
+                                     ctx->max_groups, ctx->stream),
                        "k_fwd");
             }
         }
@@ -390,13 +414,13 @@ struct Solver {
         ctx->prof_begin("bwd_sweep");
         if (!levels) {
             HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(), mode,
-                                 -1, 0, 0, ctx->timeout_ticks, ctx->stream),
+                                 -1, 0, 0, ctx->timeout_ticks, ctx->max_groups, ctx->stream),
                    "k_bwd");
         } else {
             for (int l = p.nlev - 1; l >= 0; --l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
                 HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(),
-                                     mode, l, base, cnt, ctx->timeout_ticks, ctx->stream),
+                                     mode, l, base, cnt, ctx->timeout_ticks, ctx->max_groups, ctx->stream),
                        "k_bwd");
             }
         }
@@ -413,14 +437,15 @@ struct Solver {
         if (!levels) {
             HIPCHK(launch_factor<T>(bk, (const double*)W.diag64.p, c.ax, c.ay, c.az, tau, sigma, (double*)W.inv_e64.p,
                                     inv_e32, ctrl(), (uint64_t*)W.xch.p, (int32_t*)W.nfixed.p, -1, 0, 0,
-                                    ctx->timeout_ticks, ctx->stream),
+                                    ctx->timeout_ticks, ctx->max_groups, ctx->stream),
                    "k_factor");
         } else {
             for (int l = 0; l < p.nlev; ++l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
                 HIPCHK(launch_factor<T>(bk, (const double*)W.diag64.p, c.ax, c.ay, c.az, tau, sigma,
                                         (double*)W.inv_e64.p, inv_e32, ctrl(), (uint64_t*)W.xch.p,
-                                        (int32_t*)W.nfixed.p, l, base, cnt, ctx->timeout_ticks, ctx->stream),
+                                        (int32_t*)W.nfixed.p, l, base, cnt, ctx->timeout_ticks, ctx->max_groups,
+                                        ctx->stream),
                        "k_factor");
             }
         }
@@ -445,7 +470,7 @@ struct Solver {
         for (int q = 0; q < n; ++q, ++kiter) {
             ctx->prof_begin("spmv");
             HIPCHK(launch_spmv<T>(bk, c, vec(W.vy), P[kiter & 1], P[(kiter + 1) & 1], vec(W.vq), ctrl(), stage(),
-                                  ctx->timeout_ticks, ctx->stream),
+                                  ctx->timeout_ticks, ctx->max_groups, ctx->stream),
                    "k_spmv");
             ctx->prof_end();
             if ((rc = fwd(1, P[(kiter + 1) & 1]))) return rc;
@@ -701,7 +726,7 @@ int run_project(elq_ctx* ctx, const elq_grid& dg, float dt, const elq_opts& o, e
     if (!(dt > 0.0f) || !std::isfinite(dt)) return ctx->fail(ELQ_ERR_INVALID_ARG, "dt must be positive");
     if (!check_opts(o)) return ctx->fail(ELQ_ERR_INVALID_ARG, "invalid solver options");
     if (ctx->budget) {
-        const size_t need = elq_footprint_bytes(dg.ni, dg.nj, dg.nk, o.precision, false);
+        const size_t need = elq_footprint_bytes(dg.ni, dg.nj, dg.nk, o.precision, false, -1);
         if (need > ctx->budget) return ctx->fail(ELQ_ERR_ADMISSION, "projection exceeds the context's HBM budget");
     }
     const Dense d = dense_of(dg);
@@ -762,6 +787,14 @@ void elq_ctx_destroy(elq_ctx* ctx) {
 }
 
 const char* elq_ctx_last_error(const elq_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+size_t elq_ctx_device_bytes(const elq_ctx* ctx) { return ctx ? ctx->ws.bytes() : 0; }
+
+size_t elq_projection_footprint(int32_t ni, int32_t nj, int32_t nk, int32_t precision, const float* level_set) {
+    if (!valid_dims(ni, nj, nk)) return 0;
+    const int64_t nb = level_set ? liquid_bricks(level_set, ni, nj, nk) : -1;
+    return elq_footprint_bytes(ni, nj, nk, precision, true, nb);
+}
 
 int elq_project_device(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) {
     int rc = check_grid(ctx, g);

@@ -42,6 +42,18 @@ struct Workspace {
     int32_t* cur_indices = nullptr;
     std::vector<int32_t> lvl_host;
 
+    size_t bytes() const {
+        size_t n = 0;
+        for (const DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt,
+                                &lvl_off, &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &pcoord,
+                                &plvl_cnt, &plvl_off, &plvl_fill, &plvl_list, &diag64, &b64, &inv_e64, &inv_e32,
+                                &cdiag, &cax, &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
+                                &ctrl, &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure,
+                                &s_indices, &export_buf})
+            n += b->cap;
+        return n;
+    }
+
     void release_all() {
         for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
                           &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &pcoord, &plvl_cnt,
@@ -58,6 +70,7 @@ struct elq_ctx {
     size_t budget = 0;
     hipStream_t stream = nullptr;
     uint64_t timeout_ticks = 0;
+    int max_groups = 0;  // cap on the grid of persistent kernels (0: whole device)
     std::string last_error;
     Workspace ws;
 
@@ -77,4 +90,9 @@ struct elq_ctx {
     int hip_fail(hipError_t e, const char* what);
 };
 
-size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged);
+// Device bytes a projection requests on a fresh context (nb_bound: bricks
+// holding unknowns, < 0 for all bricks; staged: host inputs are copied in).
+size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, int64_t nb_bound);
+// Bricks of the grid holding at least one cell with phi < 0 (a bound on the
+// bricks that hold unknowns).
+int64_t liquid_bricks(const float* phi, int ni, int nj, int nk);

@@ -1582,6 +1582,8 @@ constexpr int kPencilBlocksPerCU = 4;  // pencil sweeps: up to 4 workgroups (16 
 constexpr int kSpmvBlocksPerCU = 4;    // persistent SpMV: up to 4 workgroups (16 waves) per CU
 
 int device_cus() {
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
     static int cus[64] = {0};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1;
@@ -1611,7 +1613,7 @@ int occupancy(const void* kern) {
 // If another workload keeps some workgroups from starting, the bounded
 // waits end at the time limit and the launch reports the abort.
 template <typename... KArgs, typename... Args, size_t... I>
-hipError_t persistent_impl(void (*kern)(KArgs...), int items, int blocks_per_cu, hipStream_t s,
+hipError_t persistent_impl(void (*kern)(KArgs...), int items, int blocks_per_cu, int max_groups, hipStream_t s,
                            std::index_sequence<I...>, Args&&... args) {
     std::tuple<KArgs...> t(std::forward<Args>(args)...);
     void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
@@ -1620,15 +1622,18 @@ hipError_t persistent_impl(void (*kern)(KArgs...), int items, int blocks_per_cu,
     int g = (items + 3) / 4;
     if (g > per_cu * device_cus()) g = per_cu * device_cus();
     if (g > kTailMaxGroups) g = kTailMaxGroups;
+    if (max_groups > 0 && g > max_groups) g = max_groups;
     if (g < 1) g = 1;
     return hipLaunchKernel((const void*)kern, dim3(g), dim3(256), ptrs, 0, s);
 }
 
-// items: work items (one per wave, four waves per 256-thread workgroup).
+// items: work items (one per wave, four waves per 256-thread workgroup);
+// max_groups > 0 caps the grid (a share of the device for concurrent solves).
 template <typename... KArgs, typename... Args>
-hipError_t persistent(void (*kern)(KArgs...), int items, int blocks_per_cu, hipStream_t s, Args&&... args) {
+hipError_t persistent(void (*kern)(KArgs...), int items, int blocks_per_cu, int max_groups, hipStream_t s,
+                      Args&&... args) {
     static_assert(sizeof...(KArgs) == sizeof...(Args), "argument count");
-    return persistent_impl(kern, items, blocks_per_cu, s, std::index_sequence_for<KArgs...>{},
+    return persistent_impl(kern, items, blocks_per_cu, max_groups, s, std::index_sequence_for<KArgs...>{},
                            std::forward<Args>(args)...);
 }
 
@@ -1693,10 +1698,11 @@ hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const 
 template <class T>
 hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, const T* ay, const T* az, double tau,
                          double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl, uint64_t* xch, int32_t* n_fixed,
-                         int level, int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
+                         int level, int level_base, int level_count, uint64_t timeout_ticks, int max_groups,
+                         hipStream_t s) {
     if (level < 0) {
         if (bk.nb == 0) return hipSuccess;
-        return persistent(k_factor_wave<T>, bk.nb, kWaveBlocksPerCU, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
+        return persistent(k_factor_wave<T>, bk.nb, kWaveBlocksPerCU, max_groups, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
                     n_fixed, timeout_ticks);
     }
     if (level_count > 0)
@@ -1737,19 +1743,19 @@ hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const
 
 template <class T>
 hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q, Ctrl* ctrl,
-                       uint64_t* stage, uint64_t timeout_ticks, hipStream_t s) {
-    return persistent(k_spmv<T>, bk.nb, kSpmvBlocksPerCU, s, bk, c, z, p_in, p_out, q, ctrl, stage, timeout_ticks);
+                       uint64_t* stage, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
+    return persistent(k_spmv<T>, bk.nb, kSpmvBlocksPerCU, max_groups, s, bk, c, z, p_in, p_out, q, ctrl, stage, timeout_ticks);
 }
 
 template <class T>
 hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
                       double* part, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int update, int level,
-                      int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s) {
+                      int level_base, int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
     if (level < 0) {
         if (update)
-            return persistent(k_fwd_pencil<T, true>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, ctrl, xch,
+            return persistent(k_fwd_pencil<T, true>, pc.np, kPencilBlocksPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch,
                         stage, timeout_ticks);
-        return persistent(k_fwd_pencil<T, false>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
+        return persistent(k_fwd_pencil<T, false>, pc.np, kPencilBlocksPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
                     timeout_ticks);
     }
     if (level_count > 0) {
@@ -1766,9 +1772,9 @@ hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* 
 template <class T>
 hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part,
                       Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode, int level, int level_base,
-                      int level_count, uint64_t timeout_ticks, hipStream_t s) {
+                      int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
     if (level < 0)
-        return persistent(k_bwd_pencil<T>, pc.np, kPencilBlocksPerCU, s, bk, pc, c, r, y, ctrl, xch, stage, tail_mode,
+        return persistent(k_bwd_pencil<T>, pc.np, kPencilBlocksPerCU, max_groups, s, bk, pc, c, r, y, ctrl, xch, stage, tail_mode,
                     timeout_ticks);
     if (level_count > 0)
         hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
@@ -1855,19 +1861,19 @@ hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStr
     template hipError_t launch_assemble<T>(const Dense&, float, const uint8_t*, const Bricks&, double*, double*, T*, \
                                            T*, T*, T*, hipStream_t);                                                \
     template hipError_t launch_factor<T>(const Bricks&, const double*, const T*, const T*, const T*, double, double, \
-                                         double*, float*, Ctrl*, uint64_t*, int32_t*, int, int, int, uint64_t,      \
+                                         double*, float*, Ctrl*, uint64_t*, int32_t*, int, int, int, uint64_t, int, \
                                          hipStream_t);                                                              \
     template hipError_t launch_export<T>(const Dense&, const uint8_t*, const int32_t*, const Bricks&, const double*, \
                                          const double*, const T*, const T*, const T*, double*, double*, double*,    \
                                          double*, double*, hipStream_t);                                            \
     template hipError_t launch_init<T>(const Bricks&, const double*, T*, T*, T*, T*, double*, Ctrl*, hipStream_t);  \
     template hipError_t launch_spmv<T>(const Bricks&, const Coef<T>&, const T*, const T*, T*, T*, Ctrl*, uint64_t*, \
-                                       uint64_t, hipStream_t);                                                      \
+                                       uint64_t, int, hipStream_t);                                                 \
     template hipError_t launch_fwd<T>(const Bricks&, const Pencils&, const Coef<T>&, T*, T*, const T*, const T*, T*, \
-                                      double*, Ctrl*, uint64_t*, uint64_t*, int, int, int, int, uint64_t,           \
+                                      double*, Ctrl*, uint64_t*, uint64_t*, int, int, int, int, uint64_t, int,      \
                                       hipStream_t);                                                                 \
     template hipError_t launch_bwd<T>(const Bricks&, const Pencils&, const Coef<T>&, const T*, T*, double*, Ctrl*,   \
-                                      uint64_t*, uint64_t*, int, int, int, int, uint64_t, hipStream_t);             \
+                                      uint64_t*, uint64_t*, int, int, int, int, uint64_t, int, hipStream_t);        \
     template hipError_t launch_resid64<T>(const Bricks&, const double*, const T*, const T*, const T*, const double*, \
                                           const double*, double*, double*, float*, float*, float*, float*, Ctrl*,   \
                                           hipStream_t);

@@ -86,7 +86,7 @@ template <class T>
 hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, const T* ay, const T* az,
                          double tau, double sigma, double* inv_e64, float* inv_e32, Ctrl* ctrl,
                          uint64_t* xch, int32_t* n_fixed, int level, int level_base, int level_count,
-                         uint64_t timeout_ticks, hipStream_t s);
+                         uint64_t timeout_ticks, int max_groups, hipStream_t s);
 // Natural-order export of the assembled system (elq_build_matrix).
 template <class T>
 hipError_t launch_export(const Dense& d, const uint8_t* mask, const int32_t* indices, const Bricks& bk,
@@ -101,9 +101,11 @@ hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T
 hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2, uint64_t* stage,
                          uint64_t timeout_ticks, hipStream_t s);
 template <class T>
+// Persistent launches (factor wave, SpMV, pencil sweeps) take max_groups:
+// 0 = as many workgroups as the device holds, > 0 = at most that many.
 // Persistent launch; its grid tail sets alpha (SC_ALPHA).
 hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q, Ctrl* ctrl,
-                       uint64_t* stage, uint64_t timeout_ticks, hipStream_t s);
+                       uint64_t* stage, uint64_t timeout_ticks, int max_groups, hipStream_t s);
 // Forward sweep y = (E+L)^{-1} r, fused with x += alpha p, r -= alpha q and
 // |r|^2 partials when update != 0. level < 0: one persistent launch over the
 // pencils (wavefront); else the bricks of one brick level.
@@ -112,14 +114,14 @@ template <class T>
 // the level launches write per-brick partials to part instead.
 hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
                       double* part, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int update, int level,
-                      int level_base, int level_count, uint64_t timeout_ticks, hipStream_t s);
+                      int level_base, int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s);
 // Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
 template <class T>
 // The pencil launch ends in a grid tail applying tail_mode; the level
 // launches write per-brick partials to part instead.
 hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
                       uint64_t* xch, uint64_t* stage, int tail_mode, int level, int level_base, int level_count,
-                      uint64_t timeout_ticks, hipStream_t s);
+                      uint64_t timeout_ticks, int max_groups, hipStream_t s);
 // r64 = b64 - A x64 in FP64 (couplings TC read exactly); partials |r64|^2 and,
 // when r32 != nullptr, |float(r64)|^2 with r32 = float(r64); x32 and p0/p1 are
 // zeroed when non-null.

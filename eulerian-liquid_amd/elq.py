@@ -64,8 +64,12 @@ class ServerOpts(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("id", C.c_int64), ("status", C.c_int32), ("iterations", C.c_int32),
-                ("outer_iterations", C.c_int32), ("error", C.c_double), ("t_submit_ms", C.c_double),
+                ("outer_iterations", C.c_int32), ("slot", C.c_int32), ("n_unknowns", C.c_int64),
+                ("error", C.c_double), ("true_rel_residual", C.c_double), ("t_submit_ms", C.c_double),
                 ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _lib = None
@@ -106,10 +110,15 @@ def load():
     L.elq_server_destroy.argtypes = [vp]
     L.elq_server_destroy.restype = None
     L.elq_server_submit.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(i64)]
-    L.elq_server_step.argtypes = [vp]
+    L.elq_server_wait.argtypes = [vp, C.c_int]
     L.elq_server_poll.argtypes = [vp, C.POINTER(Result), C.c_int]
-    L.elq_server_footprint.argtypes = [vp, i32, i32, i32]
-    L.elq_server_footprint.restype = sz
+    L.elq_server_in_flight.argtypes = [vp]
+    L.elq_server_device_bytes.argtypes = [vp]
+    L.elq_server_device_bytes.restype = sz
+    L.elq_projection_footprint.argtypes = [i32, i32, i32, i32, vp]
+    L.elq_projection_footprint.restype = sz
+    L.elq_ctx_device_bytes.argtypes = [vp]
+    L.elq_ctx_device_bytes.restype = sz
     L.elq_scene_level_set.argtypes = [i32, i32, i32, f32, f32, i32, vp, vp]
     L.elq_scene_weights.argtypes = [i32, i32, i32, vp, vp, vp]
     L.elq_scene_add_gravity.argtypes = [i32, i32, i32, vp, f32]
@@ -314,6 +323,10 @@ class Context:
     def last_error(self) -> str:
         return self.L.elq_ctx_last_error(self.h).decode()
 
+    def device_bytes(self) -> int:
+        """Device bytes the context's workspace holds."""
+        return int(self.L.elq_ctx_device_bytes(self.h))
+
     # ---- projection
     def solve_pressure(self, s: Scene, opts: Opts | None = None, raise_on_error=True):
         """Projects s in place (u, v, w updated); returns (pressure, indices, stats)."""
@@ -423,6 +436,82 @@ class DeviceScene:
         for p in self.ptr.values():
             self.ctx.free(p)
         self.ptr = {}
+
+
+def footprint(s: Scene, precision: int = FP64, bound_by_level_set: bool = True) -> int:
+    """Device bytes a projection of s requests on a fresh context (upper bound)."""
+    phi = _ptr(s.phi) if bound_by_level_set else None
+    return int(load().elq_projection_footprint(s.ni, s.nj, s.nk, precision, phi))
+
+
+class Server:
+    """Continuous batching of projections (elq_server_*). Each submitted scene
+    is projected in place (u, v, w) and its pressure returned at poll time; the
+    scene must not be touched until then."""
+
+    def __init__(self, device: int = 0, max_slots: int = 4, max_queue: int = 4096, hbm_budget: int = 0,
+                 opts: Opts | None = None):
+        self.L = load()
+        so = ServerOpts()
+        so.hbm_budget_bytes = hbm_budget
+        so.max_slots = max_slots
+        so.max_queue = max_queue
+        so.opts = opts if opts is not None else default_opts()
+        h = C.c_void_p()
+        rc = self.L.elq_server_create(device, C.byref(so), C.byref(h))
+        if rc != OK:
+            raise ElqError(rc, f"elq_server_create: {STATUS_NAMES.get(rc, rc)}")
+        self.h = h
+        self.live = {}
+
+    def submit(self, s: Scene) -> int:
+        p = np.zeros(s.cells, np.float32)
+        g = s.grid(p, None)
+        rid = C.c_int64()
+        rc = self.L.elq_server_submit(self.h, C.byref(g), np.float32(s.dt), C.byref(rid))
+        if rc != OK:
+            raise ElqError(rc, f"elq_server_submit: {STATUS_NAMES.get(rc, rc)}")
+        self.live[rid.value] = (s, p, g)
+        return rid.value
+
+    def wait(self, timeout_ms: int = -1) -> int:
+        return int(self.L.elq_server_wait(self.h, int(timeout_ms)))
+
+    def poll(self, cap: int = 256) -> list:
+        """Finished requests: dicts with the result fields plus scene and pressure."""
+        buf = (Result * cap)()
+        n = self.L.elq_server_poll(self.h, buf, cap)
+        out = []
+        for q in range(n):
+            d = buf[q].as_dict()
+            s, p, _ = self.live.pop(d["id"])
+            d["scene"], d["pressure"] = s, p
+            out.append(d)
+        return out
+
+    def in_flight(self) -> int:
+        return int(self.L.elq_server_in_flight(self.h))
+
+    def device_bytes(self) -> int:
+        return int(self.L.elq_server_device_bytes(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            self.L.elq_server_destroy(self.h)
+            self.h = None
+            self.live = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def solve_pressure(s: Scene, opts: Opts | None = None, ctx: Context | None = None):

@@ -176,17 +176,22 @@ void elq_set_profiling_filter(elq_ctx* ctx, const char* classes);
 
 /* ------------------------------------------------------------------ serving
  * Continuous batching of independent projections (uneven grid sizes) on one
- * device. Requests are admitted into slots while their HBM footprint fits the
- * budget; all admitted requests advance together, one batched PCG iteration
- * per step, on device-resident grids and Krylov vectors. A request's result
- * does not depend on which other requests share the batch.
+ * device; no reference counterpart (the reference solves one grid per frame).
+ * A server owns max_slots solver slots, each with its own stream and
+ * device-resident workspace (grids, system, factor, Krylov vectors) that
+ * persists across requests. Requests wait in a FIFO queue and are admitted
+ * as soon as a slot is free and the device memory held by all slots, with
+ * this request's footprint, fits hbm_budget_bytes (a request that does not
+ * fit lets later, smaller ones pass at most 8 times). Admitted requests run
+ * concurrently; each gets the same fixed share of the device's workgroup
+ * slots, so its result does not depend on which requests share the device.
  */
 typedef struct elq_server elq_server;
 
 typedef struct {
-    size_t hbm_budget_bytes; /* admission limit for resident slots */
-    int32_t max_slots;       /* concurrently resident requests */
-    int32_t max_queue;       /* pending requests waiting for admission */
+    size_t hbm_budget_bytes; /* device bytes the slots may hold (0: 90% of free memory) */
+    int32_t max_slots;       /* concurrently running requests, 1..16 */
+    int32_t max_queue;       /* waiting requests; submit returns ELQ_ERR_BUSY beyond */
     elq_opts opts;           /* solver options for every request */
 } elq_server_opts;
 
@@ -195,25 +200,39 @@ typedef struct {
     int32_t status;           /* elq_status of the solve */
     int32_t iterations;
     int32_t outer_iterations;
+    int32_t slot;             /* slot that ran it */
+    int64_t n_unknowns;
     double error;
-    double t_submit_ms;       /* host clock, ms since server start */
+    double true_rel_residual;
+    double t_submit_ms;       /* host clock, ms since server creation */
     double t_admit_ms;
-    double t_done_ms;
+    double t_done_ms;         /* outputs written back to the request's arrays */
 } elq_result;
 
 int elq_server_create(int device, const elq_server_opts* opts, elq_server** out);
+/* Waits for running requests, drops queued ones, frees every slot. */
 void elq_server_destroy(elq_server* srv);
-/* Submits a projection on host grids; the grid arrays must stay valid until
- * the result is polled (u/v/w/pressure are written back at completion).
- * Returns ELQ_ERR_ADMISSION if the request alone can never fit the budget. */
+/* Queues a projection on host grids (elq_project semantics). The grid arrays
+ * must stay valid and untouched until its result has been polled. Returns
+ * ELQ_ERR_ADMISSION if the request alone exceeds the budget, ELQ_ERR_BUSY if
+ * the queue is full. */
 int elq_server_submit(elq_server* srv, const elq_grid* host_grid, float dt, int64_t* id);
-/* Advances the batch by one step (admission, one batched iteration,
- * completion). Returns the number of requests in flight after the step. */
-int elq_server_step(elq_server* srv);
-/* Pops up to cap finished results. */
+/* Blocks until at least one result can be polled or timeout_ms passes
+ * (< 0: no limit). Returns the number of results ready. */
+int elq_server_wait(elq_server* srv, int timeout_ms);
+/* Pops up to cap finished results (completion order). */
 int elq_server_poll(elq_server* srv, elq_result* out, int cap);
-/* Device bytes a request of this size occupies while resident. */
-size_t elq_server_footprint(const elq_server* srv, int32_t ni, int32_t nj, int32_t nk);
+/* Requests queued or running. */
+int elq_server_in_flight(const elq_server* srv);
+/* Device bytes the slots hold now. */
+size_t elq_server_device_bytes(const elq_server* srv);
+
+/* Device bytes a projection requests from the allocator on a fresh context,
+ * host inputs staged (an upper bound; level_set NULL counts every brick,
+ * otherwise bricks holding a cell with phi < 0). 0 for invalid sizes. */
+size_t elq_projection_footprint(int32_t ni, int32_t nj, int32_t nk, int32_t precision, const float* level_set);
+/* Device bytes a context's workspace holds now. */
+size_t elq_ctx_device_bytes(const elq_ctx* ctx);
 
 /* ------------------------------------------------------------------ scenes
  * Host-side input generation for the synthetic workloads (the simulator's own

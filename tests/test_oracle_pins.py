@@ -296,3 +296,84 @@ def test_update_skips_boundary_tangential_rows():
     assert not du[0].any() and not du[7].any()               # tangential k rows 0 and nk-1
     assert not du[:, 0].any() and not du[:, 7].any()         # tangential j rows 0 and nj-1
     assert du[1:7, 1:7, 1:8].all()
+
+
+# ----------------------------------------------------------------- scene inputs
+def test_splitmix64_known_answers():
+    # published test vector of SplitMix64 (Steele, Lea, Flood 2014; Vigna's
+    # reference implementation), seed 1234567: a wrong constant or shift fails
+    import ctypes as C
+    st = C.c_uint64(1234567)
+    got = [O.lib().orc_splitmix64(C.byref(st)) for _ in range(5)]
+    assert got == [6457827717110365317, 3203168211198807973, 9817491932198370423, 4593380528125082431,
+                   16408922859458223821]
+    st = C.c_uint64(1234567)
+    u = O.lib().orc_uniform01(C.byref(st))
+    assert u == (6457827717110365317 >> 11) * 2.0 ** -53    # top 53 bits, [0, 1)
+
+
+def test_add_gravity_uses_reference_constant_on_every_v_face():
+    # simulator.cpp:90-93 with constants.h:5-6 (9.80665): all v faces, u and w untouched
+    s = O.make_scene(6, 5, 4, 0.5, [])
+    O.add_gravity(s, 0.01)
+    assert np.all(s.v == np.float32(0.0) - np.float32(9.80665) * np.float32(0.01))
+    assert not s.u.any() and not s.w.any()
+
+
+def _ones_scene(ni, nj, nk, phi_value):
+    s = O.make_scene(ni, nj, nk, 0.5, [])
+    s.phi[:] = phi_value
+    for a in (s.uw, s.vw, s.ww, s.u, s.v, s.w):
+        a[:] = 1.0
+    return s
+
+
+def test_valid_velocity_special_cases():
+    # simulator.cpp:125-178. All liquid, all weights 1: nothing is zeroed.
+    s = _ones_scene(6, 5, 4, -1.0)
+    O.find_valid_and_zero(s)
+    assert s.u.all() and s.v.all() and s.w.all()
+    # weight 0 alone zeroes a face
+    s = _ones_scene(6, 5, 4, -1.0)
+    s.uw[7] = 0.0
+    O.find_valid_and_zero(s)
+    assert s.u[7] == 0.0 and np.count_nonzero(s.u == 0.0) == 1
+    # no liquid: exactly the faces inside the loop ranges are zeroed; ranges
+    # are [1, n_face - 1) per axis, so the last index of each axis survives
+    ni, nj, nk = 6, 5, 4
+    s = _ones_scene(ni, nj, nk, 1.0)
+    O.find_valid_and_zero(s)
+    u = s.u.reshape(nk, nj, ni + 1)
+    keep = np.ones_like(u, bool)
+    keep[1:nk - 1, 1:nj - 1, 1:ni] = False
+    assert np.array_equal(u != 0.0, keep)
+    v = s.v.reshape(nk, nj + 1, ni)
+    keep = np.ones_like(v, bool)
+    keep[1:nk - 1, 1:nj, 1:ni - 1] = False
+    assert np.array_equal(v != 0.0, keep)
+    w = s.w.reshape(nk + 1, nj, ni)
+    keep = np.ones_like(w, bool)
+    keep[1:nk, 1:nj - 1, 1:ni - 1] = False
+    assert np.array_equal(w != 0.0, keep)
+    # one liquid cell: its six faces survive besides the untouched rows
+    s = _ones_scene(ni, nj, nk, 1.0)
+    c = (2, 2, 1)  # i, j, k
+    s.phi[c[0] + ni * (c[1] + nj * c[2])] = -1.0
+    O.find_valid_and_zero(s)
+    u = s.u.reshape(nk, nj, ni + 1)
+    assert u[1, 2, 2] == 1.0 and u[1, 2, 3] == 1.0 and u[1, 2, 4] == 0.0
+    v = s.v.reshape(nk, nj + 1, ni)
+    assert v[1, 2, 2] == 1.0 and v[1, 3, 2] == 1.0 and v[1, 1, 2] == 0.0
+    w = s.w.reshape(nk + 1, nj, ni)
+    assert w[1, 2, 2] == 1.0 and w[2, 2, 2] == 1.0 and w[3, 2, 2] == 0.0
+
+
+def test_build_grid_from_vector_scatter():
+    # projection.cpp:85-91: pressure(c) = (float)x[idx(c)], background 0
+    s = O.scene_drop(12)
+    n, idx = O.assign_indices(s)
+    x = np.arange(n, dtype=np.float64) * (1.0 + 2.0 ** -40) + 0.25
+    p = O.build_grid_from_vector(s, x, idx)
+    fluid = idx != O.UNKNOWN_INDEX
+    assert np.array_equal(p[fluid], x[idx[fluid]].astype(np.float32))
+    assert not p[~fluid].any()

@@ -128,6 +128,11 @@ Plan make_plan(int ni, int nj, int nk) {
     return p;
 }
 
+// Pencil blocks of the wavefront sweeps (upper bound of the block list).
+size_t block_count(const Plan& p) {
+    return (size_t)((p.nby + kBlockY - 1) / kBlockY) * (size_t)((p.nbz + kBlockZ - 1) / kBlockZ);
+}
+
 bool valid_dims(int ni, int nj, int nk) {
     return ni >= 1 && nj >= 1 && nk >= 1 && ni <= 4092 && nj <= 4092 && nk <= 4092 &&
            (int64_t)(ni + 1) * (nj + 1) * (nk + 1) < (int64_t)INT32_MAX;
@@ -151,16 +156,17 @@ size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, i
     const size_t tsz = precision == ELQ_PRECISION_MIXED ? 4 : 8;
     const size_t C = (size_t)p.C, NB = (size_t)p.NB, NP = (size_t)p.nby * p.nbz;
     const size_t G = (size_t)scan_blocks(std::max<int64_t>(p.C, p.NB));
-    const size_t pnlev = (size_t)p.nby + p.nbz - 1;
     size_t b = 0;
     b += C * (1 + 4 + 4);                                        // mask, indices, pressure
     b += NB * (1 + 4 + 4) + 2 * G * 4 + 3 * ((size_t)p.nlev + 1) * 4 + 64;  // bricks, scans, levels, info
-    b += NP * (1 + 4 + 4 + 4) + 3 * (pnlev + 1) * 4;             // pencils
+    b += NP * (1 + 4);                                           // pencil flags, map
+    b += block_count(p) * 5;                                     // pencil blocks
     b += (size_t)nb * (8 * 4 + 4 + (size_t)kXchWords * 8 + 2 * 8);  // nbr, lvl_list, xch, partials
     b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
     b += cells * (3 * 8 + 10 * tsz + 4);                         // diag64, b64, inv_e64, coefficients, vectors, inv_e32
     if (precision == ELQ_PRECISION_MIXED) b += cells * 8;        // x64
-    if (staged) b += 4 * (C + 2 * ((size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + (size_t)ni * nj * (nk + 1)));
+    const size_t faces = (size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + (size_t)ni * nj * (nk + 1);
+    if (staged) b += 4 * (C + 2 * faces);
     return b;
 }
 
@@ -199,14 +205,11 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     HIPCHK(W.lvl_off.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
     HIPCHK(W.lvl_fill.ensure((size_t)(p.nlev + 1) * 4), "alloc lvl");
     const int64_t NP = (int64_t)p.nby * p.nbz;
-    const int pnlev = p.nby + p.nbz - 1;
     HIPCHK(W.pflag.ensure((size_t)NP), "alloc pencils");
     HIPCHK(W.pmap.ensure((size_t)NP * 4), "alloc pencils");
-    HIPCHK(W.pcoord.ensure((size_t)NP * 4), "alloc pencils");
-    HIPCHK(W.plvl_list.ensure((size_t)NP * 4), "alloc pencils");
-    HIPCHK(W.plvl_cnt.ensure((size_t)(pnlev + 1) * 4), "alloc pencils");
-    HIPCHK(W.plvl_off.ensure((size_t)(pnlev + 1) * 4), "alloc pencils");
-    HIPCHK(W.plvl_fill.ensure((size_t)(pnlev + 1) * 4), "alloc pencils");
+    const size_t NBLK = block_count(p);
+    HIPCHK(W.bkflag.ensure(NBLK), "alloc blocks");
+    HIPCHK(W.blist.ensure(NBLK * 4), "alloc blocks");
     HIPCHK(W.info.ensure(sizeof(int64_t) * 4 + sizeof(int32_t) * 2), "alloc info");
     int32_t* indices = indices_out;
     if (!indices) {
@@ -219,7 +222,6 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     int32_t* err = (int32_t*)(totals + 4);
     HIPCHK(hipMemsetAsync(W.info.p, 0, W.info.cap, s), "memset");
     HIPCHK(hipMemsetAsync(W.lvl_cnt.p, 0, (size_t)(p.nlev + 1) * 4, s), "memset");
-    HIPCHK(hipMemsetAsync(W.plvl_cnt.p, 0, (size_t)(pnlev + 1) * 4, s), "memset");
     ctx->prof_begin("setup_mask_scan");
     HIPCHK(launch_mask(d, (uint8_t*)W.mask.p, err, s), "k_mask");
     HIPCHK(launch_scan((const uint8_t*)W.mask.p, p.C, kFluid, indices, ELQ_UNKNOWN_INDEX, (int32_t*)W.scan_cnt.p,
@@ -240,12 +242,11 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     HIPCHK(launch_scan((const uint8_t*)W.pflag.p, NP, 1, (int32_t*)W.pmap.p, -1, (int32_t*)W.scan_cnt.p,
                        (int32_t*)W.scan_off.p, &totals[2], s),
            "scan pencils");
-    HIPCHK(launch_pencil_meta((const int32_t*)W.pmap.p, p.nby, p.nbz, (int32_t*)W.pcoord.p, (int32_t*)W.plvl_cnt.p, s),
-           "k_pencil_meta");
-    HIPCHK(launch_level_scan((const int32_t*)W.plvl_cnt.p, pnlev, (int32_t*)W.plvl_off.p, (int32_t*)W.plvl_fill.p, s),
-           "k_level_scan");
+    HIPCHK(launch_block_list((const int32_t*)W.pmap.p, p.nby, p.nbz, (uint8_t*)W.bkflag.p, (int32_t*)W.blist.p,
+                             &totals[3], s),
+           "k_block_list");
     ctx->prof_end();
-    int64_t host_tot[3];
+    int64_t host_tot[4];
     int32_t host_err = 0;
     HIPCHK(hipMemcpyAsync(host_tot, totals, sizeof(host_tot), hipMemcpyDeviceToHost, s), "d2h");
     HIPCHK(hipMemcpyAsync(&host_err, err, sizeof(host_err), hipMemcpyDeviceToHost, s), "d2h");
@@ -253,6 +254,7 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     W.n_unknowns = host_tot[0];
     W.nb = (int)host_tot[1];
     W.np = (int)host_tot[2];
+    W.nblk = (int)host_tot[3];
     W.setup_err = host_err;
     W.plan_ni = d.ni;
     W.plan_nj = d.nj;
@@ -279,9 +281,8 @@ Pencils make_pencils(elq_ctx* ctx, const Plan& p) {
     Workspace& W = ctx->ws;
     Pencils pc;
     pc.np = W.np;
-    pc.nlev = p.nby + p.nbz - 1;
-    pc.coord = (const int32_t*)W.pcoord.p;
-    pc.lvl_list = (const int32_t*)W.plvl_list.p;
+    pc.nblk = W.nblk;
+    pc.blk_list = (const int32_t*)W.blist.p;
     return pc;
 }
 
@@ -322,9 +323,6 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     HIPCHK(launch_brick_lists(bk.map, bk.coord, p.nbx, p.nby, p.nbz, nb, (const int32_t*)W.lvl_off.p,
                               (int32_t*)W.lvl_fill.p, (int32_t*)W.lvl_list.p, (int32_t*)W.nbr.p, s),
            "k_brick_lists");
-    HIPCHK(launch_pencil_lists((const int32_t*)W.pcoord.p, W.np, (const int32_t*)W.plvl_off.p, (int32_t*)W.plvl_fill.p,
-                               (int32_t*)W.plvl_list.p, s),
-           "k_pencil_lists");
     HIPCHK(launch_assemble<T>(d, dt, (const uint8_t*)W.mask.p, bk, (double*)W.diag64.p, (double*)W.b64.p,
                               (T*)W.cdiag.p, (T*)W.cax.p, (T*)W.cay.p, (T*)W.caz.p, s),
            "k_assemble");

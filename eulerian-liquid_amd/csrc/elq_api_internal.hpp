@@ -24,7 +24,7 @@ struct Workspace {
     // scans and brick metadata
     DevBuf scan_cnt, scan_off, bflag, map, coord, lvl_cnt, lvl_off, lvl_fill, info;
     DevBuf nbr, lvl_list, xch, stage;
-    DevBuf pflag, pmap, pcoord, plvl_cnt, plvl_off, plvl_fill, plvl_list;
+    DevBuf pflag, pmap, bkflag, blist;
     // system
     DevBuf diag64, b64, inv_e64, inv_e32, cdiag, cax, cay, caz;
     // Krylov vectors (storage type of the solve)
@@ -37,31 +37,31 @@ struct Workspace {
     int64_t n_unknowns = 0;
     int nb = 0;
     int np = 0;
+    int nblk = 0;
     int setup_err = 0;
     int plan_ni = 0, plan_nj = 0, plan_nk = 0;
     int32_t* cur_indices = nullptr;
     std::vector<int32_t> lvl_host;
 
+    template <class F>
+    void for_each(F&& f) {
+        for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
+                          &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &bkflag, &blist,
+                          &diag64, &b64, &inv_e64, &inv_e32,
+                          &cdiag, &cax, &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
+                          &ctrl, &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
+                          &export_buf})
+            f(*b);
+    }
+
     size_t bytes() const {
         size_t n = 0;
-        for (const DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt,
-                                &lvl_off, &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &pcoord,
-                                &plvl_cnt, &plvl_off, &plvl_fill, &plvl_list, &diag64, &b64, &inv_e64, &inv_e32,
-                                &cdiag, &cax, &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
-                                &ctrl, &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure,
-                                &s_indices, &export_buf})
-            n += b->cap;
+        const_cast<Workspace*>(this)->for_each([&](DevBuf& b) { n += b.cap; });
         return n;
     }
 
     void release_all() {
-        for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
-                          &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &pcoord, &plvl_cnt,
-                          &plvl_off, &plvl_fill, &plvl_list, &diag64, &b64, &inv_e64, &inv_e32, &cdiag, &cax,
-                          &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz, &ctrl,
-                          &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
-                          &export_buf})
-            b->release();
+        for_each([](DevBuf& b) { b.release(); });
     }
 };
 

@@ -50,8 +50,8 @@ __device__ __forceinline__ uint32_t move_u32(uint32_t v) {
     if constexpr (M == Mv::xp) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x101, 0xF, 0xF, true);  // row_shl:1
     if constexpr (M == Mv::ym) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x114, 0xF, 0xF, true);  // row_shr:4
     if constexpr (M == Mv::yp) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x104, 0xF, 0xF, true);  // row_shl:4
-    if constexpr (M == Mv::q3) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xFF, 0xF, 0xF, true);   // quad_perm 3333
-    if constexpr (M == Mv::q0) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x00, 0xF, 0xF, true);   // quad_perm 0000
+    if constexpr (M == Mv::q3) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xFF, 0xF, 0xF, true);  // 3333
+    if constexpr (M == Mv::q0) return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x00, 0xF, 0xF, true);  // 0000
     if constexpr (M == Mv::zm || M == Mv::zp) {
         // rows r0..r3 of v. a = ([r0 r0 r2 r2], [r1 r1 r3 r3]);
         // b = ([r0 r0 r1 r1], [r2 r2 r3 r3]).
@@ -117,7 +117,9 @@ __device__ __forceinline__ void xput_word(uint64_t* w, uint32_t lo, uint32_t e) 
     __hip_atomic_store(w, ((uint64_t)e << 32) | lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void xput(uint64_t* x, size_t s, float v, uint32_t e) { xput_word(x + s, __float_as_uint(v), e); }
+__device__ __forceinline__ void xput(uint64_t* x, size_t s, float v, uint32_t e) {
+    xput_word(x + s, __float_as_uint(v), e);
+}
 
 __device__ __forceinline__ void xput(uint64_t* x, size_t s, double v, uint32_t e) {
     const uint64_t bits = (uint64_t)__double_as_longlong(v);
@@ -776,28 +778,34 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
 }
 
 // Reduction tail of a persistent kernel (workgroup 0 waits only for
-// workgroups dispatched after it, which need nothing from it, so its waits end). Each workgroup adds its four
-// wave sums in wave order and publishes the result as a tagged word pair in
-// stage[2g]; workgroup 0 gathers the G sums (thread t takes g = t, t+256, ...
-// in that order, then a fixed tree) and applies the scalar recurrence. The
-// result depends only on the grid size, not on timing. Every thread of every
-// workgroup must call this; ctrl must not be read after it.
-__device__ void grid_tail(Ctrl* c, int mode, double wave_total, uint32_t tag, uint64_t* stage, uint64_t timeout) {
-    __shared__ double sw[4];
-    __shared__ double st[256];
+// workgroups dispatched after it, which need nothing from it, so its waits
+// end). Each workgroup of NW waves adds its wave sums in wave order and
+// publishes the result as a tagged word pair in stage[2g]; workgroup 0
+// gathers the G sums (thread t takes g = t, t+64*NW, ... in that order, then
+// a fixed tree) and applies the scalar recurrence. The result depends only on
+// the grid size, not on timing. Every thread of every workgroup must call
+// this; ctrl must not be read after it.
+template <int NW>
+__device__ void grid_tail_n(Ctrl* c, int mode, double wave_total, uint32_t tag, uint64_t* stage, uint64_t timeout) {
+    constexpr int NT = 64 * NW;
+    __shared__ double sw[NW];
+    __shared__ double st[NT];
     const int t = threadIdx.x, g = blockIdx.x, G = gridDim.x;
     if ((t & 63) == 0) sw[t >> 6] = wave_total;
     __syncthreads();
+    double own = 0.0;
+    if (t == 0)
+        for (int w = 0; w < NW; ++w) own += sw[w];
     if (g != 0) {
-        if (t == 0) xput(stage, (size_t)g * 2, ((sw[0] + sw[1]) + sw[2]) + sw[3], tag);
+        if (t == 0) xput(stage, (size_t)g * 2, own, tag);
         return;
     }
-    double s = t == 0 ? ((sw[0] + sw[1]) + sw[2]) + sw[3] : 0.0;
-    for (int k0 = 0; k0 * 256 < G; k0 += 4) {
+    double s = own;
+    for (int k0 = 0; k0 * NT < G; k0 += 4) {
         long slot[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const int gi = t + 256 * (k0 + k);
+            const int gi = t + NT * (k0 + k);
             slot[k] = (gi > 0 && gi < G) ? (long)gi * 2 : -1;
         }
         double v[4];
@@ -807,7 +815,7 @@ __device__ void grid_tail(Ctrl* c, int mode, double wave_total, uint32_t tag, ui
     }
     st[t] = s;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = NT / 2; o > 0; o >>= 1) {
         if (t < o) st[t] += st[t + o];
         __syncthreads();
     }
@@ -815,6 +823,11 @@ __device__ void grid_tail(Ctrl* c, int mode, double wave_total, uint32_t tag, ui
         c->red_epoch = tag;
         scalar_update(c, mode, st[0], 0.0);
     }
+}
+
+__device__ __forceinline__ void grid_tail(Ctrl* c, int mode, double wave_total, uint32_t tag, uint64_t* stage,
+                                          uint64_t timeout) {
+    grid_tail_n<4>(c, mode, wave_total, tag, stage, timeout);
 }
 
 // The six neighbour brick ids of brick b (wave-uniform): lanes 0..5 load
@@ -1149,20 +1162,17 @@ __device__ __forceinline__ void bput(uint64_t* x, int b, T v, uint32_t e, const 
     xput(x, bslot<T>(b, L.lane), v, e);
 }
 
-// Wavefront sweeps over x-pencils. Wave w of W walks the pencils at positions
-// w, w+W, ... of the pencil list (sorted by by+bz) and inside a pencil its
-// bricks in x order (reverse order for the backward sweep). The x neighbour
-// of a brick is the wave's previous brick: its face values come from the
-// registers through a lane shuffle. y and z neighbours belong to other pencils
-// and arrive as epoch-tagged words. Brick k+1's loads and first tag polls are
-// issued (raw, nothing consumed) before brick k is computed, so a brick whose
-// neighbours are ready costs little more than its own arithmetic. Every brick
-// a wave waits for belongs to a pencil of lower (backward: higher) level,
-// i.e. an earlier position of the list, and all waves are resident
-// (see persistent()), so every wait ends.
+// Wavefront sweeps over x-pencils. A wave walks the bricks of one pencil in x
+// order (reverse order for the backward sweep). The x neighbour of a brick is
+// the wave's previous brick: its face values come from the registers through
+// a lane move. y and z neighbours belong to other pencils and arrive as
+// epoch-tagged words (or through LDS inside a pencil block, see k_fwd_block).
+// Every brick a wave waits for belongs to a pencil of lower (backward:
+// higher) level, i.e. an earlier position of the block list, and all waves
+// are resident (see persistent()), so every wait ends.
 template <class T, bool UPDATE>
 struct FwdIn {
-    int b, bx;
+    int b, bx, ny, nz;
     T rv, xv, pv, qv, ie, axo, ayo, azo, aym_e, azm_e;
     Polls2<T> poll;
 };
@@ -1171,9 +1181,12 @@ struct FwdIn {
 // unconditional; requests a lane does not need read a valid dummy address.
 template <class T, bool UPDATE>
 __device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, int nym, int nzm, const Coef<T>& c,
-                                          const T* r, const T* x, const T* p, const T* q, const Lane& L) {
+                                          const T* r, const T* x, const T* p, const T* q, const Lane& L,
+                                          bool gy = true, bool gz = true) {
     d.b = b;
     d.bx = bx;
+    d.ny = nym;
+    d.nz = nzm;
     const size_t o = (size_t)b * kBrickCells + L.lane;
     d.rv = r[o];
     if (UPDATE) {
@@ -1192,8 +1205,8 @@ __device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, in
     const T az_e = c.az[(size_t)zb * kBrickCells + lz_lane];
     d.aym_e = wy ? ay_e : (T)0;
     d.azm_e = wz ? az_e : (T)0;
-    d.poll.want[0] = wy;
-    d.poll.want[1] = wz;
+    d.poll.want[0] = wy && gy;
+    d.poll.want[1] = wz && gz;
     d.poll.slot[0] = bslot<T>(yb, ly_lane);
     d.poll.slot[1] = bslot<T>(zb, lz_lane);
 }
@@ -1241,9 +1254,9 @@ __device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur
 
 // Forward brick: the ten hyperplanes, then publication.
 template <class T, bool UPDATE>
-__device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdPre<T, UPDATE>& pr,
-                                            const T (&got)[2], FwdCarry<T>& cy, T* y, uint64_t* xch, uint32_t epoch,
-                                            const Lane& L) {
+__device__ __forceinline__ T fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdPre<T, UPDATE>& pr,
+                                         const T (&got)[2], FwdCarry<T>& cy, T* y, uint64_t* xch, uint32_t epoch,
+                                         const Lane& L, bool publish = true) {
     const int h_own = L.lx + L.ly + L.lz;
     const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
     T yv = (T)0;
@@ -1257,82 +1270,28 @@ __device__ __forceinline__ void fwd_compute(const FwdIn<T, UPDATE>& cur, const F
             yv = t * cur.ie;
         }
     }
-    bput<T>(xch, cur.b, yv, epoch, L);
+    if (publish) bput<T>(xch, cur.b, yv, epoch, L);
     y[(size_t)cur.b * kBrickCells + L.lane] = yv;
     cy.prev_y = yv;
     cy.prev_ax = cur.axo;
     cy.prev_bx = cur.bx;
-}
-
-// Pencil loop, unrolled by two over buffers A and B so that the prefetched
-// registers are never copied (a copy would wait for the loads). Per brick:
-// everything that needs no neighbour face (coupling moves, x carry, x and r
-// updates) is done first, then the neighbour polls of
-// the brick are finished, the next brick's first poll is issued (its round
-// trip overlaps this computation) and the hyperplanes run.
-// Past the pencil's last brick the fetch repeats the current brick (valid
-// addresses, results unused), which keeps the loop free of memory branches.
-// The wave's |r|^2 terms go to the grid tail (mode SC_RR with UPDATE; without
-// UPDATE the tail only advances the epochs, SC_NOP).
-template <class T, bool UPDATE>
-__global__ __launch_bounds__(256) void k_fwd_pencil(Bricks bk, Pencils pc, Coef<T> c, T* r, T* x, const T* p,
-                                                    const T* q, T* y, Ctrl* ctrl, uint64_t* xch, uint64_t* stage,
-                                                    uint64_t timeout) {
-    if (ctrl->done) return;
-    const Lane L;
-    const int W = (int)gridDim.x * 4;
-    const uint32_t epoch = ctrl->epoch;
-    const uint32_t tag = ctrl->red_epoch + 1u;
-    const T alpha = UPDATE ? (T)ctrl->alpha : (T)0;
-    double acc = 0.0;
-    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < pc.np; pos += W) {
-        const int cc = __builtin_amdgcn_readfirstlane(pc.coord[pc.lvl_list[pos]]);
-        const int by = cc & 1023, bz = (cc >> 10) & 1023;
-        PencilIter<false> it(map_row(bk, by, bz), map_row(bk, by - 1, bz), map_row(bk, by, bz - 1), bk.nbx);
-        int bx, b, ny, nz;
-        if (!it.next(bx, b, ny, nz)) continue;
-        FwdIn<T, UPDATE> A, B;
-        FwdCarry<T> cy{(T)0, (T)0, -2, acc};
-        fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
-        A.poll.issue(xch);
-        T got[2];
-        for (;;) {
-            bool more = it.next(bx, b, ny, nz);
-            fwd_fetch<T, UPDATE>(B, b, bx, ny, nz, c, r, x, p, q, L);
-            {
-                const FwdPre<T, UPDATE> pr = fwd_pre<T, UPDATE>(A, cy, alpha, r, x, L);
-                A.poll.finish(xch, epoch, got, ctrl, timeout);
-                B.poll.issue(xch);
-                fwd_compute<T, UPDATE>(A, pr, got, cy, y, xch, epoch, L);
-            }
-            if (!more) break;
-            more = it.next(bx, b, ny, nz);
-            fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L);
-            {
-                const FwdPre<T, UPDATE> pr = fwd_pre<T, UPDATE>(B, cy, alpha, r, x, L);
-                B.poll.finish(xch, epoch, got, ctrl, timeout);
-                A.poll.issue(xch);
-                fwd_compute<T, UPDATE>(B, pr, got, cy, y, xch, epoch, L);
-            }
-            if (!more) break;
-        }
-        acc = cy.acc;
-    }
-    grid_tail(ctrl, UPDATE ? SC_RR : SC_NOP, UPDATE ? wave_sum(acc) : 0.0, tag, stage, timeout);
+    return yv;
 }
 
 template <class T>
 struct BwdIn {
-    int b, bx;
+    int b, bx, ny, nz;
     T yv, ie, axo, ayo, azo, rv;
     Polls2<T> poll;
 };
 
 template <class T>
 __device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, int nzp, const Coef<T>& c,
-                                          const T* r, const T* y, const Lane& L) {
+                                          const T* r, const T* y, const Lane& L, bool gy = true, bool gz = true) {
     d.b = b;
     d.bx = bx;
+    d.ny = nyp;
+    d.nz = nzp;
     const size_t o = (size_t)b * kBrickCells + L.lane;
     d.yv = y[o];
     d.ie = c.inv_e[o];
@@ -1341,8 +1300,8 @@ __device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, i
     d.azo = c.az[o];
     d.rv = r[o];
     const bool wy = L.ly == 3 && nyp >= 0, wz = L.lz == 3 && nzp >= 0;
-    d.poll.want[0] = wy;
-    d.poll.want[1] = wz;
+    d.poll.want[0] = wy && gy;
+    d.poll.want[1] = wz && gz;
     d.poll.slot[0] = bslot<T>(wy ? nyp : b, wy ? L.lane - 12 : L.lane);
     d.poll.slot[1] = bslot<T>(wz ? nzp : b, wz ? L.lane - 48 : L.lane);
 }
@@ -1355,8 +1314,8 @@ struct BwdCarry {
 };
 
 template <class T>
-__device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, T ex, const T (&got)[2], BwdCarry<T>& cy, T* y,
-                                            uint64_t* xch, uint32_t epoch, const Lane& L) {
+__device__ __forceinline__ T bwd_compute(const BwdIn<T>& cur, T ex, const T (&got)[2], BwdCarry<T>& cy, T* y,
+                                         uint64_t* xch, uint32_t epoch, const Lane& L, bool publish = true) {
     const int h_own = L.lx + L.ly + L.lz;
     const bool ix = L.lx < 3, iy = L.ly < 3, iz = L.lz < 3;
     T zv = (T)0;
@@ -1370,60 +1329,362 @@ __device__ __forceinline__ void bwd_compute(const BwdIn<T>& cur, T ex, const T (
             zv = fma(-cur.ie, t, cur.yv);
         }
     }
-    bput<T>(xch, cur.b, zv, epoch, L);
+    if (publish) bput<T>(xch, cur.b, zv, epoch, L);
     y[(size_t)cur.b * kBrickCells + L.lane] = zv;
     cy.prev_z = zv;
     cy.prev_bx = cur.bx;
     cy.acc += (double)cur.rv * (double)zv;
+    return zv;
 }
 
-// The wave's r.z terms go to the grid tail, which applies tail_mode
-// (SC_BETA, SC_RZ_INIT or SC_RZ_REPLACE).
+// ---------------------------------------------------------------- pencil blocks
+// A workgroup of BY x BZ waves sweeps a block of BY x BZ neighbouring
+// x-pencils, wave (dy, dz) = wave dy + BY dz taking pencil (BY Y + dy,
+// BZ Z + dz). Faces between pencils of the same block go through LDS rings
+// (one per producer wave and direction) instead of the device-wide exchange
+// buffer; only faces that leave the block are published there. Blocks are
+// taken in the order of a list sorted by block level Y + Z (then Y): block
+// position p goes to workgroup p mod G in round p / G, so a workgroup only
+// waits for blocks at earlier positions.
+//
+// Ring protocol. Entry key = round * 4096 + x position + 1 (x position: bx,
+// or 4094 - bx for the backward sweep, so keys grow in processing order);
+// every LDS word carries (key << 32 | 32 value bits), written and read as
+// single 64-bit accesses, so a reader sees a whole entry or an old one and
+// needs no fence. Slot = key mod R. A consumer announces, before it waits for
+// the faces of its brick with key k, that it needs nothing below k (prog =
+// k - 1), and at the end of a round that it needs nothing more from it. A
+// producer writes key k only once the consumer's prog >= k - R: slots of keys
+// in (prog, prog + R] are distinct, so an entry is never overwritten before
+// it has been read, and the producer can always write the key the consumer
+// waits for (no cycle between the two waits).
 template <class T>
-__global__ __launch_bounds__(256) void k_bwd_pencil(Bricks bk, Pencils pc, Coef<T> c, const T* r, T* y, Ctrl* ctrl,
-                                                    uint64_t* xch, uint64_t* stage, int tail_mode, uint64_t timeout) {
+struct Ring {
+    static constexpr int W = sizeof(T) / 4;
+    static constexpr int R = 8 / W;            // slots
+    static constexpr int kWords = R * 16 * W;  // words per ring
+};
+
+__device__ __forceinline__ uint32_t ring_key(int round, int xpos) {
+    return (uint32_t)round * 4096u + (uint32_t)xpos + 1u;
+}
+__device__ __forceinline__ uint32_t ring_end(int round) { return (uint32_t)round * 4096u + 4095u; }
+
+__device__ __forceinline__ void lds_store(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint64_t lds_load(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void prog_store(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint32_t prog_load(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Producer: waits (bounded) until the consumer no longer needs the slot of
+// key, then the lanes with on_face store their face value at index f.
+template <class T>
+__device__ __forceinline__ void ring_put(uint64_t* ring, const uint32_t* cons_prog, uint32_t& seen, uint32_t key,
+                                         bool on_face, int f, T v, Ctrl* c, uint64_t timeout) {
+    constexpr int W = Ring<T>::W, R = Ring<T>::R;
+    if (seen + R < key) {
+        uint64_t t0 = 0;
+        for (int round = 0;; ++round) {
+            seen = prog_load(cons_prog);
+            if (seen + R >= key) break;
+            if ((round & 63) == 0) {
+                if (round == 0) {
+                    t0 = wall_clock64();
+                } else {
+                    if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                    if (wall_clock64() - t0 > timeout) {
+                        __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    if (on_face) {
+        uint64_t* slot = ring + (size_t)(key % R) * (16 * W) + (size_t)f * W;
+        if constexpr (W == 1) {
+            lds_store(slot, ((uint64_t)key << 32) | __float_as_uint(v));
+        } else {
+            const uint64_t bits = (uint64_t)__double_as_longlong(v);
+            lds_store(slot, ((uint64_t)key << 32) | (uint32_t)bits);
+            lds_store(slot + 1, ((uint64_t)key << 32) | (uint32_t)(bits >> 32));
+        }
+    }
+}
+
+// Consumer: every lane reads entry f of the slot of key (lanes that do not
+// want it read the same words and accept anything) until all wanted words
+// carry key (bounded wait).
+template <class T>
+__device__ __forceinline__ T ring_get(const uint64_t* ring, uint32_t key, bool want, int f, Ctrl* c,
+                                      uint64_t timeout) {
+    constexpr int W = Ring<T>::W, R = Ring<T>::R;
+    const uint64_t* slot = ring + (size_t)(key % R) * (16 * W) + (size_t)f * W;
+    uint64_t v[W];
+    uint64_t t0 = 0;
+    for (int round = 0;; ++round) {
+        bool ok = true;
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+            v[h] = lds_load(slot + h);
+            ok = ok && (uint32_t)(v[h] >> 32) == key;
+        }
+        if (__all(ok || !want)) break;
+        if ((round & 63) == 0) {
+            if (round == 0) {
+                t0 = wall_clock64();
+            } else {
+                if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+                if (wall_clock64() - t0 > timeout) {
+                    __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (!want) return (T)0;
+    if constexpr (W == 1) {
+        return __uint_as_float((uint32_t)v[0]);
+    } else {
+        return __longlong_as_double((long long)(((uint64_t)(uint32_t)v[1] << 32) | (uint32_t)v[0]));
+    }
+}
+
+// Shared state of a block sweep: 2 rings per wave (y and z faces it
+// produces) and one progress word per wave (as a consumer).
+template <class T, int NW>
+struct BlockLds {
+    uint64_t ring[2 * NW][Ring<T>::kWords];
+    uint32_t prog[NW];
+};
+
+template <class T, int NW>
+__device__ __forceinline__ void block_lds_init(BlockLds<T, NW>& s) {
+    uint64_t* w = &s.ring[0][0];
+    for (int q = threadIdx.x; q < 2 * NW * Ring<T>::kWords; q += 64 * NW) w[q] = 0ull;
+    if (threadIdx.x < NW) s.prog[threadIdx.x] = 0u;
+    __syncthreads();
+}
+
+// Forward sweep over pencil blocks. Per pencil, a wave walks its bricks in x
+// order. Brick loads are issued two bricks ahead (three buffers in rotation,
+// so prefetched registers are never copied: a copy would wait for the loads)
+// and the -y/-z face polls one brick ahead. Per brick: everything that needs
+// no neighbour face (coupling moves, x carry, x and r updates with UPDATE) is
+// done first, then the polls are finished (faces inside the block come from
+// the LDS rings), the next brick's polls are issued and the hyperplanes run.
+// Past the pencil's last brick the fetch repeats the current brick (valid
+// addresses, results unused), which keeps the loop free of memory branches.
+// The wave's |r|^2 terms go to the grid tail (SC_RR with UPDATE; without
+// UPDATE the tail only advances the epochs, SC_NOP).
+// At most 128 VGPRs in FP32 (4 waves per SIMD: 4 workgroups of 2x2 per CU).
+template <class T, bool UPDATE, int BY, int BZ>
+__global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_block(Bricks bk, Pencils pc, Coef<T> c,
+                                                                                   T* r, T* x,
+                                                            const T* p, const T* q, T* y, Ctrl* ctrl, uint64_t* xch,
+                                                            uint64_t* stage, uint64_t timeout) {
+    constexpr int NW = BY * BZ;
+    __shared__ BlockLds<T, NW> S;
     if (ctrl->done) return;
+    block_lds_init<T, NW>(S);
     const Lane L;
-    const int W = (int)gridDim.x * 4;
+    const int w = (int)(threadIdx.x >> 6), dy = w % BY, dz = w / BY;
+    const bool gy = dy == 0, gz = dz == 0;                    // -y/-z neighbour outside the block
+    const bool out_y = dy == BY - 1, out_z = dz == BZ - 1;   // +y/+z consumer outside the block
+    const bool publish = out_y || out_z;
+    uint64_t* ring_y = S.ring[2 * w];                          // this wave's +y face, to wave (dy+1, dz)
+    uint64_t* ring_z = S.ring[2 * w + 1];                      // this wave's +z face, to wave (dy, dz+1)
+    const uint64_t* in_y = gy ? S.ring[0] : S.ring[2 * (w - 1)];
+    const uint64_t* in_z = gz ? S.ring[0] : S.ring[2 * (w - BY) + 1];
+    const uint32_t* prog_y = &S.prog[out_y ? w : w + 1];
+    const uint32_t* prog_z = &S.prog[out_z ? w : w + BY];
+    uint32_t seen_y = 0, seen_z = 0;
+    const int f_y = L.lx + 4 * L.lz, f_z = L.lx + 4 * L.ly;   // face index (producer and consumer side)
+    const uint32_t epoch = ctrl->epoch;
+    const uint32_t tag = ctrl->red_epoch + 1u;
+    const T alpha = UPDATE ? (T)ctrl->alpha : (T)0;
+    double acc = 0.0;
+    int round = 0;
+    for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
+        const int bc = pc.blk_list[pos];
+        const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
+        if (by < bk.nby && bz < bk.nbz) {
+            PencilIter<false> it(map_row(bk, by, bz), map_row(bk, by - 1, bz), map_row(bk, by, bz - 1), bk.nbx);
+            int bx, b, ny, nz;
+            if (it.next(bx, b, ny, nz)) {
+                FwdIn<T, UPDATE> A, B, C;
+                FwdCarry<T> cy{(T)0, (T)0, -2, acc};
+                fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L, gy, gz);
+                A.poll.issue(xch);
+                T got[2];
+                auto step = [&](FwdIn<T, UPDATE>& cur, FwdIn<T, UPDATE>& nxt) {
+                    const FwdPre<T, UPDATE> pr = fwd_pre<T, UPDATE>(cur, cy, alpha, r, x, L);
+                    const uint32_t key = ring_key(round, cur.bx);
+                    if (L.lane == 0) prog_store(&S.prog[w], key - 1u);
+                    cur.poll.finish(xch, epoch, got, ctrl, timeout);
+                    nxt.poll.issue(xch);
+                    if (!gy) got[0] = ring_get<T>(in_y, key, L.ly == 0 && cur.ny >= 0, f_y, ctrl, timeout);
+                    if (!gz) got[1] = ring_get<T>(in_z, key, L.lz == 0 && cur.nz >= 0, f_z, ctrl, timeout);
+                    const T yv = fwd_compute<T, UPDATE>(cur, pr, got, cy, y, xch, epoch, L, publish);
+                    if (!out_y) ring_put<T>(ring_y, prog_y, seen_y, key, L.ly == 3, f_y, yv, ctrl, timeout);
+                    if (!out_z) ring_put<T>(ring_z, prog_z, seen_z, key, L.lz == 3, f_z, yv, ctrl, timeout);
+                };
+                // three buffers in rotation: a brick's loads are issued two
+                // bricks ahead of its computation, its first polls one brick
+                // ahead (earlier polls would mostly find stale tags)
+                auto ahead = [&](FwdIn<T, UPDATE>& buf, bool prev) {
+                    const bool has = prev && it.next(bx, b, ny, nz);
+                    fwd_fetch<T, UPDATE>(buf, b, bx, ny, nz, c, r, x, p, q, L, gy, gz);
+                    return has;
+                };
+                bool hB = ahead(B, true);
+                for (;;) {
+                    const bool hC = ahead(C, hB);
+                    step(A, B);
+                    if (!hB) break;
+                    const bool hA = ahead(A, hC);
+                    step(B, C);
+                    if (!hC) break;
+                    hB = ahead(B, hA);
+                    step(C, A);
+                    if (!hA) break;
+                }
+                acc = cy.acc;
+            }
+        }
+        if (L.lane == 0) prog_store(&S.prog[w], ring_end(round));
+    }
+    grid_tail_n<NW>(ctrl, UPDATE ? SC_RR : SC_NOP, UPDATE ? wave_sum(acc) : 0.0, tag, stage, timeout);
+}
+
+// Backward sweep over pencil blocks: blocks in reverse list order, bricks in
+// descending x, +y/+z faces from LDS inside the block.
+template <class T, int BY, int BZ>
+__global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_block(Bricks bk, Pencils pc, Coef<T> c,
+                                                                                   const T* r, T* y,
+                                                            Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode,
+                                                            uint64_t timeout) {
+    constexpr int NW = BY * BZ;
+    __shared__ BlockLds<T, NW> S;
+    if (ctrl->done) return;
+    block_lds_init<T, NW>(S);
+    const Lane L;
+    const int w = (int)(threadIdx.x >> 6), dy = w % BY, dz = w / BY;
+    const bool gy = dy == BY - 1, gz = dz == BZ - 1;   // +y/+z neighbour outside the block
+    const bool out_y = dy == 0, out_z = dz == 0;       // -y/-z consumer outside the block
+    const bool publish = out_y || out_z;
+    uint64_t* ring_y = S.ring[2 * w];                   // this wave's -y face, to wave (dy-1, dz)
+    uint64_t* ring_z = S.ring[2 * w + 1];               // this wave's -z face, to wave (dy, dz-1)
+    const uint64_t* in_y = gy ? S.ring[0] : S.ring[2 * (w + 1)];
+    const uint64_t* in_z = gz ? S.ring[0] : S.ring[2 * (w + BY) + 1];
+    const uint32_t* prog_y = &S.prog[out_y ? w : w - 1];
+    const uint32_t* prog_z = &S.prog[out_z ? w : w - BY];
+    uint32_t seen_y = 0, seen_z = 0;
+    const int f_y = L.lx + 4 * L.lz, f_z = L.lx + 4 * L.ly;
     const uint32_t epoch = ctrl->epoch;
     const uint32_t tag = ctrl->red_epoch + 1u;
     double acc = 0.0;
-    for (int pos = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); pos < pc.np; pos += W) {
-        const int cc = __builtin_amdgcn_readfirstlane(pc.coord[pc.lvl_list[pc.np - 1 - pos]]);
-        const int by = cc & 1023, bz = (cc >> 10) & 1023;
-        PencilIter<true> it(map_row(bk, by, bz), map_row(bk, by + 1, bz), map_row(bk, by, bz + 1), bk.nbx);
-        int bx, b, ny, nz;
-        if (!it.next(bx, b, ny, nz)) continue;
-        BwdIn<T> A, B;
-        BwdCarry<T> cy{(T)0, -2, acc};
-        bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
-        A.poll.issue(xch);
-        T got[2];
-        for (;;) {
-            bool more = it.next(bx, b, ny, nz);
-            bwd_fetch<T>(B, b, bx, ny, nz, c, r, y, L);
-            {
-                const T pzx = move<Mv::q0>(cy.prev_z);
-                const T ex = cy.prev_bx == A.bx + 1 ? pzx : (T)0;
-                A.poll.finish(xch, epoch, got, ctrl, timeout);
-                B.poll.issue(xch);
-                bwd_compute<T>(A, ex, got, cy, y, xch, epoch, L);
-            }
-            if (!more) break;
-            more = it.next(bx, b, ny, nz);
-            bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L);
-            {
-                const T pzx = move<Mv::q0>(cy.prev_z);
-                const T ex = cy.prev_bx == B.bx + 1 ? pzx : (T)0;
-                B.poll.finish(xch, epoch, got, ctrl, timeout);
+    int round = 0;
+    for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
+        const int bc = pc.blk_list[pc.nblk - 1 - pos];
+        const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
+        if (by < bk.nby && bz < bk.nbz) {
+            PencilIter<true> it(map_row(bk, by, bz), map_row(bk, by + 1, bz), map_row(bk, by, bz + 1), bk.nbx);
+            int bx, b, ny, nz;
+            if (it.next(bx, b, ny, nz)) {
+                BwdIn<T> A, B, C;
+                BwdCarry<T> cy{(T)0, -2, acc};
+                bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L, gy, gz);
                 A.poll.issue(xch);
-                bwd_compute<T>(B, ex, got, cy, y, xch, epoch, L);
+                T got[2];
+                auto step = [&](BwdIn<T>& cur, BwdIn<T>& nxt) {
+                    const T pzx = move<Mv::q0>(cy.prev_z);
+                    const T ex = cy.prev_bx == cur.bx + 1 ? pzx : (T)0;
+                    const uint32_t key = ring_key(round, 4094 - cur.bx);
+                    if (L.lane == 0) prog_store(&S.prog[w], key - 1u);
+                    cur.poll.finish(xch, epoch, got, ctrl, timeout);
+                    nxt.poll.issue(xch);
+                    if (!gy) got[0] = ring_get<T>(in_y, key, L.ly == 3 && cur.ny >= 0, f_y, ctrl, timeout);
+                    if (!gz) got[1] = ring_get<T>(in_z, key, L.lz == 3 && cur.nz >= 0, f_z, ctrl, timeout);
+                    const T zv = bwd_compute<T>(cur, ex, got, cy, y, xch, epoch, L, publish);
+                    if (!out_y) ring_put<T>(ring_y, prog_y, seen_y, key, L.ly == 0, f_y, zv, ctrl, timeout);
+                    if (!out_z) ring_put<T>(ring_z, prog_z, seen_z, key, L.lz == 0, f_z, zv, ctrl, timeout);
+                };
+                auto ahead = [&](BwdIn<T>& buf, bool prev) {
+                    const bool has = prev && it.next(bx, b, ny, nz);
+                    bwd_fetch<T>(buf, b, bx, ny, nz, c, r, y, L, gy, gz);
+                    return has;
+                };
+                bool hB = ahead(B, true);
+                for (;;) {
+                    const bool hC = ahead(C, hB);
+                    step(A, B);
+                    if (!hB) break;
+                    const bool hA = ahead(A, hC);
+                    step(B, C);
+                    if (!hC) break;
+                    hB = ahead(B, hA);
+                    step(C, A);
+                    if (!hA) break;
+                }
+                acc = cy.acc;
             }
-            if (!more) break;
         }
-        acc = cy.acc;
+        if (L.lane == 0) prog_store(&S.prog[w], ring_end(round));
     }
-    grid_tail(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout);
+    grid_tail_n<NW>(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout);
+}
+
+// List of pencil blocks (BYv x BZv pencils) holding at least one active
+// pencil, sorted by block level Y + Z, then Y (deterministic); one workgroup.
+__global__ __launch_bounds__(1024) void k_block_list(const int32_t* pmap, int nby, int nbz, int BYv, int BZv,
+                                                     uint8_t* flag, int32_t* list, int64_t* count) {
+    __shared__ int cnt[1024];
+    __shared__ int off[1024];
+    const int NBY = (nby + BYv - 1) / BYv, NBZ = (nbz + BZv - 1) / BZv;
+    const int total = NBY * NBZ, nlev = NBY + NBZ - 1;
+    for (int i = threadIdx.x; i < total; i += 1024) {
+        const int Y = i % NBY, Z = i / NBY;
+        uint8_t any = 0;
+        for (int dz = 0; dz < BZv; ++dz)
+            for (int dy = 0; dy < BYv; ++dy) {
+                const int by = Y * BYv + dy, bz = Z * BZv + dz;
+                if (by < nby && bz < nbz && pmap[(size_t)by + (size_t)nby * bz] >= 0) any = 1;
+            }
+        flag[i] = any;
+    }
+    __syncthreads();
+    for (int l = threadIdx.x; l < nlev; l += 1024) {
+        int n = 0;
+        for (int Y = max(0, l - NBZ + 1); Y <= min(l, NBY - 1); ++Y) n += flag[(size_t)Y + (size_t)NBY * (l - Y)];
+        cnt[l] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int l = 0; l < nlev; ++l) {
+            off[l] = o;
+            o += cnt[l];
+        }
+        *count = o;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < total; i += 1024) {
+        if (!flag[i]) continue;
+        const int Y = i % NBY, Z = i / NBY, l = Y + Z;
+        int rank = 0;
+        for (int Y2 = max(0, l - NBZ + 1); Y2 < Y; ++Y2) rank += flag[(size_t)Y2 + (size_t)NBY * (l - Y2)];
+        list[off[l] + rank] = Y | (Z << 10);
+    }
 }
 
 // ---------------------------------------------------------------- pencil setup
@@ -1434,27 +1695,6 @@ __global__ __launch_bounds__(256) void k_pencil_flags(const int32_t* bmap, int n
     uint8_t any = 0;
     for (int bx = 0; bx < nbx; ++bx) any |= row[bx] >= 0 ? 1 : 0;
     pflag[pi] = any;
-}
-
-__global__ __launch_bounds__(256) void k_pencil_meta(const int32_t* pmap, int nby, int nbz, int32_t* pcoord,
-                                                     int32_t* lvl_cnt) {
-    const int64_t pi = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (pi >= (int64_t)nby * nbz) return;
-    const int id = pmap[pi];
-    if (id < 0) return;
-    const int by = (int)(pi % nby), bz = (int)(pi / nby);
-    pcoord[id] = by | (bz << 10);
-    atomicAdd(&lvl_cnt[by + bz], 1);
-}
-
-__global__ __launch_bounds__(256) void k_pencil_lists(const int32_t* pcoord, int np, const int32_t* lvl_off,
-                                                      int32_t* lvl_fill, int32_t* lvl_list) {
-    const int id = blockIdx.x * 256 + threadIdx.x;
-    if (id >= np) return;
-    const int cc = pcoord[id];
-    const int l = (cc & 1023) + ((cc >> 10) & 1023);
-    const int pos = atomicAdd(&lvl_fill[l], 1);
-    lvl_list[lvl_off[l] + pos] = id;
 }
 
 // r64 = b - A x64 in FP64 (order as the oracle's spmv: diag, -x, +x, -y, +y,
@@ -1578,8 +1818,8 @@ __global__ __launch_bounds__(256) void k_update(Dense d, float dt, const float* 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 constexpr int kWaveBlocksPerCU = 2;    // persistent factor sweep: 2 workgroups (8 waves) per CU
-constexpr int kPencilBlocksPerCU = 4;  // pencil sweeps: up to 4 workgroups (16 waves) per CU
 constexpr int kSpmvBlocksPerCU = 4;    // persistent SpMV: up to 4 workgroups (16 waves) per CU
+constexpr int kBlockGroupsPerCU = 4;   // pencil-block sweeps: up to 4 workgroups per CU
 
 int device_cus() {
     static std::mutex mu;
@@ -1592,15 +1832,15 @@ int device_cus() {
     return cus[dev] > 0 ? cus[dev] : 1;
 }
 
-// Resident workgroups per CU of a kernel at 256 threads (cached per kernel).
-int occupancy(const void* kern) {
+// Resident workgroups per CU of a kernel at the given block size (cached per kernel).
+int occupancy(const void* kern, int threads) {
     static std::mutex mu;
     static std::vector<std::pair<const void*, int>> cache;
     std::lock_guard<std::mutex> lock(mu);
     for (const auto& kv : cache)
         if (kv.first == kern) return kv.second;
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, 0) != hipSuccess || occ < 1) occ = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, threads, 0) != hipSuccess || occ < 1) occ = 1;
     cache.emplace_back(kern, occ);
     return occ;
 }
@@ -1613,18 +1853,18 @@ int occupancy(const void* kern) {
 // If another workload keeps some workgroups from starting, the bounded
 // waits end at the time limit and the launch reports the abort.
 template <typename... KArgs, typename... Args, size_t... I>
-hipError_t persistent_impl(void (*kern)(KArgs...), int items, int blocks_per_cu, int max_groups, hipStream_t s,
-                           std::index_sequence<I...>, Args&&... args) {
+hipError_t persistent_impl(void (*kern)(KArgs...), int items, int per_group, int threads, int blocks_per_cu,
+                           int max_groups, hipStream_t s, std::index_sequence<I...>, Args&&... args) {
     std::tuple<KArgs...> t(std::forward<Args>(args)...);
     void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
-    const int occ = occupancy((const void*)kern);
+    const int occ = occupancy((const void*)kern, threads);
     const int per_cu = occ < blocks_per_cu ? occ : blocks_per_cu;
-    int g = (items + 3) / 4;
+    int g = (items + per_group - 1) / per_group;
     if (g > per_cu * device_cus()) g = per_cu * device_cus();
     if (g > kTailMaxGroups) g = kTailMaxGroups;
     if (max_groups > 0 && g > max_groups) g = max_groups;
     if (g < 1) g = 1;
-    return hipLaunchKernel((const void*)kern, dim3(g), dim3(256), ptrs, 0, s);
+    return hipLaunchKernel((const void*)kern, dim3(g), dim3(threads), ptrs, 0, s);
 }
 
 // items: work items (one per wave, four waves per 256-thread workgroup);
@@ -1633,8 +1873,17 @@ template <typename... KArgs, typename... Args>
 hipError_t persistent(void (*kern)(KArgs...), int items, int blocks_per_cu, int max_groups, hipStream_t s,
                       Args&&... args) {
     static_assert(sizeof...(KArgs) == sizeof...(Args), "argument count");
-    return persistent_impl(kern, items, blocks_per_cu, max_groups, s, std::index_sequence_for<KArgs...>{},
+    return persistent_impl(kern, items, 4, 256, blocks_per_cu, max_groups, s, std::index_sequence_for<KArgs...>{},
                            std::forward<Args>(args)...);
+}
+
+// Persistent launch with one work item per workgroup of `threads` threads.
+template <typename... KArgs, typename... Args>
+hipError_t persistent_wg(void (*kern)(KArgs...), int items, int threads, int blocks_per_cu, int max_groups,
+                         hipStream_t s, Args&&... args) {
+    static_assert(sizeof...(KArgs) == sizeof...(Args), "argument count");
+    return persistent_impl(kern, items, 1, threads, blocks_per_cu, max_groups, s,
+                           std::index_sequence_for<KArgs...>{}, std::forward<Args>(args)...);
 }
 
 }  // namespace
@@ -1702,7 +1951,8 @@ hipError_t launch_factor(const Bricks& bk, const double* diag64, const T* ax, co
                          hipStream_t s) {
     if (level < 0) {
         if (bk.nb == 0) return hipSuccess;
-        return persistent(k_factor_wave<T>, bk.nb, kWaveBlocksPerCU, max_groups, s, bk, diag64, ax, ay, az, tau, sigma, inv_e64, inv_e32, ctrl, xch,
+        return persistent(k_factor_wave<T>, bk.nb, kWaveBlocksPerCU, max_groups, s, bk, diag64, ax, ay, az, tau,
+                          sigma, inv_e64, inv_e32, ctrl, xch,
                     n_fixed, timeout_ticks);
     }
     if (level_count > 0)
@@ -1744,7 +1994,8 @@ hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const
 template <class T>
 hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q, Ctrl* ctrl,
                        uint64_t* stage, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
-    return persistent(k_spmv<T>, bk.nb, kSpmvBlocksPerCU, max_groups, s, bk, c, z, p_in, p_out, q, ctrl, stage, timeout_ticks);
+    return persistent(k_spmv<T>, bk.nb, kSpmvBlocksPerCU, max_groups, s, bk, c, z, p_in, p_out, q, ctrl, stage,
+                      timeout_ticks);
 }
 
 template <class T>
@@ -1753,10 +2004,12 @@ hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* 
                       int level_base, int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
     if (level < 0) {
         if (update)
-            return persistent(k_fwd_pencil<T, true>, pc.np, kPencilBlocksPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch,
-                        stage, timeout_ticks);
-        return persistent(k_fwd_pencil<T, false>, pc.np, kPencilBlocksPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
-                    timeout_ticks);
+            return persistent_wg(k_fwd_block<T, true, kBlockY, kBlockZ>, pc.nblk, 64 * kBlockY * kBlockZ,
+                                 kBlockGroupsPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
+                                 timeout_ticks);
+        return persistent_wg(k_fwd_block<T, false, kBlockY, kBlockZ>, pc.nblk, 64 * kBlockY * kBlockZ,
+                             kBlockGroupsPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
+                             timeout_ticks);
     }
     if (level_count > 0) {
         if (update)
@@ -1774,8 +2027,8 @@ hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, con
                       Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode, int level, int level_base,
                       int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
     if (level < 0)
-        return persistent(k_bwd_pencil<T>, pc.np, kPencilBlocksPerCU, max_groups, s, bk, pc, c, r, y, ctrl, xch, stage, tail_mode,
-                    timeout_ticks);
+        return persistent_wg(k_bwd_block<T, kBlockY, kBlockZ>, pc.nblk, 64 * kBlockY * kBlockZ, kBlockGroupsPerCU,
+                             max_groups, s, bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout_ticks);
     if (level_count > 0)
         hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
                            level_base, timeout_ticks);
@@ -1785,20 +2038,6 @@ hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, con
 hipError_t launch_pencil_flags(const int32_t* bmap, int nbx, int nby, int nbz, uint8_t* pflag, hipStream_t s) {
     hipLaunchKernelGGL(k_pencil_flags, dim3(cdiv((int64_t)nby * nbz, 256)), dim3(256), 0, s, bmap, nbx, nby, nbz,
                        pflag);
-    return hipGetLastError();
-}
-
-hipError_t launch_pencil_meta(const int32_t* pmap, int nby, int nbz, int32_t* pcoord, int32_t* lvl_cnt,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_pencil_meta, dim3(cdiv((int64_t)nby * nbz, 256)), dim3(256), 0, s, pmap, nby, nbz, pcoord,
-                       lvl_cnt);
-    return hipGetLastError();
-}
-
-hipError_t launch_pencil_lists(const int32_t* pcoord, int np, const int32_t* lvl_off, int32_t* lvl_fill,
-                               int32_t* lvl_list, hipStream_t s) {
-    if (np == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pencil_lists, dim3(cdiv(np, 256)), dim3(256), 0, s, pcoord, np, lvl_off, lvl_fill, lvl_list);
     return hipGetLastError();
 }
 
@@ -1823,6 +2062,12 @@ template <typename W>
 __global__ __launch_bounds__(256) void k_copy(const W* __restrict__ src, W* __restrict__ dst, size_t n) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_block_list(const int32_t* pmap, int nby, int nbz, uint8_t* flag, int32_t* list, int64_t* count,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(k_block_list, dim3(1), dim3(1024), 0, s, pmap, nby, nbz, kBlockY, kBlockZ, flag, list, count);
+    return hipGetLastError();
 }
 
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {

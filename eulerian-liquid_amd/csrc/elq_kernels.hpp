@@ -40,10 +40,19 @@ struct Bricks {
 // dependency of the triangular sweeps stays in registers.
 struct Pencils {
     int np;                   // pencils with at least one active brick
-    int nlev;                 // pencil levels (nby + nbz - 1)
-    const int32_t* coord;     // np packed (by | bz << 10)
-    const int32_t* lvl_list;  // np, sorted by by + bz
+    int nblk;                 // pencil blocks (kBlockY x kBlockZ pencils) with an active pencil
+    const int32_t* blk_list;  // nblk packed (Y | Z << 10), sorted by Y + Z, then Y
 };
+
+// Pencil blocks of the wavefront sweeps: kBlockY x kBlockZ pencils per
+// workgroup (one wave each).
+constexpr int kBlockY = 2;
+constexpr int kBlockZ = 2;
+static_assert(kBlockY >= 2 && kBlockZ >= 2 && kBlockY * kBlockZ <= 16, "pencil block shape");
+
+// List of pencil blocks from the pencil map (one workgroup); *count = nblk.
+hipError_t launch_block_list(const int32_t* pmap, int nby, int nbz, uint8_t* flag, int32_t* list, int64_t* count,
+                             hipStream_t s);
 
 // Coefficients of the stored system (device pointers), T = storage type.
 template <class T>
@@ -70,13 +79,8 @@ hipError_t launch_level_scan(const int32_t* lvl_cnt, int nlev, int32_t* lvl_off,
 hipError_t launch_brick_lists(const int32_t* map, const int32_t* coord, int nbx, int nby, int nbz, int nb,
                               const int32_t* lvl_off, int32_t* lvl_fill, int32_t* lvl_list,
                               int32_t* nbr, hipStream_t s);
-// Pencil metadata from the brick map: pflag over (by, bz), then (after a scan
-// into pmap) coordinates, level histogram and the level-ordered list.
+// Pencil flags over (by, bz) from the brick map (a scan turns them into pmap).
 hipError_t launch_pencil_flags(const int32_t* bmap, int nbx, int nby, int nbz, uint8_t* pflag, hipStream_t s);
-hipError_t launch_pencil_meta(const int32_t* pmap, int nby, int nbz, int32_t* pcoord, int32_t* lvl_cnt,
-                              hipStream_t s);
-hipError_t launch_pencil_lists(const int32_t* pcoord, int np, const int32_t* lvl_off, int32_t* lvl_fill,
-                               int32_t* lvl_list, hipStream_t s);
 template <class T>
 hipError_t launch_assemble(const Dense& d, float dt, const uint8_t* mask, const Bricks& bk, double* diag64,
                            double* b64, T* diag, T* ax, T* ay, T* az, hipStream_t s);
@@ -108,17 +112,16 @@ hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* 
                        uint64_t* stage, uint64_t timeout_ticks, int max_groups, hipStream_t s);
 // Forward sweep y = (E+L)^{-1} r, fused with x += alpha p, r -= alpha q and
 // |r|^2 partials when update != 0. level < 0: one persistent launch over the
-// pencils (wavefront); else the bricks of one brick level.
+// pencil blocks (wavefront), ending in a grid tail (SC_RR with update, else
+// SC_NOP); else the bricks of one brick level, with per-brick partials in part.
 template <class T>
-// The pencil launch ends in a grid tail (SC_RR with update, else SC_NOP);
-// the level launches write per-brick partials to part instead.
 hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
                       double* part, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int update, int level,
                       int level_base, int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s);
 // Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
+// level < 0: persistent launch over the pencil blocks ending in a grid tail
+// that applies tail_mode; else one brick level with per-brick partials in part.
 template <class T>
-// The pencil launch ends in a grid tail applying tail_mode; the level
-// launches write per-brick partials to part instead.
 hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
                       uint64_t* xch, uint64_t* stage, int tail_mode, int level, int level_base, int level_count,
                       uint64_t timeout_ticks, int max_groups, hipStream_t s);

@@ -81,7 +81,8 @@ def load():
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"libelq.so is missing at {LIB_PATH}: build it (python -c 'import __graft_entry__ as g; g.build()')")
+        raise RuntimeError(f"libelq.so is missing at {LIB_PATH}: build it "
+                           "(python -c 'import __graft_entry__ as g; g.build()')")
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, f32, f64, sz = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double, C.c_size_t
     L.elq_default_opts.argtypes = [C.POINTER(Opts)]

@@ -923,6 +923,36 @@ int elq_update_velocity(elq_ctx* ctx, const elq_grid* g, float dt) {
     return ELQ_OK;
 }
 
+int elq_compute_weights_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, const float* solid_nodal,
+                               float* u_weight, float* v_weight, float* w_weight) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!solid_nodal || !u_weight || !v_weight || !w_weight) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(launch_weights(ni, nj, nk, solid_nodal, u_weight, v_weight, w_weight, ctx->stream), "k_weights");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_add_gravity_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* v, float dt) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!v) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(launch_gravity(v, (size_t)ni * (size_t)(nj + 1) * (size_t)nk, dt, ctx->stream), "k_gravity");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_valid_and_zero_device(elq_ctx* ctx, const elq_grid* g) {
+    const int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(launch_valid_zero(dense_of(*g), ctx->stream), "k_valid_zero");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out) {
     if (!ctx || !out) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);

@@ -2055,6 +2055,111 @@ hipError_t launch_xupdate(const Bricks& bk, double* x64, const float* x32, Ctrl*
     return hipGetLastError();
 }
 
+// ================================================================== inputs
+// The steps before the projection in the reference's frame (simulator.cpp:
+// 90-93, 125-178, 187-189 with boundary.cpp:6-34), on device grids. One
+// thread per face, 64 x 4 threads per block over (i, j), blocks over k.
+
+// level_set.h:83-163: area fraction of a square cell face inside the level
+// set from its four corner values. The corner list (bl, br, tr, tl) is
+// rotated to a canonical position; 0.5 and 0.25 are double literals in the
+// reference, so those products are evaluated in double and rounded to float
+// where the reference's float variables take them.
+__device__ float fraction_inside4(float bl, float br, float tl, float tr) {
+    const int inside = (bl < 0.0f) + (tl < 0.0f) + (br < 0.0f) + (tr < 0.0f);
+    float q[4] = {bl, br, tr, tl};
+    auto rotate = [&q]() {
+        const float t = q[0];
+        q[0] = q[1];
+        q[1] = q[2];
+        q[2] = q[3];
+        q[3] = t;
+    };
+    if (inside == 4) return 1.0f;
+    if (inside == 3) {
+        for (int n = 0; n < 4 && q[0] < 0.0f; ++n) rotate();
+        const float s0 = 1.0f - fraction_inside2(q[0], q[3]);
+        const float s1 = 1.0f - fraction_inside2(q[0], q[1]);
+        return (float)(1.0 - 0.5 * s0 * s1);
+    }
+    if (inside == 2) {
+        for (int n = 0; n < 4 && (q[0] >= 0.0f || !(q[1] < 0.0f || q[2] < 0.0f)); ++n) rotate();
+        if (q[1] < 0.0f) return (float)(0.5 * (fraction_inside2(q[0], q[3]) + fraction_inside2(q[1], q[2])));
+        const float mid = (float)(0.25 * (q[0] + q[1] + q[2] + q[3]));
+        float area = 0.0f;
+        if (mid < 0.0f) {
+            const float s1 = 1.0f - fraction_inside2(q[0], q[3]);
+            const float s3 = 1.0f - fraction_inside2(q[2], q[3]);
+            area = (float)(area + 0.5 * s1 * s3);
+            const float s2 = 1.0f - fraction_inside2(q[2], q[1]);
+            const float s0 = 1.0f - fraction_inside2(q[0], q[1]);
+            area = (float)(area + 0.5 * s0 * s2);
+            return 1.0f - area;
+        }
+        const float s0 = fraction_inside2(q[0], q[1]);
+        const float s1 = fraction_inside2(q[0], q[3]);
+        area = (float)(area + 0.5 * s0 * s1);
+        const float s2 = fraction_inside2(q[2], q[1]);
+        const float s3 = fraction_inside2(q[2], q[3]);
+        area = (float)(area + 0.5 * s2 * s3);
+        return area;
+    }
+    if (inside == 1) {
+        for (int n = 0; n < 4 && q[0] >= 0.0f; ++n) rotate();
+        const float s0 = fraction_inside2(q[0], q[3]);
+        const float s1 = fraction_inside2(q[0], q[1]);
+        return (float)(0.5 * s0 * s1);
+    }
+    return 0.0f;
+}
+
+// boundary.cpp:6-34: weight = clamp(1 - fraction of the face inside the
+// solid, 0, 1) from the nodal solid level set ((ni+1)(nj+1)(nk+1) values).
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_weights(int ni, int nj, int nk, const float* nodal, float* wgt) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int fi = ni + (AXIS == 0), fj = nj + (AXIS == 1);
+    if (i >= fi || j >= fj) return;
+    const int a = ni + 1, b = nj + 1;
+    auto N = [&](int x, int y, int z) { return nodal[cell(x, y, z, a, b)]; };
+    float f;
+    if (AXIS == 0)
+        f = fraction_inside4(N(i, j, k), N(i, j + 1, k), N(i, j, k + 1), N(i, j + 1, k + 1));
+    else if (AXIS == 1)
+        f = fraction_inside4(N(i, j, k), N(i, j, k + 1), N(i + 1, j, k), N(i + 1, j, k + 1));
+    else
+        f = fraction_inside4(N(i, j, k), N(i, j + 1, k), N(i + 1, j, k), N(i + 1, j + 1, k));
+    const float v = 1.0f - f;
+    wgt[cell(i, j, k, fi, fj)] = v < 0.0f ? 0.0f : (1.0f < v ? 1.0f : v);  // std::clamp(v, 0, 1)
+}
+
+// simulator.cpp:90-93: v -= gravity * dt on every v face (constants.h:5-6).
+__global__ __launch_bounds__(256) void k_gravity(float* v, size_t n, float g_dt) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < n) v[t] = v[t] - g_dt;
+}
+
+// simulator.cpp:125-158 then 169-178: a face is invalid when its weight is 0,
+// or, inside the ranges [1, n_face - 1) of each axis, when both cells beside
+// it are outside the liquid; invalid faces are set to 0.
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_valid_zero(Dense d) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int ni = d.ni, nj = d.nj, nk = d.nk;
+    const int fi = ni + (AXIS == 0), fj = nj + (AXIS == 1), fk = nk + (AXIS == 2);
+    if (i >= fi || j >= fj) return;
+    const size_t f = cell(i, j, k, fi, fj);
+    const float* wgt = AXIS == 0 ? d.uw : AXIS == 1 ? d.vw : d.ww;
+    float* vel = AXIS == 0 ? d.u : AXIS == 1 ? d.v : d.w;
+    bool invalid = wgt[f] == 0.0f;
+    if (i >= 1 && j >= 1 && k >= 1 && i < fi - 1 && j < fj - 1 && k < fk - 1) {
+        const size_t c1 = cell(i, j, k, ni, nj);
+        const size_t c0 = cell(i - (AXIS == 0), j - (AXIS == 1), k - (AXIS == 2), ni, nj);
+        if (d.phi[c0] >= 0.0f && d.phi[c1] >= 0.0f) invalid = true;
+    }
+    if (invalid) vel[f] = 0.0f;
+}
+
 // Plain streaming copy: 16-byte words when both pointers are 16-byte aligned,
 // 4-byte words otherwise. Its HBM counters per byte copied calibrate the
 // traffic counters of the solver kernels.
@@ -2062,6 +2167,29 @@ template <typename W>
 __global__ __launch_bounds__(256) void k_copy(const W* __restrict__ src, W* __restrict__ dst, size_t n) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_weights(int ni, int nj, int nk, const float* nodal, float* uw, float* vw, float* ww,
+                          hipStream_t s) {
+    const dim3 block(64, 4);
+    hipLaunchKernelGGL(k_weights<0>, dim3(cdiv(ni + 1, 64), cdiv(nj, 4), nk), block, 0, s, ni, nj, nk, nodal, uw);
+    hipLaunchKernelGGL(k_weights<1>, dim3(cdiv(ni, 64), cdiv(nj + 1, 4), nk), block, 0, s, ni, nj, nk, nodal, vw);
+    hipLaunchKernelGGL(k_weights<2>, dim3(cdiv(ni, 64), cdiv(nj, 4), nk + 1), block, 0, s, ni, nj, nk, nodal, ww);
+    return hipGetLastError();
+}
+
+hipError_t launch_gravity(float* v, size_t n, float dt, hipStream_t s) {
+    const float g_dt = 9.80665f * dt;  // constants::gravity<float> * dt
+    hipLaunchKernelGGL(k_gravity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, v, n, g_dt);
+    return hipGetLastError();
+}
+
+hipError_t launch_valid_zero(const Dense& d, hipStream_t s) {
+    const dim3 block(64, 4);
+    hipLaunchKernelGGL(k_valid_zero<0>, dim3(cdiv(d.ni + 1, 64), cdiv(d.nj, 4), d.nk), block, 0, s, d);
+    hipLaunchKernelGGL(k_valid_zero<1>, dim3(cdiv(d.ni, 64), cdiv(d.nj + 1, 4), d.nk), block, 0, s, d);
+    hipLaunchKernelGGL(k_valid_zero<2>, dim3(cdiv(d.ni, 64), cdiv(d.nj, 4), d.nk + 1), block, 0, s, d);
+    return hipGetLastError();
 }
 
 hipError_t launch_block_list(const int32_t* pmap, int nby, int nbz, uint8_t* flag, int32_t* list, int64_t* count,

@@ -139,6 +139,15 @@ hipError_t launch_scatter(const Dense& d, const uint8_t* mask, const Bricks& bk,
                           float* pressure, hipStream_t s);
 hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStream_t s);
 
+// ---------------------------------------------------------------- inputs
+// compute_staggered_grid_weight (boundary.cpp:6-34) from the nodal solid level set.
+hipError_t launch_weights(int ni, int nj, int nk, const float* nodal, float* uw, float* vw, float* ww,
+                          hipStream_t s);
+// add_gravity (simulator.cpp:90-93) on the n faces of v.
+hipError_t launch_gravity(float* v, size_t n, float dt, hipStream_t s);
+// find_valid_velocity + set_invalid_velocity_zero (simulator.cpp:125-178).
+hipError_t launch_valid_zero(const Dense& d, hipStream_t s);
+
 // ---------------------------------------------------------------- utility
 // Device-to-device copy of bytes (a multiple of 4) with a plain streaming kernel.
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);

@@ -102,6 +102,9 @@ def load():
     L.elq_memcpy_h2d.argtypes = [vp, vp, vp, sz]
     L.elq_memcpy_d2h.argtypes = [vp, vp, vp, sz]
     L.elq_memcpy_d2d.argtypes = [vp, vp, vp, sz]
+    L.elq_compute_weights_device.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp]
+    L.elq_add_gravity_device.argtypes = [vp, i32, i32, i32, vp, f32]
+    L.elq_valid_and_zero_device.argtypes = [vp, C.POINTER(Grid)]
     L.elq_set_profiling_filter.argtypes = [vp, C.c_char_p]
     L.elq_set_profiling_filter.restype = None
     L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]

@@ -152,6 +152,19 @@ int elq_build_matrix(elq_ctx* ctx, const elq_grid* host_grid, float dt, double* 
  * pressure grid (grid->pressure is the input here). */
 int elq_update_velocity(elq_ctx* ctx, const elq_grid* host_grid, float dt);
 
+/* The steps before the projection in the reference's frame, on device
+ * grids (same layouts), for callers that keep the whole frame in HBM:
+ * compute_staggered_grid_weight (boundary.cpp:6-34) from the nodal solid
+ * level set ((ni+1)*(nj+1)*(nk+1) floats, index i + (ni+1)*(j + (nj+1)*k));
+ * Simulator::add_gravity (simulator.cpp:90-93) on v; find_valid_velocity +
+ * set_invalid_velocity_zero (simulator.cpp:125-178), which reads level_set
+ * and the weights of device_grid and zeroes u/v/w on invalid faces. Each
+ * returns after completion. */
+int elq_compute_weights_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, const float* solid_nodal,
+                               float* u_weight, float* v_weight, float* w_weight);
+int elq_add_gravity_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* v, float dt);
+int elq_valid_and_zero_device(elq_ctx* ctx, const elq_grid* device_grid);
+
 /* Device memory helpers for callers without their own allocator. */
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out);
 int elq_device_free(elq_ctx* ctx, void* ptr);

@@ -1106,6 +1106,8 @@ __device__ __forceinline__ size_t bslot(int b, int lane) {
 // unconditional loads (unused requests point at a valid dummy word and
 // count as present). issue() starts a round; finish() completes it and
 // re-polls, all words at once, until every requested tag matches.
+constexpr int kPollSleep = 1;  // s_sleep units (64 clocks) between poll rounds
+
 template <class T>
 struct Polls2 {
     static constexpr int W = sizeof(T) / 4;
@@ -1132,7 +1134,7 @@ struct Polls2 {
         if (!__all(ready(e))) {
             const uint64_t t0 = wall_clock64();
             for (int round = 1;; ++round) {
-                __builtin_amdgcn_s_sleep(1);
+                __builtin_amdgcn_s_sleep(kPollSleep);
                 load(x);
                 if (__all(ready(e))) break;
                 if ((round & 63) == 0) {
@@ -1257,18 +1259,28 @@ template <class T, bool UPDATE>
 __device__ __forceinline__ T fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdPre<T, UPDATE>& pr,
                                          const T (&got)[2], FwdCarry<T>& cy, T* y, uint64_t* xch, uint32_t epoch,
                                          const Lane& L, bool publish = true) {
-    const int h_own = L.lx + L.ly + L.lz;
     const bool ix = L.lx > 0, iy = L.ly > 0, iz = L.lz > 0;
+    // Values from outside the brick (previous brick, -y/-z faces) are fixed
+    // for the whole brick: they go into the right-hand side first. The
+    // in-brick couplings are scaled by the pivot, so each hyperplane step is
+    // three dependent fmas. Every lane evaluates every step; a lane's value is
+    // final from its own hyperplane lx+ly+lz on (its inputs are final by then
+    // and it keeps recomputing the same value), so no per-step select is needed.
+    T rhs = pr.rv;
+    rhs = ix ? rhs : fma(-pr.axm, pr.ex, rhs);
+    rhs = iy ? rhs : fma(-pr.aym, got[0], rhs);
+    rhs = iz ? rhs : fma(-pr.azm, got[1], rhs);
+    const T b0 = rhs * cur.ie;
+    const T cx = ix ? pr.axm * cur.ie : (T)0;
+    const T cyy = iy ? pr.aym * cur.ie : (T)0;
+    const T cz = iz ? pr.azm * cur.ie : (T)0;
     T yv = (T)0;
 #pragma unroll
     for (int h = 0; h < 10; ++h) {
         const T vx = move<Mv::xm>(yv), vy = move<Mv::ym>(yv), vz = move<Mv::zm>(yv);
-        if (h == h_own) {
-            T t = fma(-pr.axm, ix ? vx : pr.ex, pr.rv);
-            t = fma(-pr.aym, iy ? vy : got[0], t);
-            t = fma(-pr.azm, iz ? vz : got[1], t);
-            yv = t * cur.ie;
-        }
+        T t = fma(-cx, vx, b0);
+        t = fma(-cyy, vy, t);
+        yv = fma(-cz, vz, t);
     }
     if (publish) bput<T>(xch, cur.b, yv, epoch, L);
     y[(size_t)cur.b * kBrickCells + L.lane] = yv;
@@ -1316,18 +1328,22 @@ struct BwdCarry {
 template <class T>
 __device__ __forceinline__ T bwd_compute(const BwdIn<T>& cur, T ex, const T (&got)[2], BwdCarry<T>& cy, T* y,
                                          uint64_t* xch, uint32_t epoch, const Lane& L, bool publish = true) {
-    const int h_own = L.lx + L.ly + L.lz;
     const bool ix = L.lx < 3, iy = L.ly < 3, iz = L.lz < 3;
+    // As in fwd_compute: outside values into the right-hand side, couplings
+    // scaled by the pivot, every lane evaluated at every step.
+    const T cx = cur.ie * cur.axo, cyy = cur.ie * cur.ayo, cz = cur.ie * cur.azo;
+    T rhs = cur.yv;
+    rhs = ix ? rhs : fma(-cx, ex, rhs);
+    rhs = iy ? rhs : fma(-cyy, got[0], rhs);
+    rhs = iz ? rhs : fma(-cz, got[1], rhs);
+    const T kx = ix ? cx : (T)0, ky = iy ? cyy : (T)0, kz = iz ? cz : (T)0;
     T zv = (T)0;
 #pragma unroll
     for (int h = 9; h >= 0; --h) {
         const T vx = move<Mv::xp>(zv), vy = move<Mv::yp>(zv), vz = move<Mv::zp>(zv);
-        if (h == h_own) {
-            T t = cur.axo * (ix ? vx : ex);
-            t = fma(cur.ayo, iy ? vy : got[0], t);
-            t = fma(cur.azo, iz ? vz : got[1], t);
-            zv = fma(-cur.ie, t, cur.yv);
-        }
+        T t = fma(-kx, vx, rhs);
+        t = fma(-ky, vy, t);
+        zv = fma(-kz, vz, t);
     }
     if (publish) bput<T>(xch, cur.b, zv, epoch, L);
     y[(size_t)cur.b * kBrickCells + L.lane] = zv;

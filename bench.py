@@ -116,6 +116,107 @@ def profiled_traffic(wname, precision, kernel):
     return None, None
 
 
+def frame_steps(ctx, ds, reps=5):
+    """The frame steps around the projection on the workload's grids, device
+    resident, each timed with HIP events on the library's stream (median of
+    reps calls; the inputs are restored between calls, untimed):
+    compute_weights from the room() solid, add_gravity, find_valid_velocity,
+    set_invalid_velocity_zero, and after the projection extrapolate_velocity
+    (10 layers, the three face grids) and find_valid_level_set +
+    extrapolate_const of the level set. Algorithmic bytes: each input and
+    output array once (a cached neighbour read counts once); for the
+    extrapolation, every flag read once plus, per extrapolated cell, its
+    value and flag written and one neighbour value read (4 + 1 + 4 B)."""
+    s, L, h = ds.s, ctx.L, ctx.h
+    ni, nj, nk = s.ni, s.nj, s.nk
+    cells = s.cells
+    nf = [(ni + 1) * nj * nk, ni * (nj + 1) * nk, ni * nj * (nk + 1)]
+    faces = sum(nf)
+    nodes = (ni + 1) * (nj + 1) * (nk + 1)
+    nodal = np.empty(nodes, np.float32)
+    L.elq_scene_room_nodal(ni, nj, nk, nodal.ctypes.data)
+    p_nodal = ctx.alloc(nodal.nbytes)
+    ctx.h2d(p_nodal, nodal)
+    p_w = [ctx.alloc(4 * n) for n in nf]
+    p_flags = [ctx.alloc(n) for n in nf]
+    p_save = [ctx.alloc(4 * n) for n in nf]
+    p_fsave = [ctx.alloc(n) for n in nf]
+    p_ls = ctx.alloc(cells)
+    p_phi = ctx.alloc(4 * cells)
+    g = ds.grid()
+    out = {}
+
+    def timed(name, call, restore=None, classes=None):
+        ms = []
+        for _ in range(reps):
+            if restore:
+                restore()
+            ctx.set_profiling(True)
+            if call() != elq.OK:
+                raise SystemExit(f"{name} failed: {ctx.last_error()}")
+            kt = ctx.kernel_times()
+            ctx.set_profiling(False)
+            ms.append(sum(v[0] for k, v in kt.items() if classes is None or k in classes))
+        return float(np.median(ms)), kt
+
+    def entry(name, ms, nbytes, kt=None):
+        e = {"ms": round(ms, 4), "algorithmic_bytes": int(nbytes), "gbs": round(nbytes / (ms / 1e3) / 1e9, 1),
+             "hbm_frac": round(nbytes / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4)}
+        if kt and len(kt) > 1:
+            e["kernels_ms"] = {k: round(v[0], 4) for k, v in kt.items()}
+        out[name] = e
+
+    ms, _ = timed("weights", lambda: L.elq_compute_weights_device(h, ni, nj, nk, p_nodal, *p_w))
+    entry("compute_weights", ms, 3 * 4 * nodes + 4 * faces)
+    v_save = ctx.alloc(4 * nf[1])
+    ctx.d2d(v_save, ds.ptr["v"], 4 * nf[1])
+    ms, _ = timed("gravity", lambda: L.elq_add_gravity_device(h, ni, nj, nk, ds.ptr["v"], np.float32(s.dt)),
+                  restore=lambda: ctx.d2d(ds.ptr["v"], v_save, 4 * nf[1]))
+    ctx.d2d(ds.ptr["v"], v_save, 4 * nf[1])
+    ctx.free(v_save)
+    entry("add_gravity", ms, 8 * nf[1])
+    ms, _ = timed("find_valid", lambda: L.elq_find_valid_velocity_device(h, g, *p_flags))
+    entry("find_valid_velocity", ms, 4 * faces + 3 * 4 * cells + faces)
+    flags = [np.empty(n, np.uint8) for n in nf]
+    for f, p in zip(flags, p_flags):
+        ctx.d2h(f, p)
+    n_invalid = sum(int(f.size - np.count_nonzero(f)) for f in flags)
+    ms, _ = timed("zero", lambda: L.elq_set_invalid_velocity_zero_device(h, g, *p_flags))
+    entry("set_invalid_velocity_zero", ms, faces + 4 * n_invalid)
+    # the velocity after the projection, as the next frame's extrapolation sees it
+    for k, p, n in zip("uvw", p_save, nf):
+        ctx.d2d(p, ds.ptr[k], 4 * n)
+    for p, q, n in zip(p_fsave, p_flags, nf):
+        ctx.d2d(p, q, n)
+
+    def restore_vel():
+        for k, p, n in zip("uvw", p_save, nf):
+            ctx.d2d(ds.ptr[k], p, 4 * n)
+        for q, p, n in zip(p_flags, p_fsave, nf):
+            ctx.d2d(q, p, n)
+
+    ms, kt = timed("extrapolate", lambda: L.elq_extrapolate_velocity_device(h, g, *p_flags, 10), restore_vel)
+    after = [np.empty(n, np.uint8) for n in nf]
+    for f, p in zip(after, p_flags):
+        ctx.d2h(f, p)
+    n_new = sum(int(np.count_nonzero(a)) - int(np.count_nonzero(b)) for a, b in zip(after, flags))
+    entry("extrapolate_velocity", ms, faces + 9 * n_new, kt)
+    out["extrapolate_velocity"]["faces"] = faces
+    out["extrapolate_velocity"]["extrapolated_faces"] = n_new
+    restore_vel()
+    ms, _ = timed("ls_valid", lambda: L.elq_find_valid_level_set_device(h, ni, nj, nk, p_nodal, p_ls))
+    entry("find_valid_level_set", ms, 4 * nodes + cells)
+    ctx.d2d(p_phi, ds.ptr["phi"], 4 * cells)
+    ms, kt = timed("extrapolate_ls", lambda: L.elq_extrapolate_const_device(h, ni, nj, nk, p_phi, p_ls, 10),
+                   restore=lambda: ctx.d2d(p_phi, ds.ptr["phi"], 4 * cells))
+    ls = np.empty(cells, np.uint8)
+    ctx.d2h(ls, p_ls)
+    entry("extrapolate_level_set", ms, cells + 9 * int(cells - np.count_nonzero(ls)), kt)
+    for p in [p_nodal, p_ls, p_phi] + p_w + p_flags + p_save + p_fsave:
+        ctx.free(p)
+    return out
+
+
 def cpu_baseline(scene, fp64_iters):
     """Oracle (FP64 restatement, single thread) on the same inputs: one timed
     call with a small and one with a larger iteration cap give setup time and
@@ -236,7 +337,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true", help="time only the FP64 drop-in (profiling runs)")
-    ap.add_argument("--no-extra", action="store_true", help="skip the MIC(0) and host-buffer measurements")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the MIC(0), frame-step and host-buffer measurements")
+    ap.add_argument("--frame-only", action="store_true",
+                    help="one untimed projection, then only the frame steps (profiling runs)")
     ap.add_argument("--serve", type=int, default=512, metavar="N",
                     help="serving stream of N requests after the single-solve benchmark (0: skip)")
     ap.add_argument("--slots", type=int, default=4, help="serving slots")
@@ -263,6 +367,12 @@ def main():
 
     ctx = elq.Context(0)
     ds = elq.DeviceScene(ctx, scene)
+    if a.frame_only:
+        project(ctx, ds, o_mixed)
+        print(json.dumps({"workload": wname, "frame_steps": frame_steps(ctx, ds)}), flush=True)
+        ds.close()
+        ctx.close()
+        return
     out = {}
     if not a.fp64_only:
         res = run_steps(ctx, ds, o_mixed, a.steps, a.warmup)
@@ -335,6 +445,8 @@ def main():
                         "solves_per_s": round(1e3 / float(np.mean(t_mic[1:])), 3),
                         "iterations": st_mic.iterations, "outer_iterations": st_mic.outer_iterations,
                         "true_rel_residual": st_mic.true_rel_residual}
+    if not a.no_extra or a.frame_only:
+        line["frame_steps"] = frame_steps(ctx, ds)
     ds.close()
     if key == "mixed" and not a.no_extra:
         # the same projection through elq_project on host buffers (inputs copied

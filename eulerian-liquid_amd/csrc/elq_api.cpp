@@ -25,6 +25,7 @@
 
 #include "../../include/elq.h"
 #include "elq_api_internal.hpp"
+#include "elq_frame.hpp"
 #include "elq_kernels.hpp"
 
 using namespace elq;
@@ -929,7 +930,11 @@ int elq_compute_weights_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk,
     if (!solid_nodal || !u_weight || !v_weight || !w_weight) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
     (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    ctx->prof_begin("weights");
     HIPCHK(launch_weights(ni, nj, nk, solid_nodal, u_weight, v_weight, w_weight, ctx->stream), "k_weights");
+    ctx->prof_end();
+    ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
 }
@@ -939,7 +944,11 @@ int elq_add_gravity_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, flo
     if (!v) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
     (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    ctx->prof_begin("gravity");
     HIPCHK(launch_gravity(v, (size_t)ni * (size_t)(nj + 1) * (size_t)nk, dt, ctx->stream), "k_gravity");
+    ctx->prof_end();
+    ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
 }
@@ -951,6 +960,173 @@ int elq_valid_and_zero_device(elq_ctx* ctx, const elq_grid* g) {
     HIPCHK(launch_valid_zero(dense_of(*g), ctx->stream), "k_valid_zero");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+}
+
+int elq_find_valid_velocity_device(elq_ctx* ctx, const elq_grid* g, uint8_t* u_valid, uint8_t* v_valid,
+                                   uint8_t* w_valid) {
+    const int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!u_valid || !v_valid || !w_valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    ctx->prof_begin("find_valid");
+    HIPCHK(launch_find_valid(dense_of(*g), u_valid, v_valid, w_valid, ctx->stream), "k_find_valid");
+    ctx->prof_end();
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_set_invalid_velocity_zero_device(elq_ctx* ctx, const elq_grid* g, const uint8_t* u_valid,
+                                         const uint8_t* v_valid, const uint8_t* w_valid) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!g || !g->u || !g->v || !g->w || !u_valid || !v_valid || !w_valid)
+        return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(g->ni, g->nj, g->nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    const Sizes z = sizes_of(g->ni, g->nj, g->nk);
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    ctx->prof_begin("zero_invalid");
+    HIPCHK(launch_zero_invalid(g->u, u_valid, z.u, ctx->stream), "k_zero_invalid");
+    HIPCHK(launch_zero_invalid(g->v, v_valid, z.v, ctx->stream), "k_zero_invalid");
+    HIPCHK(launch_zero_invalid(g->w, w_valid, z.w, ctx->stream), "k_zero_invalid");
+    ctx->prof_end();
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_find_valid_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, const float* solid_nodal,
+                                    uint8_t* valid) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!solid_nodal || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    ctx->prof_begin("level_set_valid");
+    HIPCHK(launch_level_set_valid(ni, nj, nk, solid_nodal, valid, ctx->stream), "k_level_set_valid");
+    ctx->prof_end();
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+namespace {
+
+struct ExtArgs {
+    int ni, nj, nk;
+    float* f;
+    const uint8_t* valid_in;
+    uint8_t* valid_out;
+};
+
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+// extrapolation.cpp:5-46 over up to three grids at once. The loop's early
+// return (a layer that starts with no invalid cell) needs no host check: such
+// a layer changes nothing, and its workgroups see tinv == 0 and exit. Layers
+// beyond the largest grid dimension change nothing either (every cell is then
+// within reach of a valid cell, or there is none), so they are not launched.
+int run_extrapolate(elq_ctx* ctx, const ExtArgs* a, int count, int layers) {
+    ExtSet set{};
+    size_t off[3][4];
+    size_t total = 0;
+    int64_t tiles = 0;
+    int maxdim = 0;
+    for (int q = 0; q < count; ++q) {
+        const int64_t cells = (int64_t)a[q].ni * a[q].nj * a[q].nk;
+        const int64_t tq = ext_tiles(a[q].ni, a[q].nj, a[q].nk);
+        off[q][0] = total;
+        total += align256((size_t)cells);
+        for (int f = 1; f < 4; ++f) {
+            off[q][f] = total;
+            total += align256((size_t)tq);
+        }
+        ExtGrid& g = set.g[q];
+        g.f = a[q].f;
+        g.valid_in = a[q].valid_in;
+        g.valid_out = a[q].valid_out;
+        g.ni = a[q].ni;
+        g.nj = a[q].nj;
+        g.nk = a[q].nk;
+        g.tx = (a[q].ni + kExtTX - 1) / kExtTX;
+        g.ty = (a[q].nj + kExtTY - 1) / kExtTY;
+        g.tz = (a[q].nk + kExtTZ - 1) / kExtTZ;
+        g.tile0 = (int)tiles;
+        tiles += tq;
+        maxdim = std::max({maxdim, a[q].ni, a[q].nj, a[q].nk});
+    }
+    if (tiles > INT32_MAX / 4) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid too large");
+    const size_t list_off = total;
+    total += align256((size_t)tiles * 4);
+    const size_t counter_off = total;
+    total += 256;
+    set.count = count;
+    set.tiles = (int)tiles;
+    layers = std::min(layers, maxdim);
+    if (layers <= 0) return ELQ_OK;
+    HIPCHK(ctx->ws.ext.ensure(total), "alloc");
+    uint8_t* base = (uint8_t*)ctx->ws.ext.p;
+    for (int q = 0; q < count; ++q) {
+        set.g[q].stamp = base + off[q][0];
+        set.g[q].tinv = base + off[q][1];
+        set.g[q].tfv = base + off[q][2];
+        set.g[q].tnear = base + off[q][3];
+    }
+    set.list = (int32_t*)(base + list_off);
+    set.counter = (uint32_t*)(base + counter_off);
+    ctx->kt.clear();
+    ctx->prof_begin("ext_init");
+    HIPCHK(launch_ext_init(set, ctx->stream), "k_ext_init");
+    ctx->prof_end();
+    for (int l = 0, first = 0; l < layers; ++l) {
+        if (l - first == kExtRebase) {
+            HIPCHK(launch_ext_rebase(set, ctx->stream), "k_ext_rebase");
+            first = l;
+        }
+        ctx->prof_begin("ext_layer");
+        HIPCHK(launch_ext_layer(set, l - first, l & 1, ctx->stream), "k_ext_layer");
+        ctx->prof_end();
+    }
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+}  // namespace
+
+int elq_extrapolate_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* grid, uint8_t* valid,
+                           int32_t layers) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!grid || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    (void)hipSetDevice(ctx->device);
+    const ExtArgs a{ni, nj, nk, grid, valid, valid};
+    return run_extrapolate(ctx, &a, 1, layers);
+}
+
+int elq_extrapolate_const_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* grid,
+                                 const uint8_t* valid, int32_t layers) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!grid || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    (void)hipSetDevice(ctx->device);
+    const ExtArgs a{ni, nj, nk, grid, valid, nullptr};
+    return run_extrapolate(ctx, &a, 1, layers);
+}
+
+int elq_extrapolate_velocity_device(elq_ctx* ctx, const elq_grid* g, uint8_t* u_valid, uint8_t* v_valid,
+                                    uint8_t* w_valid, int32_t layers) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!g || !g->u || !g->v || !g->w || !u_valid || !v_valid || !w_valid)
+        return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    const int ni = g->ni, nj = g->nj, nk = g->nk;
+    if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    (void)hipSetDevice(ctx->device);
+    const ExtArgs a[3] = {{ni + 1, nj, nk, g->u, u_valid, u_valid},
+                          {ni, nj + 1, nk, g->v, v_valid, v_valid},
+                          {ni, nj, nk + 1, g->w, w_valid, w_valid}};
+    return run_extrapolate(ctx, a, 3, layers);
 }
 
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out) {

@@ -1,0 +1,355 @@
+// gfx950 kernels of the frame steps around the projection.
+//
+// Reference (daichi-nama/eulerian-liquid):
+//   extrapolate, extrapolate_const     src/project/3d/fluid/extrapolation.cpp:5-51
+//   Simulator::extrapolate_velocity    src/project/3d/eulerian_liquid/simulator.cpp:119-123
+//   Simulator::find_valid_velocity     src/project/3d/eulerian_liquid/simulator.cpp:125-158
+//   Simulator::find_valid_level_set    src/project/3d/eulerian_liquid/simulator.cpp:160-167
+//   Simulator::set_invalid_velocity_zero src/project/3d/eulerian_liquid/simulator.cpp:169-178
+//   calc_cell_centered_solid_phi       src/project/3d/eulerian_liquid/simulator.cpp:191-200
+//
+// All of it is byte and float streaming work, bound by HBM (or, for the
+// extrapolation layers, by how few tiles are active); results are bit-exact
+// with the reference (built with -ffp-contract=off; the float division is
+// IEEE round-to-nearest).
+//
+// Extrapolation. In the reference each layer takes a snapshot of the flags
+// and of the values, gives every invalid cell with at least one valid
+// neighbour (of the 26 around it) the mean of those neighbours, and marks it
+// valid. Here each cell carries a byte stamp instead of its flag: 0 for a
+// cell valid from the start, l + 1 for a cell that received its value in
+// layer l, 255 for a cell still invalid. "Valid at the start of layer l" is
+// stamp <= l, and a cell written in layer l gets stamp l + 1 > l, so the
+// layer reads its snapshot in place: no reader in layer l looks at a value or
+// a stamp that layer l changes the meaning of, and no copy of the grid is
+// needed. The same holds one level up for 32x8x8 tiles: tinv says a tile
+// still holds an invalid cell, tfv is the first layer at whose start it holds
+// a valid one; a layer skips every tile with no invalid cell or with no valid
+// cell in its 3x3x3 tile neighbourhood, which leaves such a tile unchanged in
+// the reference too.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "elq_frame.hpp"
+
+namespace elq {
+namespace {
+
+constexpr uint8_t kNever = 255;
+constexpr int kHX = kExtTX + 2, kHY = kExtTY + 2, kHZ = kExtTZ + 2;
+constexpr int kHalo = kHX * kHY * kHZ;
+static_assert(kExtTX == 32 && kExtTX * kExtTY == 256, "one thread per (x, y) column of a tile");
+static_assert(kExtRebase < kNever, "stamps of one stretch of layers fit below the invalid mark");
+
+__device__ __forceinline__ size_t at(int i, int j, int k, int ni, int nj) {
+    return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
+}
+
+inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// The grid of the set that owns global tile gt (tile0 ascending), by value:
+// the fields stay scalar loads from the kernel arguments.
+__device__ __forceinline__ ExtGrid pick(const ExtSet& s, int gt) {
+    if (s.count > 2 && gt >= s.g[2].tile0) return s.g[2];
+    if (s.count > 1 && gt >= s.g[1].tile0) return s.g[1];
+    return s.g[0];
+}
+
+struct TileXYZ {
+    int t, bx, by, bz;
+};
+
+__device__ __forceinline__ TileXYZ tile_of(const ExtGrid& g, int gt) {
+    TileXYZ r;
+    r.t = gt - g.tile0;
+    r.bx = r.t % g.tx;
+    r.by = (r.t / g.tx) % g.ty;
+    r.bz = r.t / (g.tx * g.ty);
+    return r;
+}
+
+// One workgroup per tile: stamps from the caller's flags, tinv and tfv.
+// Each thread takes an (x, y) column; its eight loads are issued together.
+__global__ __launch_bounds__(256) void k_ext_init(ExtSet set) {
+    const ExtGrid g = pick(set, blockIdx.x);
+    const TileXYZ T = tile_of(g, blockIdx.x);
+    const int i = T.bx * kExtTX + (threadIdx.x & (kExtTX - 1));
+    const int j = T.by * kExtTY + (threadIdx.x / kExtTX);
+    const bool col = i < g.ni && j < g.nj;
+    uint8_t v[kExtTZ];
+#pragma unroll
+    for (int z = 0; z < kExtTZ; ++z) {
+        const int k = T.bz * kExtTZ + z;
+        v[z] = (col && k < g.nk) ? g.valid_in[at(i, j, k, g.ni, g.nj)] : 1;
+    }
+    bool inv = false, val = false;
+#pragma unroll
+    for (int z = 0; z < kExtTZ; ++z) {
+        const int k = T.bz * kExtTZ + z;
+        if (col && k < g.nk) {
+            g.stamp[at(i, j, k, g.ni, g.nj)] = v[z] ? 0 : kNever;
+            inv |= v[z] == 0;
+            val |= v[z] != 0;
+        }
+    }
+    inv = __syncthreads_or(inv);
+    val = __syncthreads_or(val);
+    if (threadIdx.x == 0) {
+        g.tinv[T.t] = inv ? 1 : 0;
+        g.tfv[T.t] = val ? 0 : kNever;
+        if (blockIdx.x == 0) set.counter[0] = 0;
+    }
+}
+
+// One thread per tile: tnear = 0 where a tile of the 3x3x3 block around it
+// holds a valid cell at the start.
+__global__ __launch_bounds__(256) void k_ext_near(ExtSet set) {
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    if (gt >= set.tiles) return;
+    const ExtGrid g = pick(set, gt);
+    const TileXYZ T = tile_of(g, gt);
+    bool near = false;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+        const int x = T.bx + q % 3 - 1, y = T.by + (q / 3) % 3 - 1, z = T.bz + q / 9 - 1;
+        if (x >= 0 && y >= 0 && z >= 0 && x < g.tx && y < g.ty && z < g.tz)
+            near |= g.tfv[x + g.tx * (y + g.ty * z)] == 0;
+    }
+    g.tnear[T.t] = near ? 0 : kNever;
+}
+
+// One thread per tile: the tiles layer l works on (an invalid cell and a
+// valid one in its 3x3x3 tile block) are appended to the list; the order of
+// the list does not matter, the tiles of a layer being independent.
+__global__ __launch_bounds__(256) void k_ext_select(ExtSet set, int l, int parity) {
+    const int gt = blockIdx.x * 256 + threadIdx.x;
+    bool act = false;
+    if (gt < set.tiles) {
+        const ExtGrid g = pick(set, gt);
+        const int t = gt - g.tile0;
+        act = g.tinv[t] != 0 && g.tnear[t] <= l;
+    }
+    const uint64_t m = __ballot(act);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&set.counter[parity], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (act) set.list[base + __popcll(m & ((1ull << lane) - 1))] = gt;
+}
+
+// extrapolation.cpp:20-44, one layer over the listed tiles. A tile and its
+// one-cell halo are staged in LDS: a 0/1 mask (valid at the layer's start)
+// and the value of each valid cell, 0.0f for the others. Each thread walks
+// one (x, y) column of the tile in z with the 3x3 neighbourhoods of three
+// planes in registers, and sums all 27 masked values in the reference's
+// order kk, jj, ii = -1..1. Adding +0.0f for an invalid neighbour leaves the
+// float sum unchanged: the sum starts at +0.0f, and a float sum of +0.0f
+// and further terms is -0.0f only when both addends are -0.0f, so the
+// running sum is never -0.0f and s + 0.0f == s. The count is the integer
+// sum of the masks; the cell takes sum / (float)count.
+__global__ __launch_bounds__(256, 5) void k_ext_layer(ExtSet set, int l, int parity) {
+    constexpr int kPer = (kHalo + 255) / 256;
+    __shared__ uint8_t s_m[kPer * 256];
+    __shared__ float s_v[kPer * 256];
+    const uint32_t count = set.counter[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0) set.counter[parity ^ 1] = 0;
+    const int lx = threadIdx.x & (kExtTX - 1), ly = threadIdx.x / kExtTX;
+    int gt_next = blockIdx.x < count ? set.list[blockIdx.x] : 0;
+    for (uint32_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const int gt = gt_next;
+        if (idx + gridDim.x < count) gt_next = set.list[idx + gridDim.x];  // in flight during this tile
+        const ExtGrid g = pick(set, gt);
+        const TileXYZ T = tile_of(g, gt);
+        const uint8_t tfv0 = g.tfv[T.t];
+        const int i0 = T.bx * kExtTX - 1, j0 = T.by * kExtTY - 1, k0 = T.bz * kExtTZ - 1;
+        // one round trip: stamps and values of the whole halo (values of
+        // invalid cells are loaded too and dropped)
+        uint8_t st[kPer];
+        float fv[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const int h = threadIdx.x + 256 * r;
+            const int hx = h % kHX, hy = (h / kHX) % kHY, hz = h / (kHX * kHY);
+            const int i = i0 + hx, j = j0 + hy, k = k0 + hz;
+            const bool in = h < kHalo && i >= 0 && j >= 0 && k >= 0 && i < g.ni && j < g.nj && k < g.nk;
+            const size_t c = in ? at(i, j, k, g.ni, g.nj) : 0;
+            st[r] = in ? g.stamp[c] : kNever;
+            fv[r] = in ? g.f[c] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {
+            const bool valid = st[r] <= l;
+            s_m[threadIdx.x + 256 * r] = valid ? 1 : 0;
+            s_v[threadIdx.x + 256 * r] = valid ? fv[r] : 0.0f;
+        }
+        __syncthreads();
+
+        const int i = T.bx * kExtTX + lx, j = T.by * kExtTY + ly;
+        bool still = false, became = false;
+        if (i < g.ni && j < g.nj) {
+            // planes hz = z, z + 1, z + 2 of the halo hold cell planes z - 1, z, z + 1:
+            // their 3x3 masked values, valid counts and centre masks
+            float v[3][9];
+            int cnt[3];
+            bool mid[3];
+            auto load_plane = [&](int p, int hz) {
+                cnt[p] = 0;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) {
+                    const int h = (lx + q % 3) + kHX * ((ly + q / 3) + kHY * hz);
+                    v[p][q] = s_v[h];
+                    cnt[p] += s_m[h];
+                }
+                mid[p] = s_m[(lx + 1) + kHX * ((ly + 1) + kHY * hz)] != 0;
+            };
+            load_plane(0, 0);
+            load_plane(1, 1);
+#pragma unroll
+            for (int z = 0; z < kExtTZ; ++z) {
+                load_plane((z + 2) % 3, z + 2);
+                const int k = T.bz * kExtTZ + z;
+                if (k < g.nk && !mid[(z + 1) % 3]) {
+                    const int n = cnt[0] + cnt[1] + cnt[2];
+                    if (n > 0) {
+                        float sum = 0.0f;
+#pragma unroll
+                        for (int dp = 0; dp < 3; ++dp)
+#pragma unroll
+                            for (int q = 0; q < 9; ++q) sum += v[(z + dp) % 3][q];
+                        const size_t c = at(i, j, k, g.ni, g.nj);
+                        g.f[c] = sum / (float)n;
+                        g.stamp[c] = (uint8_t)(l + 1);
+                        if (g.valid_out) g.valid_out[c] = 1;
+                        became = true;
+                    } else {
+                        still = true;
+                    }
+                }
+            }
+        }
+        still = __syncthreads_or(still);
+        became = __syncthreads_or(became);
+        if (threadIdx.x == 0) g.tinv[T.t] = still ? 1 : 0;
+        // A tile's first valid cells make its 3x3x3 tile block "near" from the
+        // next layer on. Writers of one layer all write l + 1, and the select
+        // pass of this layer has already read tnear.
+        if (became && tfv0 == kNever && threadIdx.x < 27) {
+            const int q = threadIdx.x;
+            const int x = T.bx + q % 3 - 1, y = T.by + (q / 3) % 3 - 1, z = T.bz + q / 9 - 1;
+            if (x >= 0 && y >= 0 && z >= 0 && x < g.tx && y < g.ty && z < g.tz) {
+                uint8_t* tn = g.tnear + (x + g.tx * (y + g.ty * z));
+                if (*tn == kNever) *tn = (uint8_t)(l + 1);
+            }
+        }
+        if (became && threadIdx.x == 0 && tfv0 == kNever) g.tfv[T.t] = (uint8_t)(l + 1);
+        __syncthreads();  // LDS is free for the next tile
+    }
+}
+
+// One workgroup per tile: stamps and tile stamps <= kExtRebase become 0.
+__global__ __launch_bounds__(256) void k_ext_rebase(ExtSet set) {
+    const ExtGrid g = pick(set, blockIdx.x);
+    const TileXYZ T = tile_of(g, blockIdx.x);
+    const int i = T.bx * kExtTX + (threadIdx.x & (kExtTX - 1));
+    const int j = T.by * kExtTY + (threadIdx.x / kExtTX);
+    if (i < g.ni && j < g.nj) {
+        for (int z = 0; z < kExtTZ; ++z) {
+            const int k = T.bz * kExtTZ + z;
+            if (k >= g.nk) break;
+            const size_t c = at(i, j, k, g.ni, g.nj);
+            if (g.stamp[c] != kNever) g.stamp[c] = 0;
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (g.tfv[T.t] != kNever) g.tfv[T.t] = 0;
+        if (g.tnear[T.t] != kNever) g.tnear[T.t] = 0;
+    }
+}
+
+// simulator.cpp:125-158 for face axis AXIS: valid iff weight != 0, then a face
+// inside the loop ranges [1, n_face - 1) whose two cells both have phi >= 0
+// is invalid.
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_find_valid(Dense d, uint8_t* valid) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    const int ni = d.ni, nj = d.nj;
+    const int fi = ni + (AXIS == 0), fj = nj + (AXIS == 1), fk = d.nk + (AXIS == 2);
+    if (i >= fi || j >= fj) return;
+    const size_t f = at(i, j, k, fi, fj);
+    const float* wgt = AXIS == 0 ? d.uw : AXIS == 1 ? d.vw : d.ww;
+    bool ok = wgt[f] != 0.0f;
+    if (i >= 1 && j >= 1 && k >= 1 && i < fi - 1 && j < fj - 1 && k < fk - 1) {
+        const size_t c1 = at(i, j, k, ni, nj);
+        const size_t c0 = at(i - (AXIS == 0), j - (AXIS == 1), k - (AXIS == 2), ni, nj);
+        if (d.phi[c0] >= 0.0f && d.phi[c1] >= 0.0f) ok = false;
+    }
+    valid[f] = ok ? 1 : 0;
+}
+
+__global__ __launch_bounds__(256) void k_zero_invalid(float* f, const uint8_t* valid, size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride)
+        if (valid[c] == 0) f[c] = 0.0f;
+}
+
+// simulator.cpp:191-200 then :160-167. The eight nodes are summed in float in
+// the reference's order; the product with the double literal 0.125 is
+// rounded to float on assignment.
+__global__ __launch_bounds__(256) void k_level_set_valid(int ni, int nj, int nk, const float* nodal,
+                                                         uint8_t* valid) {
+    const int i = blockIdx.x * 64 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y, k = blockIdx.z;
+    if (i >= ni || j >= nj) return;
+    const int mi = ni + 1, mj = nj + 1;
+    const float s = nodal[at(i, j, k, mi, mj)] + nodal[at(i + 1, j, k, mi, mj)] + nodal[at(i, j + 1, k, mi, mj)] +
+                    nodal[at(i + 1, j + 1, k, mi, mj)] + nodal[at(i, j, k + 1, mi, mj)] +
+                    nodal[at(i + 1, j, k + 1, mi, mj)] + nodal[at(i, j + 1, k + 1, mi, mj)] +
+                    nodal[at(i + 1, j + 1, k + 1, mi, mj)];
+    const float centred = (float)((double)s * 0.125);
+    valid[at(i, j, k, ni, nj)] = centred < 0.0f ? 0 : 1;
+}
+
+}  // namespace
+
+hipError_t launch_ext_init(const ExtSet& set, hipStream_t s) {
+    hipLaunchKernelGGL(k_ext_init, dim3((unsigned)set.tiles), dim3(256), 0, s, set);
+    hipLaunchKernelGGL(k_ext_near, dim3(cdiv(set.tiles, 256)), dim3(256), 0, s, set);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_layer(const ExtSet& set, int l, int parity, hipStream_t s) {
+    hipLaunchKernelGGL(k_ext_select, dim3(cdiv(set.tiles, 256)), dim3(256), 0, s, set, l, parity);
+    const unsigned groups = (unsigned)std::min<int64_t>(set.tiles, kExtLayerGroups);
+    hipLaunchKernelGGL(k_ext_layer, dim3(groups), dim3(256), 0, s, set, l, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_rebase(const ExtSet& set, hipStream_t s) {
+    hipLaunchKernelGGL(k_ext_rebase, dim3((unsigned)set.tiles), dim3(256), 0, s, set);
+    return hipGetLastError();
+}
+
+hipError_t launch_find_valid(const Dense& d, uint8_t* uv, uint8_t* vv, uint8_t* wv, hipStream_t s) {
+    const dim3 block(64, 4);
+    hipLaunchKernelGGL(k_find_valid<0>, dim3(cdiv(d.ni + 1, 64), cdiv(d.nj, 4), d.nk), block, 0, s, d, uv);
+    hipLaunchKernelGGL(k_find_valid<1>, dim3(cdiv(d.ni, 64), cdiv(d.nj + 1, 4), d.nk), block, 0, s, d, vv);
+    hipLaunchKernelGGL(k_find_valid<2>, dim3(cdiv(d.ni, 64), cdiv(d.nj, 4), d.nk + 1), block, 0, s, d, wv);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_invalid(float* f, const uint8_t* valid, size_t n, hipStream_t s) {
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_zero_invalid, dim3(blocks), dim3(256), 0, s, f, valid, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_set_valid(int ni, int nj, int nk, const float* solid_nodal, uint8_t* valid,
+                                  hipStream_t s) {
+    hipLaunchKernelGGL(k_level_set_valid, dim3(cdiv(ni, 64), cdiv(nj, 4), nk), dim3(64, 4), 0, s, ni, nj, nk,
+                       solid_nodal, valid);
+    return hipGetLastError();
+}
+
+}  // namespace elq

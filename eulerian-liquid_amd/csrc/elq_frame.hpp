@@ -1,0 +1,71 @@
+// Host-side launch wrappers for the gfx950 kernels of the frame steps around
+// the projection: validity flags of the faces and of the level set, zeroing
+// of invalid faces, and the layered extrapolation of extrapolation.cpp.
+// Every wrapper enqueues on the given stream and returns the launch status.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "elq_kernels.hpp"
+
+namespace elq {
+
+// Extrapolation tiles: 32 x 8 x 8 cells per workgroup of 256 threads.
+constexpr int kExtTX = 32;
+constexpr int kExtTY = 8;
+constexpr int kExtTZ = 8;
+// Layer stamps are bytes; every kExtRebase layers they are folded back to 0.
+constexpr int kExtRebase = 250;
+
+// Workgroups of the persistent layer kernel (tiles are taken from the
+// layer's active list in strides of this many).
+constexpr int kExtLayerGroups = 1280;
+
+// One grid of an extrapolation (device pointers).
+struct ExtGrid {
+    float* f;                 // values, updated in place
+    const uint8_t* valid_in;  // caller's 0/1 flags (read by the init pass)
+    uint8_t* valid_out;       // set to 1 where a cell receives a value (nullptr: left alone)
+    uint8_t* stamp;           // per cell: 0 valid from the start, s: valid from layer s on, 255 invalid
+    uint8_t* tinv;            // per tile: 1 while the tile holds an invalid cell
+    uint8_t* tfv;             // per tile: first layer at whose start the tile holds a valid cell (255: none)
+    uint8_t* tnear;           // per tile: first layer at whose start a tile of its 3x3x3 block holds one
+    int ni, nj, nk;
+    int tx, ty, tz;           // tiles per axis
+    int tile0;                // first global tile index of this grid
+};
+
+// Up to three grids (u, v, w) extrapolated by the same launches.
+struct ExtSet {
+    ExtGrid g[3];
+    int count;
+    int tiles;         // tiles of all grids
+    int32_t* list;     // active tiles of the current layer (global tile indices)
+    uint32_t* counter; // two list lengths, used by alternate layers
+};
+
+inline int64_t ext_tiles(int ni, int nj, int nk) {
+    return (int64_t)((ni + kExtTX - 1) / kExtTX) * ((nj + kExtTY - 1) / kExtTY) * ((nk + kExtTZ - 1) / kExtTZ);
+}
+
+// Stamps, tile flags and the first layer's neighbourhood flags from valid_in.
+hipError_t launch_ext_init(const ExtSet& set, hipStream_t s);
+// Layer l (0-based since the last rebase, l < kExtRebase; parity = absolute
+// layer index & 1): the active-tile list, then the layer over it.
+hipError_t launch_ext_layer(const ExtSet& set, int l, int parity, hipStream_t s);
+// Fold every stamp <= kExtRebase back to 0 (valid from the start).
+hipError_t launch_ext_rebase(const ExtSet& set, hipStream_t s);
+
+// find_valid_velocity (simulator.cpp:125-167): 0/1 flags of the u, v, w faces.
+hipError_t launch_find_valid(const Dense& d, uint8_t* uv, uint8_t* vv, uint8_t* wv, hipStream_t s);
+// set_invalid_velocity_zero (simulator.cpp:169-178) on one face grid of n faces.
+hipError_t launch_zero_invalid(float* f, const uint8_t* valid, size_t n, hipStream_t s);
+// calc_cell_centered_solid_phi + find_valid_level_set (simulator.cpp:160-167,
+// 191-200): a cell is valid unless the mean of its 8 solid nodes is < 0.
+hipError_t launch_level_set_valid(int ni, int nj, int nk, const float* solid_nodal, uint8_t* valid,
+                                  hipStream_t s);
+
+}  // namespace elq

@@ -122,11 +122,10 @@ int elq_scene_level_set(int32_t ni, int32_t nj, int32_t nk, float dx, float slab
     return ELQ_OK;
 }
 
-int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* uw, float* vw, float* ww) {
-    if (ni < 1 || nj < 1 || nk < 1 || !uw || !vw || !ww) return ELQ_ERR_INVALID_ARG;
+int elq_scene_room_nodal(int32_t ni, int32_t nj, int32_t nk, float* nodal) {
+    if (ni < 1 || nj < 1 || nk < 1 || !nodal) return ELQ_ERR_INVALID_ARG;
     // room() on the nodal grid of (ni+1)(nj+1)(nk+1) nodes, origin -dx/2
     const int a = ni + 1, b = nj + 1, c = nk + 1;
-    std::vector<float> nodal((size_t)a * b * c);
     for (int k = 0; k < c; ++k)
         for (int j = 0; j < b; ++j)
             for (int i = 0; i < a; ++i) {
@@ -139,6 +138,14 @@ int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* uw, float* vw, 
                     v = 1.0f;
                 nodal[at(i, j, k, a, b)] = std::min(std::numeric_limits<float>::max(), v);
             }
+    return ELQ_OK;
+}
+
+int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* uw, float* vw, float* ww) {
+    if (ni < 1 || nj < 1 || nk < 1 || !uw || !vw || !ww) return ELQ_ERR_INVALID_ARG;
+    const int a = ni + 1, b = nj + 1, c = nk + 1;
+    std::vector<float> nodal((size_t)a * b * c);
+    elq_scene_room_nodal(ni, nj, nk, nodal.data());
     auto N = [&](int i, int j, int k) { return nodal[at(i, j, k, a, b)]; };
     for (int k = 0; k < nk; ++k)
         for (int j = 0; j < nj; ++j)

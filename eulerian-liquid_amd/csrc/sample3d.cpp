@@ -1,9 +1,11 @@
-// Sample driver in the reference's call order for the projection part of a
-// frame (apps/eulerian_liquid_3d.cpp:115-119: add_gravity,
-// find_valid_velocity, set_invalid_velocity_zero, solve_pressure), on the
-// drop scene: slab y < 0.4 plus a sphere of radius 0.15 at (0.5, 0.7, 0.5),
-// room() walls. Advection, extrapolation and reinitialisation are not part of
-// this library, so consecutive frames only accumulate gravity and project.
+// Sample driver in the reference's call order for a frame
+// (apps/eulerian_liquid_3d.cpp:106-120, update_second_order) on the drop
+// scene: slab y < 0.4 plus a sphere of radius 0.15 at (0.5, 0.7, 0.5), room()
+// walls. Set-up: compute_weights, find_valid_level_set (:137-139). Per frame:
+// extrapolate_velocity, extrapolate_level_set, add_gravity,
+// find_valid_velocity, set_invalid_velocity_zero, solve_pressure. Advection
+// and reinitialisation are not part of this library, so the liquid does not
+// move between frames.
 //
 // usage: elq_sample3d [quality=64] [frames=3] [tol=1e-4] [mixed=0]
 #include <cstdio>
@@ -23,13 +25,17 @@ int main(int argc, char** argv) {
         sim.create_grid(1.0f, q, q, q);
         sim.init_scene(0.4f, {{0.5f, 0.7f, 0.5f, 0.15f}});
         sim.compute_weights();
+        sim.find_valid_level_set();
         sim.opts().tol = tol;
         sim.opts().max_iter = 10000;
         sim.opts().precision = mixed ? ELQ_PRECISION_MIXED : ELQ_PRECISION_FP64;
         const float dt = 0.01f;
         for (int f = 0; f < frames; ++f) {
+            sim.extrapolate_velocity();
+            sim.extrapolate_level_set();
             sim.add_gravity(dt);
-            sim.find_valid_velocity_and_zero_invalid();
+            sim.find_valid_velocity();
+            sim.set_invalid_velocity_zero();
             sim.solve_pressure(dt);
             const elq_stats& s = sim.last_stats();
             std::printf("frame %d: N=%lld iterations=%d error=%.3e true_rel_res=%.3e total=%.2f ms\n", f,

@@ -30,6 +30,37 @@ elq_grid make_grid(int ni, int nj, int nk, float dx, const Grid3f& level_set, St
     throw std::runtime_error(std::string("elq: ") + elq_ctx_last_error(ctx));
 }
 
+void check(elq_ctx* ctx, int rc) {
+    if (rc != ELQ_OK) raise(ctx, rc);
+}
+
+// Device copy of one host array for the duration of a call.
+class DevArray {
+   public:
+    DevArray(elq_ctx* ctx, size_t bytes) : m_ctx(ctx) {
+        check(ctx, elq_device_alloc(ctx, bytes < 4 ? 4 : bytes, &m_p));
+        m_bytes = bytes;
+    }
+    template <class T>
+    DevArray(elq_ctx* ctx, const std::vector<T>& host) : DevArray(ctx, host.size() * sizeof(T)) {
+        check(ctx, elq_memcpy_h2d(ctx, m_p, host.data(), m_bytes));
+    }
+    ~DevArray() { elq_device_free(m_ctx, m_p); }
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    template <class T>
+    T* as() const { return static_cast<T*>(m_p); }
+    template <class T>
+    void fetch(std::vector<T>& host) const {
+        check(m_ctx, elq_memcpy_d2h(m_ctx, host.data(), m_p, host.size() * sizeof(T)));
+    }
+
+   private:
+    elq_ctx* m_ctx;
+    void* m_p = nullptr;
+    size_t m_bytes = 0;
+};
+
 }  // namespace
 
 Device::Device(int device, size_t hbm_budget) {
@@ -118,6 +149,21 @@ void update_velocity_using_pressure(Device& dev, StaggeredGrid3f& velocity, cons
     if (rc != ELQ_OK) raise(dev.ctx(), rc);
 }
 
+void extrapolate(Device& dev, Grid3f& grid, Grid3c& valid, int layers) {
+    elq_ctx* c = dev.ctx();
+    DevArray g(c, grid.data), v(c, valid.data);
+    check(c, elq_extrapolate_device(c, grid.ni, grid.nj, grid.nk, g.as<float>(), v.as<uint8_t>(), layers));
+    g.fetch(grid.data);
+    v.fetch(valid.data);
+}
+
+void extrapolate_const(Device& dev, Grid3f& grid, const Grid3c& valid, int layers) {
+    elq_ctx* c = dev.ctx();
+    DevArray g(c, grid.data), v(c, valid.data);
+    check(c, elq_extrapolate_const_device(c, grid.ni, grid.nj, grid.nk, g.as<float>(), v.as<uint8_t>(), layers));
+    g.fetch(grid.data);
+}
+
 }  // namespace fluid
 
 void Simulator::create_grid(float width, int ni, int nj, int nk) {
@@ -130,6 +176,8 @@ void Simulator::create_grid(float width, int ni, int nj, int nk) {
     m_level_set.create(ni, nj, nk);
     m_indices.create(ni, nj, nk);
     m_pressure.create(ni, nj, nk);
+    m_velocity_valid.create(ni, nj, nk, 1);  // simulator.cpp:59,63: created with value 1
+    m_level_set_valid.create(ni, nj, nk, 1);
 }
 
 void Simulator::init_scene(float pool_height, const std::vector<std::array<float, 4>>& spheres) {
@@ -156,6 +204,82 @@ void Simulator::find_valid_velocity_and_zero_invalid() {
     const int rc = elq_scene_valid_and_zero(&g);
     if (rc != ELQ_OK) throw std::runtime_error("elq: invalid grid");
 }
+
+void Simulator::find_valid_velocity() {
+    elq_ctx* c = m_dev.ctx();
+    DevArray phi(c, m_level_set.data), uw(c, m_velocity_weight.u().data), vw(c, m_velocity_weight.v().data),
+        ww(c, m_velocity_weight.w().data);
+    DevArray u(c, m_velocity.u().data), v(c, m_velocity.v().data), w(c, m_velocity.w().data);
+    m_velocity_valid.create(m_ni, m_nj, m_nk);
+    DevArray fu(c, m_velocity_valid.c[0].data.size()), fv(c, m_velocity_valid.c[1].data.size()),
+        fw(c, m_velocity_valid.c[2].data.size());
+    elq_grid g{};
+    g.ni = m_ni;
+    g.nj = m_nj;
+    g.nk = m_nk;
+    g.dx = m_dx;
+    g.level_set = phi.as<float>();
+    g.u = u.as<float>();
+    g.v = v.as<float>();
+    g.w = w.as<float>();
+    g.u_weight = uw.as<float>();
+    g.v_weight = vw.as<float>();
+    g.w_weight = ww.as<float>();
+    check(c, elq_find_valid_velocity_device(c, &g, fu.as<uint8_t>(), fv.as<uint8_t>(), fw.as<uint8_t>()));
+    fu.fetch(m_velocity_valid.c[0].data);
+    fv.fetch(m_velocity_valid.c[1].data);
+    fw.fetch(m_velocity_valid.c[2].data);
+}
+
+void Simulator::set_invalid_velocity_zero() {
+    elq_ctx* c = m_dev.ctx();
+    DevArray u(c, m_velocity.u().data), v(c, m_velocity.v().data), w(c, m_velocity.w().data);
+    DevArray fu(c, m_velocity_valid.c[0].data), fv(c, m_velocity_valid.c[1].data), fw(c, m_velocity_valid.c[2].data);
+    elq_grid g{};
+    g.ni = m_ni;
+    g.nj = m_nj;
+    g.nk = m_nk;
+    g.u = u.as<float>();
+    g.v = v.as<float>();
+    g.w = w.as<float>();
+    check(c, elq_set_invalid_velocity_zero_device(c, &g, fu.as<uint8_t>(), fv.as<uint8_t>(), fw.as<uint8_t>()));
+    u.fetch(m_velocity.u().data);
+    v.fetch(m_velocity.v().data);
+    w.fetch(m_velocity.w().data);
+}
+
+void Simulator::extrapolate_velocity() {
+    elq_ctx* c = m_dev.ctx();
+    DevArray u(c, m_velocity.u().data), v(c, m_velocity.v().data), w(c, m_velocity.w().data);
+    DevArray fu(c, m_velocity_valid.c[0].data), fv(c, m_velocity_valid.c[1].data), fw(c, m_velocity_valid.c[2].data);
+    elq_grid g{};
+    g.ni = m_ni;
+    g.nj = m_nj;
+    g.nk = m_nk;
+    g.u = u.as<float>();
+    g.v = v.as<float>();
+    g.w = w.as<float>();
+    check(c, elq_extrapolate_velocity_device(c, &g, fu.as<uint8_t>(), fv.as<uint8_t>(), fw.as<uint8_t>(), 10));
+    u.fetch(m_velocity.u().data);
+    v.fetch(m_velocity.v().data);
+    w.fetch(m_velocity.w().data);
+    fu.fetch(m_velocity_valid.c[0].data);
+    fv.fetch(m_velocity_valid.c[1].data);
+    fw.fetch(m_velocity_valid.c[2].data);
+}
+
+void Simulator::find_valid_level_set() {
+    elq_ctx* c = m_dev.ctx();
+    std::vector<float> nodal((size_t)(m_ni + 1) * (m_nj + 1) * (m_nk + 1));
+    check(c, elq_scene_room_nodal(m_ni, m_nj, m_nk, nodal.data()));
+    DevArray n(c, nodal);
+    m_level_set_valid.create(m_ni, m_nj, m_nk);
+    DevArray f(c, m_level_set_valid.data.size());
+    check(c, elq_find_valid_level_set_device(c, m_ni, m_nj, m_nk, n.as<float>(), f.as<uint8_t>()));
+    f.fetch(m_level_set_valid.data);
+}
+
+void Simulator::extrapolate_level_set() { fluid::extrapolate_const(m_dev, m_level_set, m_level_set_valid, 10); }
 
 void Simulator::solve_pressure(float dt) {
     elq_grid g = make_grid(m_ni, m_nj, m_nk, m_dx, m_level_set, m_velocity, m_velocity_weight);

@@ -2,12 +2,14 @@
 // over the C ABI of include/elq.h.
 //
 // Mirrors the interface a caller of the reference uses:
-//   namespace nama::_3d::fluid       (include/nama/project/3d/fluid/projection.h:10-22)
+//   namespace nama::_3d::fluid       (include/nama/project/3d/fluid/projection.h:10-22,
+//                                     extrapolation.h:7-8)
 //     assign_indices, build_matrix, build_grid_from_vector,
-//     update_velocity_using_pressure
+//     update_velocity_using_pressure, extrapolate, extrapolate_const
 //   class Simulator                  (include/nama/project/3d/eulerian_liquid/simulator.h:16-73)
 //     create_grid, init_scene, compute_weights, add_gravity,
-//     find_valid_velocity, set_invalid_velocity_zero, solve_pressure
+//     find_valid_velocity, set_invalid_velocity_zero, solve_pressure,
+//     extrapolate_velocity, find_valid_level_set, extrapolate_level_set
 // Grids are flat std::vector<float>/<int> in the reference layout
 // (i + ni*(j + nj*k)); staggered grids are three such vectors (u, v, w).
 // Errors follow the reference: solve_pressure and assign_indices throw
@@ -51,6 +53,18 @@ struct Grid3i {
     }
 };
 
+// Array3c (array.h:350): one 0/1 byte per cell
+struct Grid3c {
+    int ni = 0, nj = 0, nk = 0;
+    std::vector<uint8_t> data;
+    void create(int a, int b, int c, uint8_t value = 0) {
+        ni = a;
+        nj = b;
+        nk = c;
+        data.assign((size_t)a * b * c, value);
+    }
+};
+
 // staggered_grid.h:100-112: u (ni+1)nj nk, v ni(nj+1)nk, w ni nj(nk+1)
 struct StaggeredGrid3f {
     std::array<Grid3f, 3> c;
@@ -65,6 +79,15 @@ struct StaggeredGrid3f {
     const Grid3f& u() const { return c[0]; }
     const Grid3f& v() const { return c[1]; }
     const Grid3f& w() const { return c[2]; }
+};
+
+struct StaggeredGrid3c {
+    std::array<Grid3c, 3> c;
+    void create(int ni, int nj, int nk, uint8_t value = 0) {
+        c[0].create(ni + 1, nj, nk, value);
+        c[1].create(ni, nj + 1, nk, value);
+        c[2].create(ni, nj, nk + 1, value);
+    }
 };
 
 // Owns one device context (device buffers persist across frames).
@@ -99,6 +122,9 @@ void build_grid_from_vector(Grid3f& grid, const std::vector<double>& vector, con
 // projection.cpp:93-134
 void update_velocity_using_pressure(Device& dev, StaggeredGrid3f& velocity, const Grid3f& pressure,
                                     const Grid3f& level_set, const StaggeredGrid3f& weight, float dx, float dt);
+// extrapolation.cpp:5-46 (grid and valid updated) and :48-51 (valid unchanged)
+void extrapolate(Device& dev, Grid3f& grid, Grid3c& valid, int layers);
+void extrapolate_const(Device& dev, Grid3f& grid, const Grid3c& valid, int layers);
 }  // namespace fluid
 
 class Simulator {
@@ -113,7 +139,12 @@ class Simulator {
     void compute_weights();
     void add_gravity(float dt);                    // simulator.cpp:90-93
     void find_valid_velocity_and_zero_invalid();   // simulator.cpp:125-158 + 169-178
+    void find_valid_velocity();                    // simulator.cpp:125-158 (m_velocity_valid)
+    void set_invalid_velocity_zero();              // simulator.cpp:169-178
     void solve_pressure(float dt);                 // simulator.cpp:96-112
+    void extrapolate_velocity();                   // simulator.cpp:119-123 (10 layers)
+    void find_valid_level_set();                   // simulator.cpp:160-167 with :191-200
+    void extrapolate_level_set();                  // simulator.cpp:115-117 (10 layers)
 
     elq_opts& opts() { return m_opts; }
     const elq_stats& last_stats() const { return m_stats; }
@@ -125,6 +156,8 @@ class Simulator {
     Grid3f m_level_set;
     Grid3i m_indices;
     Grid3f m_pressure;
+    StaggeredGrid3c m_velocity_valid;
+    Grid3c m_level_set_valid;
 
    private:
     Device& m_dev;

@@ -105,6 +105,12 @@ def load():
     L.elq_compute_weights_device.argtypes = [vp, i32, i32, i32, vp, vp, vp, vp]
     L.elq_add_gravity_device.argtypes = [vp, i32, i32, i32, vp, f32]
     L.elq_valid_and_zero_device.argtypes = [vp, C.POINTER(Grid)]
+    L.elq_find_valid_velocity_device.argtypes = [vp, C.POINTER(Grid), vp, vp, vp]
+    L.elq_set_invalid_velocity_zero_device.argtypes = [vp, C.POINTER(Grid), vp, vp, vp]
+    L.elq_find_valid_level_set_device.argtypes = [vp, i32, i32, i32, vp, vp]
+    L.elq_extrapolate_device.argtypes = [vp, i32, i32, i32, vp, vp, i32]
+    L.elq_extrapolate_const_device.argtypes = [vp, i32, i32, i32, vp, vp, i32]
+    L.elq_extrapolate_velocity_device.argtypes = [vp, C.POINTER(Grid), vp, vp, vp, i32]
     L.elq_set_profiling_filter.argtypes = [vp, C.c_char_p]
     L.elq_set_profiling_filter.restype = None
     L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]
@@ -125,6 +131,7 @@ def load():
     L.elq_ctx_device_bytes.restype = sz
     L.elq_scene_level_set.argtypes = [i32, i32, i32, f32, f32, i32, vp, vp]
     L.elq_scene_weights.argtypes = [i32, i32, i32, vp, vp, vp]
+    L.elq_scene_room_nodal.argtypes = [i32, i32, i32, vp]
     L.elq_scene_add_gravity.argtypes = [i32, i32, i32, vp, f32]
     L.elq_scene_valid_and_zero.argtypes = [C.POINTER(Grid)]
     L.elq_scene_random_spheres.argtypes = [C.POINTER(C.c_uint64), i32, f64, f64, f64, f64, f64, f64, f64, vp]
@@ -386,6 +393,24 @@ class Context:
 
     def d2d(self, dst: int, src: int, nbytes: int):
         _check_rc(self.L.elq_memcpy_d2d(self.h, dst, src, nbytes), self.h)
+
+    def extrapolate(self, grid: np.ndarray, valid: np.ndarray, dims, layers: int = 10, const: bool = False):
+        """fluid::extrapolate (or extrapolate_const) of a host float32 grid of
+        dims (ni, nj, nk) and its uint8 0/1 flags, in place, through device
+        copies and elq_extrapolate[_const]_device."""
+        ni, nj, nk = dims
+        assert grid.dtype == np.float32 and valid.dtype == np.uint8 and grid.size == valid.size == ni * nj * nk
+        pg, pv = self.alloc(max(grid.nbytes, 4)), self.alloc(max(valid.nbytes, 4))
+        try:
+            self.h2d(pg, grid)
+            self.h2d(pv, valid)
+            fn = self.L.elq_extrapolate_const_device if const else self.L.elq_extrapolate_device
+            _check_rc(fn(self.h, ni, nj, nk, pg, pv, layers), self.h)
+            self.d2h(grid, pg)
+            self.d2h(valid, pv)
+        finally:
+            self.free(pg)
+            self.free(pv)
 
     def set_profiling(self, on: bool, only: str = ""):
         """Per-kernel event timing; only: comma-separated kernel classes to time (all if empty)."""

@@ -165,6 +165,46 @@ int elq_compute_weights_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk,
 int elq_add_gravity_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* v, float dt);
 int elq_valid_and_zero_device(elq_ctx* ctx, const elq_grid* device_grid);
 
+/* The same two steps one at a time, with the flags (one byte per face, 0/1,
+ * the face grids' sizes and layouts) in device memory, as the reference keeps
+ * them in Simulator::m_velocity_valid for extrapolate_velocity:
+ * find_valid_velocity (simulator.cpp:125-158) writes them from level_set and
+ * the weights of device_grid; set_invalid_velocity_zero (simulator.cpp:169-178)
+ * zeroes u/v/w where they are 0. */
+int elq_find_valid_velocity_device(elq_ctx* ctx, const elq_grid* device_grid, uint8_t* u_valid,
+                                   uint8_t* v_valid, uint8_t* w_valid);
+int elq_set_invalid_velocity_zero_device(elq_ctx* ctx, const elq_grid* device_grid, const uint8_t* u_valid,
+                                         const uint8_t* v_valid, const uint8_t* w_valid);
+/* calc_cell_centered_solid_phi + find_valid_level_set (simulator.cpp:191-200,
+ * 160-167): valid (ni*nj*nk bytes) is 0 where the mean of a cell's 8 solid
+ * nodes is < 0, else 1. */
+int elq_find_valid_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, const float* solid_nodal,
+                                    uint8_t* valid);
+
+/* The steps after the projection: extrapolation into invalid cells
+ * (src/project/3d/fluid/extrapolation.cpp:5-51), on device memory,
+ * bit-identical with the reference. A grid of ni*nj*nk floats (index
+ * i + ni*(j + nj*k)) with one 0/1 byte per cell; for up to `layers` layers,
+ * every invalid cell with a valid neighbour among the 26 around it takes the
+ * float mean of those neighbours' values (summed with k slowest, i fastest)
+ * and becomes valid. layers <= 0 does nothing.
+ *   elq_extrapolate_device        fluid::extrapolate: grid and valid in place
+ *                                 (extrapolation.cpp:5-46)
+ *   elq_extrapolate_const_device  fluid::extrapolate_const: valid unchanged
+ *                                 (extrapolation.cpp:48-51), e.g. the level set
+ *                                 with its valid flags (simulator.cpp:115-117)
+ *   elq_extrapolate_velocity_device  Simulator::extrapolate_velocity
+ *                                 (simulator.cpp:119-123): u, v, w of
+ *                                 device_grid with their flags, all three in
+ *                                 the same launches (level_set and weights
+ *                                 are not read). */
+int elq_extrapolate_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* grid, uint8_t* valid,
+                           int32_t layers);
+int elq_extrapolate_const_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* grid,
+                                 const uint8_t* valid, int32_t layers);
+int elq_extrapolate_velocity_device(elq_ctx* ctx, const elq_grid* device_grid, uint8_t* u_valid,
+                                    uint8_t* v_valid, uint8_t* w_valid, int32_t layers);
+
 /* Device memory helpers for callers without their own allocator. */
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out);
 int elq_device_free(elq_ctx* ctx, void* ptr);
@@ -259,6 +299,9 @@ int elq_scene_level_set(int32_t ni, int32_t nj, int32_t nk, float dx, float slab
                         int32_t n_spheres, const float* spheres, float* level_set);
 int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* u_weight, float* v_weight,
                       float* w_weight);
+/* The room() solid (levelset_emitter.cpp:18-32) on the nodal grid, the input
+ * of compute_weights: (ni+1)*(nj+1)*(nk+1) floats, i fastest. */
+int elq_scene_room_nodal(int32_t ni, int32_t nj, int32_t nk, float* nodal);
 int elq_scene_add_gravity(int32_t ni, int32_t nj, int32_t nk, float* v, float dt);
 int elq_scene_valid_and_zero(const elq_grid* host_grid);
 /* SplitMix64-driven draws: spheres (centres U[c_lo,c_hi]*L_axis, radii

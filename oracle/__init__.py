@@ -28,6 +28,7 @@ UNKNOWN_INDEX = 2147483647
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 
 
 class _System(C.Structure):
@@ -54,6 +55,11 @@ def lib():
         L.orc_compute_weights.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p, _f32p]
         L.orc_add_gravity.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, C.c_float]
         L.orc_find_valid_and_zero.argtypes = [C.c_int, C.c_int, C.c_int] + [_f32p] * 7
+        L.orc_find_valid_velocity.argtypes = [C.c_int, C.c_int, C.c_int] + [_f32p] * 4 + [_u8p] * 3
+        L.orc_find_valid_level_set.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _u8p]
+        L.orc_set_invalid_zero.argtypes = [C.c_size_t, _u8p, _f32p]
+        L.orc_extrapolate.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _u8p, C.c_int]
+        L.orc_extrapolate_const.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _u8p, C.c_int]
         L.orc_assign_indices.restype = C.c_int
         L.orc_assign_indices.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _f32p, _f32p, _f32p, _i32p]
         L.orc_build_matrix.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _i32p] + \
@@ -154,6 +160,27 @@ def add_gravity(s: Scene, dt: float):
 
 def find_valid_and_zero(s: Scene):
     lib().orc_find_valid_and_zero(s.ni, s.nj, s.nk, s.phi, s.uw, s.vw, s.ww, s.u, s.v, s.w)
+
+
+def find_valid_velocity(s: Scene):
+    """0/1 flags (uint8) of the u, v and w faces."""
+    out = tuple(np.empty(a.size, np.uint8) for a in (s.u, s.v, s.w))
+    lib().orc_find_valid_velocity(s.ni, s.nj, s.nk, s.phi, s.uw, s.vw, s.ww, *out)
+    return out
+
+
+def extrapolate(grid: np.ndarray, valid: np.ndarray, dims, layers: int = 10):
+    """extrapolation.cpp:5-46 in place on a flat float32 grid of dims
+    (ni, nj, nk) and its uint8 0/1 flags."""
+    ni, nj, nk = dims
+    assert grid.size == valid.size == ni * nj * nk
+    lib().orc_extrapolate(ni, nj, nk, grid, valid, layers)
+
+
+def extrapolate_const(grid: np.ndarray, valid: np.ndarray, dims, layers: int = 10):
+    ni, nj, nk = dims
+    assert grid.size == valid.size == ni * nj * nk
+    lib().orc_extrapolate_const(ni, nj, nk, grid, valid, layers)
 
 
 def random_velocity(s: Scene, rng: Rng, amp: float):

@@ -170,13 +170,14 @@ void orc_add_gravity(int ni, int nj, int nk, float* v, float dt) {
 
 /* simulator.cpp:125-158 (find_valid_velocity) then :169-178
  * (set_invalid_velocity_zero). */
-void orc_find_valid_and_zero(int ni, int nj, int nk, const float* phi, const float* uw,
-                             const float* vw, const float* ww, float* u, float* v, float* w) {
+/* simulator.cpp:125-167 (find_valid_velocity): every face starts valid iff
+ * its weight is non-zero; then, over the reference's loop ranges [1, n-1) of
+ * each face grid, a face between two cells with phi >= 0 is invalid. */
+void orc_find_valid_velocity(int ni, int nj, int nk, const float* phi, const float* uw,
+                             const float* vw, const float* ww, uint8_t* uv, uint8_t* vv,
+                             uint8_t* wv) {
     const size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk,
                  nw = (size_t)ni * nj * (nk + 1);
-    char* uv = (char*)malloc(nu);
-    char* vv = (char*)malloc(nv);
-    char* wv = (char*)malloc(nw);
     for (size_t c = 0; c < nu; ++c) uv[c] = (uw[c] == 0.0f) ? 0 : 1;
     for (size_t c = 0; c < nv; ++c) vv[c] = (vw[c] == 0.0f) ? 0 : 1;
     for (size_t c = 0; c < nw; ++c) wv[c] = (ww[c] == 0.0f) ? 0 : 1;
@@ -194,15 +195,103 @@ void orc_find_valid_and_zero(int ni, int nj, int nk, const float* phi, const flo
             for (int i = 1; i < ni - 1; ++i)
                 if (P(i, j, k - 1) >= 0.0f && P(i, j, k) >= 0.0f) wv[IDX(i, j, k, ni, nj)] = 0;
 #undef P
-    for (size_t c = 0; c < nu; ++c)
-        if (!uv[c]) u[c] = 0.0f;
-    for (size_t c = 0; c < nv; ++c)
-        if (!vv[c]) v[c] = 0.0f;
-    for (size_t c = 0; c < nw; ++c)
-        if (!wv[c]) w[c] = 0.0f;
+}
+
+/* simulator.cpp:169-178 (set_invalid_velocity_zero), one face grid */
+void orc_set_invalid_zero(size_t n, const uint8_t* valid, float* f) {
+    for (size_t c = 0; c < n; ++c)
+        if (valid[c] == 0) f[c] = 0.0f;
+}
+
+void orc_find_valid_and_zero(int ni, int nj, int nk, const float* phi, const float* uw,
+                             const float* vw, const float* ww, float* u, float* v, float* w) {
+    const size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk,
+                 nw = (size_t)ni * nj * (nk + 1);
+    uint8_t* uv = (uint8_t*)malloc(nu);
+    uint8_t* vv = (uint8_t*)malloc(nv);
+    uint8_t* wv = (uint8_t*)malloc(nw);
+    orc_find_valid_velocity(ni, nj, nk, phi, uw, vw, ww, uv, vv, wv);
+    orc_set_invalid_zero(nu, uv, u);
+    orc_set_invalid_zero(nv, vv, v);
+    orc_set_invalid_zero(nw, wv, w);
     free(uv);
     free(vv);
     free(wv);
+}
+
+/* simulator.cpp:191-200 (calc_cell_centered_solid_phi) then :160-167
+ * (find_valid_level_set). The eight nodes are summed in float, left to right;
+ * the product with the double literal 0.125 is rounded to float when stored
+ * in the float grid; a cell is invalid iff that value is < 0. */
+void orc_find_valid_level_set(int ni, int nj, int nk, const float* nodal, uint8_t* valid) {
+    const int mi = ni + 1, mj = nj + 1;
+#define N(i, j, k) nodal[IDX(i, j, k, mi, mj)]
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const float s = N(i, j, k) + N(i + 1, j, k) + N(i, j + 1, k) + N(i + 1, j + 1, k) +
+                                N(i, j, k + 1) + N(i + 1, j, k + 1) + N(i, j + 1, k + 1) +
+                                N(i + 1, j + 1, k + 1);
+                const float centred = (float)((double)s * 0.125);
+                valid[IDX(i, j, k, ni, nj)] = (centred < 0.0f) ? 0 : 1;
+            }
+#undef N
+}
+
+/* extrapolation.cpp:5-46 (extrapolate). Each layer works from a snapshot of
+ * the valid flags (old_valid) and of the values (grid; new values go to a
+ * copy, grid_temp, that becomes grid at the end of the layer). An invalid
+ * cell takes the float mean of its valid neighbours among the 26 around it,
+ * summed in the order kk, jj, ii = -1..1 (k slowest), divided once by the
+ * count; it then becomes valid. The loop returns early when a layer starts
+ * with no invalid cell. valid holds 0/1. */
+void orc_extrapolate(int ni, int nj, int nk, float* grid, uint8_t* valid, int layers) {
+    const size_t n = (size_t)ni * nj * nk;
+    float* temp = (float*)malloc(n * sizeof(float));
+    uint8_t* old = (uint8_t*)malloc(n);
+    memcpy(temp, grid, n * sizeof(float));
+    for (int l = 0; l < layers; ++l) {
+        int should_return = 1;
+        memcpy(old, valid, n);
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < nj; ++j)
+                for (int i = 0; i < ni; ++i) {
+                    const size_t c = IDX(i, j, k, ni, nj);
+                    if (old[c]) continue;
+                    should_return = 0;
+                    float sum = 0.0f;
+                    int count = 0;
+                    for (int kk = -1; kk <= 1; ++kk)
+                        for (int jj = -1; jj <= 1; ++jj)
+                            for (int ii = -1; ii <= 1; ++ii) {
+                                const int x = i + ii, y = j + jj, z = k + kk;
+                                if (x < 0 || x >= ni || y < 0 || y >= nj || z < 0 || z >= nk) continue;
+                                const size_t q = IDX(x, y, z, ni, nj);
+                                if (old[q]) {
+                                    sum += grid[q];
+                                    ++count;
+                                }
+                            }
+                    if (count > 0) {
+                        temp[c] = sum / (float)count;
+                        valid[c] = 1;
+                    }
+                }
+        if (should_return) break;
+        memcpy(grid, temp, n * sizeof(float));
+    }
+    free(temp);
+    free(old);
+}
+
+/* extrapolation.cpp:48-51 (extrapolate_const): the caller's flags stay as
+ * they were. */
+void orc_extrapolate_const(int ni, int nj, int nk, float* grid, const uint8_t* valid, int layers) {
+    const size_t n = (size_t)ni * nj * nk;
+    uint8_t* copy = (uint8_t*)malloc(n);
+    memcpy(copy, valid, n);
+    orc_extrapolate(ni, nj, nk, grid, copy, layers);
+    free(copy);
 }
 
 /* projection.cpp:8-12 (callers guarantee in-bounds arguments) */

@@ -26,6 +26,7 @@
 #ifndef ELQ_ORACLE_H
 #define ELQ_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -56,10 +57,28 @@ void orc_compute_weights(int ni, int nj, int nk, const float* nodal,
 /* simulator.cpp:90-93 */
 void orc_add_gravity(int ni, int nj, int nk, float* v, float dt);
 
-/* simulator.cpp:125-158 then :169-178 (valid flags are not returned) */
+/* simulator.cpp:125-167 (find_valid_velocity): 0/1 flags per face */
+void orc_find_valid_velocity(int ni, int nj, int nk, const float* phi, const float* uw,
+                             const float* vw, const float* ww, uint8_t* u_valid,
+                             uint8_t* v_valid, uint8_t* w_valid);
+
+/* simulator.cpp:169-178 (set_invalid_velocity_zero) on one face grid of n faces */
+void orc_set_invalid_zero(size_t n, const uint8_t* valid, float* f);
+
+/* simulator.cpp:125-167 then :169-178 (the flags are not returned) */
 void orc_find_valid_and_zero(int ni, int nj, int nk, const float* phi,
                              const float* uw, const float* vw, const float* ww,
                              float* u, float* v, float* w);
+
+/* simulator.cpp:191-200 + 160-167: level-set flags from the nodal solid phi */
+void orc_find_valid_level_set(int ni, int nj, int nk, const float* nodal, uint8_t* valid);
+
+/* extrapolation.cpp:5-46: grid and valid (0/1) of ni*nj*nk cells, in place */
+void orc_extrapolate(int ni, int nj, int nk, float* grid, uint8_t* valid, int layers);
+
+/* extrapolation.cpp:48-51: valid is left unchanged */
+void orc_extrapolate_const(int ni, int nj, int nk, float* grid, const uint8_t* valid,
+                           int layers);
 
 /* projection.cpp:14-28. Returns N (>0), 0 for "No fluid", or -1 when an
  * unknown lies on the outer cell layer (the reference then reads outside the

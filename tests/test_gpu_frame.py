@@ -1,0 +1,193 @@
+"""The frame steps after the projection on the GPU, through the C ABI, against
+the oracle, bit for bit (values as uint32 words, flags as bytes):
+
+- elq_extrapolate_device / elq_extrapolate_const_device: extrapolate and
+  extrapolate_const (extrapolation.cpp:5-51) on grids that are not tile
+  multiples, sparse and dense valid sets, 0, 1, 3 and 10 layers, a single
+  source, all-valid and no-valid grids, and a long grid whose 260 layers
+  cross the stamp rebase;
+- elq_find_valid_velocity_device, elq_set_invalid_velocity_zero_device
+  (simulator.cpp:125-178) and elq_extrapolate_velocity_device
+  (simulator.cpp:119-123) on scenes, in the reference's order;
+- elq_find_valid_level_set_device (simulator.cpp:160-167, 191-200);
+- the C++ shim's frame loop (elq_sample3d: extrapolate_velocity,
+  extrapolate_level_set, add_gravity, find_valid_velocity,
+  set_invalid_velocity_zero, solve_pressure) runs and converges.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import elq
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    if a.dtype == np.float32:
+        return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    return np.array_equal(a, b)
+
+
+def upload(ctx, a):
+    p = ctx.alloc(max(a.nbytes, 4))
+    ctx.h2d(p, a)
+    return p
+
+
+def download(ctx, p, like):
+    out = np.empty_like(like)
+    ctx.d2h(out, p)
+    return out
+
+
+def random_case(dims, density, seed):
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(-2, 2, n).astype(np.float32)
+    v = (rng.random(n) < density).astype(np.uint8)
+    return g, v
+
+
+CASES = [((37, 21, 19), 0.02, 1, 10), ((37, 21, 19), 0.3, 2, 3), ((64, 16, 16), 0.001, 3, 10),
+         ((33, 9, 17), 0.05, 4, 1), ((5, 40, 11), 0.01, 5, 10), ((70, 30, 25), 0.0005, 6, 10),
+         ((16, 16, 16), 0.1, 7, 0), ((16, 16, 16), 0.1, 8, -3)]
+
+
+@pytest.mark.parametrize("dims,density,seed,layers", CASES)
+def test_extrapolate_bit_exact(ctx, dims, density, seed, layers):
+    g, v = random_case(dims, density, seed)
+    rg, rv = g.copy(), v.copy()
+    O.extrapolate(rg, rv, dims, layers)
+    ctx.extrapolate(g, v, dims, layers)
+    assert same(g, rg) and same(v, rv)
+
+
+@pytest.mark.parametrize("dims,density,seed,layers", CASES[:3])
+def test_extrapolate_const_bit_exact(ctx, dims, density, seed, layers):
+    g, v = random_case(dims, density, seed)
+    v0 = v.copy()
+    rg = g.copy()
+    O.extrapolate_const(rg, v.copy(), dims, layers)
+    ctx.extrapolate(g, v, dims, layers, const=True)
+    assert same(g, rg) and same(v, v0)
+
+
+def test_extrapolate_single_source_and_trivial(ctx):
+    dims = (45, 23, 31)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(9)
+    g = rng.uniform(-1, 1, n).astype(np.float32)
+    v = np.zeros(n, np.uint8)
+    v[22 + 45 * (11 + 23 * 15)] = 1
+    rg, rv = g.copy(), v.copy()
+    O.extrapolate(rg, rv, dims, 10)
+    ctx.extrapolate(g, v, dims, 10)
+    assert same(g, rg) and same(v, rv) and v.sum() == 21 ** 3
+    for fill in (0, 1):                      # nothing valid, everything valid
+        g = rng.uniform(-1, 1, n).astype(np.float32)
+        v = np.full(n, fill, np.uint8)
+        g0, v0 = g.copy(), v.copy()
+        ctx.extrapolate(g, v, dims, 10)
+        assert same(g, g0) and same(v, v0)
+
+
+def test_extrapolate_long_run_crosses_stamp_rebase(ctx):
+    dims = (260, 3, 2)
+    n = int(np.prod(dims))
+    rng = np.random.default_rng(10)
+    g = rng.uniform(-1, 1, n).astype(np.float32)
+    v = np.zeros(n, np.uint8)
+    v[0] = 1                               # the far end is 259 layers away
+    rg, rv = g.copy(), v.copy()
+    O.extrapolate(rg, rv, dims, 400)
+    ctx.extrapolate(g, v, dims, 400)
+    assert same(g, rg) and same(v, rv) and v.all()
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_velocity_frame_steps_bit_exact(ctx, seed):
+    s = elq.scene_spheres(40, seed=seed, n_spheres=6)
+    ni, nj, nk = s.ni, s.nj, s.nk
+    rng = np.random.default_rng(seed)
+    for a in (s.u, s.v, s.w):
+        a[:] = rng.uniform(-1, 1, a.size).astype(np.float32)
+    # oracle: find_valid_velocity, set_invalid_velocity_zero, extrapolate_velocity
+    flags = [f.copy() for f in O.find_valid_velocity(s)]
+    ref = [a.copy() for a in (s.u, s.v, s.w)]
+    for a, f in zip(ref, flags):
+        O.lib().orc_set_invalid_zero(a.size, f, a)
+    ref_flags = [f.copy() for f in flags]
+    for a, f, d in zip(ref, ref_flags, ((ni + 1, nj, nk), (ni, nj + 1, nk), (ni, nj, nk + 1))):
+        O.extrapolate(a, f, d, 10)
+    ptr = {k: upload(ctx, getattr(s, k)) for k in ("phi", "u", "v", "w", "uw", "vw", "ww")}
+    fp = [ctx.alloc(a.size) for a in (s.u, s.v, s.w)]
+    try:
+        g = elq.Grid()
+        g.ni, g.nj, g.nk, g.dx = ni, nj, nk, s.dx
+        g.level_set, g.u, g.v, g.w = ptr["phi"], ptr["u"], ptr["v"], ptr["w"]
+        g.u_weight, g.v_weight, g.w_weight = ptr["uw"], ptr["vw"], ptr["ww"]
+        assert ctx.L.elq_find_valid_velocity_device(ctx.h, g, *fp) == elq.OK
+        for p, f in zip(fp, flags):
+            assert same(download(ctx, p, f), f)
+        assert ctx.L.elq_set_invalid_velocity_zero_device(ctx.h, g, *fp) == elq.OK
+        assert ctx.L.elq_extrapolate_velocity_device(ctx.h, g, *fp, 10) == elq.OK
+        for k, p, r, rf in zip("uvw", fp, ref, ref_flags):
+            assert same(download(ctx, ptr[k], r), r), k
+            assert same(download(ctx, p, rf), rf), k
+        assert any(not np.array_equal(a, b) for a, b in zip(flags, ref_flags))
+    finally:
+        for p in list(ptr.values()) + fp:
+            ctx.free(p)
+
+
+def test_level_set_valid_bit_exact(ctx):
+    for (ni, nj, nk, seed) in ((20, 14, 9, None), (13, 11, 17, 3)):
+        nodal = np.empty((ni + 1) * (nj + 1) * (nk + 1), np.float32)
+        if seed is None:
+            O.lib().orc_room_nodal(ni, nj, nk, nodal)
+        else:
+            rng = np.random.default_rng(seed)
+            nodal[:] = rng.uniform(-1, 1, nodal.size).astype(np.float32)
+            nodal[rng.random(nodal.size) < 0.1] = 0.0
+        ref = np.empty(ni * nj * nk, np.uint8)
+        O.lib().orc_find_valid_level_set(ni, nj, nk, nodal, ref)
+        pn, pv = upload(ctx, nodal), ctx.alloc(ref.size)
+        try:
+            assert ctx.L.elq_find_valid_level_set_device(ctx.h, ni, nj, nk, pn, pv) == elq.OK
+            assert same(download(ctx, pv, ref), ref)
+        finally:
+            ctx.free(pn)
+            ctx.free(pv)
+
+
+def test_level_set_extrapolation_const(ctx):
+    # extrapolate_level_set (simulator.cpp:115-117): the level set of a scene
+    # with the room's level-set flags; the flags stay as they were
+    s = elq.scene_drop(48)
+    ni, nj, nk = s.ni, s.nj, s.nk
+    nodal = np.empty((ni + 1) * (nj + 1) * (nk + 1), np.float32)
+    O.lib().orc_room_nodal(ni, nj, nk, nodal)
+    flags = np.empty(ni * nj * nk, np.uint8)
+    O.lib().orc_find_valid_level_set(ni, nj, nk, nodal, flags)
+    ref = s.phi.copy()
+    O.extrapolate_const(ref, flags, (ni, nj, nk), 10)
+    phi = s.phi.copy()
+    f0 = flags.copy()
+    ctx.extrapolate(phi, flags, (ni, nj, nk), 10, const=True)
+    assert same(phi, ref) and same(flags, f0) and not same(phi, s.phi)
+
+
+def test_cpp_shim_frame_loop():
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "eulerian-liquid_amd", "elq_sample3d")
+    r = subprocess.run([exe, "32", "3", "1e-6", "1"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    frames = [ln for ln in r.stdout.splitlines() if ln.startswith("frame")]
+    assert len(frames) == 3, r.stdout
+    for ln in frames:
+        res = float(ln.split("true_rel_res=")[1].split()[0])
+        assert res <= 1e-6, ln

@@ -122,8 +122,9 @@ def frame_steps(ctx, ds, reps=5):
     reps calls; the inputs are restored between calls, untimed):
     compute_weights from the room() solid, add_gravity, find_valid_velocity,
     set_invalid_velocity_zero, and after the projection extrapolate_velocity
-    (10 layers, the three face grids) and find_valid_level_set +
-    extrapolate_const of the level set. Algorithmic bytes: each input and
+    (10 layers, the three face grids), find_valid_level_set + extrapolate_const
+    of the level set, and the semi-Lagrangian advection of the level set and
+    of the velocity (kernel time). Algorithmic bytes: each input and
     output array once (a cached neighbour read counts once); for the
     extrapolation, every flag read once plus, per extrapolated cell, its
     value and flag written and one neighbour value read (4 + 1 + 4 B)."""
@@ -212,6 +213,18 @@ def frame_steps(ctx, ds, reps=5):
     ls = np.empty(cells, np.uint8)
     ctx.d2h(ls, p_ls)
     entry("extrapolate_level_set", ms, cells + 9 * int(cells - np.count_nonzero(ls)), kt)
+    # advection of the level set, then of the velocity (apps/eulerian_liquid_3d.cpp:111-112),
+    # kernel time only (the copy of the advected grid taken before the step is not counted)
+    restore_vel()
+    ms, _ = timed("advect_ls", lambda: L.elq_advect_level_set_device(h, ni, nj, nk, np.float32(s.dx),
+                                                                     np.float32(s.dt), p_phi, ds.ptr["u"],
+                                                                     ds.ptr["v"], ds.ptr["w"]),
+                  restore=lambda: ctx.d2d(p_phi, ds.ptr["phi"], 4 * cells), classes=("advect_cells",))
+    entry("advect_level_set", ms, 8 * cells + 4 * faces)
+    ms, _ = timed("advect_vel", lambda: L.elq_advect_velocity_device(h, g, np.float32(s.dt)), restore=restore_vel,
+                  classes=("advect_faces",))
+    entry("advect_velocity", ms, 8 * faces)
+    restore_vel()
     for p in [p_nodal, p_ls, p_phi] + p_w + p_flags + p_save + p_fsave:
         ctx.free(p)
     return out

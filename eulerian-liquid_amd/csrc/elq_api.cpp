@@ -1129,6 +1129,90 @@ int elq_extrapolate_velocity_device(elq_ctx* ctx, const elq_grid* g, uint8_t* u_
     return run_extrapolate(ctx, a, 3, layers);
 }
 
+int elq_advect_staggered_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt,
+                                float* out_u, float* out_v, float* out_w, const float* val_u, const float* val_v,
+                                const float* val_w, const float* vel_u, const float* vel_v, const float* vel_w) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!out_u || !out_v || !out_w || !val_u || !val_v || !val_w || !vel_u || !vel_v || !vel_w)
+        return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk) || !(dx > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    const float* in[6] = {val_u, val_v, val_w, vel_u, vel_v, vel_w};
+    for (const float* o : {out_u, out_v, out_w})
+        for (const float* p : in)
+            if (o == p) return ctx->fail(ELQ_ERR_INVALID_ARG, "outputs must not alias inputs");
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    ctx->prof_begin("advect_faces");
+    HIPCHK(launch_advect_faces(ni, nj, nk, dx, dt, vel_u, vel_v, vel_w, val_u, val_v, val_w, out_u, out_v, out_w,
+                               ctx->stream),
+           "k_advect_faces");
+    ctx->prof_end();
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_advect_collocated_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt, float* out,
+                                 const float* val, const float* vel_u, const float* vel_v, const float* vel_w) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!out || !val || !vel_u || !vel_v || !vel_w) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk) || !(dx > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    if (out == val || out == vel_u || out == vel_v || out == vel_w)
+        return ctx->fail(ELQ_ERR_INVALID_ARG, "output must not alias inputs");
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    ctx->prof_begin("advect_cells");
+    HIPCHK(launch_advect_cells(ni, nj, nk, dx, dt, vel_u, vel_v, vel_w, val, out, ctx->stream), "k_advect_cells");
+    ctx->prof_end();
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_advect_velocity_device(elq_ctx* ctx, const elq_grid* g, float dt) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!g || !g->u || !g->v || !g->w) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(g->ni, g->nj, g->nk) || !(g->dx > 0.0f))
+        return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    const Sizes z = sizes_of(g->ni, g->nj, g->nk);
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(ctx->ws.adv.ensure(4 * (z.u + z.v + z.w)), "alloc");
+    float* t = (float*)ctx->ws.adv.p;
+    float *tu = t, *tv = t + z.u, *tw = t + z.u + z.v;
+    // m_velocity_temp = m_velocity (simulator.cpp:80)
+    HIPCHK(launch_copy(tu, g->u, 4 * z.u, ctx->stream), "k_copy");
+    HIPCHK(launch_copy(tv, g->v, 4 * z.v, ctx->stream), "k_copy");
+    HIPCHK(launch_copy(tw, g->w, 4 * z.w, ctx->stream), "k_copy");
+    ctx->kt.clear();
+    ctx->prof_begin("advect_faces");
+    HIPCHK(launch_advect_faces(g->ni, g->nj, g->nk, g->dx, dt, tu, tv, tw, tu, tv, tw, g->u, g->v, g->w, ctx->stream),
+           "k_advect_faces");
+    ctx->prof_end();
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
+int elq_advect_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt,
+                                float* level_set, const float* u, const float* v, const float* w) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!level_set || !u || !v || !w) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk) || !(dx > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    const Sizes z = sizes_of(ni, nj, nk);
+    (void)hipSetDevice(ctx->device);
+    HIPCHK(ctx->ws.adv.ensure(4 * z.c), "alloc");
+    float* t = (float*)ctx->ws.adv.p;
+    // m_level_set_temp = m_level_set (simulator.cpp:85)
+    HIPCHK(launch_copy(t, level_set, 4 * z.c, ctx->stream), "k_copy");
+    ctx->kt.clear();
+    ctx->prof_begin("advect_cells");
+    HIPCHK(launch_advect_cells(ni, nj, nk, dx, dt, u, v, w, t, level_set, ctx->stream), "k_advect_cells");
+    ctx->prof_end();
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    return ELQ_OK;
+}
+
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out) {
     if (!ctx || !out) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);

@@ -35,6 +35,8 @@ struct Workspace {
     DevBuf export_buf;
     // extrapolation: per grid, cell stamps and two tile flag arrays
     DevBuf ext;
+    // advection: copy of the advected grids taken before the step
+    DevBuf adv;
 
     int64_t n_unknowns = 0;
     int nb = 0;
@@ -52,7 +54,7 @@ struct Workspace {
                           &diag64, &b64, &inv_e64, &inv_e32,
                           &cdiag, &cax, &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
                           &ctrl, &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
-                          &export_buf, &ext})
+                          &export_buf, &ext, &adv})
             f(*b);
     }
 

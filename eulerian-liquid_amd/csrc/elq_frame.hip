@@ -7,6 +7,9 @@
 //   Simulator::find_valid_level_set    src/project/3d/eulerian_liquid/simulator.cpp:160-167
 //   Simulator::set_invalid_velocity_zero src/project/3d/eulerian_liquid/simulator.cpp:169-178
 //   calc_cell_centered_solid_phi       src/project/3d/eulerian_liquid/simulator.cpp:191-200
+//   advect_collocated_grid,            src/project/3d/fluid/advection.cpp:9-27
+//   advect_staggered_grid              (trace_rk2 include/nama/math/ode.h:30-35, lerp
+//                                      include/nama/math/interpolation.h:10-72,186-193)
 //
 // All of it is byte and float streaming work, bound by HBM (or, for the
 // extrapolation layers, by how few tiles are active); results are bit-exact
@@ -311,6 +314,119 @@ __global__ __launch_bounds__(256) void k_level_set_valid(int ni, int nj, int nk,
     valid[at(i, j, k, ni, nj)] = centred < 0.0f ? 0 : 1;
 }
 
+// ---------------------------------------------------------------- advection
+// A grid with the reference's geometry: value (i, j, k) sits at
+// ((i + 0.5) dx, (j + 0.5) dx, (k + 0.5) dx) + origin (grid.h:355-367).
+struct Sampled {
+    const float* __restrict__ a;
+    int ni, nj, nk;
+    float o0, o1, o2;
+};
+
+// The simulator's grids have origin 0; a face grid is shifted by -dx/2 on
+// its own axis: origin - vector_type(0.5, 0, 0) * dx (staggered_grid.h:109-111).
+__host__ __device__ inline Sampled face_grid(const float* a, int axis, int ni, int nj, int nk, float dx) {
+    Sampled g;
+    g.a = a;
+    g.ni = ni + (axis == 0);
+    g.nj = nj + (axis == 1);
+    g.nk = nk + (axis == 2);
+    g.o0 = 0.0f - (axis == 0 ? 0.5f : 0.0f) * dx;
+    g.o1 = 0.0f - (axis == 1 ? 0.5f : 0.0f) * dx;
+    g.o2 = 0.0f - (axis == 2 ? 0.5f : 0.0f) * dx;
+    return g;
+}
+
+// interpolation.h:10-15
+__device__ __forceinline__ int index_and_fraction(float pos, float origin, float dx, float& frac) {
+    const float t = (pos - origin - 0.5f * dx) / dx;
+    const int i = (int)floorf(t);
+    frac = t - (float)i;
+    return i;
+}
+
+__device__ __forceinline__ float lerp2(float v0, float v1, float f) { return (1.0f - f) * v0 + f * v1; }
+
+// interpolation.h:35-72 (dim 3): indices clamped to the grid (grid.h:370-379),
+// x interpolated first, then y, then z.
+__device__ __forceinline__ float trilerp_at(const Sampled& g, float x, float y, float z, float dx) {
+    float fx, fy, fz;
+    int i = index_and_fraction(x, g.o0, dx, fx);
+    int j = index_and_fraction(y, g.o1, dx, fy);
+    int k = index_and_fraction(z, g.o2, dx, fz);
+    const int i0 = min(max(i, 0), g.ni - 1), i1 = min(max(i + 1, 0), g.ni - 1);
+    const int j0 = min(max(j, 0), g.nj - 1), j1 = min(max(j + 1, 0), g.nj - 1);
+    const int k0 = min(max(k, 0), g.nk - 1), k1 = min(max(k + 1, 0), g.nk - 1);
+    const size_t r00 = (size_t)g.ni * ((size_t)j0 + (size_t)g.nj * k0);
+    const size_t r10 = (size_t)g.ni * ((size_t)j1 + (size_t)g.nj * k0);
+    const size_t r01 = (size_t)g.ni * ((size_t)j0 + (size_t)g.nj * k1);
+    const size_t r11 = (size_t)g.ni * ((size_t)j1 + (size_t)g.nj * k1);
+    const float v0 = g.a[r00 + i0], v1 = g.a[r00 + i1], v2 = g.a[r10 + i0], v3 = g.a[r10 + i1];
+    const float v4 = g.a[r01 + i0], v5 = g.a[r01 + i1], v6 = g.a[r11 + i0], v7 = g.a[r11 + i1];
+    const float b0 = lerp2(lerp2(v0, v1, fx), lerp2(v2, v3, fx), fy);
+    const float b1 = lerp2(lerp2(v4, v5, fx), lerp2(v6, v7, fx), fy);
+    return lerp2(b0, b1, fz);
+}
+
+struct VelField {
+    Sampled u, v, w;
+};
+
+// ode.h:30-35 (trace_rk2) with the step -dt (advection.cpp:13,23); the field
+// is lerp_staggered (interpolation.h:186-193).
+__device__ __forceinline__ void backtrace(const VelField& F, float dx, float dt, float& x, float& y, float& z) {
+    const float h = -dt;
+    const float vx = trilerp_at(F.u, x, y, z, dx), vy = trilerp_at(F.v, x, y, z, dx),
+                vz = trilerp_at(F.w, x, y, z, dx);
+    const float half = 0.5f * h;
+    const float mx = x + half * vx, my = y + half * vy, mz = z + half * vz;
+    const float wx = trilerp_at(F.u, mx, my, mz, dx), wy = trilerp_at(F.v, mx, my, mz, dx),
+                wz = trilerp_at(F.w, mx, my, mz, dx);
+    x = x + h * wx;
+    y = y + h * wy;
+    z = z + h * wz;
+}
+
+struct Advect {
+    int ni, nj, nk;
+    float dx, dt;
+    VelField field;   // the velocity that carries the values
+    Sampled val[3];   // values of the advected grids (face grids, or val[0] a cell grid)
+    float* out[3];
+};
+
+// advection.cpp:19-27: one thread per (i, j, k) takes the u, v and w faces
+// there (those that exist), so the three backtraces gather from the same
+// neighbourhood of the velocity grids.
+__global__ __launch_bounds__(256) void k_advect_faces(Advect A) {
+    const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y,
+              k = blockIdx.z * 2 + threadIdx.z;
+    const float dx = A.dx;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const Sampled& g = A.val[d];
+        if (i >= g.ni || j >= g.nj || k >= g.nk) continue;
+        float x = ((float)i + 0.5f) * dx + g.o0;
+        float y = ((float)j + 0.5f) * dx + g.o1;
+        float z = ((float)k + 0.5f) * dx + g.o2;
+        backtrace(A.field, dx, A.dt, x, y, z);
+        A.out[d][(size_t)i + (size_t)g.ni * ((size_t)j + (size_t)g.nj * k)] = trilerp_at(g, x, y, z, dx);
+    }
+}
+
+// advection.cpp:9-17 on the cell grid (origin 0).
+__global__ __launch_bounds__(256) void k_advect_cells(Advect A) {
+    const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y,
+              k = blockIdx.z * 2 + threadIdx.z;
+    if (i >= A.ni || j >= A.nj || k >= A.nk) return;
+    const float dx = A.dx;
+    float x = ((float)i + 0.5f) * dx + 0.0f;
+    float y = ((float)j + 0.5f) * dx + 0.0f;
+    float z = ((float)k + 0.5f) * dx + 0.0f;
+    backtrace(A.field, dx, A.dt, x, y, z);
+    A.out[0][(size_t)i + (size_t)A.ni * ((size_t)j + (size_t)A.nj * k)] = trilerp_at(A.val[0], x, y, z, dx);
+}
+
 }  // namespace
 
 hipError_t launch_ext_init(const ExtSet& set, hipStream_t s) {
@@ -328,6 +444,53 @@ hipError_t launch_ext_layer(const ExtSet& set, int l, int parity, hipStream_t s)
 
 hipError_t launch_ext_rebase(const ExtSet& set, hipStream_t s) {
     hipLaunchKernelGGL(k_ext_rebase, dim3((unsigned)set.tiles), dim3(256), 0, s, set);
+    return hipGetLastError();
+}
+
+namespace {
+VelField make_field(int ni, int nj, int nk, float dx, const float* u, const float* v, const float* w) {
+    VelField F;
+    F.u = face_grid(u, 0, ni, nj, nk, dx);
+    F.v = face_grid(v, 1, ni, nj, nk, dx);
+    F.w = face_grid(w, 2, ni, nj, nk, dx);
+    return F;
+}
+}  // namespace
+
+hipError_t launch_advect_faces(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
+                               const float* vel_w, const float* val_u, const float* val_v, const float* val_w,
+                               float* out_u, float* out_v, float* out_w, hipStream_t s) {
+    Advect A{};
+    A.ni = ni;
+    A.nj = nj;
+    A.nk = nk;
+    A.dx = dx;
+    A.dt = dt;
+    A.field = make_field(ni, nj, nk, dx, vel_u, vel_v, vel_w);
+    const VelField V = make_field(ni, nj, nk, dx, val_u, val_v, val_w);
+    A.val[0] = V.u;
+    A.val[1] = V.v;
+    A.val[2] = V.w;
+    A.out[0] = out_u;
+    A.out[1] = out_v;
+    A.out[2] = out_w;
+    hipLaunchKernelGGL(k_advect_faces, dim3(cdiv(ni + 1, 32), cdiv(nj + 1, 4), cdiv(nk + 1, 2)), dim3(32, 4, 2), 0, s,
+                       A);
+    return hipGetLastError();
+}
+
+hipError_t launch_advect_cells(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
+                               const float* vel_w, const float* val, float* out, hipStream_t s) {
+    Advect A{};
+    A.ni = ni;
+    A.nj = nj;
+    A.nk = nk;
+    A.dx = dx;
+    A.dt = dt;
+    A.field = make_field(ni, nj, nk, dx, vel_u, vel_v, vel_w);
+    A.val[0] = Sampled{val, ni, nj, nk, 0.0f, 0.0f, 0.0f};
+    A.out[0] = out;
+    hipLaunchKernelGGL(k_advect_cells, dim3(cdiv(ni, 32), cdiv(nj, 4), cdiv(nk, 2)), dim3(32, 4, 2), 0, s, A);
     return hipGetLastError();
 }
 

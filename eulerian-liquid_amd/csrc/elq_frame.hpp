@@ -68,4 +68,15 @@ hipError_t launch_zero_invalid(float* f, const uint8_t* valid, size_t n, hipStre
 hipError_t launch_level_set_valid(int ni, int nj, int nk, const float* solid_nodal, uint8_t* valid,
                                   hipStream_t s);
 
+// advect_staggered_grid (advection.cpp:19-27): out_{u,v,w} = val_{u,v,w}
+// sampled at the RK2 backtrace (step -dt) of each face through the face
+// velocities vel_{u,v,w}. Outputs must not alias any input.
+hipError_t launch_advect_faces(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
+                               const float* vel_w, const float* val_u, const float* val_v, const float* val_w,
+                               float* out_u, float* out_v, float* out_w, hipStream_t s);
+// advect_collocated_grid (advection.cpp:9-17): out = val (cell grid) sampled
+// at the RK2 backtrace of each cell centre. out must not alias any input.
+hipError_t launch_advect_cells(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
+                               const float* vel_w, const float* val, float* out, hipStream_t s);
+
 }  // namespace elq

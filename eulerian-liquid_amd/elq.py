@@ -111,6 +111,10 @@ def load():
     L.elq_extrapolate_device.argtypes = [vp, i32, i32, i32, vp, vp, i32]
     L.elq_extrapolate_const_device.argtypes = [vp, i32, i32, i32, vp, vp, i32]
     L.elq_extrapolate_velocity_device.argtypes = [vp, C.POINTER(Grid), vp, vp, vp, i32]
+    L.elq_advect_staggered_device.argtypes = [vp, i32, i32, i32, f32, f32] + [vp] * 9
+    L.elq_advect_collocated_device.argtypes = [vp, i32, i32, i32, f32, f32] + [vp] * 5
+    L.elq_advect_velocity_device.argtypes = [vp, C.POINTER(Grid), f32]
+    L.elq_advect_level_set_device.argtypes = [vp, i32, i32, i32, f32, f32, vp, vp, vp, vp]
     L.elq_set_profiling_filter.argtypes = [vp, C.c_char_p]
     L.elq_set_profiling_filter.restype = None
     L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]

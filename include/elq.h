@@ -205,6 +205,32 @@ int elq_extrapolate_const_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t n
 int elq_extrapolate_velocity_device(elq_ctx* ctx, const elq_grid* device_grid, uint8_t* u_valid,
                                     uint8_t* v_valid, uint8_t* w_valid, int32_t layers);
 
+/* Semi-Lagrangian advection (src/project/3d/fluid/advection.cpp:9-27), on
+ * device memory, bit-identical with the reference: each sample point is
+ * traced back with the RK2 midpoint rule over -dt (include/nama/math/ode.h:30-35)
+ * through the trilinearly interpolated face velocities, and the advected
+ * grid is interpolated trilinearly there, indices clamped to the grid
+ * (include/nama/math/interpolation.h:10-72,186-193). Cell (i,j,k) sits at
+ * ((i+0.5)dx, (j+0.5)dx, (k+0.5)dx); face grids are shifted by -dx/2 on
+ * their own axis. Outputs must not alias inputs.
+ *   elq_advect_staggered_device   fluid::advect_staggered_grid: out_{u,v,w} =
+ *                                 val_{u,v,w} carried by vel_{u,v,w}
+ *   elq_advect_collocated_device  fluid::advect_collocated_grid: out (cell
+ *                                 grid) = val carried by vel_{u,v,w}
+ *   elq_advect_velocity_device    Simulator::advect_velocity (simulator.cpp:79-82):
+ *                                 u, v, w of device_grid advected by themselves,
+ *                                 in place (the library keeps the copy)
+ *   elq_advect_level_set_device   Simulator::advect_level_set (simulator.cpp:84-87):
+ *                                 level_set advected by u, v, w, in place */
+int elq_advect_staggered_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt,
+                                float* out_u, float* out_v, float* out_w, const float* val_u, const float* val_v,
+                                const float* val_w, const float* vel_u, const float* vel_v, const float* vel_w);
+int elq_advect_collocated_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt, float* out,
+                                 const float* val, const float* vel_u, const float* vel_v, const float* vel_w);
+int elq_advect_velocity_device(elq_ctx* ctx, const elq_grid* device_grid, float dt);
+int elq_advect_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt,
+                                float* level_set, const float* u, const float* v, const float* w);
+
 /* Device memory helpers for callers without their own allocator. */
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out);
 int elq_device_free(elq_ctx* ctx, void* ptr);

@@ -56,6 +56,9 @@ def lib():
         L.orc_add_gravity.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, C.c_float]
         L.orc_find_valid_and_zero.argtypes = [C.c_int, C.c_int, C.c_int] + [_f32p] * 7
         L.orc_find_valid_velocity.argtypes = [C.c_int, C.c_int, C.c_int] + [_f32p] * 4 + [_u8p] * 3
+        L.orc_advect_staggered.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float] + [_f32p] * 9
+        L.orc_advect_velocity.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float] + [_f32p] * 6
+        L.orc_advect_level_set.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float] + [_f32p] * 5
         L.orc_find_valid_level_set.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _u8p]
         L.orc_set_invalid_zero.argtypes = [C.c_size_t, _u8p, _f32p]
         L.orc_extrapolate.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _u8p, C.c_int]
@@ -181,6 +184,18 @@ def extrapolate_const(grid: np.ndarray, valid: np.ndarray, dims, layers: int = 1
     ni, nj, nk = dims
     assert grid.size == valid.size == ni * nj * nk
     lib().orc_extrapolate_const(ni, nj, nk, grid, valid, layers)
+
+
+def advect_velocity(s: Scene, dt: float):
+    """Simulator::advect_velocity (simulator.cpp:79-82) in place on s.u/v/w."""
+    u0, v0, w0 = s.u.copy(), s.v.copy(), s.w.copy()
+    lib().orc_advect_velocity(s.ni, s.nj, s.nk, np.float32(s.dx), np.float32(dt), u0, v0, w0, s.u, s.v, s.w)
+
+
+def advect_level_set(s: Scene, dt: float):
+    """Simulator::advect_level_set (simulator.cpp:84-87) in place on s.phi."""
+    phi0 = s.phi.copy()
+    lib().orc_advect_level_set(s.ni, s.nj, s.nk, np.float32(s.dx), np.float32(dt), phi0, s.u, s.v, s.w, s.phi)
 
 
 def random_velocity(s: Scene, rng: Rng, amp: float):

@@ -219,6 +219,131 @@ void orc_find_valid_and_zero(int ni, int nj, int nk, const float* phi, const flo
     free(wv);
 }
 
+/* ---------------------------------------------------------------- advection
+ * A 3-D grid of floats with the reference's Grid3 geometry: value (i,j,k) at
+ * ((i + 0.5) dx, (j + 0.5) dx, (k + 0.5) dx) + origin (grid.h:355-367). The
+ * simulator's grids have origin 0; the face grids are shifted by -dx/2 on
+ * their own axis (staggered_grid.h:109-111). */
+typedef struct {
+    const float* a;
+    int ni, nj, nk;
+    float o[3];
+} orc_field;
+
+static orc_field face_field(const float* a, int axis, int ni, int nj, int nk, float dx) {
+    orc_field f;
+    f.a = a;
+    f.ni = ni + (axis == 0);
+    f.nj = nj + (axis == 1);
+    f.nk = nk + (axis == 2);
+    /* origin - vector_type(0.5, 0, 0) * dx with origin 0: 0 - 0.5f * dx on the
+     * axis, 0 - 0 * dx = +0 elsewhere */
+    for (int c = 0; c < 3; ++c) f.o[c] = 0.0f - ((c == axis) ? 0.5f : 0.0f) * dx;
+    return f;
+}
+
+/* interpolation.h:10-15 (calc_index_and_fraction) */
+static int index_and_fraction(float pos, float origin, float dx, float* frac) {
+    const float t = (pos - origin - 0.5f * dx) / dx;
+    const int i = (int)floorf(t);
+    *frac = t - (float)i;
+    return i;
+}
+
+/* grid.h:370-379 (extrapolate: indices clamped to the grid) */
+static float clamped(const orc_field* g, int i, int j, int k) {
+    i = i < 0 ? 0 : (i > g->ni - 1 ? g->ni - 1 : i);
+    j = j < 0 ? 0 : (j > g->nj - 1 ? g->nj - 1 : j);
+    k = k < 0 ? 0 : (k > g->nk - 1 ? g->nk - 1 : k);
+    return g->a[IDX(i, j, k, g->ni, g->nj)];
+}
+
+/* interpolation.h:17-21 */
+static float lerp2(float v0, float v1, float f) { return (1.0f - f) * v0 + f * v1; }
+
+/* interpolation.h:35-72 (dim 3) with bilerp/trilerp of :23-33: x first, then
+ * y, then z */
+static float trilerp_at(const orc_field* g, const float x[3], float dx) {
+    float fx, fy, fz;
+    const int i = index_and_fraction(x[0], g->o[0], dx, &fx);
+    const int j = index_and_fraction(x[1], g->o[1], dx, &fy);
+    const int k = index_and_fraction(x[2], g->o[2], dx, &fz);
+    const float v0 = clamped(g, i, j, k), v1 = clamped(g, i + 1, j, k);
+    const float v2 = clamped(g, i, j + 1, k), v3 = clamped(g, i + 1, j + 1, k);
+    const float v4 = clamped(g, i, j, k + 1), v5 = clamped(g, i + 1, j, k + 1);
+    const float v6 = clamped(g, i, j + 1, k + 1), v7 = clamped(g, i + 1, j + 1, k + 1);
+    const float b0 = lerp2(lerp2(v0, v1, fx), lerp2(v2, v3, fx), fy);
+    const float b1 = lerp2(lerp2(v4, v5, fx), lerp2(v6, v7, fx), fy);
+    return lerp2(b0, b1, fz);
+}
+
+/* ode.h:8-19 (StaggeredGridField) + interpolation.h:186-193 (lerp_staggered) */
+static void velocity_at(const orc_field vel[3], const float x[3], float dx, float out[3]) {
+    for (int d = 0; d < 3; ++d) out[d] = trilerp_at(&vel[d], x, dx);
+}
+
+/* ode.h:30-35 (trace_rk2) with dt = -dt (advection.cpp:13,23) */
+static void backtrace(const orc_field vel[3], const float pos[3], float dt, float dx, float out[3]) {
+    const float h = -dt;
+    float v[3], mid[3];
+    velocity_at(vel, pos, dx, v);
+    const float half = 0.5f * h;
+    for (int c = 0; c < 3; ++c) mid[c] = pos[c] + half * v[c];
+    velocity_at(vel, mid, dx, v);
+    for (int c = 0; c < 3; ++c) out[c] = pos[c] + h * v[c];
+}
+
+/* advection.cpp:19-27 (advect_staggered_grid): out_* = val_* sampled at the
+ * RK2 backtrace of each face through vel_*. */
+void orc_advect_staggered(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
+                          const float* vel_w, const float* val_u, const float* val_v, const float* val_w,
+                          float* out_u, float* out_v, float* out_w) {
+    orc_field vel[3] = {face_field(vel_u, 0, ni, nj, nk, dx), face_field(vel_v, 1, ni, nj, nk, dx),
+                        face_field(vel_w, 2, ni, nj, nk, dx)};
+    orc_field val[3] = {face_field(val_u, 0, ni, nj, nk, dx), face_field(val_v, 1, ni, nj, nk, dx),
+                        face_field(val_w, 2, ni, nj, nk, dx)};
+    float* out[3] = {out_u, out_v, out_w};
+    for (int d = 0; d < 3; ++d) {
+        const orc_field* g = &val[d];
+        for (int k = 0; k < g->nk; ++k)
+            for (int j = 0; j < g->nj; ++j)
+                for (int i = 0; i < g->ni; ++i) {
+                    const float pos[3] = {((float)i + 0.5f) * dx + g->o[0], ((float)j + 0.5f) * dx + g->o[1],
+                                          ((float)k + 0.5f) * dx + g->o[2]};
+                    float back[3];
+                    backtrace(vel, pos, dt, dx, back);
+                    out[d][IDX(i, j, k, g->ni, g->nj)] = trilerp_at(g, back, dx);
+                }
+    }
+}
+
+/* Simulator::advect_velocity (simulator.cpp:79-82): the velocity field and
+ * the advected values are both the copy taken before the step (u0, v0, w0);
+ * u, v, w receive the result. */
+void orc_advect_velocity(int ni, int nj, int nk, float dx, float dt, const float* u0, const float* v0,
+                         const float* w0, float* u, float* v, float* w) {
+    orc_advect_staggered(ni, nj, nk, dx, dt, u0, v0, w0, u0, v0, w0, u, v, w);
+}
+
+/* advection.cpp:9-17 (advect_collocated_grid) as Simulator::advect_level_set
+ * calls it (simulator.cpp:84-87): phi0 is the copy before the step, the
+ * velocity grids are the current ones; phi receives the result. */
+void orc_advect_level_set(int ni, int nj, int nk, float dx, float dt, const float* phi0, const float* u,
+                          const float* v, const float* w, float* phi) {
+    orc_field vel[3] = {face_field(u, 0, ni, nj, nk, dx), face_field(v, 1, ni, nj, nk, dx),
+                        face_field(w, 2, ni, nj, nk, dx)};
+    orc_field g = {phi0, ni, nj, nk, {0.0f, 0.0f, 0.0f}};
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const float pos[3] = {((float)i + 0.5f) * dx + 0.0f, ((float)j + 0.5f) * dx + 0.0f,
+                                      ((float)k + 0.5f) * dx + 0.0f};
+                float back[3];
+                backtrace(vel, pos, dt, dx, back);
+                phi[IDX(i, j, k, ni, nj)] = trilerp_at(&g, back, dx);
+            }
+}
+
 /* simulator.cpp:191-200 (calc_cell_centered_solid_phi) then :160-167
  * (find_valid_level_set). The eight nodes are summed in float, left to right;
  * the product with the double literal 0.125 is rounded to float when stored

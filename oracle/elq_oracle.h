@@ -70,6 +70,20 @@ void orc_find_valid_and_zero(int ni, int nj, int nk, const float* phi,
                              const float* uw, const float* vw, const float* ww,
                              float* u, float* v, float* w);
 
+/* advection.cpp:19-27: out_* = val_* carried by vel_* (semi-Lagrangian RK2) */
+void orc_advect_staggered(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
+                          const float* vel_w, const float* val_u, const float* val_v, const float* val_w,
+                          float* out_u, float* out_v, float* out_w);
+
+/* advection.cpp:19-27 via simulator.cpp:79-82: semi-Lagrangian RK2
+ * advection of the face grids u0, v0, w0 by themselves into u, v, w */
+void orc_advect_velocity(int ni, int nj, int nk, float dx, float dt, const float* u0, const float* v0,
+                         const float* w0, float* u, float* v, float* w);
+
+/* advection.cpp:9-17 via simulator.cpp:84-87: phi0 advected by u, v, w into phi */
+void orc_advect_level_set(int ni, int nj, int nk, float dx, float dt, const float* phi0, const float* u,
+                          const float* v, const float* w, float* phi);
+
 /* simulator.cpp:191-200 + 160-167: level-set flags from the nodal solid phi */
 void orc_find_valid_level_set(int ni, int nj, int nk, const float* nodal, uint8_t* valid);
 

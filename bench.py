@@ -124,10 +124,14 @@ def frame_steps(ctx, ds, reps=5):
     set_invalid_velocity_zero, and after the projection extrapolate_velocity
     (10 layers, the three face grids), find_valid_level_set + extrapolate_const
     of the level set, and the semi-Lagrangian advection of the level set and
-    of the velocity (kernel time). Algorithmic bytes: each input and
+    of the velocity (kernel time), and the reinitialisation of the level set
+    (events over its kernels; wall_ms adds the host's pass-count reads).
+    Algorithmic bytes: each input and
     output array once (a cached neighbour read counts once); for the
     extrapolation, every flag read once plus, per extrapolated cell, its
-    value and flag written and one neighbour value read (4 + 1 + 4 B)."""
+    value and flag written and one neighbour value read (4 + 1 + 4 B); for
+    the reinitialisation, the level set and flags read and the level set
+    written once (9 B per cell)."""
     s, L, h = ds.s, ctx.L, ctx.h
     ni, nj, nk = s.ni, s.nj, s.nk
     cells = s.cells
@@ -225,6 +229,21 @@ def frame_steps(ctx, ds, reps=5):
                   classes=("advect_faces",))
     entry("advect_velocity", ms, 8 * faces)
     restore_vel()
+    # reinitialisation of the level set with the room's flags (simulator.cpp:181-183)
+    import ctypes as C
+    left, passes = C.c_int64(0), C.c_int32(0)
+    wall = []
+
+    def rei():
+        t0 = time.perf_counter()
+        rc = L.elq_reinitialize_device(h, ni, nj, nk, np.float32(s.dx), p_phi, p_ls, C.byref(left), C.byref(passes))
+        wall.append((time.perf_counter() - t0) * 1e3)
+        return rc
+
+    ms, kt = timed("reinit", rei, restore=lambda: ctx.d2d(p_phi, ds.ptr["phi"], 4 * cells))
+    entry("reinitialize", ms, 9 * cells, kt)
+    out["reinitialize"].update({"wall_ms": round(float(np.median(wall)), 4), "passes": passes.value,
+                                "unreached": left.value})
     for p in [p_nodal, p_ls, p_phi] + p_w + p_flags + p_save + p_fsave:
         ctx.free(p)
     return out

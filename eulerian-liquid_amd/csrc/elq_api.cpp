@@ -1213,6 +1213,87 @@ int elq_advect_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk
     return ELQ_OK;
 }
 
+int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float* level_set,
+                            const uint8_t* valid, int64_t* unreached, int32_t* passes) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!level_set || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
+    if (!valid_dims(ni, nj, nk) || !(dx > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
+    (void)hipSetDevice(ctx->device);
+    const size_t cells = (size_t)ni * nj * nk;
+    ReiSet r{};
+    r.ni = ni;
+    r.nj = nj;
+    r.nk = nk;
+    r.dx = dx;
+    r.tx = (ni + kExtTX - 1) / kExtTX;
+    r.ty = (nj + kExtTY - 1) / kExtTY;
+    r.tz = (nk + kExtTZ - 1) / kExtTZ;
+    r.tiles = r.tx * r.ty * r.tz;
+    size_t off = 0;
+    const size_t o_temp = off;
+    off += align256(4 * cells);
+    const size_t o_stamp = off;
+    off += align256(2 * cells);
+    const size_t o_tfar = off;
+    off += align256((size_t)r.tiles);
+    const size_t o_tfix = off;
+    off += align256(2 * (size_t)r.tiles);
+    const size_t o_tnear = off;
+    off += align256(2 * (size_t)r.tiles);
+    const size_t o_list = off;
+    off += align256(4 * (size_t)r.tiles);
+    const size_t o_counter = off;
+    off += 256;
+    HIPCHK(ctx->ws.rei.ensure(off), "alloc");
+    uint8_t* base = (uint8_t*)ctx->ws.rei.p;
+    r.phi = level_set;
+    r.temp = (const float*)(base + o_temp);
+    r.valid = valid;
+    r.stamp = (uint16_t*)(base + o_stamp);
+    r.tfar = base + o_tfar;
+    r.tfix = (uint16_t*)(base + o_tfix);
+    r.tnear = (uint16_t*)(base + o_tnear);
+    r.list = (int32_t*)(base + o_list);
+    r.counter = (uint32_t*)(base + o_counter);
+    ctx->kt.clear();
+    // grid_temp = grid (reinitialization.cpp:26)
+    HIPCHK(launch_copy(base + o_temp, level_set, 4 * cells, ctx->stream), "k_copy");
+    ctx->prof_begin("rei_init");
+    HIPCHK(launch_rei_init(r, ctx->stream), "k_rei_init");
+    ctx->prof_end();
+    // Passes in batches; after each batch the host reads how many cells the
+    // batch's last pass fixed. A pass that fixes nothing ends the march: no
+    // later pass could change anything (the reference then returns, or, with
+    // cells that no path reaches, loops forever). Passes launched after the
+    // end find no tile to work on.
+    constexpr int kBatch = 32;
+    int64_t first = 0;  // first pass of the current stamp epoch
+    int64_t l = 0;
+    uint32_t last_fixed = 1;
+    while (last_fixed != 0 && l <= (int64_t)cells) {
+        for (int q = 0; q < kBatch; ++q, ++l) {
+            if (l - first == kReiRebase) {
+                HIPCHK(launch_rei_rebase(r, ctx->stream), "k_rei_rebase");
+                first = l;
+            }
+            ctx->prof_begin("rei_pass");
+            HIPCHK(launch_rei_pass(r, (int)(l - first), (int)(l & 1), ctx->stream), "k_rei_pass");
+            ctx->prof_end();
+        }
+        HIPCHK(hipMemcpyAsync(&last_fixed, r.counter + 2 + ((l - 1) & 1), 4, hipMemcpyDeviceToHost, ctx->stream),
+               "read counter");
+        HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    }
+    HIPCHK(launch_rei_finish(r, ctx->stream), "k_rei_finish");
+    uint32_t fin[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(fin, r.counter + 4, 8, hipMemcpyDeviceToHost, ctx->stream), "read counter");
+    ctx->prof_collect();
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    if (unreached) *unreached = fin[0];
+    if (passes) *passes = (int32_t)(first + fin[1]);
+    return ELQ_OK;
+}
+
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out) {
     if (!ctx || !out) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);

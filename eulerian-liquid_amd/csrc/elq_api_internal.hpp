@@ -37,6 +37,8 @@ struct Workspace {
     DevBuf ext;
     // advection: copy of the advected grids taken before the step
     DevBuf adv;
+    // reinitialisation: level-set copy, stamps, tile flags, list, counters
+    DevBuf rei;
 
     int64_t n_unknowns = 0;
     int nb = 0;
@@ -54,7 +56,7 @@ struct Workspace {
                           &diag64, &b64, &inv_e64, &inv_e32,
                           &cdiag, &cax, &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
                           &ctrl, &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
-                          &export_buf, &ext, &adv})
+                          &export_buf, &ext, &adv, &rei})
             f(*b);
     }
 

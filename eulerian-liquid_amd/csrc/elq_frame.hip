@@ -7,6 +7,9 @@
 //   Simulator::find_valid_level_set    src/project/3d/eulerian_liquid/simulator.cpp:160-167
 //   Simulator::set_invalid_velocity_zero src/project/3d/eulerian_liquid/simulator.cpp:169-178
 //   calc_cell_centered_solid_phi       src/project/3d/eulerian_liquid/simulator.cpp:191-200
+//   reinitialize_fast_marching         src/project/3d/fluid/reinitialization.cpp:19-156
+//                                      (calc_sdf include/nama/math/level_set.h:165-264,
+//                                      vertex_interp include/nama/common/meshing.h:12-17)
 //   advect_collocated_grid,            src/project/3d/fluid/advection.cpp:9-27
 //   advect_staggered_grid              (trace_rk2 include/nama/math/ode.h:30-35, lerp
 //                                      include/nama/math/interpolation.h:10-72,186-193)
@@ -33,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 
 #include "elq_frame.hpp"
 
@@ -427,6 +431,345 @@ __global__ __launch_bounds__(256) void k_advect_cells(Advect A) {
     A.out[0][(size_t)i + (size_t)A.ni * ((size_t)j + (size_t)A.nj * k)] = trilerp_at(A.val[0], x, y, z, dx);
 }
 
+// ---------------------------------------------------------- reinitialisation
+// The reference marches from the interface one face-neighbour layer per pass:
+// FAR cells next to a FIX cell become NEAR, take an upwind distance from the
+// FIX neighbours' values (a snapshot, grid_temp), and become FIX. As in the
+// extrapolation, a 16-bit stamp per cell stands for the label and the
+// snapshot: FIX at the start of pass l is stamp <= l, and a cell fixed in pass
+// l gets l + 1, so each pass runs in place over the tiles it can change.
+
+struct P3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ P3 sub3(P3 a, P3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// Eigen's cross() for 3-vectors
+__device__ __forceinline__ P3 cross3(P3 u, P3 v) {
+    return {u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x};
+}
+
+// Eigen's norm() of a 3-vector: the fixed-size reduction splits the squares
+// in halves, x0 + (x1 + x2)
+__device__ __forceinline__ float norm3(P3 v) { return sqrtf(v.x * v.x + (v.y * v.y + v.z * v.z)); }
+
+// meshing.h:12-17 with isolevel 0
+__device__ __forceinline__ P3 vertex_interp(P3 p0, P3 p1, float l0, float l1) {
+    const float mu = -(l0 - 0.0f) / (l1 - l0);
+    return {p0.x + mu * (p1.x - p0.x), p0.y + mu * (p1.y - p0.y), p0.z + mu * (p1.z - p0.z)};
+}
+
+// level_set.h:212-237: distance from x0 to the zero set of the linear
+// interpolant on the edges x0-x1, x0-x2, x0-x3 (1, 2 or 3 crossings).
+__device__ float tet_distance(P3 x0, P3 x1, P3 x2, P3 x3, float f0, float f1, float f2, float f3) {
+    if (f0 == 0.0f) return 0.0f;
+    const bool b1 = f0 * f1 <= 0.0f, b2 = f0 * f2 <= 0.0f, b3 = f0 * f3 <= 0.0f;
+    const int n = (int)b1 + (int)b2 + (int)b3;
+    if (n == 0) return FLT_MAX;
+    // edges without a crossing may divide by zero here; their points are not used
+    const P3 p1 = vertex_interp(x0, x1, f0, f1), p2 = vertex_interp(x0, x2, f0, f2), p3 = vertex_interp(x0, x3, f0, f3);
+    const P3 first = b1 ? p1 : (b2 ? p2 : p3);
+    const P3 second = b1 ? (b2 ? p2 : p3) : p3;
+    if (n == 1) return norm3(sub3(first, x0));
+    if (n == 2) {  // level_set.h:172-177
+        const P3 b = sub3(second, first);
+        return norm3(cross3(sub3(x0, first), b)) / norm3(b);
+    }
+    // level_set.h:179-189
+    const P3 nv = cross3(sub3(p2, p1), sub3(p3, p1));
+    const float d = -(nv.x * p1.x + nv.y * p1.y + nv.z * p1.z);
+    return fabsf(nv.x * x0.x + nv.y * x0.y + nv.z * x0.z + d) / sqrtf(nv.x * nv.x + nv.y * nv.y + nv.z * nv.z);
+}
+
+__device__ __forceinline__ float rei_at(const float* a, int i, int j, int k, int ni, int nj, int nk) {
+    i = min(max(i, 0), ni - 1);
+    j = min(max(j, 0), nj - 1);
+    k = min(max(k, 0), nk - 1);
+    return a[at(i, j, k, ni, nj)];
+}
+
+// reinitialization.cpp:27-62 for one valid cell: FIX if phi changes sign (or
+// touches 0) towards a face neighbour (indices clamped); a FIX cell whose
+// phi is not 0 takes calc_sdf over the 8 tetrahedra of its 7-point star
+// (level_set.h:251-264). Star positions are Vector3f((i + 0.5) * dx, ...):
+// double products rounded to float.
+__device__ bool rei_interface(const ReiSet& r, int i, int j, int k, float& out) {
+    const int ni = r.ni, nj = r.nj, nk = r.nk;
+    const float* t = r.temp;
+    const float p = t[at(i, j, k, ni, nj)];
+    const float pxm = rei_at(t, i - 1, j, k, ni, nj, nk), pxp = rei_at(t, i + 1, j, k, ni, nj, nk);
+    const float pym = rei_at(t, i, j - 1, k, ni, nj, nk), pyp = rei_at(t, i, j + 1, k, ni, nj, nk);
+    const float pzm = rei_at(t, i, j, k - 1, ni, nj, nk), pzp = rei_at(t, i, j, k + 1, ni, nj, nk);
+    const bool flag = p * pxm <= 0.0f || p * pxp <= 0.0f || p * pym <= 0.0f || p * pyp <= 0.0f ||
+                      p * pzm <= 0.0f || p * pzp <= 0.0f;
+    if (!flag) return false;
+    out = p;
+    if (p != 0.0f) {
+        const double d = (double)r.dx;
+        const float xc = (float)((i + 0.5) * d), yc = (float)((j + 0.5) * d), zc = (float)((k + 0.5) * d);
+        const P3 x[7] = {{xc, yc, zc},
+                         {(float)(((i + 1) + 0.5) * d), yc, zc},
+                         {xc, (float)(((j + 1) + 0.5) * d), zc},
+                         {(float)(((i - 1) + 0.5) * d), yc, zc},
+                         {xc, (float)(((j - 1) + 0.5) * d), zc},
+                         {xc, yc, (float)(((k + 1) + 0.5) * d)},
+                         {xc, yc, (float)(((k - 1) + 0.5) * d)}};
+        const float f[7] = {p, pxp, pyp, pxm, pym, pzp, pzm};
+        float l = FLT_MAX;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int a = q + 1, b = (q + 1) % 4 + 1;  // star pairs (1,2) (2,3) (3,4) (4,1)
+#pragma unroll
+            for (int zz = 5; zz <= 6; ++zz) {
+                const float dist = tet_distance(x[0], x[a], x[b], x[zz], f[0], f[a], f[b], f[zz]);
+                l = (dist < l) ? dist : l;  // std::min(l, dist)
+            }
+        }
+        out = p < 0.0f ? -l : l;
+    }
+    return true;
+}
+
+__device__ __forceinline__ TileXYZ rei_tile(const ReiSet& r, int t) {
+    TileXYZ T;
+    T.t = t;
+    T.bx = t % r.tx;
+    T.by = (t / r.tx) % r.ty;
+    T.bz = t / (r.tx * r.ty);
+    return T;
+}
+
+// One workgroup per tile: interface cells, stamps, tfar and tfix.
+__global__ __launch_bounds__(256) void k_rei_init(ReiSet r) {
+    const TileXYZ T = rei_tile(r, blockIdx.x);
+    const int i = T.bx * kExtTX + (threadIdx.x & (kExtTX - 1));
+    const int j = T.by * kExtTY + (threadIdx.x / kExtTX);
+    bool far_any = false, fix_any = false;
+    if (i < r.ni && j < r.nj) {
+        for (int z = 0; z < kExtTZ; ++z) {
+            const int k = T.bz * kExtTZ + z;
+            if (k >= r.nk) break;
+            const size_t c = at(i, j, k, r.ni, r.nj);
+            uint16_t st;
+            float v;
+            if (r.valid[c] == 0) {
+                st = kReiUnknown;
+            } else if (rei_interface(r, i, j, k, v)) {
+                r.phi[c] = v;
+                st = 0;
+                fix_any = true;
+            } else {
+                st = kReiFar;
+                far_any = true;
+            }
+            r.stamp[c] = st;
+        }
+    }
+    far_any = __syncthreads_or(far_any);
+    fix_any = __syncthreads_or(fix_any);
+    if (threadIdx.x == 0) {
+        r.tfar[T.t] = far_any ? 1 : 0;
+        r.tfix[T.t] = fix_any ? 0 : kReiFar;
+        if (blockIdx.x == 0)
+            for (int q = 0; q < 6; ++q) r.counter[q] = 0;
+    }
+}
+
+// One thread per tile: tnear = 0 where the tile or a face neighbour holds a
+// FIX cell at the start.
+__global__ __launch_bounds__(256) void k_rei_near(ReiSet r) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= r.tiles) return;
+    const TileXYZ T = rei_tile(r, t);
+    bool near = r.tfix[t] == 0;
+    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int x = T.bx + off[q][0], y = T.by + off[q][1], z = T.bz + off[q][2];
+        if (x >= 0 && y >= 0 && z >= 0 && x < r.tx && y < r.ty && z < r.tz)
+            near |= r.tfix[x + r.tx * (y + r.ty * z)] == 0;
+    }
+    r.tnear[t] = near ? 0 : kReiFar;
+}
+
+// One thread per tile: the tiles pass l can change (a cell not reached yet
+// and a FIX cell in the tile or a face neighbour) are listed.
+__global__ __launch_bounds__(256) void k_rei_select(ReiSet r, int l, int parity) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) r.counter[2 + parity] = 0;
+    const bool act = t < r.tiles && r.tfar[t] != 0 && r.tnear[t] <= l;
+    const uint64_t m = __ballot(act);
+    if (!m) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&r.counter[parity], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (act) r.list[base + __popcll(m & ((1ull << lane) - 1))] = t;
+}
+
+// reinitialization.cpp:80-140 for one NEAR cell: one FIX value per axis (-x
+// before +x), signs folded, sorted, then the 1-, 2- or 3-neighbour upwind
+// distance with the reference's float and double steps. Asserts are off in
+// the reference's release build: mixed signs proceed unflipped.
+__device__ float rei_update(bool hx, float vx, bool hy, float vy, bool hz, float vz, float dx) {
+    const int m = (int)hx + (int)hy + (int)hz;
+    float a0 = hx ? vx : (hy ? vy : vz);
+    float a1 = hx ? (hy ? vy : vz) : vz;
+    float a2 = vz;
+    const bool all_ge = (!hx || vx >= 0.0f) && (!hy || vy >= 0.0f) && (!hz || vz >= 0.0f);
+    const bool all_le = (!hx || vx <= 0.0f) && (!hy || vy <= 0.0f) && (!hz || vz <= 0.0f);
+    const bool flip = !all_ge && all_le;
+    if (flip) {
+        a0 = -a0;
+        a1 = -a1;
+        a2 = -a2;
+    }
+    // ascending, as an insertion sort
+    if (m >= 2 && a1 < a0) {
+        const float t = a0;
+        a0 = a1;
+        a1 = t;
+    }
+    if (m == 3 && a2 < a1) {
+        float t = a1;
+        a1 = a2;
+        a2 = t;
+        if (a1 < a0) {
+            t = a0;
+            a0 = a1;
+            a1 = t;
+        }
+    }
+    float d = a0 + dx;
+    if (m >= 2 && d > a1) d = (float)(0.5 * (double)(a0 + a1 + sqrtf(2 * dx * dx - (a1 - a0) * (a1 - a0))));
+    if (m == 3 && d > a2) {
+        const float s = a0 + a1 + a2;
+        float q = s * s - 3.0f * (a0 * a0 + a1 * a1 + a2 * a2 - dx * dx);
+        q = (0.0f < q) ? q : 0.0f;  // std::max(0.0f, q)
+        d = (float)(0.333333 * (double)(a0 + a1 + a2 + sqrtf(q)));
+    }
+    return flip ? -d : d;
+}
+
+// One pass over the listed tiles (persistent). A tile and a one-cell halo of
+// stamps and values are staged in LDS; cells outside the grid count as not
+// FIX, as the reference's clamped label lookup returns the cell itself.
+__global__ __launch_bounds__(256) void k_rei_pass(ReiSet r, int l, int parity) {
+    constexpr int kPer = (kHalo + 255) / 256;
+    __shared__ uint16_t s_st[kPer * 256];
+    __shared__ float s_v[kPer * 256];
+    const uint32_t count = r.counter[parity];
+    if (blockIdx.x == 0 && threadIdx.x == 0) r.counter[parity ^ 1] = 0;
+    const int lx = threadIdx.x & (kExtTX - 1), ly = threadIdx.x / kExtTX;
+    int t_next = blockIdx.x < count ? r.list[blockIdx.x] : 0;
+    for (uint32_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
+        const int tt = t_next;
+        if (idx + gridDim.x < count) t_next = r.list[idx + gridDim.x];
+        const TileXYZ T = rei_tile(r, tt);
+        const uint16_t tfix0 = r.tfix[tt];
+        const int i0 = T.bx * kExtTX - 1, j0 = T.by * kExtTY - 1, k0 = T.bz * kExtTZ - 1;
+        uint16_t st[kPer];
+        float fv[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int h = threadIdx.x + 256 * q;
+            const int hx = h % kHX, hy = (h / kHX) % kHY, hz = h / (kHX * kHY);
+            const int i = i0 + hx, j = j0 + hy, k = k0 + hz;
+            const bool in = h < kHalo && i >= 0 && j >= 0 && k >= 0 && i < r.ni && j < r.nj && k < r.nk;
+            const size_t c = in ? at(i, j, k, r.ni, r.nj) : 0;
+            st[q] = in ? r.stamp[c] : kReiFar;
+            fv[q] = in ? r.phi[c] : 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            s_st[threadIdx.x + 256 * q] = st[q];
+            s_v[threadIdx.x + 256 * q] = fv[q];
+        }
+        __syncthreads();
+
+        const int i = T.bx * kExtTX + lx, j = T.by * kExtTY + ly;
+        bool still = false, became = false;
+        uint32_t fixed = 0;
+        if (i < r.ni && j < r.nj) {
+            for (int z = 0; z < kExtTZ; ++z) {
+                const int k = T.bz * kExtTZ + z;
+                if (k >= r.nk) break;
+                const int h = (lx + 1) + kHX * ((ly + 1) + kHY * (z + 1));
+                if (s_st[h] != kReiFar) continue;
+                const bool fxm = s_st[h - 1] <= l, fxp = s_st[h + 1] <= l;
+                const bool fym = s_st[h - kHX] <= l, fyp = s_st[h + kHX] <= l;
+                const bool fzm = s_st[h - kHX * kHY] <= l, fzp = s_st[h + kHX * kHY] <= l;
+                if (!(fxm || fxp || fym || fyp || fzm || fzp)) {
+                    still = true;
+                    continue;
+                }
+                const float vx = fxm ? s_v[h - 1] : s_v[h + 1];
+                const float vy = fym ? s_v[h - kHX] : s_v[h + kHX];
+                const float vz = fzm ? s_v[h - kHX * kHY] : s_v[h + kHX * kHY];
+                const size_t c = at(i, j, k, r.ni, r.nj);
+                r.phi[c] = rei_update(fxm || fxp, vx, fym || fyp, vy, fzm || fzp, vz, r.dx);
+                r.stamp[c] = (uint16_t)(l + 1);
+                became = true;
+                ++fixed;
+            }
+        }
+        still = __syncthreads_or(still);
+        became = __syncthreads_or(became);
+        // cells fixed by this tile, summed over its waves
+        fixed = __reduce_add_sync(~0ull, fixed);
+        if ((threadIdx.x & 63) == 0 && fixed) atomicAdd(&r.counter[2 + parity], fixed);
+        if (threadIdx.x == 0) r.tfar[tt] = still ? 1 : 0;
+        if (became && tfix0 == kReiFar && threadIdx.x < 7) {
+            const int q = threadIdx.x;
+            int x = T.bx, y = T.by, z = T.bz;
+            if (q == 1) x -= 1;
+            if (q == 2) x += 1;
+            if (q == 3) y -= 1;
+            if (q == 4) y += 1;
+            if (q == 5) z -= 1;
+            if (q == 6) z += 1;
+            if (x >= 0 && y >= 0 && z >= 0 && x < r.tx && y < r.ty && z < r.tz) {
+                uint16_t* tn = r.tnear + (x + r.tx * (y + r.ty * z));
+                if (*tn == kReiFar) *tn = (uint16_t)(l + 1);
+            }
+        }
+        if (became && threadIdx.x == 0 && tfix0 == kReiFar) r.tfix[tt] = (uint16_t)(l + 1);
+        __syncthreads();  // LDS is free for the next tile
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rei_rebase(ReiSet r) {
+    const size_t n = (size_t)r.ni * r.nj * r.nk;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+        const uint16_t s = r.stamp[c];
+        if (s != kReiFar && s != kReiUnknown) r.stamp[c] = 0;
+    }
+    for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)r.tiles; t += stride) {
+        if (r.tfix[t] != kReiFar) r.tfix[t] = 0;
+        if (r.tnear[t] != kReiFar) r.tnear[t] = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_rei_finish(ReiSet r) {
+    const size_t n = (size_t)r.ni * r.nj * r.nk;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    uint32_t far = 0, top = 0;
+    for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
+        const uint16_t s = r.stamp[c];
+        if (s == kReiFar) ++far;
+        else if (s != kReiUnknown) top = max(top, (uint32_t)s);
+    }
+    far = __reduce_add_sync(~0ull, far);
+    top = __reduce_max_sync(~0ull, top);
+    if ((threadIdx.x & 63) == 0) {
+        if (far) atomicAdd(&r.counter[4], far);
+        if (top) atomicMax(&r.counter[5], top);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_ext_init(const ExtSet& set, hipStream_t s) {
@@ -491,6 +834,29 @@ hipError_t launch_advect_cells(int ni, int nj, int nk, float dx, float dt, const
     A.val[0] = Sampled{val, ni, nj, nk, 0.0f, 0.0f, 0.0f};
     A.out[0] = out;
     hipLaunchKernelGGL(k_advect_cells, dim3(cdiv(ni, 32), cdiv(nj, 4), cdiv(nk, 2)), dim3(32, 4, 2), 0, s, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_rei_init(const ReiSet& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_rei_init, dim3((unsigned)r.tiles), dim3(256), 0, s, r);
+    hipLaunchKernelGGL(k_rei_near, dim3(cdiv(r.tiles, 256)), dim3(256), 0, s, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_rei_pass(const ReiSet& r, int l, int parity, hipStream_t s) {
+    hipLaunchKernelGGL(k_rei_select, dim3(cdiv(r.tiles, 256)), dim3(256), 0, s, r, l, parity);
+    const unsigned groups = (unsigned)std::min<int64_t>(r.tiles, kExtLayerGroups);
+    hipLaunchKernelGGL(k_rei_pass, dim3(groups), dim3(256), 0, s, r, l, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_rei_rebase(const ReiSet& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_rei_rebase, dim3(2048), dim3(256), 0, s, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_rei_finish(const ReiSet& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_rei_finish, dim3(2048), dim3(256), 0, s, r);
     return hipGetLastError();
 }
 

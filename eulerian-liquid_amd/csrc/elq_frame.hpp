@@ -79,4 +79,37 @@ hipError_t launch_advect_faces(int ni, int nj, int nk, float dx, float dt, const
 hipError_t launch_advect_cells(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
                                const float* vel_w, const float* val, float* out, hipStream_t s);
 
+// Reinitialisation (reinitialization.cpp:19-156): same 32x8x8 tiles; cell
+// stamps are 16-bit (kReiFar: not reached yet, kReiUnknown: invalid cell,
+// s: fixed from the start of pass s on). Every kReiRebase passes the stamps
+// are folded back to 0.
+constexpr uint16_t kReiFar = 0xFFFF;
+constexpr uint16_t kReiUnknown = 0xFFFE;
+constexpr int kReiRebase = 60000;
+
+struct ReiSet {
+    float* phi;            // level set, updated in place
+    const float* temp;     // copy of phi taken before the interface pass
+    const uint8_t* valid;  // 0/1 flags
+    uint16_t* stamp;
+    uint8_t* tfar;         // per tile: 1 while the tile holds a cell not reached yet
+    uint16_t* tfix;        // per tile: first pass at whose start it holds a fixed cell (kReiFar: none)
+    uint16_t* tnear;       // per tile: same over the tile and its 6 face neighbours
+    int32_t* list;         // tiles of the current pass
+    uint32_t* counter;     // [0..1] list lengths, [2..3] cells fixed by alternate passes,
+                           // [4] cells left unreached, [5] largest stamp
+    int ni, nj, nk;
+    int tx, ty, tz;
+    int tiles;
+    float dx;
+};
+
+// Interface cells (their geometric distance) and stamps; tile flags.
+hipError_t launch_rei_init(const ReiSet& r, hipStream_t s);
+// Pass l (0-based since the last rebase; parity = absolute pass & 1).
+hipError_t launch_rei_pass(const ReiSet& r, int l, int parity, hipStream_t s);
+hipError_t launch_rei_rebase(const ReiSet& r, hipStream_t s);
+// counter[4] = cells never reached, counter[5] = largest stamp (passes that fixed cells).
+hipError_t launch_rei_finish(const ReiSet& r, hipStream_t s);
+
 }  // namespace elq

@@ -115,6 +115,7 @@ def load():
     L.elq_advect_collocated_device.argtypes = [vp, i32, i32, i32, f32, f32] + [vp] * 5
     L.elq_advect_velocity_device.argtypes = [vp, C.POINTER(Grid), f32]
     L.elq_advect_level_set_device.argtypes = [vp, i32, i32, i32, f32, f32, vp, vp, vp, vp]
+    L.elq_reinitialize_device.argtypes = [vp, i32, i32, i32, f32, vp, vp, C.POINTER(i64), C.POINTER(i32)]
     L.elq_set_profiling_filter.argtypes = [vp, C.c_char_p]
     L.elq_set_profiling_filter.restype = None
     L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]
@@ -415,6 +416,25 @@ class Context:
         finally:
             self.free(pg)
             self.free(pv)
+
+    def reinitialize(self, phi: np.ndarray, valid: np.ndarray, dims, dx: float):
+        """fluid::reinitialize_fast_marching of a host float32 level set of
+        dims (ni, nj, nk) with uint8 0/1 flags, in place, through device copies
+        and elq_reinitialize_device; returns (unreached cells, passes)."""
+        ni, nj, nk = dims
+        assert phi.dtype == np.float32 and valid.dtype == np.uint8 and phi.size == valid.size == ni * nj * nk
+        pp, pv = self.alloc(max(phi.nbytes, 4)), self.alloc(max(valid.nbytes, 4))
+        try:
+            self.h2d(pp, phi)
+            self.h2d(pv, valid)
+            left, passes = C.c_int64(0), C.c_int32(0)
+            _check_rc(self.L.elq_reinitialize_device(self.h, ni, nj, nk, np.float32(dx), pp, pv, C.byref(left),
+                                                     C.byref(passes)), self.h)
+            self.d2h(phi, pp)
+        finally:
+            self.free(pp)
+            self.free(pv)
+        return int(left.value), int(passes.value)
 
     def set_profiling(self, on: bool, only: str = ""):
         """Per-kernel event timing; only: comma-separated kernel classes to time (all if empty)."""

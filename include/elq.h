@@ -231,6 +231,20 @@ int elq_advect_velocity_device(elq_ctx* ctx, const elq_grid* device_grid, float 
 int elq_advect_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt,
                                 float* level_set, const float* u, const float* v, const float* w);
 
+/* fluid::reinitialize_fast_marching (src/project/3d/fluid/reinitialization.cpp:19-156),
+ * Simulator::reinitialize (simulator.cpp:181-183), on device memory,
+ * bit-identical with the reference: cells whose level set changes sign
+ * towards a face neighbour take their geometric distance to the
+ * interpolated zero set (include/nama/math/level_set.h:165-264); the other
+ * valid cells (valid: ni*nj*nk bytes, 0/1) are reached one face-neighbour
+ * layer per pass and take the first-order upwind distance from the cells
+ * fixed before. level_set is updated in place. *unreached (optional): valid
+ * cells no path of valid cells connects to the interface; they keep their
+ * value (the reference loops forever on them). *passes (optional): marching
+ * passes that fixed cells. */
+int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float* level_set,
+                            const uint8_t* valid, int64_t* unreached, int32_t* passes);
+
 /* Device memory helpers for callers without their own allocator. */
 int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out);
 int elq_device_free(elq_ctx* ctx, void* ptr);

@@ -59,6 +59,8 @@ def lib():
         L.orc_advect_staggered.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float] + [_f32p] * 9
         L.orc_advect_velocity.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float] + [_f32p] * 6
         L.orc_advect_level_set.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float] + [_f32p] * 5
+        L.orc_reinitialize.restype = C.c_long
+        L.orc_reinitialize.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, _f32p, _u8p, C.POINTER(C.c_int)]
         L.orc_find_valid_level_set.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _u8p]
         L.orc_set_invalid_zero.argtypes = [C.c_size_t, _u8p, _f32p]
         L.orc_extrapolate.argtypes = [C.c_int, C.c_int, C.c_int, _f32p, _u8p, C.c_int]
@@ -196,6 +198,15 @@ def advect_level_set(s: Scene, dt: float):
     """Simulator::advect_level_set (simulator.cpp:84-87) in place on s.phi."""
     phi0 = s.phi.copy()
     lib().orc_advect_level_set(s.ni, s.nj, s.nk, np.float32(s.dx), np.float32(dt), phi0, s.u, s.v, s.w, s.phi)
+
+
+def reinitialize(phi: np.ndarray, valid: np.ndarray, dims, dx: float):
+    """reinitialization.cpp:19-156 in place on a flat float32 level set;
+    returns (FAR cells left, marching passes)."""
+    ni, nj, nk = dims
+    passes = C.c_int(0)
+    left = lib().orc_reinitialize(ni, nj, nk, np.float32(dx), phi, valid, C.byref(passes))
+    return int(left), passes.value
 
 
 def random_velocity(s: Scene, rng: Rng, amp: float):

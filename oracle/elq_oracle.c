@@ -344,6 +344,211 @@ void orc_advect_level_set(int ni, int nj, int nk, float dx, float dt, const floa
             }
 }
 
+/* ------------------------------------------------------- reinitialisation
+ * Eigen's Vector3f operations as the reference uses them: cross() is the
+ * usual three 2x2 determinants; norm() is sqrt(squaredNorm()), and Eigen's
+ * fixed-size reduction of a 3-vector splits it in halves, so the squares
+ * are summed as x0 + (x1 + x2). */
+static void sub3(const float a[3], const float b[3], float out[3]) {
+    for (int c = 0; c < 3; ++c) out[c] = a[c] - b[c];
+}
+
+static void cross3(const float u[3], const float v[3], float out[3]) {
+    out[0] = u[1] * v[2] - u[2] * v[1];
+    out[1] = u[2] * v[0] - u[0] * v[2];
+    out[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+static float norm3(const float v[3]) { return sqrtf(v[0] * v[0] + (v[1] * v[1] + v[2] * v[2])); }
+
+/* meshing.h:12-17 (vertex_interp) with isolevel 0 */
+static void vertex_interp(const float p0[3], const float p1[3], float l0, float l1, float out[3]) {
+    const float mu = -(l0 - 0.0f) / (l1 - l0);
+    for (int c = 0; c < 3; ++c) out[c] = p0[c] + mu * (p1[c] - p0[c]);
+}
+
+/* level_set.h:172-177 */
+static float line_distance(const float x0[3], const float x1[3], const float x2[3]) {
+    float a[3], b[3], n[3];
+    sub3(x0, x1, a);
+    sub3(x2, x1, b);
+    cross3(a, b, n);
+    const float area = norm3(n);
+    const float l = norm3(b);
+    return area / l;
+}
+
+/* level_set.h:179-189 */
+static float plane_distance(const float x0[3], const float x1[3], const float x2[3], const float x3[3]) {
+    float u[3], v[3], n[3];
+    sub3(x2, x1, u);
+    sub3(x3, x1, v);
+    cross3(u, v, n);
+    const float a = n[0], b = n[1], c = n[2];
+    const float d = -(n[0] * x1[0] + n[1] * x1[1] + n[2] * x1[2]);
+    return fabsf(a * x0[0] + b * x0[1] + c * x0[2] + d) / sqrtf(a * a + b * b + c * c);
+}
+
+/* level_set.h:212-237 (calc_distance_from_zero, tetrahedron x0..x3) */
+static float tet_distance(const float* x0, const float* x1, const float* x2, const float* x3, float phi0,
+                          float phi1, float phi2, float phi3) {
+    if (phi0 == 0.0f) return 0.0f;
+    float xi[3][3];
+    int n = 0;
+    if (phi0 * phi1 <= 0.0f) vertex_interp(x0, x1, phi0, phi1, xi[n++]);
+    if (phi0 * phi2 <= 0.0f) vertex_interp(x0, x2, phi0, phi2, xi[n++]);
+    if (phi0 * phi3 <= 0.0f) vertex_interp(x0, x3, phi0, phi3, xi[n++]);
+    if (n == 1) {
+        float d[3];
+        sub3(xi[0], x0, d);
+        return norm3(d);
+    }
+    if (n == 2) return line_distance(x0, xi[0], xi[1]);
+    if (n == 3) return plane_distance(x0, xi[0], xi[1], xi[2]);
+    return FLT_MAX;
+}
+
+/* level_set.h:251-264 (calc_sdf, 7 points: centre, +x, +y, -x, -y, +z, -z) */
+static float calc_sdf7(const float x[7][3], const float phi[7]) {
+    float l = FLT_MAX;
+    static const int c[5] = {1, 2, 3, 4, 1};
+    for (int i = 0; i < 4; ++i)
+        for (int j = 5; j <= 6; ++j) {
+            const float t = tet_distance(x[0], x[c[i]], x[c[i + 1]], x[j], phi[0], phi[c[i]], phi[c[i + 1]], phi[j]);
+            l = (t < l) ? t : l; /* std::min(l, t) */
+        }
+    return phi[0] < 0.0f ? -l : l;
+}
+
+static int clampi(int v, int hi) { return v < 0 ? 0 : (v > hi ? hi : v); }
+
+#define RL_FIX 0
+#define RL_NEAR 1
+#define RL_FAR 2
+#define RL_UNKNOWN 3
+
+/* reinitialization.cpp:19-156 (reinitialize_fast_marching). grid is the
+ * level set (in place), valid its 0/1 flags. The reference loops while any
+ * FAR cell remains; a FAR cell with no path to the interface would keep it
+ * looping forever, so this restatement also stops when a pass changes no
+ * label, and returns the number of FAR cells left (0 whenever the reference
+ * terminates). *passes receives the number of marching passes that changed
+ * labels. */
+long orc_reinitialize(int ni, int nj, int nk, float dx, float* grid, const uint8_t* valid, int* passes) {
+    const size_t n = (size_t)ni * nj * nk;
+    float* temp = (float*)malloc(n * sizeof(float));
+    uint8_t* label = (uint8_t*)malloc(n);
+    memcpy(temp, grid, n * sizeof(float));
+#define T(i, j, k) temp[IDX(clampi(i, ni - 1), clampi(j, nj - 1), clampi(k, nk - 1), ni, nj)]
+#define LB(i, j, k) label[IDX(clampi(i, ni - 1), clampi(j, nj - 1), clampi(k, nk - 1), ni, nj)]
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const size_t c = IDX(i, j, k, ni, nj);
+                if (valid[c] == 0) {
+                    label[c] = RL_UNKNOWN;
+                    continue;
+                }
+                const float p = temp[c];
+                const int flag = p * T(i - 1, j, k) <= 0.0f || p * T(i + 1, j, k) <= 0.0f ||
+                                 p * T(i, j - 1, k) <= 0.0f || p * T(i, j + 1, k) <= 0.0f ||
+                                 p * T(i, j, k - 1) <= 0.0f || p * T(i, j, k + 1) <= 0.0f;
+                if (flag) {
+                    if (grid[c] != 0.0f) {
+                        /* Vector3f((i + 0.5) * dx, ...): double products rounded to float */
+                        const double d = (double)dx;
+                        const float xc = (float)((i + 0.5) * d), yc = (float)((j + 0.5) * d),
+                                    zc = (float)((k + 0.5) * d);
+                        const float xp = (float)(((i + 1) + 0.5) * d), xm = (float)(((i - 1) + 0.5) * d);
+                        const float yp = (float)(((j + 1) + 0.5) * d), ym = (float)(((j - 1) + 0.5) * d);
+                        const float zp = (float)(((k + 1) + 0.5) * d), zm = (float)(((k - 1) + 0.5) * d);
+                        const float x[7][3] = {{xc, yc, zc}, {xp, yc, zc}, {xc, yp, zc}, {xm, yc, zc},
+                                               {xc, ym, zc}, {xc, yc, zp}, {xc, yc, zm}};
+                        const float phi[7] = {T(i, j, k),     T(i + 1, j, k), T(i, j + 1, k), T(i - 1, j, k),
+                                              T(i, j - 1, k), T(i, j, k + 1), T(i, j, k - 1)};
+                        grid[c] = calc_sdf7(x, phi);
+                    }
+                    label[c] = RL_FIX;
+                } else {
+                    label[c] = RL_FAR;
+                }
+            }
+    int npass = 0;
+    long far_left = 0;
+    for (;;) {
+        memcpy(temp, grid, n * sizeof(float));
+        long far = 0, near = 0;
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < nj; ++j)
+                for (int i = 0; i < ni; ++i) {
+                    const size_t c = IDX(i, j, k, ni, nj);
+                    if (label[c] != RL_FAR) continue;
+                    ++far;
+                    if (LB(i - 1, j, k) == RL_FIX || LB(i + 1, j, k) == RL_FIX || LB(i, j - 1, k) == RL_FIX ||
+                        LB(i, j + 1, k) == RL_FIX || LB(i, j, k - 1) == RL_FIX || LB(i, j, k + 1) == RL_FIX) {
+                        label[c] = RL_NEAR;
+                        ++near;
+                    }
+                }
+        if (far == 0) break;
+        if (near == 0) { /* the reference would loop forever from here */
+            far_left = far;
+            break;
+        }
+        ++npass;
+        for (int k = 0; k < nk; ++k)
+            for (int j = 0; j < nj; ++j)
+                for (int i = 0; i < ni; ++i) {
+                    const size_t c = IDX(i, j, k, ni, nj);
+                    if (label[c] != RL_NEAR) continue;
+                    float p[3];
+                    int m = 0;
+                    if (LB(i - 1, j, k) == RL_FIX) p[m++] = T(i - 1, j, k);
+                    else if (LB(i + 1, j, k) == RL_FIX) p[m++] = T(i + 1, j, k);
+                    if (LB(i, j - 1, k) == RL_FIX) p[m++] = T(i, j - 1, k);
+                    else if (LB(i, j + 1, k) == RL_FIX) p[m++] = T(i, j + 1, k);
+                    if (LB(i, j, k - 1) == RL_FIX) p[m++] = T(i, j, k - 1);
+                    else if (LB(i, j, k + 1) == RL_FIX) p[m++] = T(i, j, k + 1);
+                    /* mode: 0 all >= 0, 1 all <= 0, else neither (asserts are off in the
+                     * reference's release build, so that case proceeds like mode 0) */
+                    int all_ge = 1, all_le = 1;
+                    for (int q = 0; q < m; ++q) {
+                        all_ge &= p[q] >= 0.0f;
+                        all_le &= p[q] <= 0.0f;
+                    }
+                    const int mode = all_ge ? 0 : (all_le ? 1 : -1);
+                    if (mode == 1)
+                        for (int q = 0; q < m; ++q) p[q] = -p[q];
+                    /* std::sort, ascending */
+                    for (int a = 1; a < m; ++a)
+                        for (int b = a; b > 0 && p[b] < p[b - 1]; --b) {
+                            const float t = p[b];
+                            p[b] = p[b - 1];
+                            p[b - 1] = t;
+                        }
+                    float d = p[0] + dx;
+                    if (m >= 2 && d > p[1])
+                        d = (float)(0.5 * (double)(p[0] + p[1] + sqrtf(2 * dx * dx - (p[1] - p[0]) * (p[1] - p[0]))));
+                    if (m == 3 && d > p[2]) {
+                        const float s = p[0] + p[1] + p[2];
+                        float q = s * s - 3.0f * (p[0] * p[0] + p[1] * p[1] + p[2] * p[2] - dx * dx);
+                        q = (0.0f < q) ? q : 0.0f; /* std::max(0.0f, q) */
+                        d = (float)(0.333333 * (double)(p[0] + p[1] + p[2] + sqrtf(q)));
+                    }
+                    if (mode == 1) d = -d;
+                    grid[c] = d;
+                }
+        for (size_t c = 0; c < n; ++c)
+            if (label[c] == RL_NEAR) label[c] = RL_FIX;
+    }
+#undef T
+#undef LB
+    free(temp);
+    free(label);
+    if (passes) *passes = npass;
+    return far_left;
+}
+
 /* simulator.cpp:191-200 (calc_cell_centered_solid_phi) then :160-167
  * (find_valid_level_set). The eight nodes are summed in float, left to right;
  * the product with the double literal 0.125 is rounded to float when stored

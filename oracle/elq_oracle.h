@@ -84,6 +84,11 @@ void orc_advect_velocity(int ni, int nj, int nk, float dx, float dt, const float
 void orc_advect_level_set(int ni, int nj, int nk, float dx, float dt, const float* phi0, const float* u,
                           const float* v, const float* w, float* phi);
 
+/* reinitialization.cpp:19-156: level set grid (in place) with its 0/1 flags.
+ * Returns the FAR cells left when a pass changes nothing (0 whenever the
+ * reference terminates); *passes = marching passes that changed labels. */
+long orc_reinitialize(int ni, int nj, int nk, float dx, float* grid, const uint8_t* valid, int* passes);
+
 /* simulator.cpp:191-200 + 160-167: level-set flags from the nodal solid phi */
 void orc_find_valid_level_set(int ni, int nj, int nk, const float* nodal, uint8_t* valid);
 

@@ -1,0 +1,93 @@
+"""Level-set reinitialisation on the GPU (reinitialization.cpp:19-156),
+through the C ABI, against the oracle, bit for bit (float32 words), with the
+same count of passes and of unreached cells:
+
+- planar fronts along an axis, at 45 degrees and along a body diagonal (the
+  1-, 2- and 3-neighbour upwind formulas and the geometric interface
+  distance);
+- meeting fronts (the -x/-y/-z neighbour is taken first);
+- a sphere with noise on a room() grid, flags from find_valid_level_set, and
+  a pocket of cells walled off by invalid cells;
+- the 128^3 drop scene's level set after one advection step (about 100
+  passes, several host batches).
+"""
+import numpy as np
+import pytest
+
+import elq
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def check(ctx, phi, valid, dims, dx):
+    ref = phi.copy()
+    r_left, r_passes = O.reinitialize(ref, valid, dims, dx)
+    got = phi.copy()
+    left, passes = ctx.reinitialize(got, valid, dims, dx)
+    assert (left, passes) == (r_left, r_passes)
+    assert same(got, ref), np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))[:10]
+    return passes
+
+
+def xyz(dims, dx):
+    ni, nj, nk = dims
+    k, j, i = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(ni), indexing="ij")
+    return ((i + 0.5) * dx).reshape(-1), ((j + 0.5) * dx).reshape(-1), ((k + 0.5) * dx).reshape(-1)
+
+
+@pytest.mark.parametrize("normal,scale", [((0, 1, 0), 3.0), ((1, 1, 0), 2.0), ((1, 1, 1), 1.7), ((-1, 2, 1), 0.6)])
+def test_planar_fronts_bit_exact(ctx, normal, scale):
+    dims, dx = (40, 36, 44), 1.0 / 40
+    x, y, z = xyz(dims, dx)
+    nv = np.array(normal, float) / np.linalg.norm(normal)
+    c = nv @ np.array([0.5, 0.45, 0.55]) + 0.3 * dx
+    phi = (scale * (nv[0] * x + nv[1] * y + nv[2] * z - c)).astype(np.float32)
+    assert check(ctx, phi, np.ones(phi.size, np.uint8), dims, dx) > 10
+
+
+def test_meeting_fronts_bit_exact(ctx):
+    dims, dx = (33, 30, 35), 1.0 / 32
+    x, y, z = xyz(dims, dx)
+    phi = (2.0 * np.minimum(np.minimum(x - 0.31, 0.72 - x), np.minimum(y - 0.2, 0.83 - z))).astype(np.float32)
+    check(ctx, phi, np.ones(phi.size, np.uint8), dims, dx)
+
+
+def test_sphere_with_noise_on_room_and_pocket(ctx):
+    dims, dx = (48, 40, 44), 1.0 / 48
+    ni, nj, nk = dims
+    x, y, z = xyz(dims, dx)
+    rng = np.random.default_rng(3)
+    phi = (np.sqrt((x - 0.5) ** 2 + (y - 0.4) ** 2 + (z - 0.45) ** 2) - 0.25).astype(np.float32)
+    phi *= rng.uniform(0.5, 1.5, phi.size).astype(np.float32)
+    k, j, i = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(ni), indexing="ij")
+    shell = (np.abs(i - 40) <= 2) & (np.abs(j - 33) <= 2) & (np.abs(k - 36) <= 2)
+    pocket = (np.abs(i - 40) <= 1) & (np.abs(j - 33) <= 1) & (np.abs(k - 36) <= 1)
+    phi[(rng.random(phi.size) < 0.01) & ~shell.reshape(-1)] = 0.0   # no interface inside the pocket
+    nodal = np.empty((ni + 1) * (nj + 1) * (nk + 1), np.float32)
+    O.lib().orc_room_nodal(ni, nj, nk, nodal)
+    valid = np.empty(phi.size, np.uint8)
+    O.lib().orc_find_valid_level_set(ni, nj, nk, nodal, valid)
+    # a walled-off pocket of air cells
+    valid[(shell & ~pocket).reshape(-1)] = 0
+    ref = phi.copy()
+    left, _ = O.reinitialize(ref, valid, dims, dx)
+    assert left == 27
+    check(ctx, phi, valid, dims, dx)
+
+
+def test_drop128_after_advection_bit_exact(ctx):
+    s = elq.scene_drop(128)
+    ctx.solve_pressure(s, elq.default_opts(tol=1e-6, max_iter=5000, precision=elq.MIXED))
+    phi = np.empty_like(s.phi)
+    O.lib().orc_advect_level_set(s.ni, s.nj, s.nk, np.float32(s.dx), np.float32(s.dt), s.phi, s.u, s.v, s.w, phi)
+    ni, nj, nk = s.ni, s.nj, s.nk
+    nodal = np.empty((ni + 1) * (nj + 1) * (nk + 1), np.float32)
+    O.lib().orc_room_nodal(ni, nj, nk, nodal)
+    valid = np.empty(phi.size, np.uint8)
+    O.lib().orc_find_valid_level_set(ni, nj, nk, nodal, valid)
+    assert check(ctx, phi, valid, (ni, nj, nk), s.dx) > 40

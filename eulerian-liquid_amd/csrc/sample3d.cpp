@@ -2,10 +2,9 @@
 // (apps/eulerian_liquid_3d.cpp:106-120, update_second_order) on the drop
 // scene: slab y < 0.4 plus a sphere of radius 0.15 at (0.5, 0.7, 0.5), room()
 // walls. Set-up: compute_weights, find_valid_level_set (:137-139). Per frame:
-// extrapolate_velocity, extrapolate_level_set, add_gravity,
-// find_valid_velocity, set_invalid_velocity_zero, solve_pressure. Advection
-// and reinitialisation are not part of this library, so the liquid does not
-// move between frames.
+// extrapolate_velocity, extrapolate_level_set, advect_level_set,
+// advect_velocity, reinitialize, add_gravity, find_valid_velocity,
+// set_invalid_velocity_zero, solve_pressure, all on the GPU through the C ABI.
 //
 // usage: elq_sample3d [quality=64] [frames=3] [tol=1e-4] [mixed=0]
 #include <cstdio>
@@ -33,6 +32,9 @@ int main(int argc, char** argv) {
         for (int f = 0; f < frames; ++f) {
             sim.extrapolate_velocity();
             sim.extrapolate_level_set();
+            sim.advect_level_set(dt);
+            sim.advect_velocity(dt);
+            sim.reinitialize();
             sim.add_gravity(dt);
             sim.find_valid_velocity();
             sim.set_invalid_velocity_zero();

@@ -34,6 +34,11 @@ void check(elq_ctx* ctx, int rc) {
     if (rc != ELQ_OK) raise(ctx, rc);
 }
 
+template <class G>
+size_t bytes_of(const G& g) {
+    return g.data.size() * sizeof(g.data[0]);
+}
+
 // Device copy of one host array for the duration of a call.
 class DevArray {
    public:
@@ -164,6 +169,45 @@ void extrapolate_const(Device& dev, Grid3f& grid, const Grid3c& valid, int layer
     g.fetch(grid.data);
 }
 
+void advect_collocated_grid(Device& dev, Grid3f& grid, const Grid3f& grid_temp, const StaggeredGrid3f& velocity,
+                            float dx, float dt) {
+    elq_ctx* c = dev.ctx();
+    DevArray src(c, grid_temp.data), u(c, velocity.u().data), v(c, velocity.v().data), w(c, velocity.w().data);
+    DevArray out(c, grid_temp.data.size() * sizeof(float));
+    check(c, elq_advect_collocated_device(c, grid_temp.ni, grid_temp.nj, grid_temp.nk, dx, dt, out.as<float>(),
+                                          src.as<float>(), u.as<float>(), v.as<float>(), w.as<float>()));
+    grid.create(grid_temp.ni, grid_temp.nj, grid_temp.nk);
+    out.fetch(grid.data);
+}
+
+void advect_staggered_grid(Device& dev, StaggeredGrid3f& grid, const StaggeredGrid3f& grid_temp,
+                           const StaggeredGrid3f& velocity, float dx, float dt) {
+    elq_ctx* c = dev.ctx();
+    const int ni = grid_temp.v().ni, nj = grid_temp.u().nj, nk = grid_temp.u().nk;
+    DevArray au(c, grid_temp.u().data), av(c, grid_temp.v().data), aw(c, grid_temp.w().data);
+    DevArray vu(c, velocity.u().data), vv(c, velocity.v().data), vw(c, velocity.w().data);
+    DevArray ou(c, bytes_of(grid_temp.u())), ov(c, bytes_of(grid_temp.v())), ow(c, bytes_of(grid_temp.w()));
+    check(c, elq_advect_staggered_device(c, ni, nj, nk, dx, dt, ou.as<float>(), ov.as<float>(), ow.as<float>(),
+                                         au.as<float>(), av.as<float>(), aw.as<float>(), vu.as<float>(),
+                                         vv.as<float>(), vw.as<float>()));
+    grid.create(ni, nj, nk);
+    ou.fetch(grid.u().data);
+    ov.fetch(grid.v().data);
+    ow.fetch(grid.w().data);
+}
+
+void reinitialize_fast_marching(Device& dev, Grid3f& grid, const Grid3c& valid, float dx) {
+    elq_ctx* c = dev.ctx();
+    DevArray g(c, grid.data), f(c, valid.data);
+    int64_t unreached = 0;
+    check(c, elq_reinitialize_device(c, grid.ni, grid.nj, grid.nk, dx, g.as<float>(), f.as<uint8_t>(), &unreached,
+                                     nullptr));
+    g.fetch(grid.data);
+    if (unreached)
+        throw std::runtime_error("reinitialize: " + std::to_string(unreached) +
+                                 " valid cells have no path to the interface");
+}
+
 }  // namespace fluid
 
 void Simulator::create_grid(float width, int ni, int nj, int nk) {
@@ -280,6 +324,18 @@ void Simulator::find_valid_level_set() {
 }
 
 void Simulator::extrapolate_level_set() { fluid::extrapolate_const(m_dev, m_level_set, m_level_set_valid, 10); }
+
+void Simulator::advect_velocity(float dt) {
+    const StaggeredGrid3f temp = m_velocity;  // m_velocity_temp = m_velocity (simulator.cpp:80)
+    fluid::advect_staggered_grid(m_dev, m_velocity, temp, temp, m_dx, dt);
+}
+
+void Simulator::advect_level_set(float dt) {
+    const Grid3f temp = m_level_set;  // m_level_set_temp = m_level_set (simulator.cpp:85)
+    fluid::advect_collocated_grid(m_dev, m_level_set, temp, m_velocity, m_dx, dt);
+}
+
+void Simulator::reinitialize() { fluid::reinitialize_fast_marching(m_dev, m_level_set, m_level_set_valid, m_dx); }
 
 void Simulator::solve_pressure(float dt) {
     elq_grid g = make_grid(m_ni, m_nj, m_nk, m_dx, m_level_set, m_velocity, m_velocity_weight);

@@ -5,11 +5,14 @@
 //   namespace nama::_3d::fluid       (include/nama/project/3d/fluid/projection.h:10-22,
 //                                     extrapolation.h:7-8)
 //     assign_indices, build_matrix, build_grid_from_vector,
-//     update_velocity_using_pressure, extrapolate, extrapolate_const
+//     update_velocity_using_pressure, extrapolate, extrapolate_const,
+//     advect_staggered_grid, advect_collocated_grid (advection.h:8-11),
+//     reinitialize_fast_marching (reinitialization.h)
 //   class Simulator                  (include/nama/project/3d/eulerian_liquid/simulator.h:16-73)
 //     create_grid, init_scene, compute_weights, add_gravity,
 //     find_valid_velocity, set_invalid_velocity_zero, solve_pressure,
-//     extrapolate_velocity, find_valid_level_set, extrapolate_level_set
+//     extrapolate_velocity, find_valid_level_set, extrapolate_level_set,
+//     advect_velocity, advect_level_set, reinitialize
 // Grids are flat std::vector<float>/<int> in the reference layout
 // (i + ni*(j + nj*k)); staggered grids are three such vectors (u, v, w).
 // Errors follow the reference: solve_pressure and assign_indices throw
@@ -125,6 +128,15 @@ void update_velocity_using_pressure(Device& dev, StaggeredGrid3f& velocity, cons
 // extrapolation.cpp:5-46 (grid and valid updated) and :48-51 (valid unchanged)
 void extrapolate(Device& dev, Grid3f& grid, Grid3c& valid, int layers);
 void extrapolate_const(Device& dev, Grid3f& grid, const Grid3c& valid, int layers);
+// advection.cpp:9-27: grid = grid_temp carried by velocity (dx: cell size)
+void advect_collocated_grid(Device& dev, Grid3f& grid, const Grid3f& grid_temp, const StaggeredGrid3f& velocity,
+                            float dx, float dt);
+void advect_staggered_grid(Device& dev, StaggeredGrid3f& grid, const StaggeredGrid3f& grid_temp,
+                           const StaggeredGrid3f& velocity, float dx, float dt);
+// reinitialization.cpp:19-156 (grid_temp is scratch in the reference; here
+// the library keeps its own). Throws std::runtime_error when a valid cell
+// has no path to the interface: the reference would not return.
+void reinitialize_fast_marching(Device& dev, Grid3f& grid, const Grid3c& valid, float dx);
 }  // namespace fluid
 
 class Simulator {
@@ -145,6 +157,9 @@ class Simulator {
     void extrapolate_velocity();                   // simulator.cpp:119-123 (10 layers)
     void find_valid_level_set();                   // simulator.cpp:160-167 with :191-200
     void extrapolate_level_set();                  // simulator.cpp:115-117 (10 layers)
+    void advect_velocity(float dt);                // simulator.cpp:79-82
+    void advect_level_set(float dt);               // simulator.cpp:84-87
+    void reinitialize();                           // simulator.cpp:181-183
 
     elq_opts& opts() { return m_opts; }
     const elq_stats& last_stats() const { return m_stats; }

@@ -8,6 +8,9 @@
  *   fluid::build_matrix(...)                     projection.cpp:30-83 / projection.h:13-16
  *   fluid::build_grid_from_vector(...)           projection.cpp:85-91 / projection.h:18-19
  *   fluid::update_velocity_using_pressure(...)   projection.cpp:93-134 / projection.h:21-22
+ * and the other steps of the reference's frame on device memory (weights,
+ * gravity, validity, extrapolation, advection, reinitialisation; each entry
+ * point below cites the function it replaces).
  *
  * Plain C types only: pointers, sizes, POD structs. No exceptions cross this
  * boundary; every entry point returns an elq_status. Grids use the

@@ -273,6 +273,48 @@ def cpu_baseline(scene, fp64_iters):
             "est_s_per_solve": round(est, 2)}
 
 
+def cpu_baseline_frame(ctx):
+    """The oracle's frame steps (single thread) on a bounded sample, the
+    128^3 version of the workload's scene (about 4 s of CPU work), beside the
+    GPU's times for the same sample: extrapolate_velocity (10 layers),
+    advect_level_set, advect_velocity and reinitialize."""
+    import oracle as O
+    s = elq.scene_spheres(128, seed=1, n_spheres=32)
+    n = s.ni
+    dims = ((n + 1, n, n), (n, n + 1, n), (n, n, n + 1))
+    cpu = {}
+    flags = O.find_valid_velocity(s)
+    vel = [a.copy() for a in (s.u, s.v, s.w)]
+    t0 = time.perf_counter()
+    for a, f, d in zip(vel, flags, dims):
+        O.extrapolate(a, f.copy(), d, 10)
+    cpu["extrapolate_velocity"] = time.perf_counter() - t0
+    phi = np.empty_like(s.phi)
+    t0 = time.perf_counter()
+    O.lib().orc_advect_level_set(n, n, n, np.float32(s.dx), np.float32(s.dt), s.phi, s.u, s.v, s.w, phi)
+    cpu["advect_level_set"] = time.perf_counter() - t0
+    out = [np.empty_like(a) for a in (s.u, s.v, s.w)]
+    t0 = time.perf_counter()
+    O.lib().orc_advect_velocity(n, n, n, np.float32(s.dx), np.float32(s.dt), s.u, s.v, s.w, *out)
+    cpu["advect_velocity"] = time.perf_counter() - t0
+    nodal = np.empty((n + 1) ** 3, np.float32)
+    O.lib().orc_room_nodal(n, n, n, nodal)
+    ls = np.empty(n ** 3, np.uint8)
+    O.lib().orc_find_valid_level_set(n, n, n, nodal, ls)
+    phi = s.phi.copy()
+    t0 = time.perf_counter()
+    O.reinitialize(phi, ls, (n, n, n), s.dx)
+    cpu["reinitialize"] = time.perf_counter() - t0
+    ds = elq.DeviceScene(ctx, s)
+    gpu = frame_steps(ctx, ds, reps=3)
+    ds.close()
+    return {"sample": "spheres128 seed 1 (the workload's generator at 128^3), oracle single thread vs the GPU "
+                      "on the same sample",
+            "cores": 1, "kind": "port",
+            "steps": {k: {"cpu_s": round(v, 3), "gpu_ms": gpu[k]["ms"],
+                          "speedup": round(v * 1e3 / gpu[k]["ms"], 1)} for k, v in cpu.items()}}
+
+
 def copy_calibration(nbytes, reps):
     """Copies nbytes device-to-device with the library's streaming kernel
     (reads nbytes, writes nbytes) so that per-dispatch FETCH_SIZE/WRITE_SIZE
@@ -497,6 +539,10 @@ def main():
     if not a.no_cpu:
         fp64_iters = out["fp64"][1][-1]["iterations"] if "fp64" in out else iters
         line["cpu_baseline"] = cpu_baseline(scene, fp64_iters)
+        if not a.no_extra:
+            ctx2 = elq.Context(0)
+            line["cpu_baseline"]["frame_steps"] = cpu_baseline_frame(ctx2)
+            ctx2.close()
     print(json.dumps(line), flush=True)
 
 

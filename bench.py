@@ -65,18 +65,18 @@ def project(ctx, ds, opts):
 
 
 def run_steps(ctx, ds, opts, steps, warmup):
-    """One untimed step with HIP events around every kernel class (the
-    per-class breakdown; it is the first warm-up step when warmup >= 1), the
-    remaining warm-up steps, then the timed steps with events around the
+    """The warm-up steps (the first one also allocates the context's device
+    buffers), one untimed step with HIP events around every kernel class (the
+    per-class breakdown), then the timed steps with events around the
     dominant kernel class only (its average launch duration gives the
     roofline; the other classes run without events)."""
+    for _ in range(warmup):
+        project(ctx, ds, opts)
     ctx.set_profiling(True)
     project(ctx, ds, opts)
     breakdown = ctx.kernel_times()
     ctx.set_profiling(False)
     dominant = max((k for k in breakdown if k in BYTES_PER_UNKNOWN), key=lambda k: breakdown[k][0])
-    for _ in range(max(warmup - 1, 0)):
-        project(ctx, ds, opts)
     ctx.set_profiling(True, only=dominant)
     times, stats, dom = [], [], [0.0, 0]
     for _ in range(steps):

@@ -1503,6 +1503,16 @@ __device__ __forceinline__ void block_lds_init(BlockLds<T, NW>& s) {
 // The wave's |r|^2 terms go to the grid tail (SC_RR with UPDATE; without
 // UPDATE the tail only advances the epochs, SC_NOP).
 // At most 128 VGPRs in FP32 (4 waves per SIMD: 4 workgroups of 2x2 per CU).
+// Issue priority of a sweep wave from its block's place in the current
+// round of the block list: blocks early in the list feed the ones after them.
+__device__ __forceinline__ void sweep_priority(int pos, int round) {
+    const int q = (4 * (pos - round * (int)gridDim.x)) / (int)gridDim.x;
+    if (q <= 0) __builtin_amdgcn_s_setprio(3);
+    else if (q == 1) __builtin_amdgcn_s_setprio(2);
+    else if (q == 2) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 template <class T, bool UPDATE, int BY, int BZ>
 __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_block(Bricks bk, Pencils pc, Coef<T> c,
                                                                                    T* r, T* x,
@@ -1532,6 +1542,7 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
     int round = 0;
     for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
         const int bc = pc.blk_list[pos];
+        sweep_priority(pos, round);
         const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
         if (by < bk.nby && bz < bk.nbz) {
             PencilIter<false> it(map_row(bk, by, bz), map_row(bk, by - 1, bz), map_row(bk, by, bz - 1), bk.nbx);
@@ -1612,6 +1623,7 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
     int round = 0;
     for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
         const int bc = pc.blk_list[pc.nblk - 1 - pos];
+        sweep_priority(pos, round);
         const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
         if (by < bk.nby && bz < bk.nbz) {
             PencilIter<true> it(map_row(bk, by, bz), map_row(bk, by + 1, bz), map_row(bk, by, bz + 1), bk.nbx);

@@ -129,9 +129,11 @@ Plan make_plan(int ni, int nj, int nk) {
     return p;
 }
 
-// Pencil blocks of the wavefront sweeps (upper bound of the block list).
+// Pencil blocks of the wavefront sweeps: the block lists of every shape.
 size_t block_count(const Plan& p) {
-    return (size_t)((p.nby + kBlockY - 1) / kBlockY) * (size_t)((p.nbz + kBlockZ - 1) / kBlockZ);
+    size_t n = 0;
+    for (int k = 0; k < kShapeCount; ++k) n += shape_blocks(p.nby, p.nbz, k);
+    return n;
 }
 
 bool valid_dims(int ni, int nj, int nk) {
@@ -211,7 +213,7 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     const size_t NBLK = block_count(p);
     HIPCHK(W.bkflag.ensure(NBLK), "alloc blocks");
     HIPCHK(W.blist.ensure(NBLK * 4), "alloc blocks");
-    HIPCHK(W.info.ensure(sizeof(int64_t) * 4 + sizeof(int32_t) * 2), "alloc info");
+    HIPCHK(W.info.ensure(sizeof(int64_t) * (3 + kShapeCount) + sizeof(int32_t) * 2), "alloc info");
     int32_t* indices = indices_out;
     if (!indices) {
         HIPCHK(W.indices.ensure((size_t)p.C * 4), "alloc indices");
@@ -220,7 +222,7 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     W.cur_indices = indices;
     hipStream_t s = ctx->stream;
     int64_t* totals = (int64_t*)W.info.p;
-    int32_t* err = (int32_t*)(totals + 4);
+    int32_t* err = (int32_t*)(totals + 3 + kShapeCount);
     HIPCHK(hipMemsetAsync(W.info.p, 0, W.info.cap, s), "memset");
     HIPCHK(hipMemsetAsync(W.lvl_cnt.p, 0, (size_t)(p.nlev + 1) * 4, s), "memset");
     ctx->prof_begin("setup_mask_scan");
@@ -243,11 +245,11 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     HIPCHK(launch_scan((const uint8_t*)W.pflag.p, NP, 1, (int32_t*)W.pmap.p, -1, (int32_t*)W.scan_cnt.p,
                        (int32_t*)W.scan_off.p, &totals[2], s),
            "scan pencils");
-    HIPCHK(launch_block_list((const int32_t*)W.pmap.p, p.nby, p.nbz, (uint8_t*)W.bkflag.p, (int32_t*)W.blist.p,
-                             &totals[3], s),
+    HIPCHK(launch_block_lists((const int32_t*)W.pmap.p, p.nby, p.nbz, (uint8_t*)W.bkflag.p, (int32_t*)W.blist.p,
+                              &totals[3], s),
            "k_block_list");
     ctx->prof_end();
-    int64_t host_tot[4];
+    int64_t host_tot[3 + kShapeCount];
     int32_t host_err = 0;
     HIPCHK(hipMemcpyAsync(host_tot, totals, sizeof(host_tot), hipMemcpyDeviceToHost, s), "d2h");
     HIPCHK(hipMemcpyAsync(&host_err, err, sizeof(host_err), hipMemcpyDeviceToHost, s), "d2h");
@@ -255,7 +257,12 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
     W.n_unknowns = host_tot[0];
     W.nb = (int)host_tot[1];
     W.np = (int)host_tot[2];
-    W.nblk = (int)host_tot[3];
+    size_t base = 0;
+    for (int k = 0; k < kShapeCount; ++k) {
+        W.nblk[k] = (int)host_tot[3 + k];
+        W.blk_base[k] = base;
+        base += shape_blocks(p.nby, p.nbz, k);
+    }
     W.setup_err = host_err;
     W.plan_ni = d.ni;
     W.plan_nj = d.nj;
@@ -278,12 +285,20 @@ Bricks make_bricks(elq_ctx* ctx, const Plan& p) {
     return bk;
 }
 
-Pencils make_pencils(elq_ctx* ctx, const Plan& p) {
+// Pencil-block shape of the sweeps: the first choice while its blocks fit in
+// one round of four workgroups per CU, else the second (FP32: 1x4, else 2x2;
+// FP64: 2x2, else 2x4). FP64 sweeps hold two workgroups per CU, so there the
+// first choice covers up to two rounds. The choice depends only on the
+// active pencils, the precision and the context's workgroup cap.
+Pencils make_pencils(elq_ctx* ctx, const Plan& p, bool dbl) {
     Workspace& W = ctx->ws;
+    const int a = dbl ? kShape2x2 : kShape1x4, b = dbl ? kShape2x4 : kShape2x2;
+    const int shape = W.nblk[a] <= sweep_round_groups(ctx->max_groups) ? a : b;
     Pencils pc;
     pc.np = W.np;
-    pc.nblk = W.nblk;
-    pc.blk_list = (const int32_t*)W.blist.p;
+    pc.shape = shape;
+    pc.nblk = W.nblk[shape];
+    pc.blk_list = (const int32_t*)W.blist.p + W.blk_base[shape];
     return pc;
 }
 
@@ -586,7 +601,7 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     S.ctx = ctx;
     S.p = p;
     S.bk = make_bricks(ctx, p);
-    S.pc = make_pencils(ctx, p);
+    S.pc = make_pencils(ctx, p, sizeof(T) == 8);
     S.o = &o;
     S.levels = levels;
     S.c.diag = (const T*)W.cdiag.p;

@@ -9,6 +9,8 @@
 #include <utility>
 #include <vector>
 
+#include "elq_kernels.hpp"
+
 // Device buffer that only grows (hipMalloc on growth, reused otherwise).
 struct DevBuf {
     void* p = nullptr;
@@ -43,7 +45,8 @@ struct Workspace {
     int64_t n_unknowns = 0;
     int nb = 0;
     int np = 0;
-    int nblk = 0;
+    int nblk[elq::kShapeCount] = {};       // pencil blocks of each shape
+    size_t blk_base[elq::kShapeCount] = {};  // offset of each shape's block list
     int setup_err = 0;
     int plan_ni = 0, plan_nj = 0, plan_nk = 0;
     int32_t* cur_indices = nullptr;

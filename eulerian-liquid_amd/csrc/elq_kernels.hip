@@ -1502,7 +1502,7 @@ __device__ __forceinline__ void block_lds_init(BlockLds<T, NW>& s) {
 // addresses, results unused), which keeps the loop free of memory branches.
 // The wave's |r|^2 terms go to the grid tail (SC_RR with UPDATE; without
 // UPDATE the tail only advances the epochs, SC_NOP).
-// At most 128 VGPRs in FP32 (4 waves per SIMD: 4 workgroups of 2x2 per CU).
+// At most 128 VGPRs in FP32 (4 waves per SIMD: 4 workgroups of 4 waves per CU).
 // Issue priority of a sweep wave from its block's place in the current
 // round of the block list: blocks early in the list feed the ones after them.
 __device__ __forceinline__ void sweep_priority(int pos, int round) {
@@ -1519,6 +1519,7 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
                                                             const T* p, const T* q, T* y, Ctrl* ctrl, uint64_t* xch,
                                                             uint64_t* stage, uint64_t timeout) {
     constexpr int NW = BY * BZ;
+    static_assert((NW & (NW - 1)) == 0, "grid_tail_n reduces a power-of-two number of waves");
     __shared__ BlockLds<T, NW> S;
     if (ctrl->done) return;
     block_lds_init<T, NW>(S);
@@ -1601,6 +1602,7 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
                                                             Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode,
                                                             uint64_t timeout) {
     constexpr int NW = BY * BZ;
+    static_assert((NW & (NW - 1)) == 0, "grid_tail_n reduces a power-of-two number of waves");
     __shared__ BlockLds<T, NW> S;
     if (ctrl->done) return;
     block_lds_init<T, NW>(S);
@@ -1672,12 +1674,23 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
     grid_tail_n<NW>(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout);
 }
 
-// List of pencil blocks (BYv x BZv pencils) holding at least one active
-// pencil, sorted by block level Y + Z, then Y (deterministic); one workgroup.
-__global__ __launch_bounds__(1024) void k_block_list(const int32_t* pmap, int nby, int nbz, int BYv, int BZv,
-                                                     uint8_t* flag, int32_t* list, int64_t* count) {
-    __shared__ int cnt[1024];
-    __shared__ int off[1024];
+// Lists of pencil blocks, one workgroup per block shape: the blocks (BY x BZ
+// pencils) holding at least one active pencil, sorted by block level Y + Z,
+// then Y (deterministic). Workgroup s writes flag + base[s], list + base[s]
+// and count[s]. Levels: NBY + NBZ - 1 <= 2045 (nby, nbz <= 1023).
+constexpr int kBlockLevels = 2048;
+struct ShapeLists {
+    int by[kShapeCount], bz[kShapeCount];
+    int64_t base[kShapeCount];
+};
+__global__ __launch_bounds__(1024) void k_block_list(const int32_t* pmap, int nby, int nbz, ShapeLists sl,
+                                                     uint8_t* flag_all, int32_t* list_all, int64_t* count_all) {
+    __shared__ int cnt[kBlockLevels];
+    __shared__ int off[kBlockLevels];
+    const int BYv = sl.by[blockIdx.x], BZv = sl.bz[blockIdx.x];
+    uint8_t* flag = flag_all + sl.base[blockIdx.x];
+    int32_t* list = list_all + sl.base[blockIdx.x];
+    int64_t* count = count_all + blockIdx.x;
     const int NBY = (nby + BYv - 1) / BYv, NBZ = (nbz + BZv - 1) / BZv;
     const int total = NBY * NBZ, nlev = NBY + NBZ - 1;
     for (int i = threadIdx.x; i < total; i += 1024) {
@@ -2026,18 +2039,46 @@ hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* 
                       timeout_ticks);
 }
 
+// Persistent sweeps of one pencil-block shape.
+template <class T, int S>
+hipError_t launch_fwd_shape(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q,
+                            T* y, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int update, uint64_t timeout_ticks,
+                            int max_groups, hipStream_t s) {
+    constexpr int BY = kShapeY[S], BZ = kShapeZ[S];
+    if (update)
+        return persistent_wg(k_fwd_block<T, true, BY, BZ>, pc.nblk, 64 * BY * BZ, kBlockGroupsPerCU, max_groups, s,
+                             bk, pc, c, r, x, p, q, y, ctrl, xch, stage, timeout_ticks);
+    return persistent_wg(k_fwd_block<T, false, BY, BZ>, pc.nblk, 64 * BY * BZ, kBlockGroupsPerCU, max_groups, s, bk,
+                         pc, c, r, x, p, q, y, ctrl, xch, stage, timeout_ticks);
+}
+
+template <class T, int S>
+hipError_t launch_bwd_shape(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, Ctrl* ctrl,
+                            uint64_t* xch, uint64_t* stage, int tail_mode, uint64_t timeout_ticks, int max_groups,
+                            hipStream_t s) {
+    constexpr int BY = kShapeY[S], BZ = kShapeZ[S];
+    return persistent_wg(k_bwd_block<T, BY, BZ>, pc.nblk, 64 * BY * BZ, kBlockGroupsPerCU, max_groups, s, bk, pc, c,
+                         r, y, ctrl, xch, stage, tail_mode, timeout_ticks);
+}
+
 template <class T>
 hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
                       double* part, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int update, int level,
                       int level_base, int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
     if (level < 0) {
-        if (update)
-            return persistent_wg(k_fwd_block<T, true, kBlockY, kBlockZ>, pc.nblk, 64 * kBlockY * kBlockZ,
-                                 kBlockGroupsPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
-                                 timeout_ticks);
-        return persistent_wg(k_fwd_block<T, false, kBlockY, kBlockZ>, pc.nblk, 64 * kBlockY * kBlockZ,
-                             kBlockGroupsPerCU, max_groups, s, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
-                             timeout_ticks);
+        switch (pc.shape) {
+        case kShape2x2:
+            return launch_fwd_shape<T, kShape2x2>(bk, pc, c, r, x, p, q, y, ctrl, xch, stage, update, timeout_ticks,
+                                                  max_groups, s);
+        case kShape1x4:
+            return launch_fwd_shape<T, kShape1x4>(bk, pc, c, r, x, p, q, y, ctrl, xch, stage, update, timeout_ticks,
+                                                  max_groups, s);
+        case kShape2x4:
+            return launch_fwd_shape<T, kShape2x4>(bk, pc, c, r, x, p, q, y, ctrl, xch, stage, update, timeout_ticks,
+                                                  max_groups, s);
+        default:
+            return hipErrorInvalidValue;
+        }
     }
     if (level_count > 0) {
         if (update)
@@ -2054,13 +2095,32 @@ template <class T>
 hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part,
                       Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode, int level, int level_base,
                       int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
-    if (level < 0)
-        return persistent_wg(k_bwd_block<T, kBlockY, kBlockZ>, pc.nblk, 64 * kBlockY * kBlockZ, kBlockGroupsPerCU,
-                             max_groups, s, bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout_ticks);
+    if (level < 0) {
+        switch (pc.shape) {
+        case kShape2x2:
+            return launch_bwd_shape<T, kShape2x2>(bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout_ticks,
+                                                  max_groups, s);
+        case kShape1x4:
+            return launch_bwd_shape<T, kShape1x4>(bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout_ticks,
+                                                  max_groups, s);
+        case kShape2x4:
+            return launch_bwd_shape<T, kShape2x4>(bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout_ticks,
+                                                  max_groups, s);
+        default:
+            return hipErrorInvalidValue;
+        }
+    }
     if (level_count > 0)
         hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
                            level_base, timeout_ticks);
     return hipGetLastError();
+}
+
+int sweep_round_groups(int max_groups) {
+    int g = kBlockGroupsPerCU * device_cus();
+    if (g > kTailMaxGroups) g = kTailMaxGroups;
+    if (max_groups > 0 && g > max_groups) g = max_groups;
+    return g < 1 ? 1 : g;
 }
 
 hipError_t launch_pencil_flags(const int32_t* bmap, int nbx, int nby, int nbz, uint8_t* pflag, hipStream_t s) {
@@ -2220,9 +2280,18 @@ hipError_t launch_valid_zero(const Dense& d, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_block_list(const int32_t* pmap, int nby, int nbz, uint8_t* flag, int32_t* list, int64_t* count,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_block_list, dim3(1), dim3(1024), 0, s, pmap, nby, nbz, kBlockY, kBlockZ, flag, list, count);
+hipError_t launch_block_lists(const int32_t* pmap, int nby, int nbz, uint8_t* flag, int32_t* list, int64_t* count,
+                              hipStream_t s) {
+    if (nby > 1023 || nbz > 1023) return hipErrorInvalidValue;
+    ShapeLists sl;
+    int64_t base = 0;
+    for (int k = 0; k < kShapeCount; ++k) {
+        sl.by[k] = kShapeY[k];
+        sl.bz[k] = kShapeZ[k];
+        sl.base[k] = base;
+        base += (int64_t)shape_blocks(nby, nbz, k);
+    }
+    hipLaunchKernelGGL(k_block_list, dim3(kShapeCount), dim3(1024), 0, s, pmap, nby, nbz, sl, flag, list, count);
     return hipGetLastError();
 }
 

@@ -40,19 +40,32 @@ struct Bricks {
 // dependency of the triangular sweeps stays in registers.
 struct Pencils {
     int np;                   // pencils with at least one active brick
-    int nblk;                 // pencil blocks (kBlockY x kBlockZ pencils) with an active pencil
+    int shape;                // pencil-block shape (BlockShape) of the sweeps
+    int nblk;                 // pencil blocks of that shape with an active pencil
     const int32_t* blk_list;  // nblk packed (Y | Z << 10), sorted by Y + Z, then Y
 };
 
-// Pencil blocks of the wavefront sweeps: kBlockY x kBlockZ pencils per
-// workgroup (one wave each).
-constexpr int kBlockY = 2;
-constexpr int kBlockZ = 2;
-static_assert(kBlockY >= 2 && kBlockZ >= 2 && kBlockY * kBlockZ <= 16, "pencil block shape");
+// Pencil-block shapes of the wavefront sweeps: BY x BZ pencils per workgroup,
+// one wave each. Faces between the waves of a block go through LDS; faces
+// between blocks through the exchange words in HBM. BY * BZ is a power of
+// two (the grid tail's reduction tree over the waves of a workgroup).
+enum BlockShape : int { kShape2x2 = 0, kShape1x4 = 1, kShape2x4 = 2, kShapeCount = 3 };
+constexpr int kShapeY[kShapeCount] = {2, 1, 2};
+constexpr int kShapeZ[kShapeCount] = {2, 4, 4};
 
-// List of pencil blocks from the pencil map (one workgroup); *count = nblk.
-hipError_t launch_block_list(const int32_t* pmap, int nby, int nbz, uint8_t* flag, int32_t* list, int64_t* count,
-                             hipStream_t s);
+// Pencil blocks of one shape over an nby x nbz pencil grid.
+inline size_t shape_blocks(int nby, int nbz, int shape) {
+    return (size_t)((nby + kShapeY[shape] - 1) / kShapeY[shape]) * (size_t)((nbz + kShapeZ[shape] - 1) / kShapeZ[shape]);
+}
+
+// Block lists of every shape from the pencil map, shape k at offset
+// sum_{s<k} shape_blocks(nby, nbz, s) of flag and list; count[k] = its nblk.
+hipError_t launch_block_lists(const int32_t* pmap, int nby, int nbz, uint8_t* flag, int32_t* list, int64_t* count,
+                              hipStream_t s);
+// Workgroups of one round of a sweep at four workgroups per CU (the FP32
+// residency of every shape up to 4 waves; FP64 sweeps hold two), capped by
+// max_groups > 0.
+int sweep_round_groups(int max_groups);
 
 // Coefficients of the stored system (device pointers), T = storage type.
 template <class T>

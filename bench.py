@@ -42,16 +42,18 @@ B_ITER = lambda sm, sv: 12 * sm + 16 * sv  # per PCG iteration and unknown
 B_OUTER = 64  # per refinement step and unknown (FP64 residual + x update)
 
 
+WORKLOADS = {
+    "spheres256": lambda seed: elq.scene_spheres(256, seed=seed, n_spheres=32),
+    "drop128": lambda seed: elq.scene_drop(128),
+    "drop64": lambda seed: elq.scene_drop(64),
+    "spheres512": lambda seed: elq.scene_spheres(512, seed=seed, n_spheres=128),
+}
+
+
 def workload(name: str, seed: int):
-    if name == "cfg3":
-        return elq.scene_spheres(256, seed=seed, n_spheres=32), "spheres256"
-    if name == "cfg2":
-        return elq.scene_drop(128), "drop128"
-    if name == "cfg1":
-        return elq.scene_drop(64), "drop64"
-    if name == "cfg5":
-        return elq.scene_spheres(512, seed=seed, n_spheres=128), "spheres512"
-    raise SystemExit(f"unknown workload {name}")
+    if name not in WORKLOADS:
+        raise SystemExit(f"unknown workload {name} (one of {', '.join(WORKLOADS)})")
+    return WORKLOADS[name](seed), name
 
 
 def project(ctx, ds, opts):
@@ -404,7 +406,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cfg3")
+    ap.add_argument("--workload", default="spheres256", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--precond", default="ic0", choices=["ic0", "mic0"])
     ap.add_argument("--schedule", default="wavefront", choices=["wavefront", "levels"])

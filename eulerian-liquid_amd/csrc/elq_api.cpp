@@ -1251,9 +1251,7 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     off += align256(2 * cells);
     const size_t o_tfar = off;
     off += align256((size_t)r.tiles);
-    const size_t o_tfix = off;
-    off += align256(2 * (size_t)r.tiles);
-    const size_t o_tnear = off;
+    const size_t o_tnew = off;
     off += align256(2 * (size_t)r.tiles);
     const size_t o_list = off;
     off += align256(4 * (size_t)r.tiles);
@@ -1266,8 +1264,7 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     r.valid = valid;
     r.stamp = (uint16_t*)(base + o_stamp);
     r.tfar = base + o_tfar;
-    r.tfix = (uint16_t*)(base + o_tfix);
-    r.tnear = (uint16_t*)(base + o_tnear);
+    r.tnew = (uint16_t*)(base + o_tnew);
     r.list = (int32_t*)(base + o_list);
     r.counter = (uint32_t*)(base + o_counter);
     ctx->kt.clear();
@@ -1357,6 +1354,13 @@ void elq_set_profiling_filter(elq_ctx* ctx, const char* classes) {
     if (!ctx) return;
     ctx->prof_filter.clear();
     if (classes && *classes) ctx->prof_filter = "," + std::string(classes) + ",";
+}
+
+int elq_set_max_groups(elq_ctx* ctx, int max_groups) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (max_groups < 0) return ctx->fail(ELQ_ERR_INVALID_ARG, "max_groups must be >= 0");
+    ctx->max_groups = max_groups;
+    return ELQ_OK;
 }
 
 int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, double* ms, int64_t* launches, int cap) {

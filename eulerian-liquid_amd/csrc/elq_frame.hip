@@ -540,7 +540,7 @@ __device__ __forceinline__ TileXYZ rei_tile(const ReiSet& r, int t) {
     return T;
 }
 
-// One workgroup per tile: interface cells, stamps, tfar and tfix.
+// One workgroup per tile: interface cells, stamps, tfar and tnew.
 __global__ __launch_bounds__(256) void k_rei_init(ReiSet r) {
     const TileXYZ T = rei_tile(r, blockIdx.x);
     const int i = T.bx * kExtTX + (threadIdx.x & (kExtTX - 1));
@@ -570,35 +570,32 @@ __global__ __launch_bounds__(256) void k_rei_init(ReiSet r) {
     fix_any = __syncthreads_or(fix_any);
     if (threadIdx.x == 0) {
         r.tfar[T.t] = far_any ? 1 : 0;
-        r.tfix[T.t] = fix_any ? 0 : kReiFar;
+        r.tnew[T.t] = fix_any ? 0 : kReiFar;
         if (blockIdx.x == 0)
             for (int q = 0; q < 6; ++q) r.counter[q] = 0;
     }
 }
 
-// One thread per tile: tnear = 0 where the tile or a face neighbour holds a
-// FIX cell at the start.
-__global__ __launch_bounds__(256) void k_rei_near(ReiSet r) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= r.tiles) return;
-    const TileXYZ T = rei_tile(r, t);
-    bool near = r.tfix[t] == 0;
-    const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-        const int x = T.bx + off[q][0], y = T.by + off[q][1], z = T.bz + off[q][2];
-        if (x >= 0 && y >= 0 && z >= 0 && x < r.tx && y < r.ty && z < r.tz)
-            near |= r.tfix[x + r.tx * (y + r.ty * z)] == 0;
-    }
-    r.tnear[t] = near ? 0 : kReiFar;
-}
-
-// One thread per tile: the tiles pass l can change (a cell not reached yet
-// and a FIX cell in the tile or a face neighbour) are listed.
+// One thread per tile: the tiles pass l can change are listed. A cell not
+// reached yet is fixed in the first pass that sees a FIX neighbour, so pass l
+// can only fix cells next to cells fixed just before it (stamp == l): tiles
+// with a cell not reached yet whose own tnew or a face neighbour's is l.
 __global__ __launch_bounds__(256) void k_rei_select(ReiSet r, int l, int parity) {
     const int t = blockIdx.x * 256 + threadIdx.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) r.counter[2 + parity] = 0;
-    const bool act = t < r.tiles && r.tfar[t] != 0 && r.tnear[t] <= l;
+    bool act = false;
+    if (t < r.tiles && r.tfar[t] != 0) {
+        const TileXYZ T = rei_tile(r, t);
+        const uint16_t lv = (uint16_t)l;
+        act = r.tnew[t] == lv;
+        const int off[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const int x = T.bx + off[q][0], y = T.by + off[q][1], z = T.bz + off[q][2];
+            if (x >= 0 && y >= 0 && z >= 0 && x < r.tx && y < r.ty && z < r.tz)
+                act |= r.tnew[x + r.tx * (y + r.ty * z)] == lv;
+        }
+    }
     const uint64_t m = __ballot(act);
     if (!m) return;
     const int lane = threadIdx.x & 63;
@@ -668,7 +665,6 @@ __global__ __launch_bounds__(256) void k_rei_pass(ReiSet r, int l, int parity) {
         const int tt = t_next;
         if (idx + gridDim.x < count) t_next = r.list[idx + gridDim.x];
         const TileXYZ T = rei_tile(r, tt);
-        const uint16_t tfix0 = r.tfix[tt];
         const int i0 = T.bx * kExtTX - 1, j0 = T.by * kExtTY - 1, k0 = T.bz * kExtTZ - 1;
         uint16_t st[kPer];
         float fv[kPer];
@@ -721,21 +717,7 @@ __global__ __launch_bounds__(256) void k_rei_pass(ReiSet r, int l, int parity) {
         fixed = __reduce_add_sync(~0ull, fixed);
         if ((threadIdx.x & 63) == 0 && fixed) atomicAdd(&r.counter[2 + parity], fixed);
         if (threadIdx.x == 0) r.tfar[tt] = still ? 1 : 0;
-        if (became && tfix0 == kReiFar && threadIdx.x < 7) {
-            const int q = threadIdx.x;
-            int x = T.bx, y = T.by, z = T.bz;
-            if (q == 1) x -= 1;
-            if (q == 2) x += 1;
-            if (q == 3) y -= 1;
-            if (q == 4) y += 1;
-            if (q == 5) z -= 1;
-            if (q == 6) z += 1;
-            if (x >= 0 && y >= 0 && z >= 0 && x < r.tx && y < r.ty && z < r.tz) {
-                uint16_t* tn = r.tnear + (x + r.tx * (y + r.ty * z));
-                if (*tn == kReiFar) *tn = (uint16_t)(l + 1);
-            }
-        }
-        if (became && threadIdx.x == 0 && tfix0 == kReiFar) r.tfix[tt] = (uint16_t)(l + 1);
+        if (became && threadIdx.x == 0) r.tnew[tt] = (uint16_t)(l + 1);
         __syncthreads();  // LDS is free for the next tile
     }
 }
@@ -748,8 +730,7 @@ __global__ __launch_bounds__(256) void k_rei_rebase(ReiSet r) {
         if (s != kReiFar && s != kReiUnknown) r.stamp[c] = 0;
     }
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)r.tiles; t += stride) {
-        if (r.tfix[t] != kReiFar) r.tfix[t] = 0;
-        if (r.tnear[t] != kReiFar) r.tnear[t] = 0;
+        if (r.tnew[t] != kReiFar) r.tnew[t] = 0;  // every fixed cell now has stamp 0
     }
 }
 
@@ -839,7 +820,6 @@ hipError_t launch_advect_cells(int ni, int nj, int nk, float dx, float dt, const
 
 hipError_t launch_rei_init(const ReiSet& r, hipStream_t s) {
     hipLaunchKernelGGL(k_rei_init, dim3((unsigned)r.tiles), dim3(256), 0, s, r);
-    hipLaunchKernelGGL(k_rei_near, dim3(cdiv(r.tiles, 256)), dim3(256), 0, s, r);
     return hipGetLastError();
 }
 

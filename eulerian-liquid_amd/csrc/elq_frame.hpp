@@ -93,8 +93,8 @@ struct ReiSet {
     const uint8_t* valid;  // 0/1 flags
     uint16_t* stamp;
     uint8_t* tfar;         // per tile: 1 while the tile holds a cell not reached yet
-    uint16_t* tfix;        // per tile: first pass at whose start it holds a fixed cell (kReiFar: none)
-    uint16_t* tnear;       // per tile: same over the tile and its 6 face neighbours
+    uint16_t* tnew;        // per tile: latest pass at whose start it holds cells fixed just
+                           // before (stamp == that pass; kReiFar: none yet)
     int32_t* list;         // tiles of the current pass
     uint32_t* counter;     // [0..1] list lengths, [2..3] cells fixed by alternate passes,
                            // [4] cells left unreached, [5] largest stamp

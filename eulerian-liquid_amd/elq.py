@@ -121,6 +121,7 @@ def load():
     L.elq_last_kernel_times.argtypes = [vp, C.c_char_p, sz, C.POINTER(f64), C.POINTER(i64), C.c_int]
     L.elq_set_profiling.argtypes = [vp, C.c_int]
     L.elq_set_profiling.restype = None
+    L.elq_set_max_groups.argtypes = [vp, C.c_int]
     L.elq_server_create.argtypes = [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]
     L.elq_server_destroy.argtypes = [vp]
     L.elq_server_destroy.restype = None
@@ -435,6 +436,10 @@ class Context:
             self.free(pp)
             self.free(pv)
         return int(left.value), int(passes.value)
+
+    def set_max_groups(self, max_groups: int):
+        """Caps the persistent kernels' grid (0: whole device)."""
+        _check_rc(self.L.elq_set_max_groups(self.h, int(max_groups)), self.h)
 
     def set_profiling(self, on: bool, only: str = ""):
         """Per-kernel event timing; only: comma-separated kernel classes to time (all if empty)."""

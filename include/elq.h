@@ -269,6 +269,13 @@ void elq_set_profiling(elq_ctx* ctx, int enable);
 /* Restricts event timing to the named kernel classes (comma-separated, e.g.
  * "fwd_sweep_update"); NULL or "" times every class. */
 void elq_set_profiling_filter(elq_ctx* ctx, const char* classes);
+/* Caps the grid of the persistent solver kernels at max_groups workgroups
+ * (0: as many as the device holds), a share of the device for solves that
+ * run beside other work; the serving slots use it. No reference
+ * counterpart. Results depend on the cap (the reduction tree and the
+ * sweeps' block shape follow the grid), never on timing. Returns
+ * ELQ_ERR_INVALID_ARG for a negative cap. */
+int elq_set_max_groups(elq_ctx* ctx, int max_groups);
 
 /* ------------------------------------------------------------------ serving
  * Continuous batching of independent projections (uneven grid sizes) on one

@@ -109,6 +109,28 @@ def test_projection_mixed(ctx, name):
     assert st.outer_iterations >= 2
 
 
+@pytest.mark.parametrize("cap", [4, 8])
+@pytest.mark.parametrize("precision, tol", [(elq.MIXED, 2e-5), (elq.FP64, 2e-6)])
+def test_projection_capped_grid(precision, tol, cap):
+    """A workgroup cap below the pencil-block count: the sweeps switch to
+    their second block shape (FP32 2x2, FP64 2x4) and walk the block list in
+    several rounds, the SpMV and factor in more bricks per wave; the solution
+    meets the same bounds as the uncapped solve. A negative cap is refused."""
+    c = elq.Context(0)
+    try:
+        with pytest.raises(elq.ElqError):
+            c.set_max_groups(-1)
+        c.set_max_groups(cap)
+        s = SCENES["drop32"]()
+        g = s.copy()
+        o = elq.default_opts(tol=1e-8, max_iter=5000, precision=precision)
+        p, idx, st = c.solve_pressure(g, o)
+        assert st.status == elq.OK and st.converged, st.as_dict()
+        _check_solution(s, p, idx, st, tol)
+    finally:
+        c.close()
+
+
 def test_reference_defaults_iteration_semantics(ctx):
     """tol 1e-4 / maxit 1000 (simulator.cpp:105-106): Eigen error() below tol."""
     s = SCENES["drop32"]()

@@ -1255,6 +1255,14 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     off += align256(2 * (size_t)r.tiles);
     const size_t o_list = off;
     off += align256(4 * (size_t)r.tiles);
+    const size_t o_dist = off;
+    off += align256(2 * cells);
+    const size_t o_dcount = off;
+    off += align256(4 * (size_t)(kReiDistMax + 2));
+    const size_t o_dfill = off;
+    off += align256(4 * (size_t)(kReiDistMax + 2));
+    const size_t o_cells = off;
+    off += align256(4 * cells);
     const size_t o_counter = off;
     off += 256;
     HIPCHK(ctx->ws.rei.ensure(off), "alloc");
@@ -1266,6 +1274,10 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     r.tfar = base + o_tfar;
     r.tnew = (uint16_t*)(base + o_tnew);
     r.list = (int32_t*)(base + o_list);
+    r.dist = (uint16_t*)(base + o_dist);
+    r.dcount = (uint32_t*)(base + o_dcount);
+    r.dfill = (uint32_t*)(base + o_dfill);
+    r.cells = (uint32_t*)(base + o_cells);
     r.counter = (uint32_t*)(base + o_counter);
     ctx->kt.clear();
     // grid_temp = grid (reinitialization.cpp:26)
@@ -1273,6 +1285,34 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     ctx->prof_begin("rei_init");
     HIPCHK(launch_rei_init(r, ctx->stream), "k_rei_init");
     ctx->prof_end();
+    // Where no invalid cell lengthens a path, pass l fixes the cells at
+    // taxicab distance l + 1 from the interface: one launch per pass over
+    // those cells only.
+    ctx->prof_begin("rei_dist");
+    HIPCHK(launch_rei_dist(r, ctx->stream), "k_rei_dist");
+    ctx->prof_end();
+    uint32_t chk[2] = {1, 0};
+    HIPCHK(hipMemcpyAsync(chk, r.counter + 6, 8, hipMemcpyDeviceToHost, ctx->stream), "read counter");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    if (chk[0] == 0) {
+        ctx->prof_begin("rei_lists");
+        HIPCHK(launch_rei_lists(r, ctx->stream), "k_rei_lists");
+        ctx->prof_end();
+        std::vector<uint32_t> offs(chk[1] + 2);
+        HIPCHK(hipMemcpyAsync(offs.data(), r.dcount, offs.size() * 4, hipMemcpyDeviceToHost, ctx->stream),
+               "read offsets");
+        HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+        for (uint32_t t = 1; t <= chk[1]; ++t) {
+            ctx->prof_begin("rei_layer");
+            HIPCHK(launch_rei_layer(r, offs[t], offs[t + 1], (int)t, ctx->stream), "k_rei_layer");
+            ctx->prof_end();
+        }
+        ctx->prof_collect();
+        HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+        if (unreached) *unreached = 0;
+        if (passes) *passes = (int32_t)chk[1];
+        return ELQ_OK;
+    }
     // Passes in batches; after each batch the host reads how many cells the
     // batch's last pass fixed. A pass that fixes nothing ends the march: no
     // later pass could change anything (the reference then returns, or, with

@@ -572,8 +572,9 @@ __global__ __launch_bounds__(256) void k_rei_init(ReiSet r) {
         r.tfar[T.t] = far_any ? 1 : 0;
         r.tnew[T.t] = fix_any ? 0 : kReiFar;
         if (blockIdx.x == 0)
-            for (int q = 0; q < 6; ++q) r.counter[q] = 0;
+            for (int q = 0; q < 8; ++q) r.counter[q] = 0;
     }
+    for (int q = blockIdx.x * 256 + threadIdx.x; q < kReiDistMax + 2; q += gridDim.x * 256) r.dcount[q] = 0;
 }
 
 // One thread per tile: the tiles pass l can change are listed. A cell not
@@ -722,6 +723,276 @@ __global__ __launch_bounds__(256) void k_rei_pass(ReiSet r, int l, int parity) {
     }
 }
 
+// ---- passes ordered by distance
+// Pass l fixes exactly the valid cells whose shortest path to an interface
+// cell through valid cells has l + 1 steps. Where no invalid cell lies in the
+// way, that length is the taxicab distance, computed by three separable
+// passes; a check proves it cell by cell (a valid non-interface cell at
+// distance d has a valid neighbour at d - 1, so by induction a valid path of
+// d steps exists, and no path is shorter than the taxicab distance). Then
+// each pass touches only its own cells. Otherwise the tile passes run.
+constexpr uint16_t kDistInf = 0xFFFF;
+constexpr int kDistBig = 1 << 28;
+
+__device__ __forceinline__ uint16_t dist_step(uint16_t prev) { return prev >= kDistInf - 1 ? kDistInf : prev + 1; }
+
+// x: one wave per grid row, prefix and suffix minima over 64-cell chunks:
+// d(i) = min(i + min_{i' <= i}(g - i'), min_{i' >= i}(g + i') - i), g = 0 at
+// interface cells.
+__global__ __launch_bounds__(256) void k_rei_dist_x(ReiSet r) {
+    const int line = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    if (line >= r.nj * r.nk) return;
+    const int lane = threadIdx.x & 63;
+    const size_t row = (size_t)line * (size_t)r.ni;
+    int carry = kDistBig;
+    for (int c0 = 0; c0 < r.ni; c0 += 64) {
+        const int i = c0 + lane;
+        int h = (i < r.ni && r.stamp[row + i] == 0) ? -i : kDistBig;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(h, o, 64);
+            if (lane >= o) h = min(h, t);
+        }
+        h = min(h, carry);
+        carry = __shfl(h, 63, 64);
+        if (i < r.ni) r.dist[row + i] = h >= kDistBig / 2 ? kDistInf : (uint16_t)(i + h);
+    }
+    carry = kDistBig;
+    for (int c0 = ((r.ni - 1) / 64) * 64; c0 >= 0; c0 -= 64) {
+        const int i = c0 + lane;
+        int h = (i < r.ni && r.stamp[row + i] == 0) ? i : kDistBig;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_down(h, o, 64);
+            if (lane + o < 64) h = min(h, t);
+        }
+        h = min(h, carry);
+        carry = __shfl(h, 0, 64);
+        if (i < r.ni && h < kDistBig / 2) {
+            const uint16_t b = (uint16_t)(h - i);
+            if (b < r.dist[row + i]) r.dist[row + i] = b;
+        }
+    }
+}
+
+// y (AXIS 1) or z (AXIS 2): one thread per column, a forward and a backward
+// running minimum, loads in chunks of 16 so they overlap.
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_rei_dist_yz(ReiSet r) {
+    const int i = blockIdx.x * 64 + (int)(threadIdx.x & 63);
+    const int o = blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+    const int n = AXIS == 1 ? r.nj : r.nk;
+    if (i >= r.ni || o >= (AXIS == 1 ? r.nk : r.nj)) return;
+    const size_t stride = AXIS == 1 ? (size_t)r.ni : (size_t)r.ni * r.nj;
+    const size_t base = AXIS == 1 ? (size_t)i + (size_t)r.ni * r.nj * o : (size_t)i + (size_t)r.ni * o;
+    uint16_t* __restrict__ d = r.dist + base;
+    constexpr int C = 16;
+    uint16_t prev = kDistInf;
+    for (int a0 = 0; a0 < n; a0 += C) {
+        uint16_t v[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) v[q] = a0 + q < n ? d[(size_t)(a0 + q) * stride] : kDistInf;
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const uint16_t s1 = dist_step(prev);
+            v[q] = v[q] < s1 ? v[q] : s1;
+            prev = v[q];
+        }
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+            if (a0 + q < n) d[(size_t)(a0 + q) * stride] = v[q];
+    }
+    prev = kDistInf;
+    for (int a1 = n; a1 > 0; a1 -= C) {
+        uint16_t v[C];
+#pragma unroll
+        for (int q = 0; q < C; ++q) v[q] = a1 - 1 - q >= 0 ? d[(size_t)(a1 - 1 - q) * stride] : kDistInf;
+#pragma unroll
+        for (int q = 0; q < C; ++q) {
+            const uint16_t s1 = dist_step(prev);
+            v[q] = v[q] < s1 ? v[q] : s1;
+            prev = v[q];
+        }
+#pragma unroll
+        for (int q = 0; q < C; ++q)
+            if (a1 - 1 - q >= 0) d[(size_t)(a1 - 1 - q) * stride] = v[q];
+    }
+}
+
+// Whether a neighbour (in the grid, valid) lies at distance want.
+__device__ __forceinline__ bool rei_at_dist(const ReiSet& r, int i, int j, int k, uint16_t want) {
+    if (i < 0 || j < 0 || k < 0 || i >= r.ni || j >= r.nj || k >= r.nk) return false;
+    const size_t c = at(i, j, k, r.ni, r.nj);
+    return r.stamp[c] != kReiUnknown && r.dist[c] == want;
+}
+
+// The cells not reached yet are bucketed by distance per chunk of
+// kReiBucketCells cells (one workgroup): distances inside a chunk span less
+// than kReiBins (neighbouring cells differ by at most 1, and a chunk covers at
+// most a grid row plus a few cells of the next ones), so a chunk counts its
+// cells per distance in LDS, with one global atomic per distinct distance.
+constexpr int kReiBucketThreads = 1024;
+constexpr int kReiBucketPer = 4;
+constexpr int kReiBucketCells = kReiBucketThreads * kReiBucketPer;
+constexpr int kReiBins = 8192;
+
+struct ReiChunk {
+    size_t c[kReiBucketPer];
+    uint32_t d[kReiBucketPer];   // distance of a cell not reached yet, else 0
+    uint32_t rank[kReiBucketPer];
+};
+
+// Loads a chunk's distances (0 for cells that are not FAR), its smallest
+// distance (*dmin) and the span; sets LDS bins[0..span) to the per-distance
+// counts and rank[] to each cell's place among equal distances. Returns the
+// span, or 0 when the chunk holds no FAR cell.
+__device__ int rei_chunk_bins(const ReiSet& r, ReiChunk& ch, uint32_t* bins, uint32_t* red, uint32_t& dmin) {
+    const size_t n = (size_t)r.ni * r.nj * r.nk;
+    const size_t c0 = (size_t)blockIdx.x * kReiBucketCells;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0;
+#pragma unroll
+    for (int q = 0; q < kReiBucketPer; ++q) {
+        ch.c[q] = c0 + (size_t)q * kReiBucketThreads + threadIdx.x;
+        const bool far = ch.c[q] < n && r.stamp[ch.c[q]] == kReiFar;
+        ch.d[q] = far ? r.dist[ch.c[q]] : 0u;
+        if (far) {
+            lo = min(lo, ch.d[q]);
+            hi = max(hi, ch.d[q]);
+        }
+    }
+    lo = __reduce_min_sync(~0ull, lo);
+    hi = __reduce_max_sync(~0ull, hi);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = lo;
+        red[32 + (threadIdx.x >> 6)] = hi;
+    }
+    __syncthreads();
+    lo = 0xFFFFFFFFu;
+    hi = 0;
+    for (int w = 0; w < kReiBucketThreads / 64; ++w) {
+        lo = min(lo, red[w]);
+        hi = max(hi, red[32 + w]);
+    }
+    if (lo > hi) return 0;
+    dmin = lo;
+    const int span = (int)(hi - lo + 1);
+    if (span > kReiBins) return -1;
+    for (int b = threadIdx.x; b < span; b += kReiBucketThreads) bins[b] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kReiBucketPer; ++q)
+        if (ch.d[q]) ch.rank[q] = atomicAdd(&bins[ch.d[q] - lo], 1u);
+    __syncthreads();
+    return span;
+}
+
+// Check, and counts per distance of the cells that pass it.
+__global__ __launch_bounds__(kReiBucketThreads) void k_rei_dist_check(ReiSet r) {
+    __shared__ uint32_t bins[kReiBins];
+    __shared__ uint32_t red[64];
+    ReiChunk ch;
+    uint32_t bad = 0, top = 0;
+    // the check: a FAR cell needs a finite distance >= 1 and a valid neighbour one step closer
+#pragma unroll
+    for (int q = 0; q < kReiBucketPer; ++q) {
+        const size_t n = (size_t)r.ni * r.nj * r.nk;
+        const size_t c = (size_t)blockIdx.x * kReiBucketCells + (size_t)q * kReiBucketThreads + threadIdx.x;
+        if (c >= n || r.stamp[c] != kReiFar) continue;
+        const uint16_t dv = r.dist[c];
+        bool ok = dv != 0 && dv != kDistInf;
+        if (ok) {
+            const int i = (int)(c % r.ni), j = (int)((c / r.ni) % r.nj), k = (int)(c / ((size_t)r.ni * r.nj));
+            const uint16_t w = dv - 1;
+            ok = rei_at_dist(r, i - 1, j, k, w) || rei_at_dist(r, i + 1, j, k, w) ||
+                 rei_at_dist(r, i, j - 1, k, w) || rei_at_dist(r, i, j + 1, k, w) ||
+                 rei_at_dist(r, i, j, k - 1, w) || rei_at_dist(r, i, j, k + 1, w);
+        }
+        if (!ok) ++bad;
+        else top = max(top, (uint32_t)dv);
+    }
+    bad = __reduce_add_sync(~0ull, bad);
+    top = __reduce_max_sync(~0ull, top);
+    if ((threadIdx.x & 63) == 0) {
+        if (bad) atomicAdd(&r.counter[6], bad);
+        if (top) atomicMax(&r.counter[7], top);
+    }
+    uint32_t dmin = 0;
+    const int span = rei_chunk_bins(r, ch, bins, red, dmin);
+    if (span < 0) {
+        if (threadIdx.x == 0) atomicAdd(&r.counter[6], 1u);  // not expected; the tile passes run instead
+        return;
+    }
+    for (int b = threadIdx.x; b < span; b += kReiBucketThreads)
+        if (bins[b] && dmin + b <= (uint32_t)kReiDistMax) atomicAdd(&r.dcount[dmin + b], bins[b]);
+}
+
+// Exclusive scan of dcount[0..kReiDistMax + 1] in place (one workgroup);
+// dfill gets the same offsets.
+__global__ __launch_bounds__(1024) void k_rei_dscan(ReiSet r) {
+    constexpr int N = kReiDistMax + 2, PER = (N + 1023) / 1024;
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    uint32_t v[PER], sum = 0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = t * PER + q;
+        v[q] = idx < N ? r.dcount[idx] : 0u;
+        sum += v[q];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t add = t >= o ? part[t - o] : 0u;
+        __syncthreads();
+        part[t] += add;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int idx = t * PER + q;
+        if (idx < N) {
+            r.dcount[idx] = run;
+            r.dfill[idx] = run;
+        }
+        run += v[q];
+    }
+}
+
+// Lists the cells not reached yet by distance (after a passed check, every
+// one has a distance in 1..kReiDistMax): a chunk reserves one run of slots
+// per distance and keeps its cells together inside it.
+__global__ __launch_bounds__(kReiBucketThreads) void k_rei_scatter(ReiSet r) {
+    __shared__ uint32_t bins[kReiBins];
+    __shared__ uint32_t red[64];
+    ReiChunk ch;
+    uint32_t dmin = 0;
+    const int span = rei_chunk_bins(r, ch, bins, red, dmin);
+    if (span <= 0) return;
+    for (int b = threadIdx.x; b < span; b += kReiBucketThreads)
+        if (bins[b]) bins[b] = atomicAdd(&r.dfill[dmin + b], bins[b]);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kReiBucketPer; ++q)
+        if (ch.d[q]) r.cells[bins[ch.d[q] - dmin] + ch.rank[q]] = (uint32_t)ch.c[q];
+}
+
+// One pass, one listed cell per thread: the cell takes its FIX neighbours
+// (valid, distance target - 1) exactly as k_rei_pass does.
+__global__ __launch_bounds__(256) void k_rei_layer(ReiSet r, uint32_t begin, uint32_t end, uint32_t target) {
+    const uint32_t e = begin + blockIdx.x * 256 + threadIdx.x;
+    if (e >= end) return;
+    const size_t c = r.cells[e];
+    const uint16_t w = (uint16_t)(target - 1);
+    const int i = (int)(c % r.ni), j = (int)((c / r.ni) % r.nj), k = (int)(c / ((size_t)r.ni * r.nj));
+    const bool fxm = rei_at_dist(r, i - 1, j, k, w), fxp = rei_at_dist(r, i + 1, j, k, w);
+    const bool fym = rei_at_dist(r, i, j - 1, k, w), fyp = rei_at_dist(r, i, j + 1, k, w);
+    const bool fzm = rei_at_dist(r, i, j, k - 1, w), fzp = rei_at_dist(r, i, j, k + 1, w);
+    const size_t sx = 1, sy = (size_t)r.ni, sz = (size_t)r.ni * r.nj;
+    const float vx = fxm ? r.phi[c - sx] : (fxp ? r.phi[c + sx] : 0.0f);
+    const float vy = fym ? r.phi[c - sy] : (fyp ? r.phi[c + sy] : 0.0f);
+    const float vz = fzm ? r.phi[c - sz] : (fzp ? r.phi[c + sz] : 0.0f);
+    r.phi[c] = rei_update(fxm || fxp, vx, fym || fyp, vy, fzm || fzp, vz, r.dx);
+}
+
 __global__ __launch_bounds__(256) void k_rei_rebase(ReiSet r) {
     const size_t n = (size_t)r.ni * r.nj * r.nk;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -827,6 +1098,29 @@ hipError_t launch_rei_pass(const ReiSet& r, int l, int parity, hipStream_t s) {
     hipLaunchKernelGGL(k_rei_select, dim3(cdiv(r.tiles, 256)), dim3(256), 0, s, r, l, parity);
     const unsigned groups = (unsigned)std::min<int64_t>(r.tiles, kExtLayerGroups);
     hipLaunchKernelGGL(k_rei_pass, dim3(groups), dim3(256), 0, s, r, l, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_rei_dist(const ReiSet& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_rei_dist_x, dim3(cdiv((int64_t)r.nj * r.nk, 4)), dim3(256), 0, s, r);
+    hipLaunchKernelGGL(k_rei_dist_yz<1>, dim3(cdiv(r.ni, 64), cdiv(r.nk, 4)), dim3(256), 0, s, r);
+    hipLaunchKernelGGL(k_rei_dist_yz<2>, dim3(cdiv(r.ni, 64), cdiv(r.nj, 4)), dim3(256), 0, s, r);
+    const int64_t n = (int64_t)r.ni * r.nj * r.nk;
+    hipLaunchKernelGGL(k_rei_dist_check, dim3(cdiv(n, kReiBucketCells)), dim3(kReiBucketThreads), 0, s, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s) {
+    hipLaunchKernelGGL(k_rei_dscan, dim3(1), dim3(1024), 0, s, r);
+    hipLaunchKernelGGL(k_rei_scatter, dim3(cdiv((int64_t)r.ni * r.nj * r.nk, kReiBucketCells)),
+                       dim3(kReiBucketThreads), 0, s, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, int target, hipStream_t s) {
+    if (end <= begin) return hipSuccess;
+    hipLaunchKernelGGL(k_rei_layer, dim3(cdiv((int64_t)(end - begin), 256)), dim3(256), 0, s, r, begin, end,
+                       (uint32_t)target);
     return hipGetLastError();
 }
 

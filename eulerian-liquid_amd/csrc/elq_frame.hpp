@@ -96,8 +96,13 @@ struct ReiSet {
     uint16_t* tnew;        // per tile: latest pass at whose start it holds cells fixed just
                            // before (stamp == that pass; kReiFar: none yet)
     int32_t* list;         // tiles of the current pass
+    uint16_t* dist;        // per cell: taxicab distance to the nearest interface cell
+    uint32_t* dcount;      // per distance 0..kReiDistMax: cells not reached yet, then list offsets
+    uint32_t* dfill;       // per distance: next free list slot
+    uint32_t* cells;       // cells not reached yet, grouped by distance
     uint32_t* counter;     // [0..1] list lengths, [2..3] cells fixed by alternate passes,
-                           // [4] cells left unreached, [5] largest stamp
+                           // [4] cells left unreached, [5] largest stamp,
+                           // [6] cells where dist is not the march's pass, [7] largest dist
     int ni, nj, nk;
     int tx, ty, tz;
     int tiles;
@@ -106,6 +111,18 @@ struct ReiSet {
 
 // Interface cells (their geometric distance) and stamps; tile flags.
 hipError_t launch_rei_init(const ReiSet& r, hipStream_t s);
+// dist: taxicab distance transform of the interface cells (three separable
+// passes), then a check that it equals the march's pass count at every
+// valid cell (counter[6] = failures, counter[7] = largest distance) and a
+// count of the cells at each distance (dcount).
+constexpr int kReiDistMax = 3 * 4092;
+hipError_t launch_rei_dist(const ReiSet& r, hipStream_t s);
+// After the check passed: dcount becomes list offsets (exclusive scan over
+// 0..kReiDistMax + 1) and the cells not reached yet are listed by distance.
+hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s);
+// The cells of one pass (list entries [begin, end), all at distance target)
+// from their FIX neighbours (valid, distance target - 1).
+hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, int target, hipStream_t s);
 // Pass l (0-based since the last rebase; parity = absolute pass & 1).
 hipError_t launch_rei_pass(const ReiSet& r, int l, int parity, hipStream_t s);
 hipError_t launch_rei_rebase(const ReiSet& r, hipStream_t s);

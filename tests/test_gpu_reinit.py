@@ -8,8 +8,14 @@ same count of passes and of unreached cells:
 - meeting fronts (the -x/-y/-z neighbour is taken first);
 - a sphere with noise on a room() grid, flags from find_valid_level_set, and
   a pocket of cells walled off by invalid cells;
+- a wall of invalid cells with a hole, so that cells behind it are reached
+  by a detour (longer than their taxicab distance);
 - the 128^3 drop scene's level set after one advection step (about 100
   passes, several host batches).
+
+The library takes one of two routes, both checked here: passes ordered by
+the taxicab distance to the interface where that distance is the pass count
+(every test but the pocket and the wall), tile passes otherwise.
 """
 import numpy as np
 import pytest
@@ -24,12 +30,19 @@ def same(a, b):
     return np.array_equal(a.view(np.uint32), b.view(np.uint32))
 
 
-def check(ctx, phi, valid, dims, dx):
+def check(ctx, phi, valid, dims, dx, route="rei_layer"):
     ref = phi.copy()
     r_left, r_passes = O.reinitialize(ref, valid, dims, dx)
     got = phi.copy()
-    left, passes = ctx.reinitialize(got, valid, dims, dx)
+    ctx.set_profiling(True)
+    try:
+        left, passes = ctx.reinitialize(got, valid, dims, dx)
+        kinds = ctx.kernel_times()
+    finally:
+        ctx.set_profiling(False)
     assert (left, passes) == (r_left, r_passes)
+    if r_passes:
+        assert route in kinds, sorted(kinds)
     assert same(got, ref), np.flatnonzero(got.view(np.uint32) != ref.view(np.uint32))[:10]
     return passes
 
@@ -77,7 +90,22 @@ def test_sphere_with_noise_on_room_and_pocket(ctx):
     ref = phi.copy()
     left, _ = O.reinitialize(ref, valid, dims, dx)
     assert left == 27
-    check(ctx, phi, valid, dims, dx)
+    check(ctx, phi, valid, dims, dx, route="rei_pass")
+
+
+def test_wall_with_hole_detour_bit_exact(ctx):
+    dims, dx = (36, 32, 40), 1.0 / 36
+    ni, nj, nk = dims
+    x, y, z = xyz(dims, dx)
+    phi = (1.3 * (y - 0.3)).astype(np.float32)   # liquid below y = 0.3
+    k, j, i = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(ni), indexing="ij")
+    wall = (j == 16) & ~((np.abs(i - 6) <= 1) & (np.abs(k - 30) <= 1))
+    valid = np.ones(phi.size, np.uint8)
+    valid[wall.reshape(-1)] = 0
+    ref = phi.copy()
+    left, passes = O.reinitialize(ref, valid, dims, dx)
+    assert left == 0 and passes > 40   # 21 passes reach the far corner without the wall
+    check(ctx, phi, valid, dims, dx, route="rei_pass")
 
 
 def test_drop128_after_advection_bit_exact(ctx):

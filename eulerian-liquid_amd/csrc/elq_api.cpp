@@ -1039,24 +1039,24 @@ size_t align256(size_t n) { return (n + 255) / 256 * 256; }
 
 // extrapolation.cpp:5-46 over up to three grids at once. The loop's early
 // return (a layer that starts with no invalid cell) needs no host check: such
-// a layer changes nothing, and its workgroups see tinv == 0 and exit. Layers
-// beyond the largest grid dimension change nothing either (every cell is then
-// within reach of a valid cell, or there is none), so they are not launched.
+// a layer adds no cell to the plane and lists nothing. Layers beyond the
+// largest grid dimension change nothing either (every cell is then within
+// reach of a valid cell, or there is none), so they are not launched.
 int run_extrapolate(elq_ctx* ctx, const ExtArgs* a, int count, int layers) {
     ExtSet set{};
-    size_t off[3][4];
+    size_t off[3][3];
     size_t total = 0;
-    int64_t tiles = 0;
+    int64_t words = 0;
     int maxdim = 0;
     for (int q = 0; q < count; ++q) {
         const int64_t cells = (int64_t)a[q].ni * a[q].nj * a[q].nk;
-        const int64_t tq = ext_tiles(a[q].ni, a[q].nj, a[q].nk);
+        const int64_t wq = ext_words(a[q].ni, a[q].nj, a[q].nk);
         off[q][0] = total;
-        total += align256((size_t)cells);
-        for (int f = 1; f < 4; ++f) {
-            off[q][f] = total;
-            total += align256((size_t)tq);
-        }
+        total += align256((size_t)wq * 8);
+        off[q][1] = total;
+        total += align256((size_t)wq * 8);
+        off[q][2] = total;
+        total += align256((size_t)cells * 4);
         ExtGrid& g = set.g[q];
         g.f = a[q].f;
         g.valid_in = a[q].valid_in;
@@ -1064,43 +1064,34 @@ int run_extrapolate(elq_ctx* ctx, const ExtArgs* a, int count, int layers) {
         g.ni = a[q].ni;
         g.nj = a[q].nj;
         g.nk = a[q].nk;
-        g.tx = (a[q].ni + kExtTX - 1) / kExtTX;
-        g.ty = (a[q].nj + kExtTY - 1) / kExtTY;
-        g.tz = (a[q].nk + kExtTZ - 1) / kExtTZ;
-        g.tile0 = (int)tiles;
-        tiles += tq;
+        g.wpr = ext_wpr(a[q].ni);
+        g.slot = q;
+        g.word0 = words;
+        g.words = wq;
+        words += (wq + kExtWordAlign - 1) / kExtWordAlign * kExtWordAlign;
         maxdim = std::max({maxdim, a[q].ni, a[q].nj, a[q].nk});
     }
-    if (tiles > INT32_MAX / 4) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid too large");
-    const size_t list_off = total;
-    total += align256((size_t)tiles * 4);
     const size_t counter_off = total;
     total += 256;
     set.count = count;
-    set.tiles = (int)tiles;
+    set.words = words;
     layers = std::min(layers, maxdim);
     if (layers <= 0) return ELQ_OK;
     HIPCHK(ctx->ws.ext.ensure(total), "alloc");
     uint8_t* base = (uint8_t*)ctx->ws.ext.p;
     for (int q = 0; q < count; ++q) {
-        set.g[q].stamp = base + off[q][0];
-        set.g[q].tinv = base + off[q][1];
-        set.g[q].tfv = base + off[q][2];
-        set.g[q].tnear = base + off[q][3];
+        set.g[q].plane[0] = (uint64_t*)(base + off[q][0]);
+        set.g[q].plane[1] = (uint64_t*)(base + off[q][1]);
+        set.g[q].list = (uint32_t*)(base + off[q][2]);
     }
-    set.list = (int32_t*)(base + list_off);
     set.counter = (uint32_t*)(base + counter_off);
     ctx->kt.clear();
     ctx->prof_begin("ext_init");
     HIPCHK(launch_ext_init(set, ctx->stream), "k_ext_init");
     ctx->prof_end();
-    for (int l = 0, first = 0; l < layers; ++l) {
-        if (l - first == kExtRebase) {
-            HIPCHK(launch_ext_rebase(set, ctx->stream), "k_ext_rebase");
-            first = l;
-        }
+    for (int l = 0; l < layers; ++l) {
         ctx->prof_begin("ext_layer");
-        HIPCHK(launch_ext_layer(set, l - first, l & 1, ctx->stream), "k_ext_layer");
+        HIPCHK(launch_ext_layer(set, l, ctx->stream), "k_ext_layer");
         ctx->prof_end();
     }
     ctx->prof_collect();

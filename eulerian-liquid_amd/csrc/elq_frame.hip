@@ -14,25 +14,21 @@
 //   advect_staggered_grid              (trace_rk2 include/nama/math/ode.h:30-35, lerp
 //                                      include/nama/math/interpolation.h:10-72,186-193)
 //
-// All of it is byte and float streaming work, bound by HBM (or, for the
-// extrapolation layers, by how few tiles are active); results are bit-exact
-// with the reference (built with -ffp-contract=off; the float division is
-// IEEE round-to-nearest).
+// Most of it is byte and float streaming work, bound by HBM; the layered
+// steps (extrapolation, reinitialisation) are bound by their dependent
+// layers. Results are bit-exact with the reference (built with
+// -ffp-contract=off; the float division is IEEE round-to-nearest).
 //
 // Extrapolation. In the reference each layer takes a snapshot of the flags
 // and of the values, gives every invalid cell with at least one valid
 // neighbour (of the 26 around it) the mean of those neighbours, and marks it
-// valid. Here each cell carries a byte stamp instead of its flag: 0 for a
-// cell valid from the start, l + 1 for a cell that received its value in
-// layer l, 255 for a cell still invalid. "Valid at the start of layer l" is
-// stamp <= l, and a cell written in layer l gets stamp l + 1 > l, so the
-// layer reads its snapshot in place: no reader in layer l looks at a value or
-// a stamp that layer l changes the meaning of, and no copy of the grid is
-// needed. The same holds one level up for 32x8x8 tiles: tinv says a tile
-// still holds an invalid cell, tfv is the first layer at whose start it holds
-// a valid one; a layer skips every tile with no invalid cell or with no valid
-// cell in its 3x3x3 tile neighbourhood, which leaves such a tile unchanged in
-// the reference too.
+// valid. Here the flags are bit planes: the cells a layer adds are exactly
+// the 3x3x3 box dilation of the plane minus the plane, so a first launch
+// dilates the plane (into the second buffer) and lists the added cells, and
+// a second launch computes only those cells, reading neighbours that were
+// valid at the layer's start, which that launch does not write. No copy of
+// the values is needed.
+
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -43,11 +39,9 @@
 namespace elq {
 namespace {
 
-constexpr uint8_t kNever = 255;
 constexpr int kHX = kExtTX + 2, kHY = kExtTY + 2, kHZ = kExtTZ + 2;
 constexpr int kHalo = kHX * kHY * kHZ;
 static_assert(kExtTX == 32 && kExtTX * kExtTY == 256, "one thread per (x, y) column of a tile");
-static_assert(kExtRebase < kNever, "stamps of one stretch of layers fit below the invalid mark");
 
 __device__ __forceinline__ size_t at(int i, int j, int k, int ni, int nj) {
     return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
@@ -55,224 +49,214 @@ __device__ __forceinline__ size_t at(int i, int j, int k, int ni, int nj) {
 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
-// The grid of the set that owns global tile gt (tile0 ascending), by value:
-// the fields stay scalar loads from the kernel arguments.
-__device__ __forceinline__ ExtGrid pick(const ExtSet& s, int gt) {
-    if (s.count > 2 && gt >= s.g[2].tile0) return s.g[2];
-    if (s.count > 1 && gt >= s.g[1].tile0) return s.g[1];
-    return s.g[0];
+
+// Grid q of the set, field by field: selecting whole structs by a runtime
+// index makes the compiler keep the array in scratch memory.
+template <class T>
+__device__ __forceinline__ T sel3(int q, T a, T b, T c) {
+    return q == 2 ? c : (q == 1 ? b : a);
 }
+__device__ __forceinline__ ExtGrid pick_slot(const ExtSet& s, int q) {
+    ExtGrid g;
+#define ELQ_SEL(field) g.field = sel3(q, s.g[0].field, s.g[1].field, s.g[2].field)
+    ELQ_SEL(f);
+    ELQ_SEL(valid_in);
+    ELQ_SEL(valid_out);
+    ELQ_SEL(plane[0]);
+    ELQ_SEL(plane[1]);
+    ELQ_SEL(list);
+    ELQ_SEL(ni);
+    ELQ_SEL(nj);
+    ELQ_SEL(nk);
+    ELQ_SEL(wpr);
+    ELQ_SEL(slot);
+    ELQ_SEL(word0);
+    ELQ_SEL(words);
+#undef ELQ_SEL
+    return g;
+}
+
+// The grid of the set that owns word gw (word0 ascending). The callers' words
+// of one wave lie in one grid, so the index is made wave-uniform and the
+// fields stay scalar loads from the kernel arguments (a per-lane choice
+// would read them with vector loads for every word).
+__device__ __forceinline__ ExtGrid pick(const ExtSet& s, int64_t gw) {
+    const int q = (s.count > 2 && gw >= s.g[2].word0) ? 2 : ((s.count > 1 && gw >= s.g[1].word0) ? 1 : 0);
+    return pick_slot(s, __builtin_amdgcn_readfirstlane(q));
+}
+
 
 struct TileXYZ {
     int t, bx, by, bz;
 };
 
-__device__ __forceinline__ TileXYZ tile_of(const ExtGrid& g, int gt) {
-    TileXYZ r;
-    r.t = gt - g.tile0;
-    r.bx = r.t % g.tx;
-    r.by = (r.t / g.tx) % g.ty;
-    r.bz = r.t / (g.tx * g.ty);
-    return r;
-}
-
-// One workgroup per tile: stamps from the caller's flags, tinv and tfv.
-// Each thread takes an (x, y) column; its eight loads are issued together.
+// Waves walk the words (stride: all waves of the grid); lane b reads the
+// caller's flag of cell 64 x + b, a ballot makes the word.
 __global__ __launch_bounds__(256) void k_ext_init(ExtSet set) {
-    const ExtGrid g = pick(set, blockIdx.x);
-    const TileXYZ T = tile_of(g, blockIdx.x);
-    const int i = T.bx * kExtTX + (threadIdx.x & (kExtTX - 1));
-    const int j = T.by * kExtTY + (threadIdx.x / kExtTX);
-    const bool col = i < g.ni && j < g.nj;
-    uint8_t v[kExtTZ];
-#pragma unroll
-    for (int z = 0; z < kExtTZ; ++z) {
-        const int k = T.bz * kExtTZ + z;
-        v[z] = (col && k < g.nk) ? g.valid_in[at(i, j, k, g.ni, g.nj)] : 1;
-    }
-    bool inv = false, val = false;
-#pragma unroll
-    for (int z = 0; z < kExtTZ; ++z) {
-        const int k = T.bz * kExtTZ + z;
-        if (col && k < g.nk) {
-            g.stamp[at(i, j, k, g.ni, g.nj)] = v[z] ? 0 : kNever;
-            inv |= v[z] == 0;
-            val |= v[z] != 0;
-        }
-    }
-    inv = __syncthreads_or(inv);
-    val = __syncthreads_or(val);
-    if (threadIdx.x == 0) {
-        g.tinv[T.t] = inv ? 1 : 0;
-        g.tfv[T.t] = val ? 0 : kNever;
-        if (blockIdx.x == 0) set.counter[0] = 0;
-    }
-}
-
-// One thread per tile: tnear = 0 where a tile of the 3x3x3 block around it
-// holds a valid cell at the start.
-__global__ __launch_bounds__(256) void k_ext_near(ExtSet set) {
-    const int gt = blockIdx.x * 256 + threadIdx.x;
-    if (gt >= set.tiles) return;
-    const ExtGrid g = pick(set, gt);
-    const TileXYZ T = tile_of(g, gt);
-    bool near = false;
-#pragma unroll
-    for (int q = 0; q < 27; ++q) {
-        const int x = T.bx + q % 3 - 1, y = T.by + (q / 3) % 3 - 1, z = T.bz + q / 9 - 1;
-        if (x >= 0 && y >= 0 && z >= 0 && x < g.tx && y < g.ty && z < g.tz)
-            near |= g.tfv[x + g.tx * (y + g.ty * z)] == 0;
-    }
-    g.tnear[T.t] = near ? 0 : kNever;
-}
-
-// One thread per tile: the tiles layer l works on (an invalid cell and a
-// valid one in its 3x3x3 tile block) are appended to the list; the order of
-// the list does not matter, the tiles of a layer being independent.
-__global__ __launch_bounds__(256) void k_ext_select(ExtSet set, int l, int parity) {
-    const int gt = blockIdx.x * 256 + threadIdx.x;
-    bool act = false;
-    if (gt < set.tiles) {
-        const ExtGrid g = pick(set, gt);
-        const int t = gt - g.tile0;
-        act = g.tinv[t] != 0 && g.tnear[t] <= l;
-    }
-    const uint64_t m = __ballot(act);
-    if (!m) return;
     const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&set.counter[parity], (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (act) set.list[base + __popcll(m & ((1ull << lane) - 1))] = gt;
-}
-
-// extrapolation.cpp:20-44, one layer over the listed tiles. A tile and its
-// one-cell halo are staged in LDS: a 0/1 mask (valid at the layer's start)
-// and the value of each valid cell, 0.0f for the others. Each thread walks
-// one (x, y) column of the tile in z with the 3x3 neighbourhoods of three
-// planes in registers, and sums all 27 masked values in the reference's
-// order kk, jj, ii = -1..1. Adding +0.0f for an invalid neighbour leaves the
-// float sum unchanged: the sum starts at +0.0f, and a float sum of +0.0f
-// and further terms is -0.0f only when both addends are -0.0f, so the
-// running sum is never -0.0f and s + 0.0f == s. The count is the integer
-// sum of the masks; the cell takes sum / (float)count.
-__global__ __launch_bounds__(256, 5) void k_ext_layer(ExtSet set, int l, int parity) {
-    constexpr int kPer = (kHalo + 255) / 256;
-    __shared__ uint8_t s_m[kPer * 256];
-    __shared__ float s_v[kPer * 256];
-    const uint32_t count = set.counter[parity];
-    if (blockIdx.x == 0 && threadIdx.x == 0) set.counter[parity ^ 1] = 0;
-    const int lx = threadIdx.x & (kExtTX - 1), ly = threadIdx.x / kExtTX;
-    int gt_next = blockIdx.x < count ? set.list[blockIdx.x] : 0;
-    for (uint32_t idx = blockIdx.x; idx < count; idx += gridDim.x) {
-        const int gt = gt_next;
-        if (idx + gridDim.x < count) gt_next = set.list[idx + gridDim.x];  // in flight during this tile
-        const ExtGrid g = pick(set, gt);
-        const TileXYZ T = tile_of(g, gt);
-        const uint8_t tfv0 = g.tfv[T.t];
-        const int i0 = T.bx * kExtTX - 1, j0 = T.by * kExtTY - 1, k0 = T.bz * kExtTZ - 1;
-        // one round trip: stamps and values of the whole halo (values of
-        // invalid cells are loaded too and dropped)
-        uint8_t st[kPer];
-        float fv[kPer];
+    const int64_t wave = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6), waves = (int64_t)gridDim.x * 4;
+    if (wave == 0 && lane == 0)
+        for (int q = 0; q < 6; ++q) set.counter[q] = 0;
+    // eight words per step (one grid: grids start at multiples of
+    // kExtWordAlign words), their loads issued together
+    for (int64_t gw0 = wave * 8; gw0 < set.words; gw0 += waves * 8) {
+        const ExtGrid g = pick(set, gw0);
+        const int64_t w0 = gw0 - g.word0;
+        bool v[8];
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const int h = threadIdx.x + 256 * r;
-            const int hx = h % kHX, hy = (h / kHX) % kHY, hz = h / (kHX * kHY);
-            const int i = i0 + hx, j = j0 + hy, k = k0 + hz;
-            const bool in = h < kHalo && i >= 0 && j >= 0 && k >= 0 && i < g.ni && j < g.nj && k < g.nk;
-            const size_t c = in ? at(i, j, k, g.ni, g.nj) : 0;
-            st[r] = in ? g.stamp[c] : kNever;
-            fv[r] = in ? g.f[c] : 0.0f;
+        for (int q = 0; q < 8; ++q) {
+            const int64_t w = w0 + q;
+            const uint32_t row = (uint32_t)w / (uint32_t)g.wpr;
+            const int x = (int)((uint32_t)w - row * (uint32_t)g.wpr) * 64 + lane;
+            // unconditional load (cell 0 stands in outside the plane)
+            const bool in = w < g.words && x < g.ni;
+            const uint8_t b = g.valid_in[in ? (size_t)row * g.ni + x : 0];
+            v[q] = in && b != 0;
         }
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {
-            const bool valid = st[r] <= l;
-            s_m[threadIdx.x + 256 * r] = valid ? 1 : 0;
-            s_v[threadIdx.x + 256 * r] = valid ? fv[r] : 0.0f;
+        for (int q = 0; q < 8; ++q) {
+            const uint64_t bits = __ballot(v[q]);
+            if (lane == 0 && w0 + q < g.words) g.plane[0][w0 + q] = bits;
         }
-        __syncthreads();
-
-        const int i = T.bx * kExtTX + lx, j = T.by * kExtTY + ly;
-        bool still = false, became = false;
-        if (i < g.ni && j < g.nj) {
-            // planes hz = z, z + 1, z + 2 of the halo hold cell planes z - 1, z, z + 1:
-            // their 3x3 masked values, valid counts and centre masks
-            float v[3][9];
-            int cnt[3];
-            bool mid[3];
-            auto load_plane = [&](int p, int hz) {
-                cnt[p] = 0;
-#pragma unroll
-                for (int q = 0; q < 9; ++q) {
-                    const int h = (lx + q % 3) + kHX * ((ly + q / 3) + kHY * hz);
-                    v[p][q] = s_v[h];
-                    cnt[p] += s_m[h];
-                }
-                mid[p] = s_m[(lx + 1) + kHX * ((ly + 1) + kHY * hz)] != 0;
-            };
-            load_plane(0, 0);
-            load_plane(1, 1);
-#pragma unroll
-            for (int z = 0; z < kExtTZ; ++z) {
-                load_plane((z + 2) % 3, z + 2);
-                const int k = T.bz * kExtTZ + z;
-                if (k < g.nk && !mid[(z + 1) % 3]) {
-                    const int n = cnt[0] + cnt[1] + cnt[2];
-                    if (n > 0) {
-                        float sum = 0.0f;
-#pragma unroll
-                        for (int dp = 0; dp < 3; ++dp)
-#pragma unroll
-                            for (int q = 0; q < 9; ++q) sum += v[(z + dp) % 3][q];
-                        const size_t c = at(i, j, k, g.ni, g.nj);
-                        g.f[c] = sum / (float)n;
-                        g.stamp[c] = (uint8_t)(l + 1);
-                        if (g.valid_out) g.valid_out[c] = 1;
-                        became = true;
-                    } else {
-                        still = true;
-                    }
-                }
-            }
-        }
-        still = __syncthreads_or(still);
-        became = __syncthreads_or(became);
-        if (threadIdx.x == 0) g.tinv[T.t] = still ? 1 : 0;
-        // A tile's first valid cells make its 3x3x3 tile block "near" from the
-        // next layer on. Writers of one layer all write l + 1, and the select
-        // pass of this layer has already read tnear.
-        if (became && tfv0 == kNever && threadIdx.x < 27) {
-            const int q = threadIdx.x;
-            const int x = T.bx + q % 3 - 1, y = T.by + (q / 3) % 3 - 1, z = T.bz + q / 9 - 1;
-            if (x >= 0 && y >= 0 && z >= 0 && x < g.tx && y < g.ty && z < g.tz) {
-                uint8_t* tn = g.tnear + (x + g.tx * (y + g.ty * z));
-                if (*tn == kNever) *tn = (uint8_t)(l + 1);
-            }
-        }
-        if (became && threadIdx.x == 0 && tfv0 == kNever) g.tfv[T.t] = (uint8_t)(l + 1);
-        __syncthreads();  // LDS is free for the next tile
     }
 }
 
-// One workgroup per tile: stamps and tile stamps <= kExtRebase become 0.
-__global__ __launch_bounds__(256) void k_ext_rebase(ExtSet set) {
-    const ExtGrid g = pick(set, blockIdx.x);
-    const TileXYZ T = tile_of(g, blockIdx.x);
-    const int i = T.bx * kExtTX + (threadIdx.x & (kExtTX - 1));
-    const int j = T.by * kExtTY + (threadIdx.x / kExtTX);
-    if (i < g.ni && j < g.nj) {
-        for (int z = 0; z < kExtTZ; ++z) {
-            const int k = T.bz * kExtTZ + z;
-            if (k >= g.nk) break;
-            const size_t c = at(i, j, k, g.ni, g.nj);
-            if (g.stamp[c] != kNever) g.stamp[c] = 0;
-        }
+// Layer l, one thread per word: the 3x3x3 box dilation of the plane (x by
+// shifts with the neighbour words' edge bits, y and z by OR over rows), the
+// added cells appended to the grid's list (one atomic per workgroup; a
+// workgroup's words all belong to one grid).
+constexpr int kExtDilateThreads = 1024;
+__global__ __launch_bounds__(kExtDilateThreads) void k_ext_dilate(ExtSet set, int l) {
+    __shared__ uint32_t s_tot[kExtDilateThreads / 64 + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t gw = (int64_t)blockIdx.x * kExtDilateThreads + threadIdx.x;  // set.words: whole workgroups
+    const ExtGrid g = pick(set, gw);
+    const int64_t w = gw - g.word0;
+    // selects, not an index: a dynamic index into the by-value struct would
+    // put it in scratch memory
+    const uint64_t* src = (l & 1) ? g.plane[1] : g.plane[0];
+    uint64_t* dst = (l & 1) ? g.plane[0] : g.plane[1];
+    // padding words past the grid's plane compute on its last word and
+    // write nothing; every lane takes part in the shuffles below
+    const bool real = w < g.words;
+    const int64_t wr = real ? w : g.words - 1;
+    const uint32_t row = (uint32_t)wr / (uint32_t)g.wpr;  // planes hold fewer than 2^31 words
+    const int xw = (int)((uint32_t)wr - row * (uint32_t)g.wpr);
+    const int j = (int)(row % (uint32_t)g.nj), k = (int)(row / (uint32_t)g.nj);
+    const int64_t cell0 = (int64_t)row * g.ni + (int64_t)xw * 64;
+    const uint64_t old = src[wr];
+    // the nine rows' words, all loaded before any is used
+    uint64_t c[9];
+    bool in[9];
+    int64_t rw[9];
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int jj = j + r % 3 - 1, kk = k + r / 3 - 1;
+        in[r] = jj >= 0 && jj < g.nj && kk >= 0 && kk < g.nk;
+        rw[r] = in[r] ? ((int64_t)jj + (int64_t)g.nj * kk) * g.wpr + xw : wr;
+        c[r] = src[rw[r]];  // unconditional (a row outside the grid reads the centre one)
+        c[r] = in[r] ? c[r] : 0ull;
     }
+    // a row's neighbour words are the neighbour lanes' (lanes hold consecutive
+    // words); lanes 0 and 63 load theirs when the row goes on past the wave
+    uint64_t acc = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        uint64_t lft = __shfl_up(c[r], 1, 64), rgt = __shfl_down(c[r], 1, 64);
+        if (lane == 0) lft = (in[r] && xw > 0) ? src[rw[r] - 1] : 0ull;      // rare: rows cross waves
+        if (lane == 63) rgt = (in[r] && xw + 1 < g.wpr) ? src[rw[r] + 1] : 0ull;
+        if (xw == 0) lft = 0ull;
+        if (xw + 1 == g.wpr) rgt = 0ull;
+        acc |= c[r] | (c[r] << 1) | (c[r] >> 1) | (lft >> 63) | (rgt << 63);
+    }
+    const int rem = g.ni - xw * 64;
+    if (rem < 64) acc &= (1ull << rem) - 1ull;
+    const uint64_t add = real ? acc & ~old : 0ull;
+    if (real) dst[w] = old | add;
+    const uint32_t n = (uint32_t)__popcll(add);
+    uint32_t incl = n;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    // wave totals -> workgroup offsets -> one atomic
+    if (lane == 63) s_tot[wave] = incl;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        if (g.tfv[T.t] != kNever) g.tfv[T.t] = 0;
-        if (g.tnear[T.t] != kNever) g.tnear[T.t] = 0;
+        uint32_t run = 0;
+        for (int q = 0; q < kExtDilateThreads / 64; ++q) {
+            const uint32_t t = s_tot[q];
+            s_tot[q] = run;
+            run += t;
+        }
+        s_tot[kExtDilateThreads / 64] = run ? atomicAdd(&set.counter[2 * g.slot + (l & 1)], run) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = s_tot[kExtDilateThreads / 64] + s_tot[wave];
+    // the wave writes one lane's cells at a time, lane b of the wave taking
+    // bit b, so each store is a run of consecutive slots
+    const uint32_t first = base + incl - n;
+    uint64_t lanes = __ballot(add != 0ull);
+    while (lanes) {
+        const int src = __ffsll((unsigned long long)lanes) - 1;
+        lanes &= lanes - 1ull;
+        const uint64_t word = __shfl(add, src, 64);
+        const uint32_t at0 = __shfl(first, src, 64);
+        const int64_t c0 = __shfl(cell0, src, 64);
+        if ((word >> lane) & 1ull)
+            g.list[at0 + (uint32_t)__popcll(word & ((1ull << lane) - 1ull))] = (uint32_t)(c0 + lane);
+    }
+}
+
+// extrapolation.cpp:20-44 for the cells layer l added (blockIdx.y = grid):
+// the sum over the neighbours valid at the layer's start, in the reference's
+// order kk, jj, ii = -1..1, divided by their count. The neighbours read are
+// not written by this launch.
+__global__ __launch_bounds__(256) void k_ext_value(ExtSet set, int l) {
+    const int q = blockIdx.y;
+    if (q >= set.count) return;
+    const ExtGrid g = pick_slot(set, q);
+    const uint32_t n = set.counter[2 * q + (l & 1)];
+    if (blockIdx.x == 0 && threadIdx.x == 0) set.counter[2 * q + ((l & 1) ^ 1)] = 0;  // next layer's list
+    const uint64_t* pl = (l & 1) ? g.plane[1] : g.plane[0];
+    const size_t sj = (size_t)g.ni, sk = (size_t)g.ni * g.nj;
+    for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < n; e += gridDim.x * 256) {
+        const uint32_t c = g.list[e];
+        const int i = (int)(c % (uint32_t)g.ni), j = (int)((c / (uint32_t)g.ni) % (uint32_t)g.nj);
+        const int k = (int)(c / ((uint32_t)g.ni * (uint32_t)g.nj));
+        // row by row in the reference's order kk, jj (then ii); flags and values
+        // are loaded unconditionally (neighbours outside the grid read the
+        // cell itself and are masked off), so the loads do not wait on the
+        // flags. +0.0f for a neighbour that is not valid leaves the sum
+        // unchanged (it starts at +0.0f, so it is never -0.0f).
+        const uint32_t* pw = reinterpret_cast<const uint32_t*>(pl);
+        const uint32_t wpr2 = 2u * (uint32_t)g.wpr;
+        const bool has_m = i > 0, has_p = i + 1 < g.ni;
+        float sum = 0.0f;
+        int cnt = 0;
+#pragma unroll
+        for (int r = 0; r < 9; ++r) {
+            const int dj = r % 3 - 1, dk = r / 3 - 1;
+            const int jj = j + dj, kk = k + dk;
+            const bool row_in = jj >= 0 && jj < g.nj && kk >= 0 && kk < g.nk;
+            const uint32_t rc = row_in ? c + (uint32_t)((int)sj * dj + (int)sk * dk) : c;
+            const uint32_t rowi = row_in ? (uint32_t)jj + (uint32_t)g.nj * (uint32_t)kk
+                                         : (uint32_t)j + (uint32_t)g.nj * (uint32_t)k;
+            const uint32_t* rowp = pw + rowi * wpr2;
+            const uint32_t im = has_m ? i - 1 : i, ip = has_p ? i + 1 : i;
+            // unconditional loads, masked afterwards
+            const uint32_t wm = rowp[im >> 5], w0 = rowp[(uint32_t)i >> 5], wp = rowp[ip >> 5];
+            const uint32_t bm = (row_in && has_m) ? (wm >> (im & 31)) & 1u : 0u;
+            const uint32_t b0 = row_in ? (w0 >> (i & 31)) & 1u : 0u;
+            const uint32_t bp = (row_in && has_p) ? (wp >> (ip & 31)) & 1u : 0u;
+            const float vm = g.f[has_m ? rc - 1u : rc], v0 = g.f[rc], vp = g.f[has_p ? rc + 1u : rc];
+            sum += bm ? vm : 0.0f;
+            sum += b0 ? v0 : 0.0f;
+            sum += bp ? vp : 0.0f;
+            cnt += (int)(bm + b0 + bp);
+        }
+        g.f[c] = sum / (float)cnt;
+        if (g.valid_out) g.valid_out[c] = 1;
     }
 }
 
@@ -818,10 +802,21 @@ __global__ __launch_bounds__(256) void k_rei_dist_yz(ReiSet r) {
 }
 
 // Whether a neighbour (in the grid, valid) lies at distance want.
-__device__ __forceinline__ bool rei_at_dist(const ReiSet& r, int i, int j, int k, uint16_t want) {
-    if (i < 0 || j < 0 || k < 0 || i >= r.ni || j >= r.nj || k >= r.nk) return false;
-    const size_t c = at(i, j, k, r.ni, r.nj);
-    return r.stamp[c] != kReiUnknown && r.dist[c] == want;
+// Branch-free: a neighbour outside the grid reads the cell itself (c) and is
+// masked off, so the loads of several lookups go out together.
+__device__ __forceinline__ bool rei_at_dist(const ReiSet& r, uint32_t c, bool inside, int32_t delta,
+                                            uint16_t want) {
+    const uint32_t nb = inside ? (uint32_t)((int32_t)c + delta) : c;
+    const uint16_t st = r.stamp[nb], dv = r.dist[nb];
+    return inside & (st != kReiUnknown) & (dv == want);
+}
+
+// The six face neighbours of cell c = (i, j, k) at distance want.
+__device__ __forceinline__ uint32_t rei_fix6(const ReiSet& r, uint32_t c, int i, int j, int k, uint16_t want) {
+    const int32_t sj = r.ni, sk = r.ni * r.nj;
+    return (uint32_t)rei_at_dist(r, c, i > 0, -1, want) | (uint32_t)rei_at_dist(r, c, i + 1 < r.ni, 1, want) << 1 |
+           (uint32_t)rei_at_dist(r, c, j > 0, -sj, want) << 2 | (uint32_t)rei_at_dist(r, c, j + 1 < r.nj, sj, want) << 3 |
+           (uint32_t)rei_at_dist(r, c, k > 0, -sk, want) << 4 | (uint32_t)rei_at_dist(r, c, k + 1 < r.nk, sk, want) << 5;
 }
 
 // The cells not reached yet are bucketed by distance per chunk of
@@ -830,7 +825,7 @@ __device__ __forceinline__ bool rei_at_dist(const ReiSet& r, int i, int j, int k
 // most a grid row plus a few cells of the next ones), so a chunk counts its
 // cells per distance in LDS, with one global atomic per distinct distance.
 constexpr int kReiBucketThreads = 1024;
-constexpr int kReiBucketPer = 4;
+constexpr int kReiBucketPer = 8;
 constexpr int kReiBucketCells = kReiBucketThreads * kReiBucketPer;
 constexpr int kReiBins = 8192;
 
@@ -889,7 +884,7 @@ __global__ __launch_bounds__(kReiBucketThreads) void k_rei_dist_check(ReiSet r) 
     __shared__ uint32_t bins[kReiBins];
     __shared__ uint32_t red[64];
     ReiChunk ch;
-    uint32_t bad = 0, top = 0;
+    uint32_t bad = 0;
     // the check: a FAR cell needs a finite distance >= 1 and a valid neighbour one step closer
 #pragma unroll
     for (int q = 0; q < kReiBucketPer; ++q) {
@@ -899,21 +894,15 @@ __global__ __launch_bounds__(kReiBucketThreads) void k_rei_dist_check(ReiSet r) 
         const uint16_t dv = r.dist[c];
         bool ok = dv != 0 && dv != kDistInf;
         if (ok) {
-            const int i = (int)(c % r.ni), j = (int)((c / r.ni) % r.nj), k = (int)(c / ((size_t)r.ni * r.nj));
-            const uint16_t w = dv - 1;
-            ok = rei_at_dist(r, i - 1, j, k, w) || rei_at_dist(r, i + 1, j, k, w) ||
-                 rei_at_dist(r, i, j - 1, k, w) || rei_at_dist(r, i, j + 1, k, w) ||
-                 rei_at_dist(r, i, j, k - 1, w) || rei_at_dist(r, i, j, k + 1, w);
+            const uint32_t cu = (uint32_t)c, row = cu / (uint32_t)r.ni;
+            const int i = (int)(cu - row * (uint32_t)r.ni), j = (int)(row % (uint32_t)r.nj),
+                      k = (int)(row / (uint32_t)r.nj);
+            ok = rei_fix6(r, cu, i, j, k, (uint16_t)(dv - 1)) != 0;
         }
         if (!ok) ++bad;
-        else top = max(top, (uint32_t)dv);
     }
     bad = __reduce_add_sync(~0ull, bad);
-    top = __reduce_max_sync(~0ull, top);
-    if ((threadIdx.x & 63) == 0) {
-        if (bad) atomicAdd(&r.counter[6], bad);
-        if (top) atomicMax(&r.counter[7], top);
-    }
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(&r.counter[6], bad);
     uint32_t dmin = 0;
     const int span = rei_chunk_bins(r, ch, bins, red, dmin);
     if (span < 0) {
@@ -925,7 +914,7 @@ __global__ __launch_bounds__(kReiBucketThreads) void k_rei_dist_check(ReiSet r) 
 }
 
 // Exclusive scan of dcount[0..kReiDistMax + 1] in place (one workgroup);
-// dfill gets the same offsets.
+// dfill gets the same offsets, counter[7] the largest distance counted.
 __global__ __launch_bounds__(1024) void k_rei_dscan(ReiSet r) {
     constexpr int N = kReiDistMax + 2, PER = (N + 1023) / 1024;
     __shared__ uint32_t part[1024];
@@ -946,6 +935,7 @@ __global__ __launch_bounds__(1024) void k_rei_dscan(ReiSet r) {
         __syncthreads();
     }
     uint32_t run = part[t] - sum;
+    int top = 0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int idx = t * PER + q;
@@ -953,8 +943,18 @@ __global__ __launch_bounds__(1024) void k_rei_dscan(ReiSet r) {
             r.dcount[idx] = run;
             r.dfill[idx] = run;
         }
+        if (v[q]) top = idx;
         run += v[q];
     }
+    // counter[7]: the largest distance with a cell (the pass count)
+    __syncthreads();
+    part[t] = (uint32_t)top;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (t < o) part[t] = max(part[t], part[t + o]);
+        __syncthreads();
+    }
+    if (t == 0) r.counter[7] = part[0];
 }
 
 // Lists the cells not reached yet by distance (after a passed check, every
@@ -980,12 +980,11 @@ __global__ __launch_bounds__(kReiBucketThreads) void k_rei_scatter(ReiSet r) {
 __global__ __launch_bounds__(256) void k_rei_layer(ReiSet r, uint32_t begin, uint32_t end, uint32_t target) {
     const uint32_t e = begin + blockIdx.x * 256 + threadIdx.x;
     if (e >= end) return;
-    const size_t c = r.cells[e];
-    const uint16_t w = (uint16_t)(target - 1);
-    const int i = (int)(c % r.ni), j = (int)((c / r.ni) % r.nj), k = (int)(c / ((size_t)r.ni * r.nj));
-    const bool fxm = rei_at_dist(r, i - 1, j, k, w), fxp = rei_at_dist(r, i + 1, j, k, w);
-    const bool fym = rei_at_dist(r, i, j - 1, k, w), fyp = rei_at_dist(r, i, j + 1, k, w);
-    const bool fzm = rei_at_dist(r, i, j, k - 1, w), fzp = rei_at_dist(r, i, j, k + 1, w);
+    const uint32_t cu = r.cells[e], row = cu / (uint32_t)r.ni;
+    const size_t c = cu;
+    const int i = (int)(cu - row * (uint32_t)r.ni), j = (int)(row % (uint32_t)r.nj), k = (int)(row / (uint32_t)r.nj);
+    const uint32_t fx = rei_fix6(r, cu, i, j, k, (uint16_t)(target - 1));
+    const bool fxm = fx & 1u, fxp = fx & 2u, fym = fx & 4u, fyp = fx & 8u, fzm = fx & 16u, fzp = fx & 32u;
     const size_t sx = 1, sy = (size_t)r.ni, sz = (size_t)r.ni * r.nj;
     const float vx = fxm ? r.phi[c - sx] : (fxp ? r.phi[c + sx] : 0.0f);
     const float vy = fym ? r.phi[c - sy] : (fyp ? r.phi[c + sy] : 0.0f);
@@ -1025,20 +1024,13 @@ __global__ __launch_bounds__(256) void k_rei_finish(ReiSet r) {
 }  // namespace
 
 hipError_t launch_ext_init(const ExtSet& set, hipStream_t s) {
-    hipLaunchKernelGGL(k_ext_init, dim3((unsigned)set.tiles), dim3(256), 0, s, set);
-    hipLaunchKernelGGL(k_ext_near, dim3(cdiv(set.tiles, 256)), dim3(256), 0, s, set);
+    hipLaunchKernelGGL(k_ext_init, dim3(cdiv(set.words, 4 * 8)), dim3(256), 0, s, set);
     return hipGetLastError();
 }
 
-hipError_t launch_ext_layer(const ExtSet& set, int l, int parity, hipStream_t s) {
-    hipLaunchKernelGGL(k_ext_select, dim3(cdiv(set.tiles, 256)), dim3(256), 0, s, set, l, parity);
-    const unsigned groups = (unsigned)std::min<int64_t>(set.tiles, kExtLayerGroups);
-    hipLaunchKernelGGL(k_ext_layer, dim3(groups), dim3(256), 0, s, set, l, parity);
-    return hipGetLastError();
-}
-
-hipError_t launch_ext_rebase(const ExtSet& set, hipStream_t s) {
-    hipLaunchKernelGGL(k_ext_rebase, dim3((unsigned)set.tiles), dim3(256), 0, s, set);
+hipError_t launch_ext_layer(const ExtSet& set, int l, hipStream_t s) {
+    hipLaunchKernelGGL(k_ext_dilate, dim3(cdiv(set.words, kExtDilateThreads)), dim3(kExtDilateThreads), 0, s, set, l);
+    hipLaunchKernelGGL(k_ext_value, dim3(1024, (unsigned)set.count), dim3(256), 0, s, set, l);
     return hipGetLastError();
 }
 
@@ -1107,11 +1099,11 @@ hipError_t launch_rei_dist(const ReiSet& r, hipStream_t s) {
     hipLaunchKernelGGL(k_rei_dist_yz<2>, dim3(cdiv(r.ni, 64), cdiv(r.nj, 4)), dim3(256), 0, s, r);
     const int64_t n = (int64_t)r.ni * r.nj * r.nk;
     hipLaunchKernelGGL(k_rei_dist_check, dim3(cdiv(n, kReiBucketCells)), dim3(kReiBucketThreads), 0, s, r);
+    hipLaunchKernelGGL(k_rei_dscan, dim3(1), dim3(1024), 0, s, r);
     return hipGetLastError();
 }
 
 hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s) {
-    hipLaunchKernelGGL(k_rei_dscan, dim3(1), dim3(1024), 0, s, r);
     hipLaunchKernelGGL(k_rei_scatter, dim3(cdiv((int64_t)r.ni * r.nj * r.nk, kReiBucketCells)),
                        dim3(kReiBucketThreads), 0, s, r);
     return hipGetLastError();

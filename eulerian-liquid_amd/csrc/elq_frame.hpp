@@ -13,51 +13,55 @@
 
 namespace elq {
 
-// Extrapolation tiles: 32 x 8 x 8 cells per workgroup of 256 threads.
+// Tiles of the reinitialisation's tile passes: 32 x 8 x 8 cells per
+// workgroup of 256 threads.
 constexpr int kExtTX = 32;
 constexpr int kExtTY = 8;
 constexpr int kExtTZ = 8;
-// Layer stamps are bytes; every kExtRebase layers they are folded back to 0.
-constexpr int kExtRebase = 250;
 
-// Workgroups of the persistent layer kernel (tiles are taken from the
-// layer's active list in strides of this many).
+// Workgroups of the persistent tile-pass kernel (tiles are taken from the
+// pass's active list in strides of this many).
 constexpr int kExtLayerGroups = 1280;
 
-// One grid of an extrapolation (device pointers).
+// Extrapolation (extrapolation.cpp:5-51) over bit planes: bit x % 64 of
+// word (row, x / 64), row = j + nj k, is set where cell (x, j, k) is valid at
+// the start of a layer. A layer dilates the plane by the 3x3x3 box and lists
+// the cells it adds; a second launch gives each listed cell the mean of its
+// neighbours that were valid at the layer's start.
 struct ExtGrid {
     float* f;                 // values, updated in place
     const uint8_t* valid_in;  // caller's 0/1 flags (read by the init pass)
     uint8_t* valid_out;       // set to 1 where a cell receives a value (nullptr: left alone)
-    uint8_t* stamp;           // per cell: 0 valid from the start, s: valid from layer s on, 255 invalid
-    uint8_t* tinv;            // per tile: 1 while the tile holds an invalid cell
-    uint8_t* tfv;             // per tile: first layer at whose start the tile holds a valid cell (255: none)
-    uint8_t* tnear;           // per tile: first layer at whose start a tile of its 3x3x3 block holds one
+    uint64_t* plane[2];       // validity at the start of even / odd layers
+    uint32_t* list;           // cells added by the current layer
     int ni, nj, nk;
-    int tx, ty, tz;           // tiles per axis
-    int tile0;                // first global tile index of this grid
+    int wpr;                  // words per row
+    int slot;                 // index of this grid in its set
+    int64_t word0;            // first word in the set's word space (a multiple of kExtWordAlign)
+    int64_t words;            // rows * wpr
 };
 
 // Up to three grids (u, v, w) extrapolated by the same launches.
 struct ExtSet {
     ExtGrid g[3];
     int count;
-    int tiles;         // tiles of all grids
-    int32_t* list;     // active tiles of the current layer (global tile indices)
-    uint32_t* counter; // two list lengths, used by alternate layers
+    int64_t words;      // word space of all grids, each padded to a multiple of kExtWordAlign
+    uint32_t* counter;  // [2 q + (layer & 1)]: length of grid q's list
 };
 
-inline int64_t ext_tiles(int ni, int nj, int nk) {
-    return (int64_t)((ni + kExtTX - 1) / kExtTX) * ((nj + kExtTY - 1) / kExtTY) * ((nk + kExtTZ - 1) / kExtTZ);
-}
+// Grids start at multiples of this many words in a set's word space: a
+// workgroup of the dilation (one thread per word) stays inside one grid.
+constexpr int kExtWordAlign = 1024;
 
-// Stamps, tile flags and the first layer's neighbourhood flags from valid_in.
+// Words per row and words of a grid's plane.
+inline int ext_wpr(int ni) { return (ni + 63) / 64; }
+inline int64_t ext_words(int ni, int nj, int nk) { return (int64_t)ext_wpr(ni) * nj * nk; }
+
+// Plane of layer 0 from valid_in; list lengths cleared.
 hipError_t launch_ext_init(const ExtSet& set, hipStream_t s);
-// Layer l (0-based since the last rebase, l < kExtRebase; parity = absolute
-// layer index & 1): the active-tile list, then the layer over it.
-hipError_t launch_ext_layer(const ExtSet& set, int l, int parity, hipStream_t s);
-// Fold every stamp <= kExtRebase back to 0 (valid from the start).
-hipError_t launch_ext_rebase(const ExtSet& set, hipStream_t s);
+// Layer l: dilation (plane l & 1 -> plane (l & 1) ^ 1, list of added cells),
+// then the added cells' values.
+hipError_t launch_ext_layer(const ExtSet& set, int l, hipStream_t s);
 
 // find_valid_velocity (simulator.cpp:125-167): 0/1 flags of the u, v, w faces.
 hipError_t launch_find_valid(const Dense& d, uint8_t* uv, uint8_t* vv, uint8_t* wv, hipStream_t s);
@@ -113,12 +117,12 @@ struct ReiSet {
 hipError_t launch_rei_init(const ReiSet& r, hipStream_t s);
 // dist: taxicab distance transform of the interface cells (three separable
 // passes), then a check that it equals the march's pass count at every
-// valid cell (counter[6] = failures, counter[7] = largest distance) and a
-// count of the cells at each distance (dcount).
+// valid cell (counter[6] = failures), the cells counted per distance and
+// the counts turned into list offsets in dcount and dfill (exclusive scan
+// over 0..kReiDistMax + 1; counter[7] = largest distance).
 constexpr int kReiDistMax = 3 * 4092;
 hipError_t launch_rei_dist(const ReiSet& r, hipStream_t s);
-// After the check passed: dcount becomes list offsets (exclusive scan over
-// 0..kReiDistMax + 1) and the cells not reached yet are listed by distance.
+// After the check passed: the cells not reached yet listed by distance.
 hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s);
 // The cells of one pass (list entries [begin, end), all at distance target)
 // from their FIX neighbours (valid, distance target - 1).

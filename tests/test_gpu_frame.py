@@ -4,8 +4,8 @@ the oracle, bit for bit (values as uint32 words, flags as bytes):
 - elq_extrapolate_device / elq_extrapolate_const_device: extrapolate and
   extrapolate_const (extrapolation.cpp:5-51) on grids that are not tile
   multiples, sparse and dense valid sets, 0, 1, 3 and 10 layers, a single
-  source, all-valid and no-valid grids, and a long grid whose 260 layers
-  cross the stamp rebase;
+  source, all-valid and no-valid grids, and a long grid (260 layers, rows
+  of five 64-cell words of the validity bit planes);
 - elq_find_valid_velocity_device, elq_set_invalid_velocity_zero_device
   (simulator.cpp:125-178) and elq_extrapolate_velocity_device
   (simulator.cpp:119-123) on scenes, in the reference's order;
@@ -95,7 +95,7 @@ def test_extrapolate_single_source_and_trivial(ctx):
         assert same(g, g0) and same(v, v0)
 
 
-def test_extrapolate_long_run_crosses_stamp_rebase(ctx):
+def test_extrapolate_long_run_across_words(ctx):
     dims = (260, 3, 2)
     n = int(np.prod(dims))
     rng = np.random.default_rng(10)

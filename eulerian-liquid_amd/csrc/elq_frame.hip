@@ -418,10 +418,12 @@ __global__ __launch_bounds__(256) void k_advect_cells(Advect A) {
 // ---------------------------------------------------------- reinitialisation
 // The reference marches from the interface one face-neighbour layer per pass:
 // FAR cells next to a FIX cell become NEAR, take an upwind distance from the
-// FIX neighbours' values (a snapshot, grid_temp), and become FIX. As in the
-// extrapolation, a 16-bit stamp per cell stands for the label and the
-// snapshot: FIX at the start of pass l is stamp <= l, and a cell fixed in pass
-// l gets l + 1, so each pass runs in place over the tiles it can change.
+// FIX neighbours' values (a snapshot, grid_temp), and become FIX. A 16-bit
+// stamp per cell stands for the label and the snapshot: FIX at the start of
+// pass l is stamp <= l, and a cell fixed in pass l gets l + 1, so the tile
+// passes run in place over the tiles they can change. Where the pass of every
+// cell is its taxicab distance to the interface (checked cell by cell), the
+// passes instead run over per-distance cell lists (the distance route below).
 
 struct P3 {
     float x, y, z;

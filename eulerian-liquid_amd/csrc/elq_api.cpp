@@ -761,7 +761,7 @@ void elq_default_opts(elq_opts* o) {
     o->precond = ELQ_PRECOND_IC0;
     o->tau = 0.97;
     o->sigma = 0.25;
-    o->inner_tol = 0.1;
+    o->inner_tol = 0.01;
     o->schedule = ELQ_SCHED_WAVEFRONT;
     o->check_every = 8;
 }

@@ -93,6 +93,21 @@ def test_sphere_with_noise_on_room_and_pocket(ctx):
     check(ctx, phi, valid, dims, dx, route="rei_pass")
 
 
+@pytest.mark.parametrize("flags", ["all_valid", "none_valid"])
+def test_no_interface(ctx, flags):
+    """No sign change anywhere: every valid cell is unreached (the reference
+    would loop forever), no pass runs; with no valid cell nothing happens."""
+    dims, dx = (20, 18, 22), 1.0 / 20
+    n = int(np.prod(dims))
+    phi = np.full(n, 0.3, np.float32)
+    valid = np.ones(n, np.uint8) if flags == "all_valid" else np.zeros(n, np.uint8)
+    got = phi.copy()
+    left, passes = ctx.reinitialize(got, valid, dims, dx)
+    ref = phi.copy()
+    assert (left, passes) == O.reinitialize(ref, valid, dims, dx) == ((n if flags == "all_valid" else 0), 0)
+    assert same(got, ref)
+
+
 def test_wall_with_hole_detour_bit_exact(ctx):
     dims, dx = (36, 32, 40), 1.0 / 36
     ni, nj, nk = dims

@@ -860,8 +860,9 @@ __device__ __forceinline__ size_t halo_offset(const Nbr6& nb, int d, size_t own,
 // through lane shuffles. All loads are issued unconditionally and at once.
 // Accumulation order: diag, -x, +x, -y, +y, -z, +z.
 template <class T>
-__device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in,
-                                             T* p_out, T* q, T beta, const Lane& L) {
+__device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef<T>& c, const T* __restrict__ z,
+                                             const T* __restrict__ p_in, T* __restrict__ p_out, T* __restrict__ q,
+                                             T beta, const Lane& L) {
     const Nbr6 nb = load_nbr6(bk, b);
     const size_t o = (size_t)b * kBrickCells + L.lane;
     const int lc[3] = {L.lx, L.ly, L.lz};
@@ -906,8 +907,10 @@ __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef
     return (double)pc * (double)acc;
 }
 
-// Persistent: wave w of W takes bricks w, w+W, ...; its p.q terms accumulate
-// per lane in that order; the grid tail computes alpha.
+// Persistent: wave w of W takes bricks w, w+W, ..., two per loop step so
+// that the second brick's loads overlap the first's arithmetic (the
+// restrict-qualified arrays let the compiler move them); its p.q terms
+// accumulate per lane in brick order; the grid tail computes alpha.
 template <class T>
 __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
                                               Ctrl* ctrl, uint64_t* stage, uint64_t timeout) {
@@ -917,8 +920,10 @@ __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, 
     const Lane L;
     const int W = (int)gridDim.x * 4;
     double acc = 0.0;
-    for (int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6)); b < bk.nb; b += W)
+    for (int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6)); b < bk.nb; b += 2 * W) {
         acc += spmv_brick<T>(b, bk, c, z, p_in, p_out, q, beta, L);
+        if (b + W < bk.nb) acc += spmv_brick<T>(b + W, bk, c, z, p_in, p_out, q, beta, L);
+    }
     grid_tail(ctrl, SC_ALPHA, wave_sum(acc), tag, stage, timeout);
 }
 

@@ -249,11 +249,15 @@ int setup_phase_a(elq_ctx* ctx, const Dense& d, int32_t* indices_out) {
                               &totals[3], s),
            "k_block_list");
     ctx->prof_end();
+    // totals and the error word are adjacent in info: one export, one sync
     int64_t host_tot[3 + kShapeCount];
     int32_t host_err = 0;
-    HIPCHK(hipMemcpyAsync(host_tot, totals, sizeof(host_tot), hipMemcpyDeviceToHost, s), "d2h");
-    HIPCHK(hipMemcpyAsync(&host_err, err, sizeof(host_err), hipMemcpyDeviceToHost, s), "d2h");
+    const size_t info_bytes = sizeof(int64_t) * (3 + kShapeCount) + sizeof(int32_t) * 2;
+    static_assert(sizeof(int64_t) * (3 + kShapeCount) + sizeof(int32_t) * 2 <= kHostMirrorBytes, "info mirror");
+    HIPCHK(launch_export_words(totals, info_bytes, ctx->ctrl_host, s), "k_export_words");
     HIPCHK(hipStreamSynchronize(s), "sync");
+    std::memcpy(host_tot, ctx->ctrl_host, sizeof(host_tot));
+    std::memcpy(&host_err, (const char*)ctx->ctrl_host + sizeof(host_tot), sizeof(host_err));
     W.n_unknowns = host_tot[0];
     W.nb = (int)host_tot[1];
     W.np = (int)host_tot[2];
@@ -368,9 +372,14 @@ int upload_ctrl(elq_ctx* ctx, const Ctrl& c) {
     return ELQ_OK;
 }
 
+// The control block reaches the host through a kernel that stores it into
+// pinned host memory: a poll then costs one small launch and the stream
+// synchronisation, without a copy-engine transfer.
 int read_ctrl(elq_ctx* ctx, Ctrl& c) {
-    HIPCHK(hipMemcpyAsync(&c, ctx->ws.ctrl.p, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    static_assert(sizeof(Ctrl) % 4 == 0 && sizeof(Ctrl) <= kHostMirrorBytes, "control block mirror");
+    HIPCHK(launch_export_words(ctx->ws.ctrl.p, sizeof(Ctrl), ctx->ctrl_host, ctx->stream), "k_export_words");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
+    std::memcpy(&c, ctx->ctrl_host, sizeof(Ctrl));
     return ELQ_OK;
 }
 
@@ -782,6 +791,12 @@ int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
         delete ctx;
         return ELQ_ERR_DEVICE;
     }
+    // pinned host copy of the control block, written by a kernel at each poll
+    if (hipHostMalloc(&ctx->ctrl_host, kHostMirrorBytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return ELQ_ERR_DEVICE;
+    }
     int rate_khz = 0;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
         rate_khz = 100000;
@@ -797,6 +812,7 @@ void elq_ctx_destroy(elq_ctx* ctx) {
     ctx->ws.release_all();
     for (auto ev : ctx->events) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(ctx->stream);
+    if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
     delete ctx;
 }
 

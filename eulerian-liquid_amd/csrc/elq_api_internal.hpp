@@ -11,6 +11,9 @@
 
 #include "elq_kernels.hpp"
 
+// Bytes of the pinned host mirror a context keeps for its polls.
+constexpr size_t kHostMirrorBytes = 4096;
+
 // Device buffer that only grows (hipMalloc on growth, reused otherwise).
 struct DevBuf {
     void* p = nullptr;
@@ -78,6 +81,7 @@ struct elq_ctx {
     int device = 0;
     size_t budget = 0;
     hipStream_t stream = nullptr;
+    void* ctrl_host = nullptr;  // pinned host mirror (kHostMirrorBytes): control block, setup totals
     uint64_t timeout_ticks = 0;
     int max_groups = 0;  // cap on the grid of persistent kernels (0: whole device)
     std::string last_error;

@@ -726,6 +726,12 @@ __device__ void scalar_update(Ctrl* c, int mode, double s, double s2) {
     }
 }
 
+// 32-bit words into host-visible memory with system-scope stores (read by
+// the host after a stream synchronisation).
+__global__ __launch_bounds__(64) void k_export_words(const uint32_t* src, uint32_t* out, int words) {
+    for (int q = threadIdx.x; q < words; q += 64) __hip_atomic_store(out + q, src[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, const double* parts, int n,
                                                            const double* parts2, int nblk, uint64_t* stage,
                                                            uint64_t timeout) {
@@ -2027,6 +2033,12 @@ hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T
 int scalar_blocks(int n) {
     const int g = (n + kRedPerBlock - 1) / kRedPerBlock;
     return g < 1 ? 1 : (g > kRedBlocks ? kRedBlocks : g);
+}
+
+hipError_t launch_export_words(const void* src, size_t bytes, void* host, hipStream_t s) {
+    if (bytes % 4 != 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_export_words, dim3(1), dim3(64), 0, s, (const uint32_t*)src, (uint32_t*)host, (int)(bytes / 4));
+    return hipGetLastError();
 }
 
 hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2, uint64_t* stage,

@@ -115,6 +115,10 @@ template <class T>
 hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T* p1, double* part,
                        Ctrl* ctrl, hipStream_t s);
 // stage: 4*kRedBlocks tagged words (zero-initialised once, never reset).
+// Copies bytes (a multiple of 4) of device memory into pinned host memory
+// with a small kernel (system-scope stores), for polls that would otherwise
+// wait on a copy-engine transfer.
+hipError_t launch_export_words(const void* src, size_t bytes, void* host, hipStream_t s);
 hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const double* parts2, uint64_t* stage,
                          uint64_t timeout_ticks, hipStream_t s);
 template <class T>

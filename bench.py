@@ -437,7 +437,7 @@ def main():
           flush=True)
     precond = elq.IC0 if a.precond == "ic0" else elq.MIC0
     schedule = elq.WAVEFRONT if a.schedule == "wavefront" else elq.LEVELS
-    base = dict(tol=1e-8, max_iter=20000, precond=precond, schedule=schedule, check_every=16)
+    base = dict(tol=1e-8, max_iter=20000, precond=precond, schedule=schedule, check_every=32)
     o_mixed = elq.default_opts(precision=elq.MIXED, **base)
     o_fp64 = elq.default_opts(precision=elq.FP64, **base)
 

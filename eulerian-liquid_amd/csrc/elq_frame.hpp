@@ -1,6 +1,7 @@
 // Host-side launch wrappers for the gfx950 kernels of the frame steps around
 // the projection: validity flags of the faces and of the level set, zeroing
-// of invalid faces, and the layered extrapolation of extrapolation.cpp.
+// of invalid faces, the layered extrapolation of extrapolation.cpp, the
+// semi-Lagrangian advection and the level-set reinitialisation.
 // Every wrapper enqueues on the given stream and returns the launch status.
 #pragma once
 
@@ -83,10 +84,10 @@ hipError_t launch_advect_faces(int ni, int nj, int nk, float dx, float dt, const
 hipError_t launch_advect_cells(int ni, int nj, int nk, float dx, float dt, const float* vel_u, const float* vel_v,
                                const float* vel_w, const float* val, float* out, hipStream_t s);
 
-// Reinitialisation (reinitialization.cpp:19-156): same 32x8x8 tiles; cell
-// stamps are 16-bit (kReiFar: not reached yet, kReiUnknown: invalid cell,
-// s: fixed from the start of pass s on). Every kReiRebase passes the stamps
-// are folded back to 0.
+// Reinitialisation (reinitialization.cpp:19-156). Cell stamps are 16-bit
+// (kReiFar: not reached yet, kReiUnknown: invalid cell, s: fixed from the
+// start of pass s on). The tile passes (32x8x8 tiles) fold the stamps back to
+// 0 every kReiRebase passes; the distance route has no stamps to fold.
 constexpr uint16_t kReiFar = 0xFFFF;
 constexpr uint16_t kReiUnknown = 0xFFFE;
 constexpr int kReiRebase = 60000;
@@ -127,7 +128,7 @@ hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s);
 // The cells of one pass (list entries [begin, end), all at distance target)
 // from their FIX neighbours (valid, distance target - 1).
 hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, int target, hipStream_t s);
-// Pass l (0-based since the last rebase; parity = absolute pass & 1).
+// Tile route. Pass l (0-based since the last rebase; parity = absolute pass & 1).
 hipError_t launch_rei_pass(const ReiSet& r, int l, int parity, hipStream_t s);
 hipError_t launch_rei_rebase(const ReiSet& r, hipStream_t s);
 // counter[4] = cells never reached, counter[5] = largest stamp (passes that fixed cells).

@@ -424,7 +424,7 @@ def main():
                     help="only run the library's plain copy kernel on MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the projection runs on one GPU here; --gpus must be 1"}))
         raise SystemExit(2)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration << 20, max(a.steps, 1))

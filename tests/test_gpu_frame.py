@@ -109,8 +109,10 @@ def test_extrapolate_long_run_across_words(ctx):
 
 
 @pytest.mark.parametrize("seed", [1, 2])
-def test_velocity_frame_steps_bit_exact(ctx, seed):
-    s = elq.scene_spheres(40, seed=seed, n_spheres=6)
+@pytest.mark.parametrize("n", [40, 129])
+def test_velocity_frame_steps_bit_exact(ctx, seed, n):
+    # n = 129: rows of 129 to 130 faces, three 64-cell words with a short last one
+    s = elq.scene_spheres(n, seed=seed, n_spheres=6)
     ni, nj, nk = s.ni, s.nj, s.nk
     rng = np.random.default_rng(seed)
     for a in (s.u, s.v, s.w):

@@ -333,43 +333,93 @@ __device__ __forceinline__ int index_and_fraction(float pos, float origin, float
     return i;
 }
 
+// Index and fraction of one coordinate on one axis of a grid.
+struct Coord {
+    int i;
+    float f;
+};
+
+__device__ __forceinline__ Coord coord_at(float pos, float origin, float dx) {
+    Coord c;
+    c.i = index_and_fraction(pos, origin, dx, c.f);
+    return c;
+}
+
 __device__ __forceinline__ float lerp2(float v0, float v1, float f) { return (1.0f - f) * v0 + f * v1; }
 
 // interpolation.h:35-72 (dim 3): indices clamped to the grid (grid.h:370-379),
 // x interpolated first, then y, then z.
-__device__ __forceinline__ float trilerp_at(const Sampled& g, float x, float y, float z, float dx) {
-    float fx, fy, fz;
-    int i = index_and_fraction(x, g.o0, dx, fx);
-    int j = index_and_fraction(y, g.o1, dx, fy);
-    int k = index_and_fraction(z, g.o2, dx, fz);
+// Indices are 32-bit: every grid has fewer than 2^31 values. The upper
+// neighbour on an axis is the lower one or the next, so its offset is a
+// select, not a product.
+__device__ __forceinline__ float trilerp(const Sampled& g, Coord cx, Coord cy, Coord cz) {
+    const int i = cx.i, j = cy.i, k = cz.i;
+    const float fx = cx.f, fy = cy.f, fz = cz.f;
     const int i0 = min(max(i, 0), g.ni - 1), i1 = min(max(i + 1, 0), g.ni - 1);
     const int j0 = min(max(j, 0), g.nj - 1), j1 = min(max(j + 1, 0), g.nj - 1);
     const int k0 = min(max(k, 0), g.nk - 1), k1 = min(max(k + 1, 0), g.nk - 1);
-    const size_t r00 = (size_t)g.ni * ((size_t)j0 + (size_t)g.nj * k0);
-    const size_t r10 = (size_t)g.ni * ((size_t)j1 + (size_t)g.nj * k0);
-    const size_t r01 = (size_t)g.ni * ((size_t)j0 + (size_t)g.nj * k1);
-    const size_t r11 = (size_t)g.ni * ((size_t)j1 + (size_t)g.nj * k1);
-    const float v0 = g.a[r00 + i0], v1 = g.a[r00 + i1], v2 = g.a[r10 + i0], v3 = g.a[r10 + i1];
-    const float v4 = g.a[r01 + i0], v5 = g.a[r01 + i1], v6 = g.a[r11 + i0], v7 = g.a[r11 + i1];
+    const uint32_t row = (uint32_t)g.ni, plane = (uint32_t)g.ni * (uint32_t)g.nj;
+    const uint32_t r00 = (uint32_t)k0 * plane + (uint32_t)j0 * row;
+    const uint32_t dj = j1 > j0 ? row : 0u, dk = k1 > k0 ? plane : 0u;
+    const uint32_t r10 = r00 + dj, r01 = r00 + dk, r11 = r01 + dj;
+    const uint32_t a0 = (uint32_t)i0, a1 = (uint32_t)i1;
+    const float v0 = g.a[r00 + a0], v1 = g.a[r00 + a1], v2 = g.a[r10 + a0], v3 = g.a[r10 + a1];
+    const float v4 = g.a[r01 + a0], v5 = g.a[r01 + a1], v6 = g.a[r11 + a0], v7 = g.a[r11 + a1];
     const float b0 = lerp2(lerp2(v0, v1, fx), lerp2(v2, v3, fx), fy);
     const float b1 = lerp2(lerp2(v4, v5, fx), lerp2(v6, v7, fx), fy);
     return lerp2(b0, b1, fz);
 }
 
+__device__ __forceinline__ float trilerp_at(const Sampled& g, float x, float y, float z, float dx) {
+    return trilerp(g, coord_at(x, g.o0, dx), coord_at(y, g.o1, dx), coord_at(z, g.o2, dx));
+}
+
+// The velocity's face grids (face_grid, origin 0 shifted by -dx/2 on the
+// grid's own axis).
 struct VelField {
     Sampled u, v, w;
 };
 
-// ode.h:30-35 (trace_rk2) with the step -dt (advection.cpp:13,23); the field
-// is lerp_staggered (interpolation.h:186-193).
-__device__ __forceinline__ void backtrace(const VelField& F, float dx, float dt, float& x, float& y, float& z) {
+// A point's coordinates on the velocity grids. On each axis one grid is
+// shifted (origin 0 - dx/2, s[]) and the two others are not (origin 0, c[]),
+// so those two share the division of interpolation.h:13.
+struct StagCoords {
+    Coord s[3], c[3];
+};
+
+__device__ __forceinline__ float shifted_origin(float dx) { return 0.0f - 0.5f * dx; }
+
+__device__ __forceinline__ StagCoords stag_coords(float x, float y, float z, float dx) {
+    const float os = shifted_origin(dx);
+    StagCoords p;
+    p.s[0] = coord_at(x, os, dx);
+    p.s[1] = coord_at(y, os, dx);
+    p.s[2] = coord_at(z, os, dx);
+    p.c[0] = coord_at(x, 0.0f, dx);
+    p.c[1] = coord_at(y, 0.0f, dx);
+    p.c[2] = coord_at(z, 0.0f, dx);
+    return p;
+}
+
+// lerp_staggered (interpolation.h:186-193) at the point with coordinates p.
+__device__ __forceinline__ void lerp_staggered(const VelField& F, const StagCoords& p, float& vx, float& vy,
+                                               float& vz) {
+    vx = trilerp(F.u, p.s[0], p.c[1], p.c[2]);
+    vy = trilerp(F.v, p.c[0], p.s[1], p.c[2]);
+    vz = trilerp(F.w, p.c[0], p.c[1], p.s[2]);
+}
+
+// ode.h:30-35 (trace_rk2) with the step -dt (advection.cpp:13,23) from the
+// point (x, y, z), whose coordinates on the velocity grids are p0.
+__device__ __forceinline__ void backtrace(const VelField& F, const StagCoords& p0, float dx, float dt, float& x,
+                                          float& y, float& z) {
     const float h = -dt;
-    const float vx = trilerp_at(F.u, x, y, z, dx), vy = trilerp_at(F.v, x, y, z, dx),
-                vz = trilerp_at(F.w, x, y, z, dx);
+    float vx, vy, vz;
+    lerp_staggered(F, p0, vx, vy, vz);
     const float half = 0.5f * h;
     const float mx = x + half * vx, my = y + half * vy, mz = z + half * vz;
-    const float wx = trilerp_at(F.u, mx, my, mz, dx), wy = trilerp_at(F.v, mx, my, mz, dx),
-                wz = trilerp_at(F.w, mx, my, mz, dx);
+    float wx, wy, wz;
+    lerp_staggered(F, stag_coords(mx, my, mz, dx), wx, wy, wz);
     x = x + h * wx;
     y = y + h * wy;
     z = z + h * wz;
@@ -385,19 +435,37 @@ struct Advect {
 
 // advection.cpp:19-27: one thread per (i, j, k) takes the u, v and w faces
 // there (those that exist), so the three backtraces gather from the same
-// neighbourhood of the velocity grids.
+// neighbourhood of the velocity grids. Face u starts at (cx + os, cy, cz),
+// v at (cx, cy + os, cz) and w at (cx, cy, cz + os), with c* the cell
+// centre's coordinates and os the shifted origin: the start points share
+// their coordinates two by two, and so do their divisions.
 __global__ __launch_bounds__(256) void k_advect_faces(Advect A) {
     const int i = blockIdx.x * 32 + threadIdx.x, j = blockIdx.y * 4 + threadIdx.y,
               k = blockIdx.z * 2 + threadIdx.z;
-    const float dx = A.dx;
+    const float dx = A.dx, os = shifted_origin(dx);
+    const float c[3] = {((float)i + 0.5f) * dx, ((float)j + 0.5f) * dx, ((float)k + 0.5f) * dx};
+    const float sh[3] = {c[0] + os, c[1] + os, c[2] + os};
+    // coordinates of the centre (cc*) and shifted (sc*) values on both kinds of grid
+    Coord cc[3][2], sc[3][2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        cc[a][0] = coord_at(c[a], os, dx);
+        cc[a][1] = coord_at(c[a], 0.0f, dx);
+        sc[a][0] = coord_at(sh[a], os, dx);
+        sc[a][1] = coord_at(sh[a], 0.0f, dx);
+    }
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const Sampled& g = A.val[d];
         if (i >= g.ni || j >= g.nj || k >= g.nk) continue;
-        float x = ((float)i + 0.5f) * dx + g.o0;
-        float y = ((float)j + 0.5f) * dx + g.o1;
-        float z = ((float)k + 0.5f) * dx + g.o2;
-        backtrace(A.field, dx, A.dt, x, y, z);
+        StagCoords p0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            p0.s[a] = a == d ? sc[a][0] : cc[a][0];
+            p0.c[a] = a == d ? sc[a][1] : cc[a][1];
+        }
+        float x = d == 0 ? sh[0] : c[0], y = d == 1 ? sh[1] : c[1], z = d == 2 ? sh[2] : c[2];
+        backtrace(A.field, p0, dx, A.dt, x, y, z);
         A.out[d][(size_t)i + (size_t)g.ni * ((size_t)j + (size_t)g.nj * k)] = trilerp_at(g, x, y, z, dx);
     }
 }
@@ -411,7 +479,7 @@ __global__ __launch_bounds__(256) void k_advect_cells(Advect A) {
     float x = ((float)i + 0.5f) * dx + 0.0f;
     float y = ((float)j + 0.5f) * dx + 0.0f;
     float z = ((float)k + 0.5f) * dx + 0.0f;
-    backtrace(A.field, dx, A.dt, x, y, z);
+    backtrace(A.field, stag_coords(x, y, z, dx), dx, A.dt, x, y, z);
     A.out[0][(size_t)i + (size_t)A.ni * ((size_t)j + (size_t)A.nj * k)] = trilerp_at(A.val[0], x, y, z, dx);
 }
 

@@ -1270,6 +1270,8 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     off += align256(4 * (size_t)(kReiDistMax + 2));
     const size_t o_cells = off;
     off += align256(4 * cells);
+    const size_t o_fixm = off;
+    off += align256(cells);
     const size_t o_counter = off;
     off += 256;
     HIPCHK(ctx->ws.rei.ensure(off), "alloc");
@@ -1285,6 +1287,7 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     r.dcount = (uint32_t*)(base + o_dcount);
     r.dfill = (uint32_t*)(base + o_dfill);
     r.cells = (uint32_t*)(base + o_cells);
+    r.fixm = base + o_fixm;
     r.counter = (uint32_t*)(base + o_counter);
     ctx->kt.clear();
     // grid_temp = grid (reinitialization.cpp:26)
@@ -1311,7 +1314,7 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
         HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
         for (uint32_t t = 1; t <= chk[1]; ++t) {
             ctx->prof_begin("rei_layer");
-            HIPCHK(launch_rei_layer(r, offs[t], offs[t + 1], (int)t, ctx->stream), "k_rei_layer");
+            HIPCHK(launch_rei_layer(r, offs[t], offs[t + 1], ctx->stream), "k_rei_layer");
             ctx->prof_end();
         }
         ctx->prof_collect();

@@ -1041,19 +1041,25 @@ __global__ __launch_bounds__(kReiBucketThreads) void k_rei_scatter(ReiSet r) {
         if (bins[b]) bins[b] = atomicAdd(&r.dfill[dmin + b], bins[b]);
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < kReiBucketPer; ++q)
-        if (ch.d[q]) r.cells[bins[ch.d[q] - dmin] + ch.rank[q]] = (uint32_t)ch.c[q];
+    for (int q = 0; q < kReiBucketPer; ++q) {
+        if (!ch.d[q]) continue;
+        const uint32_t cu = (uint32_t)ch.c[q], row = cu / (uint32_t)r.ni;
+        const int i = (int)(cu - row * (uint32_t)r.ni), j = (int)(row % (uint32_t)r.nj),
+                  k = (int)(row / (uint32_t)r.nj);
+        const uint32_t e = bins[ch.d[q] - dmin] + ch.rank[q];
+        r.cells[e] = cu;
+        r.fixm[e] = (uint8_t)rei_fix6(r, cu, i, j, k, (uint16_t)(ch.d[q] - 1));
+    }
 }
 
 // One pass, one listed cell per thread: the cell takes its FIX neighbours
-// (valid, distance target - 1) exactly as k_rei_pass does.
-__global__ __launch_bounds__(256) void k_rei_layer(ReiSet r, uint32_t begin, uint32_t end, uint32_t target) {
+// (valid, distance target - 1, listed beside it by k_rei_scatter) exactly as
+// k_rei_pass does.
+__global__ __launch_bounds__(256) void k_rei_layer(ReiSet r, uint32_t begin, uint32_t end) {
     const uint32_t e = begin + blockIdx.x * 256 + threadIdx.x;
     if (e >= end) return;
-    const uint32_t cu = r.cells[e], row = cu / (uint32_t)r.ni;
-    const size_t c = cu;
-    const int i = (int)(cu - row * (uint32_t)r.ni), j = (int)(row % (uint32_t)r.nj), k = (int)(row / (uint32_t)r.nj);
-    const uint32_t fx = rei_fix6(r, cu, i, j, k, (uint16_t)(target - 1));
+    const size_t c = r.cells[e];
+    const uint32_t fx = r.fixm[e];
     const bool fxm = fx & 1u, fxp = fx & 2u, fym = fx & 4u, fyp = fx & 8u, fzm = fx & 16u, fzp = fx & 32u;
     const size_t sx = 1, sy = (size_t)r.ni, sz = (size_t)r.ni * r.nj;
     const float vx = fxm ? r.phi[c - sx] : (fxp ? r.phi[c + sx] : 0.0f);
@@ -1179,10 +1185,9 @@ hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, int target, hipStream_t s) {
+hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, hipStream_t s) {
     if (end <= begin) return hipSuccess;
-    hipLaunchKernelGGL(k_rei_layer, dim3(cdiv((int64_t)(end - begin), 256)), dim3(256), 0, s, r, begin, end,
-                       (uint32_t)target);
+    hipLaunchKernelGGL(k_rei_layer, dim3(cdiv((int64_t)(end - begin), 256)), dim3(256), 0, s, r, begin, end);
     return hipGetLastError();
 }
 

@@ -105,6 +105,7 @@ struct ReiSet {
     uint32_t* dcount;      // per distance 0..kReiDistMax: cells not reached yet, then list offsets
     uint32_t* dfill;       // per distance: next free list slot
     uint32_t* cells;       // cells not reached yet, grouped by distance
+    uint8_t* fixm;         // per listed cell: its face neighbours one step closer (bits -x +x -y +y -z +z)
     uint32_t* counter;     // [0..1] list lengths, [2..3] cells fixed by alternate passes,
                            // [4] cells left unreached, [5] largest stamp,
                            // [6] cells where dist is not the march's pass, [7] largest dist
@@ -123,11 +124,12 @@ hipError_t launch_rei_init(const ReiSet& r, hipStream_t s);
 // over 0..kReiDistMax + 1; counter[7] = largest distance).
 constexpr int kReiDistMax = 3 * 4092;
 hipError_t launch_rei_dist(const ReiSet& r, hipStream_t s);
-// After the check passed: the cells not reached yet listed by distance.
+// After the check passed: the cells not reached yet listed by distance, each
+// with the mask of its FIX neighbours (valid, one step closer) in fixm.
 hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s);
-// The cells of one pass (list entries [begin, end), all at distance target)
-// from their FIX neighbours (valid, distance target - 1).
-hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, int target, hipStream_t s);
+// The cells of one pass (list entries [begin, end), all at one distance)
+// from their FIX neighbours.
+hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, hipStream_t s);
 // Tile route. Pass l (0-based since the last rebase; parity = absolute pass & 1).
 hipError_t launch_rei_pass(const ReiSet& r, int l, int parity, hipStream_t s);
 hipError_t launch_rei_rebase(const ReiSet& r, hipStream_t s);

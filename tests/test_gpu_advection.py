@@ -4,8 +4,8 @@ ABI, against the oracle, bit for bit (float32 words):
 - elq_advect_velocity_device / elq_advect_level_set_device in the
   simulator's form (simulator.cpp:79-87) on seeded scenes with random face
   velocities, a non-dyadic dx (1/40), an anisotropic grid that is no tile
-  multiple, and a step long enough that backtraces leave the grid (clamped
-  indices);
+  multiple, a step long enough that backtraces leave the grid (clamped
+  indices), and grids with an axis of one cell;
 - elq_advect_staggered_device / elq_advect_collocated_device with separate
   carried values and carrying velocity;
 - the 128^3 drop scene after one projection, both advections;
@@ -48,7 +48,9 @@ def random_fields(ni, nj, nk, seed, amp):
 
 
 CASES = [((40, 40, 40), 1 / 40, 0.01, 1.0, 1), ((23, 17, 29), 1 / 29, 0.02, 0.8, 2),
-         ((16, 16, 16), 1 / 16, 0.5, 1.0, 3)]   # the last one leaves the grid
+         ((16, 16, 16), 1 / 16, 0.5, 1.0, 3),    # leaves the grid
+         # an axis of one cell (every index on it clamps to 0), large non-dyadic steps
+         ((1, 7, 5), 1 / 7, 0.05, 1.0, 4), ((33, 1, 2), 0.37, 0.9, 2.0, 5), ((3, 2, 1), 0.3, 0.2, 1.5, 6)]
 
 
 @pytest.mark.parametrize("dims,dx,dt,amp,seed", CASES)

@@ -54,7 +54,10 @@ def random_case(dims, density, seed):
 
 CASES = [((37, 21, 19), 0.02, 1, 10), ((37, 21, 19), 0.3, 2, 3), ((64, 16, 16), 0.001, 3, 10),
          ((33, 9, 17), 0.05, 4, 1), ((5, 40, 11), 0.01, 5, 10), ((70, 30, 25), 0.0005, 6, 10),
-         ((16, 16, 16), 0.1, 7, 0), ((16, 16, 16), 0.1, 8, -3)]
+         ((16, 16, 16), 0.1, 7, 0), ((16, 16, 16), 0.1, 8, -3),
+         # one-cell axes; a row of exactly one 64-cell word and one of 65 cells
+         ((1, 30, 20), 0.02, 9, 10), ((25, 1, 1), 0.08, 11, 10), ((64, 5, 1), 0.01, 11, 10),
+         ((65, 1, 7), 0.01, 12, 10)]
 
 
 @pytest.mark.parametrize("dims,density,seed,layers", CASES)

@@ -11,7 +11,8 @@ same count of passes and of unreached cells:
 - a wall of invalid cells with a hole, so that cells behind it are reached
   by a detour (longer than their taxicab distance);
 - the 128^3 drop scene's level set after one advection step (about 100
-  passes, several host batches).
+  passes, several host batches);
+- grids one or two cells thick on some axis.
 
 The library takes one of two routes, both checked here: passes ordered by
 the taxicab distance to the interface where that distance is the pass count
@@ -91,6 +92,17 @@ def test_sphere_with_noise_on_room_and_pocket(ctx):
     left, _ = O.reinitialize(ref, valid, dims, dx)
     assert left == 27
     check(ctx, phi, valid, dims, dx, route="rei_pass")
+
+
+@pytest.mark.parametrize("dims", [(1, 30, 25), (27, 1, 19), (2, 3, 70), (65, 2, 1)])
+def test_thin_grids_bit_exact(ctx, dims):
+    """Grids one or two cells thick on some axis: the neighbour masks of the
+    distance route never point across those faces."""
+    dx = 1.0 / 30
+    x, y, z = xyz(dims, dx)
+    rng = np.random.default_rng(sum(dims))
+    phi = ((np.sin(7 * x + 3 * y) + np.cos(5 * z) - 0.4) * rng.uniform(0.5, 1.5, x.size)).astype(np.float32)
+    assert check(ctx, phi, np.ones(phi.size, np.uint8), dims, dx) >= 10
 
 
 @pytest.mark.parametrize("flags", ["all_valid", "none_valid"])

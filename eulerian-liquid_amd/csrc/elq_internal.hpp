@@ -66,6 +66,7 @@ struct alignas(64) Ctrl {
     double last_err;    // sqrt(rr / bnorm2) at stop
     double rz_div;      // rz the next beta divides by (mixed residual replacement)
     double rr_ref;      // |r|^2 right after the last residual replacement
+    double rr_rec;      // recursive |r|^2 just before the last residual replacement
     int32_t iter;       // Krylov iterations of the current (inner) solve
     int32_t iter_total; // summed over refinement steps
     int32_t max_iter;

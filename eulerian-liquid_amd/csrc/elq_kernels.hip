@@ -595,6 +595,10 @@ __global__ __launch_bounds__(256) void k_init(Bricks bk, const double* b64, T* r
     if (lane_id() == 0) part[b] = s;
 }
 
+// Mixed precision: a residual replacement restarts the search direction when
+// the FP64 |r|^2 exceeds the recursive one by more than this factor.
+constexpr double kRestartGap2 = 4.0;
+
 // Scalar recurrences of Eigen's conjugate_gradient (see the oracle's orc_pcg)
 // and of the mixed-precision residual replacement. Stage 1: workgroup g sums
 // the partials of a fixed contiguous segment; stage 2: workgroup 0 gathers the
@@ -700,6 +704,7 @@ __device__ void scalar_update(Ctrl* c, int mode, double s, double s2) {
                 c->last_err = sqrt(s / c->bnorm2);
                 if (!isfinite(s)) c->status = 3;
             } else {
+                c->rr_rec = c->rr;
                 c->rr = s2;
                 c->rr_ref = s2;
                 c->threshold = thr;
@@ -710,7 +715,11 @@ __device__ void scalar_update(Ctrl* c, int mode, double s, double s2) {
         }
         case SC_RZ_REPLACE:
             if (!was_done) {
-                c->beta = s / c->rz_div;
+                // Keep the search direction (beta = rz_new / rz_old) while the
+                // FP64 residual is close to the recursive one; where it is far
+                // above (the recursion ran on past what FP32 can resolve), that
+                // beta would make p almost the old direction, so restart.
+                c->beta = c->rr64 > kRestartGap2 * c->rr_rec ? 0.0 : s / c->rz_div;
                 c->rz = s;
                 if (!isfinite(c->beta)) {
                     c->done = 1;

@@ -109,6 +109,23 @@ def test_projection_mixed(ctx, name):
     assert st.outer_iterations >= 2
 
 
+@pytest.mark.parametrize("inner_tol", [0.01, 0.001])
+def test_mixed_replacement_far_above_recursion_restarts(ctx, inner_tol):
+    """Polled every 32 iterations with a small inner tolerance, the FP32
+    recursion runs on well past what it resolves, so the FP64 residual that
+    replaces it is far larger; keeping the search direction there stalled the
+    64^3 drop scene at ~3e-5 for thousands of iterations (5,603 in all).
+    Restarting on such a replacement keeps the count near the FP64 one."""
+    s = elq.scene_drop(64)
+    g = s.copy()
+    o64 = elq.default_opts(tol=1e-8, max_iter=20000, precision=elq.FP64)
+    _, _, st64 = ctx.solve_pressure(s.copy(), o64)
+    o = elq.default_opts(tol=1e-8, max_iter=20000, precision=elq.MIXED, inner_tol=inner_tol, check_every=32)
+    p, idx, st = ctx.solve_pressure(g, o)
+    assert st.status == elq.OK and st.converged and st.true_rel_residual <= 1e-8, st.as_dict()
+    assert st.iterations <= 1.4 * st64.iterations, (st.iterations, st64.iterations)
+
+
 @pytest.mark.parametrize("cap", [4, 8])
 @pytest.mark.parametrize("precision, tol", [(elq.MIXED, 2e-5), (elq.FP64, 2e-6)])
 def test_projection_capped_grid(precision, tol, cap):

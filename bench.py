@@ -10,7 +10,9 @@ pressure scatter, velocity update) on inputs already resident in HBM.
 
 Prints one JSON line: solves/s, the dominant kernel's roofline (HIP events on
 the library's stream, algorithmic bytes per launch), the FP64 GPU drop-in on
-the same inputs, and the CPU oracle timed on a bounded sample.
+the same inputs, max|p - p_FP64|, the largest weighted divergence left in a
+fluid cell relative to the input's largest (both precisions), and the CPU
+oracle timed on a bounded sample.
 """
 import argparse
 import json
@@ -64,6 +66,27 @@ def project(ctx, ds, opts):
     if rc != elq.OK:
         raise SystemExit(f"projection failed: {elq.STATUS_NAMES.get(rc)} {ctx.last_error()}")
     return dt_ms, st
+
+
+def weighted_divergence(s, u, v, w):
+    """Cell-centred weighted divergence sum over faces of +/- weight * u / dx
+    (the right-hand side of build_matrix up to its -1/dx factor), FP64."""
+    ni, nj, nk = s.ni, s.nj, s.nk
+    uu = (u * s.uw).astype(np.float64).reshape(nk, nj, ni + 1)
+    vv = (v * s.vw).astype(np.float64).reshape(nk, nj + 1, ni)
+    ww = (w * s.ww).astype(np.float64).reshape(nk + 1, nj, ni)
+    return (((uu[:, :, 1:] - uu[:, :, :-1]) + (vv[:, 1:, :] - vv[:, :-1, :]) + (ww[1:] - ww[:-1])) / s.dx).reshape(-1)
+
+
+def fluid_divergence(ds):
+    """After the last projection on ds: the largest |weighted divergence| over
+    the fluid cells (those with an index), relative to the largest over all
+    cells of the input velocity."""
+    s = ds.s
+    fluid = ds.fetch("indices", np.int32) != elq.UNKNOWN_INDEX
+    before = float(np.abs(weighted_divergence(s, s.u, s.v, s.w)).max())
+    after = float(np.abs(weighted_divergence(s, ds.fetch("u"), ds.fetch("v"), ds.fetch("w"))[fluid]).max())
+    return after / before if before else 0.0
 
 
 def run_steps(ctx, ds, opts, steps, warmup):
@@ -453,6 +476,7 @@ def main():
     if not a.fp64_only:
         res = run_steps(ctx, ds, o_mixed, a.steps, a.warmup)
         p_mixed = ds.fetch("pressure")
+        div_mixed = fluid_divergence(ds)
         print(f"# mixed: {[round(t, 1) for t in res[0]]} ms, iters {res[1][-1]['iterations']} "
               f"outer {res[1][-1]['outer_iterations']}", file=sys.stderr, flush=True)
         out["mixed"] = res
@@ -460,6 +484,7 @@ def main():
         steps64 = a.steps if a.fp64_only else max(1, min(a.steps, 2))
         res = run_steps(ctx, ds, o_fp64, steps64, a.warmup if a.fp64_only else 1)
         p_fp64 = ds.fetch("pressure")
+        div_fp64 = fluid_divergence(ds)
         print(f"# fp64: {[round(t, 1) for t in res[0]]} ms, iters {res[1][-1]['iterations']}", file=sys.stderr,
               flush=True)
         out["fp64"] = res
@@ -493,7 +518,8 @@ def main():
         "config": {"workload": f"{wname} seed {a.seed} ({scene.ni}^3 slab+spheres, rel.res 1e-8)",
                    "precision": key, "precond": a.precond, "schedule": a.schedule, "unknowns": n,
                    "bricks": st["n_bricks"], "iterations": iters, "outer_iterations": outer,
-                   "true_rel_residual": st["true_rel_residual"]},
+                   "true_rel_residual": st["true_rel_residual"],
+                   "max_fluid_divergence_rel": div_fp64 if key == "fp64" else div_mixed},
         "hbm_gbs_whole_solve": round(total_bytes / (ms_step / 1e3) / 1e9, 1),
         "hbm_frac_whole_solve": round(total_bytes / (ms_step / 1e3) / 1e9 / PEAK_HBM_GBS, 4),
         "roofline": roof,
@@ -505,7 +531,8 @@ def main():
         scale = float(np.abs(p_fp64).max())
         line["fp64_drop_in"] = {"solves_per_s": round(1000.0 / ms64, 4), "ms_per_solve": round(ms64, 2),
                                 "iterations": s64[-1]["iterations"],
-                                "true_rel_residual": s64[-1]["true_rel_residual"]}
+                                "true_rel_residual": s64[-1]["true_rel_residual"],
+                                "max_fluid_divergence_rel": div_fp64}
         line["speedup_mixed_vs_fp64"] = round(ms64 / ms_step, 3)
         line["max_abs_p_minus_p_fp64"] = float(np.abs(p_mixed - p_fp64).max())
         line["max_rel_p_minus_p_fp64"] = float(np.abs(p_mixed - p_fp64).max()) / scale if scale else 0.0

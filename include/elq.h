@@ -92,7 +92,9 @@ typedef struct {
     double sigma;      /* pivot safeguard: pivot t <= sigma*diag -> diag */
     double inner_tol;  /* MIXED: the FP32 recursive residual is replaced by the FP64
                           residual b - A x each time it has dropped by this factor
-                          (default 0.01) and whenever it meets the stopping test */
+                          (default 0.01; checked at the polls) and whenever it meets
+                          the stopping test; a replacement far above the recursive
+                          residual restarts the search direction */
     int32_t schedule;  /* elq_schedule */
     int32_t check_every; /* PCG iterations between host convergence polls */
 } elq_opts;

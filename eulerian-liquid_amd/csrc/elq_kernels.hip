@@ -391,6 +391,9 @@ __global__ __launch_bounds__(256) void k_brick_lists(const int32_t* map, const i
 // neighbours are visited in the reference's order -x,+x,-y,+y,-z,+z; diag is a
 // double sum of float terms; b accumulates float products in double and is
 // scaled by (double)(dx/dt) with dx/dt a float division done on the host.
+// The FP32 copy of the diagonal holds only its ghost part g = diag + (sum of
+// the row's off-diagonals) = the sum of the air faces' w / theta terms, 0 in
+// the interior, for the FP32 SpMV's difference form.
 template <class T>
 __global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const uint8_t* mask, Bricks bk,
                                                   double* diag64, double* b64, T* diag, T* ax, T* ay, T* az) {
@@ -402,7 +405,7 @@ __global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const ui
               k = ((cc >> 20) & 1023) * kBrick + L.lz;
     const int ni = d.ni, nj = d.nj, nk = d.nk;
     const size_t o = (size_t)b * kBrickCells + L.lane;
-    double dg = 0.0, bb = 0.0;
+    double dg = 0.0, gh = 0.0, bb = 0.0;
     float cpl[3] = {0.0f, 0.0f, 0.0f};
     const bool inside = i < ni && j < nj && k < nk;
     const size_t c = inside ? cell(i, j, k, ni, nj) : 0;
@@ -428,6 +431,7 @@ __global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const ui
                 } else {
                     const float theta = clamp_theta(fraction_inside2(phic, d.phi[cn]));
                     dg += (double)(wgt[dir] / theta);
+                    gh += (double)(wgt[dir] / theta);
                 }
             }
             if (dir & 1)
@@ -439,7 +443,7 @@ __global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const ui
     }
     diag64[o] = dg;
     b64[o] = bb;
-    diag[o] = (T)dg;
+    diag[o] = sizeof(T) == 4 ? (T)gh : (T)dg;
     ax[o] = (T)cpl[0];
     ay[o] = (T)cpl[1];
     az[o] = (T)cpl[2];
@@ -873,7 +877,7 @@ __device__ __forceinline__ size_t halo_offset(const Nbr6& nb, int d, size_t own,
 // p = z + beta p_old (ping-pong buffers), q = A p, p.q partials. Cross-brick
 // neighbour values of p are rebuilt from z and p_old; in-brick ones come
 // through lane shuffles. All loads are issued unconditionally and at once.
-// Accumulation order: diag, -x, +x, -y, +y, -z, +z.
+// Accumulation order: diag (FP32: its ghost part), -x, +x, -y, +y, -z, +z.
 template <class T>
 __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef<T>& c, const T* __restrict__ z,
                                              const T* __restrict__ p_in, T* __restrict__ p_out, T* __restrict__ q,
@@ -911,13 +915,28 @@ __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef
     const T axm = L.lx > 0 ? cxm : (has[0] ? axe : (T)0);
     const T aym = L.ly > 0 ? cym : (has[2] ? aye : (T)0);
     const T azm = L.lz > 0 ? czm : (has[4] ? aze : (T)0);
-    T acc = dg * pc;
-    acc = fma(axm, p_xm, acc);
-    acc = fma(axo, p_xp, acc);
-    acc = fma(aym, p_ym, acc);
-    acc = fma(ayo, p_yp, acc);
-    acc = fma(azm, p_zm, acc);
-    acc = fma(azo, p_zp, acc);
+    T acc;
+    if constexpr (sizeof(T) == 4) {
+        // difference form g p_i + sum a_ij (p_j - p_i), g the ghost part of
+        // the diagonal: for a smooth p the sum of diag p_i and the
+        // off-diagonal terms cancels to a small value that FP32 would round
+        // away, while the differences are exact for close values
+        acc = dg * pc;
+        acc = fma(axm, p_xm - pc, acc);
+        acc = fma(axo, p_xp - pc, acc);
+        acc = fma(aym, p_ym - pc, acc);
+        acc = fma(ayo, p_yp - pc, acc);
+        acc = fma(azm, p_zm - pc, acc);
+        acc = fma(azo, p_zp - pc, acc);
+    } else {
+        acc = dg * pc;
+        acc = fma(axm, p_xm, acc);
+        acc = fma(axo, p_xp, acc);
+        acc = fma(aym, p_ym, acc);
+        acc = fma(ayo, p_yp, acc);
+        acc = fma(azm, p_zm, acc);
+        acc = fma(azo, p_zp, acc);
+    }
     q[o] = acc;
     return (double)pc * (double)acc;
 }

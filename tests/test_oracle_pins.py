@@ -129,6 +129,31 @@ def test_matrix_invariants():
         assert row[r] == 6.0 and sorted(row[row < 0]) == [-1.0] * 6
 
 
+@pytest.mark.parametrize("make", [lambda: O.scene_drop(16), lambda: O.scene_spheres(16, seed=11, n_spheres=6)])
+def test_ghost_part_of_the_diagonal(make):
+    """The FP32 SpMV of the mixed solve stores g = diag + (sum of the row's
+    off-diagonals) and computes g p_i + sum a_ij (p_j - p_i). That is the
+    same operator as long as g is the row sum of A: g = 0 exactly where all
+    six neighbours are unknowns, g > 0 only where a neighbour is not, and the
+    off-diagonals are float values (exact in FP32)."""
+    s = make()
+    n, idx, sys_ = _system(s)
+    A = _csr(sys_)
+    off = A - sp.diags(A.diagonal())
+    nbrs = np.diff(off.tocsr().indptr)
+    g = np.asarray(A.sum(axis=1)).ravel()
+    assert g.min() >= 0.0
+    assert np.all(g[nbrs == 6] == 0.0) and np.count_nonzero(nbrs == 6) > 0
+    assert np.all(nbrs[g > 0] < 6) and np.count_nonzero(g > 0) > 0
+    vals = off.tocsr().data
+    assert np.array_equal(vals.astype(np.float32).astype(np.float64), vals)
+    # the difference form gives A p on a smooth field to FP64 rounding
+    p = np.linspace(1.0, 2.0, n)
+    i, j = off.nonzero()
+    q = g * p + np.bincount(i, weights=np.asarray(off[i, j]).ravel() * (p[j] - p[i]), minlength=n)
+    assert np.abs(q - A @ p).max() <= 1e-12 * np.abs(A).max() * np.abs(p).max()
+
+
 def _divergence(s):
     """Weighted divergence sum_faces (+/-) w u / dx, cell-centred, [k, j, i]."""
     ni, nj, nk = s.ni, s.nj, s.nk

@@ -536,25 +536,14 @@ __device__ float tet_distance(P3 x0, P3 x1, P3 x2, P3 x3, float f0, float f1, fl
     return fabsf(nv.x * x0.x + nv.y * x0.y + nv.z * x0.z + d) / sqrtf(nv.x * nv.x + nv.y * nv.y + nv.z * nv.z);
 }
 
-__device__ __forceinline__ float rei_at(const float* a, int i, int j, int k, int ni, int nj, int nk) {
-    i = min(max(i, 0), ni - 1);
-    j = min(max(j, 0), nj - 1);
-    k = min(max(k, 0), nk - 1);
-    return a[at(i, j, k, ni, nj)];
-}
-
 // reinitialization.cpp:27-62 for one valid cell: FIX if phi changes sign (or
 // touches 0) towards a face neighbour (indices clamped); a FIX cell whose
 // phi is not 0 takes calc_sdf over the 8 tetrahedra of its 7-point star
 // (level_set.h:251-264). Star positions are Vector3f((i + 0.5) * dx, ...):
-// double products rounded to float.
-__device__ bool rei_interface(const ReiSet& r, int i, int j, int k, float& out) {
-    const int ni = r.ni, nj = r.nj, nk = r.nk;
-    const float* t = r.temp;
-    const float p = t[at(i, j, k, ni, nj)];
-    const float pxm = rei_at(t, i - 1, j, k, ni, nj, nk), pxp = rei_at(t, i + 1, j, k, ni, nj, nk);
-    const float pym = rei_at(t, i, j - 1, k, ni, nj, nk), pyp = rei_at(t, i, j + 1, k, ni, nj, nk);
-    const float pzm = rei_at(t, i, j, k - 1, ni, nj, nk), pzp = rei_at(t, i, j, k + 1, ni, nj, nk);
+// double products rounded to float. The caller gives p and its six face
+// neighbours (indices clamped to the grid).
+__device__ bool rei_interface_at(const ReiSet& r, int i, int j, int k, float p, float pxm, float pxp, float pym,
+                                 float pyp, float pzm, float pzp, float& out) {
     const bool flag = p * pxm <= 0.0f || p * pxp <= 0.0f || p * pym <= 0.0f || p * pyp <= 0.0f ||
                       p * pzm <= 0.0f || p * pzp <= 0.0f;
     if (!flag) return false;
@@ -594,22 +583,46 @@ __device__ __forceinline__ TileXYZ rei_tile(const ReiSet& r, int t) {
     return T;
 }
 
-// One workgroup per tile: interface cells, stamps, tfar and tnew.
+// One workgroup per tile: interface cells, stamps, tfar and tnew. A thread
+// takes a z column of the tile and issues the loads of all its cells first
+// (flags, the column with one cell beyond each end, the four side columns;
+// indices clamped, rows past the grid read row nk - 1 and are not used).
 __global__ __launch_bounds__(256) void k_rei_init(ReiSet r) {
     const TileXYZ T = rei_tile(r, blockIdx.x);
     const int i = T.bx * kExtTX + (threadIdx.x & (kExtTX - 1));
     const int j = T.by * kExtTY + (threadIdx.x / kExtTX);
     bool far_any = false, fix_any = false;
     if (i < r.ni && j < r.nj) {
+        const int ni = r.ni, nj = r.nj, nk = r.nk, k0 = T.bz * kExtTZ;
+        const float* t = r.temp;
+        const int im = max(i - 1, 0), ip = min(i + 1, ni - 1), jm = max(j - 1, 0), jp = min(j + 1, nj - 1);
+        uint8_t val[kExtTZ];
+        float col[kExtTZ + 2], sxm[kExtTZ], sxp[kExtTZ], sym[kExtTZ], syp[kExtTZ];
+#pragma unroll
+        for (int z = 0; z < kExtTZ + 2; ++z) {
+            const int k = min(max(k0 + z - 1, 0), nk - 1);
+            col[z] = t[at(i, j, k, ni, nj)];
+        }
+#pragma unroll
         for (int z = 0; z < kExtTZ; ++z) {
-            const int k = T.bz * kExtTZ + z;
-            if (k >= r.nk) break;
-            const size_t c = at(i, j, k, r.ni, r.nj);
+            const int k = min(k0 + z, nk - 1);
+            val[z] = r.valid[at(i, j, k, ni, nj)];
+            sxm[z] = t[at(im, j, k, ni, nj)];
+            sxp[z] = t[at(ip, j, k, ni, nj)];
+            sym[z] = t[at(i, jm, k, ni, nj)];
+            syp[z] = t[at(i, jp, k, ni, nj)];
+        }
+#pragma unroll
+        for (int z = 0; z < kExtTZ; ++z) {
+            const int k = k0 + z;
+            if (k >= nk) break;
+            const size_t c = at(i, j, k, ni, nj);
             uint16_t st;
             float v;
-            if (r.valid[c] == 0) {
+            if (val[z] == 0) {
                 st = kReiUnknown;
-            } else if (rei_interface(r, i, j, k, v)) {
+            } else if (rei_interface_at(r, i, j, k, col[z + 1], sxm[z], sxp[z], sym[z], syp[z], col[z], col[z + 2],
+                                        v)) {
                 r.phi[c] = v;
                 st = 0;
                 fix_any = true;

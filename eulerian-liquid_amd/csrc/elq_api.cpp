@@ -800,7 +800,8 @@ int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
     int rate_khz = 0;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess || rate_khz <= 0)
         rate_khz = 100000;
-    ctx->timeout_ticks = (uint64_t)rate_khz * 1000ull * 4ull;  // 4 s per wait
+    ctx->clock_hz = (uint64_t)rate_khz * 1000ull;
+    ctx->timeout_ticks = ctx->clock_hz * 4ull;  // 4 s per wait
     *out = ctx;
     return ELQ_OK;
 }
@@ -817,6 +818,13 @@ void elq_ctx_destroy(elq_ctx* ctx) {
 }
 
 const char* elq_ctx_last_error(const elq_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int elq_set_wait_limit(elq_ctx* ctx, double seconds) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (!(seconds >= 0.0) || seconds > 3600.0) return ctx->fail(ELQ_ERR_INVALID_ARG, "wait limit out of range");
+    ctx->timeout_ticks = (uint64_t)(seconds * (double)ctx->clock_hz);
+    return ELQ_OK;
+}
 
 size_t elq_ctx_device_bytes(const elq_ctx* ctx) { return ctx ? ctx->ws.bytes() : 0; }
 
@@ -932,6 +940,55 @@ int elq_build_matrix(elq_ctx* ctx, const elq_grid* g, float dt, double* diag, do
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     double* outs[5] = {diag, ax, ay, az, b};
     for (int q = 0; q < 5; ++q) HIPCHK(hipMemcpy(outs[q], e + q * n, n * 8, hipMemcpyDeviceToHost), "d2h");
+    return ELQ_OK;
+}
+
+int elq_ic_factor(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, double* inv_e) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!inv_e) return ctx->fail(ELQ_ERR_INVALID_ARG, "null output");
+    if (!(dt > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "dt must be positive");
+    elq_opts o;
+    if (opts)
+        o = *opts;
+    else
+        elq_default_opts(&o);
+    if (!check_opts(o)) return ctx->fail(ELQ_ERR_INVALID_ARG, "invalid solver options");
+    (void)hipSetDevice(ctx->device);
+    elq_grid dg;
+    if ((rc = stage_inputs(ctx, *g, dg))) return rc;
+    const Dense d = dense_of(dg);
+    if ((rc = setup_phase_a(ctx, d, nullptr))) return rc;
+    Workspace& W = ctx->ws;
+    if (W.n_unknowns == 0) return ctx->fail(ELQ_ERR_NO_FLUID, "No fluid");
+    if (W.setup_err & kErrBoundary) return ctx->fail(ELQ_ERR_BOUNDARY, "unknown on the outer cell layer");
+    const bool levels = o.schedule == ELQ_SCHED_LEVELS;
+    if ((rc = setup_phase_b<double>(ctx, d, dt, levels))) return rc;
+    const Plan p = make_plan(d.ni, d.nj, d.nk);
+    Solver<double> S;
+    S.ctx = ctx;
+    S.p = p;
+    S.bk = make_bricks(ctx, p);
+    S.pc = make_pencils(ctx, p, true);
+    S.o = &o;
+    S.levels = levels;
+    S.c.diag = (const double*)W.cdiag.p;
+    S.c.ax = (const double*)W.cax.p;
+    S.c.ay = (const double*)W.cay.p;
+    S.c.az = (const double*)W.caz.p;
+    S.c.inv_e = (const double*)W.inv_e64.p;
+    Ctrl hc;
+    ctrl_host_init(hc, o, 0);
+    if ((rc = upload_ctrl(ctx, hc))) return rc;
+    if ((rc = S.factor(o.precond == ELQ_PRECOND_MIC0 ? o.tau : 0.0, o.sigma, nullptr))) return rc;
+    const size_t n = (size_t)W.n_unknowns;
+    HIPCHK(W.export_buf.ensure(n * 8), "alloc");
+    HIPCHK(launch_export_cells(d, (const uint8_t*)W.mask.p, W.cur_indices, S.bk, (const double*)W.inv_e64.p,
+                               (double*)W.export_buf.p, ctx->stream),
+           "k_export_cells");
+    if ((rc = read_ctrl(ctx, hc))) return rc;
+    if (hc.abort) return ctx->fail(ELQ_ERR_DEVICE, "wavefront wait exceeded its time limit");
+    HIPCHK(hipMemcpy(inv_e, W.export_buf.p, n * 8, hipMemcpyDeviceToHost), "d2h");
     return ELQ_OK;
 }
 

@@ -83,6 +83,7 @@ struct elq_ctx {
     hipStream_t stream = nullptr;
     void* ctrl_host = nullptr;  // pinned host mirror (kHostMirrorBytes): control block, setup totals
     uint64_t timeout_ticks = 0;
+    uint64_t clock_hz = 0;  // device wall clock (wall_clock64) rate
     int max_groups = 0;  // cap on the grid of persistent kernels (0: whole device)
     std::string last_error;
     Workspace ws;

@@ -96,6 +96,11 @@ __device__ __forceinline__ float fraction_inside2(float l, float r) {
 // std::max(theta, 0.01f) as <algorithm> defines it
 __device__ __forceinline__ float clamp_theta(float t) { return (t < 0.01f) ? 0.01f : t; }
 
+// Tag generations (the sweep epoch and the reduction tag) advance by one and
+// skip 0: exchange and stage words start zeroed, so a generation of 0 would
+// accept a word that was never published.
+__device__ __forceinline__ uint32_t next_tag(uint32_t t) { return t + 1u == 0u ? 1u : t + 1u; }
+
 __device__ __forceinline__ size_t cell(int i, int j, int k, int ni, int nj) {
     return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
 }
@@ -582,6 +587,22 @@ __global__ __launch_bounds__(256) void k_export(Dense d, const uint8_t* mask, co
     ob[n] = b64[o];
 }
 
+// One FP64 value per brick cell (e.g. the IC pivots) into the reference's
+// unknown numbering: out[indices[c]] = v[brick slot of c].
+__global__ __launch_bounds__(256) void k_export_cells(Dense d, const uint8_t* mask, const int32_t* indices, Bricks bk,
+                                                      const double* v, double* out) {
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= bk.nb) return;
+    const Lane L;
+    const int cc = bk.coord[b];
+    const int i = (cc & 1023) * kBrick + L.lx, j = ((cc >> 10) & 1023) * kBrick + L.ly,
+              k = ((cc >> 20) & 1023) * kBrick + L.lz;
+    if (i >= d.ni || j >= d.nj || k >= d.nk) return;
+    const size_t c = cell(i, j, k, d.ni, d.nj);
+    if (!(mask[c] & kFluid)) return;
+    out[indices[c]] = v[(size_t)b * kBrickCells + L.lane];
+}
+
 // ================================================================== Krylov
 // x0 = 0, r = b (Eigen: residual = rhs - mat*x with x = 0), |b|^2 partials.
 template <class T>
@@ -609,7 +630,7 @@ constexpr double kRestartGap2 = 4.0;
 // segment sums (tagged words, no cache-wide fences) and adds them in segment
 // order. The result is deterministic for a given number of partials.
 __device__ void scalar_update(Ctrl* c, int mode, double s, double s2) {
-    c->epoch += 1u;
+    c->epoch = next_tag(c->epoch);
     c->ticket = 0u;
     const bool was_done = c->done != 0;
     switch (mode) {
@@ -630,6 +651,11 @@ __device__ void scalar_update(Ctrl* c, int mode, double s, double s2) {
             } else if (s < c->threshold) {
                 c->done = 1;
                 c->converged = 1;
+                c->last_err = sqrt(c->rr / c->bnorm2);
+            } else if (c->max_iter <= 0) {
+                // Eigen's loop body never runs: x = 0, iterations() = 0,
+                // error() = sqrt(|r|^2 / |b|^2) = 1
+                c->done = 1;
                 c->last_err = sqrt(c->rr / c->bnorm2);
             }
             break;
@@ -752,7 +778,7 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
     __shared__ double sh2[kReduceThreads];
     __shared__ double gs[kRedBlocks], gs2[kRedBlocks];
     const int t = threadIdx.x, g = blockIdx.x;
-    const uint32_t tag = c->red_epoch + 1u;
+    const uint32_t tag = next_tag(c->red_epoch);
     const int seg = (n + nblk - 1) / nblk;
     const int lo = g * seg, hi = min(n, lo + seg);
     double s = 0.0, s2 = 0.0;
@@ -949,7 +975,7 @@ template <class T>
 __global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
                                               Ctrl* ctrl, uint64_t* stage, uint64_t timeout) {
     if (ctrl->done) return;
-    const uint32_t tag = ctrl->red_epoch + 1u;
+    const uint32_t tag = next_tag(ctrl->red_epoch);
     const T beta = (T)ctrl->beta;
     const Lane L;
     const int W = (int)gridDim.x * 4;
@@ -1576,7 +1602,7 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
     uint32_t seen_y = 0, seen_z = 0;
     const int f_y = L.lx + 4 * L.lz, f_z = L.lx + 4 * L.ly;   // face index (producer and consumer side)
     const uint32_t epoch = ctrl->epoch;
-    const uint32_t tag = ctrl->red_epoch + 1u;
+    const uint32_t tag = next_tag(ctrl->red_epoch);
     const T alpha = UPDATE ? (T)ctrl->alpha : (T)0;
     double acc = 0.0;
     int round = 0;
@@ -1659,7 +1685,7 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
     uint32_t seen_y = 0, seen_z = 0;
     const int f_y = L.lx + 4 * L.lz, f_z = L.lx + 4 * L.ly;
     const uint32_t epoch = ctrl->epoch;
-    const uint32_t tag = ctrl->red_epoch + 1u;
+    const uint32_t tag = next_tag(ctrl->red_epoch);
     double acc = 0.0;
     int round = 0;
     for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
@@ -2047,6 +2073,13 @@ hipError_t launch_export(const Dense& d, const uint8_t* mask, const int32_t* ind
                          double* odiag, double* oax, double* oay, double* oaz, double* ob, hipStream_t s) {
     hipLaunchKernelGGL(k_export<T>, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, d, mask, indices, bk, diag64, b64, ax,
                        ay, az, odiag, oax, oay, oaz, ob);
+    return hipGetLastError();
+}
+
+hipError_t launch_export_cells(const Dense& d, const uint8_t* mask, const int32_t* indices, const Bricks& bk,
+                               const double* v, double* out, hipStream_t s) {
+    if (bk.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_export_cells, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, d, mask, indices, bk, v, out);
     return hipGetLastError();
 }
 

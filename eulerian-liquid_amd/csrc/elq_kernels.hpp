@@ -110,6 +110,10 @@ hipError_t launch_export(const Dense& d, const uint8_t* mask, const int32_t* ind
                          const double* diag64, const double* b64, const T* ax, const T* ay, const T* az,
                          double* odiag, double* oax, double* oay, double* oaz, double* ob, hipStream_t s);
 
+// Per-brick-cell FP64 array (e.g. inv_e64) into the natural numbering.
+hipError_t launch_export_cells(const Dense& d, const uint8_t* mask, const int32_t* indices, const Bricks& bk,
+                               const double* v, double* out, hipStream_t s);
+
 // ---------------------------------------------------------------- Krylov
 template <class T>
 hipError_t launch_init(const Bricks& bk, const double* b64, T* r, T* x, T* p0, T* p1, double* part,

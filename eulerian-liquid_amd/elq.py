@@ -16,8 +16,10 @@ missing or no gfx950 device is present; there is no CPU fallback.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -97,6 +99,7 @@ def load():
     L.elq_assign_indices.argtypes = [vp, C.POINTER(Grid), vp, C.POINTER(i64)]
     L.elq_build_matrix.argtypes = [vp, C.POINTER(Grid), f32, vp, vp, vp, vp, vp]
     L.elq_update_velocity.argtypes = [vp, C.POINTER(Grid), f32]
+    L.elq_ic_factor.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(Opts), vp]
     L.elq_device_alloc.argtypes = [vp, sz, C.POINTER(vp)]
     L.elq_device_free.argtypes = [vp, vp]
     L.elq_memcpy_h2d.argtypes = [vp, vp, vp, sz]
@@ -122,6 +125,7 @@ def load():
     L.elq_set_profiling.argtypes = [vp, C.c_int]
     L.elq_set_profiling.restype = None
     L.elq_set_max_groups.argtypes = [vp, C.c_int]
+    L.elq_set_wait_limit.argtypes = [vp, f64]
     L.elq_server_create.argtypes = [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]
     L.elq_server_destroy.argtypes = [vp]
     L.elq_server_destroy.restype = None
@@ -269,6 +273,24 @@ def scene_spheres(n: int, seed: int, n_spheres: int = 32, dt: float = 0.01, vel_
     return s
 
 
+def scene_clamp(n: int = 16, seed: int = 5, dt: float = 0.01) -> Scene:
+    """A slab whose top lies 1e-5 above the centres of cell row floor(0.35 n)
+    and a sphere 1e-5 beyond the centres of the cells three cells from its
+    centre cell (n/2, n/2, n/2) on each axis: the faces from those cells
+    outwards have theta ~ 1e-4, below the ghost-fluid clamp of 0.01
+    (projection.cpp:59-63,100-131), on every axis. Face velocities
+    U[-0.5, 0.5]; validity + zeroing."""
+    dx = np.float32(np.float32(1.0) / np.float32(n))
+    c = np.float32((n // 2 + 0.5) * dx)
+    slab = np.float32((int(0.35 * n) + 0.5) * dx + 1e-5)
+    r = np.float32(3 * dx + 1e-5)
+    s = make_scene(n, n, n, float(slab), [(c, c, c, r)], dt)
+    Rng(seed).velocity(s, 0.5)
+    valid_and_zero(s)
+    s.meta["workload"] = f"clamp{n}"
+    return s
+
+
 SERVE_DIMS = (32, 48, 64, 96, 128)
 
 
@@ -288,6 +310,33 @@ def scene_request(seed: int, dt: float = 0.01) -> Scene:
     valid_and_zero(s)
     s.meta["workload"] = f"request_s{seed}_{ni}x{nj}x{nk}"
     return s
+
+
+# --------------------------------------------------------------------------- teardown order
+# Library objects still alive when the interpreter exits are destroyed here,
+# in an explicit order, while the HIP runtime is still intact: servers first
+# (elq_server_destroy joins their slot threads, which run HIP calls and hold
+# per-thread runtime state), then contexts. Python runs this hook before it
+# tears down modules, and long before exit() runs the runtime's own static
+# destructors; left to the garbage collector, a server could outlive the
+# interpreter with its threads still attached to the runtime, or be destroyed
+# at an arbitrary point of the shutdown. __del__ does nothing once the hook
+# has started.
+_live_servers: "weakref.WeakSet" = weakref.WeakSet()
+_live_contexts: "weakref.WeakSet" = weakref.WeakSet()
+_shutting_down = False
+
+
+def _shutdown():
+    global _shutting_down
+    _shutting_down = True
+    for srv in list(_live_servers):
+        srv.close()
+    for ctx in list(_live_contexts):
+        ctx.close()
+
+
+atexit.register(_shutdown)
 
 
 # --------------------------------------------------------------------------- context
@@ -319,13 +368,17 @@ class Context:
             raise RuntimeError(f"elq_ctx_create failed ({STATUS_NAMES.get(rc, rc)}): no usable gfx950 device")
         self.h = h
         self.L = L
+        _live_contexts.add(self)
 
     def close(self):
         if getattr(self, "h", None):
             self.L.elq_ctx_destroy(self.h)
             self.h = None
+        _live_contexts.discard(self)
 
     def __del__(self):
+        if _shutting_down:
+            return
         try:
             self.close()
         except Exception:
@@ -376,6 +429,14 @@ class Context:
         rc = self.L.elq_build_matrix(self.h, C.byref(g), np.float32(s.dt), *[_ptr(a) for a in out])
         _check_rc(rc, self.h)
         return dict(zip(("diag", "ax", "ay", "az", "b"), out))
+
+    def ic_factor(self, s: Scene, n: int, opts: Opts | None = None):
+        """IC(0)/MIC(0) pivot inverses 1/e (FP64) in the reference's unknown numbering."""
+        out = np.zeros(n, np.float64)
+        g = s.grid()
+        rc = self.L.elq_ic_factor(self.h, C.byref(g), np.float32(s.dt), C.byref(opts) if opts else None, _ptr(out))
+        _check_rc(rc, self.h)
+        return out
 
     def update_velocity_using_pressure(self, s: Scene, pressure: np.ndarray):
         pressure = np.ascontiguousarray(pressure, np.float32)
@@ -440,6 +501,10 @@ class Context:
     def set_max_groups(self, max_groups: int):
         """Caps the persistent kernels' grid (0: whole device)."""
         _check_rc(self.L.elq_set_max_groups(self.h, int(max_groups)), self.h)
+
+    def set_wait_limit(self, seconds: float):
+        """Bound of every device-side wait (default 4 s); see elq_set_wait_limit."""
+        _check_rc(self.L.elq_set_wait_limit(self.h, float(seconds)), self.h)
 
     def set_profiling(self, on: bool, only: str = ""):
         """Per-kernel event timing; only: comma-separated kernel classes to time (all if empty)."""
@@ -521,6 +586,7 @@ class Server:
             raise ElqError(rc, f"elq_server_create: {STATUS_NAMES.get(rc, rc)}")
         self.h = h
         self.live = {}
+        _live_servers.add(self)
 
     def submit(self, s: Scene) -> int:
         p = np.zeros(s.cells, np.float32)
@@ -558,8 +624,11 @@ class Server:
             self.L.elq_server_destroy(self.h)
             self.h = None
             self.live = {}
+        _live_servers.discard(self)
 
     def __del__(self):
+        if _shutting_down:
+            return
         try:
             self.close()
         except Exception:

@@ -129,6 +129,15 @@ void elq_ctx_destroy(elq_ctx* ctx);
 /* Last error message of this context (static storage of the context). */
 const char* elq_ctx_last_error(const elq_ctx* ctx);
 
+/* Bounds every device-side wait of the persistent kernels (a workgroup waiting
+ * for another's face values or partial sums) to `seconds` of device wall
+ * clock; default 4. A wait that runs out raises the context's abort flag,
+ * every other wait then ends at once, and the call returns ELQ_ERR_DEVICE
+ * ("wavefront wait exceeded its time limit"); the next call on the context
+ * starts afresh. No reference counterpart (the reference has no device
+ * waits). 0 ends every wait that is not satisfied at its first check. */
+int elq_set_wait_limit(elq_ctx* ctx, double seconds);
+
 /* Simulator::solve_pressure(dt) on host grids (simulator.cpp:96-112): copies
  * inputs to HBM, projects, copies u/v/w (and pressure/indices) back. */
 int elq_project(elq_ctx* ctx, const elq_grid* host_grid, float dt, const elq_opts* opts,
@@ -152,6 +161,16 @@ int elq_assign_indices(elq_ctx* ctx, const elq_grid* host_grid, int32_t* indices
  * elq_assign_indices). */
 int elq_build_matrix(elq_ctx* ctx, const elq_grid* host_grid, float dt, double* diag, double* ax,
                      double* ay, double* az, double* b);
+
+/* The preconditioner set-up of simulator.cpp:102-107 (solver.compute(A) of
+ * Eigen's IncompleteCholesky<double>), in the form this library factors: the
+ * natural-order IC(0) (opts->precond ELQ_PRECOND_MIC0: MIC(0) with opts->tau;
+ * pivot safeguard opts->sigma) of the matrix elq_build_matrix exports, on
+ * host grids. Writes the pivot inverses 1/e, FP64, in the reference's
+ * unknown numbering (n_unknowns entries, from elq_assign_indices): the
+ * factor M = (E+L) E^{-1} (E+L^T) with L the strict lower triangle of A.
+ * opts NULL: defaults. */
+int elq_ic_factor(elq_ctx* ctx, const elq_grid* host_grid, float dt, const elq_opts* opts, double* inv_e);
 
 /* projection.cpp:93-134 on host grids: u/v/w updated in place from a given
  * pressure grid (grid->pressure is the input here). */

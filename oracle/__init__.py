@@ -247,6 +247,29 @@ def scene_spheres(n: int, seed: int, n_spheres: int = 32, dt: float = 0.01, vel_
     return s
 
 
+def clamp_geometry(n: int):
+    """Slab and sphere of the clamp scene: the slab top lies 1e-5 above the
+    centres of cell row j0 = floor(0.35 n), the sphere (centre at the centre
+    of cell (n/2, n/2, n/2)) 1e-5 beyond the centres of the cells three cells
+    from it on each axis, so the faces from those cells outwards have theta
+    ~ 1e-4 < 0.01 (clamped) on every axis."""
+    dx = np.float32(np.float32(1.0) / np.float32(n))
+    c = np.float32((n // 2 + 0.5) * dx)
+    slab = np.float32((int(0.35 * n) + 0.5) * dx + 1e-5)
+    r = np.float32(3 * dx + 1e-5)
+    return float(slab), [(c, c, c, r)]
+
+
+def scene_clamp(n: int = 16, seed: int = 5, dt: float = 0.01) -> Scene:
+    """Ghost-fluid theta clamp scene (projection.cpp:59-63,100-131): the
+    clamp_geometry level set, face velocities U[-0.5, 0.5], validity + zeroing."""
+    slab, sph = clamp_geometry(n)
+    s = make_scene(n, n, n, slab, sph, dt)
+    random_velocity(s, Rng(seed), 0.5)
+    find_valid_and_zero(s)
+    return s
+
+
 # --------------------------------------------------------------------------- linear system
 @dataclass
 class System:

@@ -1,6 +1,6 @@
 """Writes the golden fixtures of the projection from the CPU oracle.
 
-Usage: python tests/golden/make_golden.py   (run from the repository root)
+Usage: python tests/golden/make_golden.py [name ...]   (run from the repository root)
 
 Each projection fixture holds the inputs of one projection (reference
 layout, float32) and the oracle's outputs: indices, the assembled system in
@@ -29,6 +29,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SCENES = {
     "drop12": lambda: O.scene_drop(12),
     "spheres16": lambda: O.scene_spheres(16, seed=21, n_spheres=5),
+    # interface 1e-5 from cell centres on every axis: ghost-fluid faces with
+    # theta < 0.01, i.e. the clamp of projection.cpp:59-63,100-131
+    "clamp16": lambda: O.scene_clamp(16),
 }
 
 
@@ -71,10 +74,16 @@ def frame_fixture():
 
 
 if __name__ == "__main__":
+    # optional arguments: the fixture names to (re)write; default all
+    want = set(sys.argv[1:])
     for name, make in SCENES.items():
+        if want and name not in want:
+            continue
         path = os.path.join(HERE, f"{name}.npz")
         np.savez_compressed(path, **fixture(make()))
         print("wrote", path, os.path.getsize(path), "bytes")
+    if want and "frame16" not in want:
+        sys.exit(0)
     path = os.path.join(HERE, "frame16.npz")
     np.savez_compressed(path, **frame_fixture())
     print("wrote", path, os.path.getsize(path), "bytes")

@@ -1,8 +1,8 @@
 """The GPU against the committed golden fixtures (tests/golden/*.npz, written
 by the oracle through tests/golden/make_golden.py), through the C ABI:
 
-- projection fixtures: indices, A and b bit for bit (elq_assign_indices,
-  elq_build_matrix), the pressure of elq_project at rel.res 1e-10 within
+- projection fixtures: indices, A, b and the IC(0) pivots bit for bit
+  (elq_assign_indices, elq_build_matrix, elq_ic_factor), the pressure of elq_project at rel.res 1e-10 within
   2e-6 of the fixture's (FP64; mixed 2e-5) relative to max|p|, and the
   updated velocities within that bound times 2 dt / (0.01 dx), the largest
   factor the update applies to a pressure difference;
@@ -43,7 +43,10 @@ def test_projection_fixture(ctx, path):
     np.testing.assert_array_equal(idx, z["indices"])
     m = ctx.build_matrix(s, n)
     for k in ("diag", "ax", "ay", "az", "b"):
-        np.testing.assert_array_equal(m[k], z[k], err_msg=k)
+        np.testing.assert_array_equal(m[k].view(np.uint64), z[k].view(np.uint64), err_msg=k)
+    # IC(0) pivots of the preconditioner set-up (simulator.cpp:102-107), bit for bit
+    inv_e = ctx.ic_factor(s, n)
+    np.testing.assert_array_equal(inv_e.view(np.uint64), z["inv_e"].view(np.uint64))
     scale = np.abs(z["pressure"]).max()
     for prec, tol in ((elq.FP64, 2e-6), (elq.MIXED, 2e-5)):
         t = scene(z)

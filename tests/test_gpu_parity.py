@@ -28,6 +28,9 @@ SCENES = {
     "spheres24": lambda: elq.scene_spheres(24, seed=7, n_spheres=6),
     "aniso": lambda: elq.scene_request(3),
     "odd": lambda: _odd(),
+    # interface 1e-5 from cell centres: ghost-fluid faces with theta < 0.01 on
+    # every axis, so A, b and the update go through the clamp (projection.cpp:59-63,100-131)
+    "clamp16": lambda: elq.scene_clamp(16),
 }
 
 
@@ -60,7 +63,7 @@ def test_build_matrix_bit_exact(ctx, name):
         np.testing.assert_array_equal(m[k].view(np.uint64), getattr(ref, k).view(np.uint64), err_msg=k)
 
 
-@pytest.mark.parametrize("name", ["drop32", "spheres24"])
+@pytest.mark.parametrize("name", ["drop32", "spheres24", "clamp16"])
 def test_update_velocity_bit_exact(ctx, name):
     s = SCENES[name]()
     rng = np.random.default_rng(1)
@@ -85,7 +88,7 @@ def _check_solution(s, p, idx, st, tol_rel):
 
 
 @pytest.mark.parametrize("schedule", [elq.WAVEFRONT, elq.LEVELS])
-@pytest.mark.parametrize("name", ["drop32", "spheres24", "aniso", "odd"])
+@pytest.mark.parametrize("name", ["drop32", "spheres24", "aniso", "odd", "clamp16"])
 def test_projection_fp64(ctx, name, schedule):
     s = SCENES[name]()
     g = s.copy()
@@ -98,7 +101,7 @@ def test_projection_fp64(ctx, name, schedule):
         assert np.abs(getattr(g, f) - getattr(ref, f)).max() <= 2e-6 * umax, f
 
 
-@pytest.mark.parametrize("name", ["drop32", "spheres24", "aniso"])
+@pytest.mark.parametrize("name", ["drop32", "spheres24", "aniso", "clamp16"])
 def test_projection_mixed(ctx, name):
     s = SCENES[name]()
     g = s.copy()

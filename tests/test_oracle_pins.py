@@ -15,6 +15,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import oracle as O
+from scene_checks import clamped_faces
 
 
 # ----------------------------------------------------------------- fraction_inside
@@ -183,6 +184,66 @@ def test_uniform_flow_has_zero_rhs_in_open_cells():
     idx3 = idx.reshape(12, 12, 12)
     for (k, j, i) in [(5, 5, 5), (6, 4, 7)]:
         assert sys_.b[idx3[k, j, i]] == 0.0
+
+
+# ----------------------------------------------------------------- ghost-fluid theta clamp
+# projection.cpp:59-63 (assembly) and :100-131 (update): theta =
+# max(fraction_inside(phi_c, phi_n), 0.01f). An interface that passes close to
+# a cell centre gives theta ~ 1e-4; the clamp turns it into 0.01. The known
+# answers below are worked out by hand from those lines (w = 1, float 1/0.01f
+# rounds to exactly 100): an oracle without the clamp (w/theta ~ 1e4 per
+# face), with min in place of max (the theta = 0.5 face would give 100, not
+# 2) or with another constant fails them.
+def _isolated_cell(axis, up):
+    """8^3 room scene, all air (phi = 1) except cell (3,3,3) at phi = -1e-4;
+    its neighbour across face (axis, up) sits at phi = +1e-4 (theta = 0.5
+    exactly), the other five at phi = 1 (theta ~ 1e-4, clamped)."""
+    n = 8
+    s = O.make_scene(n, n, n, None, [])
+    phi = np.ones((n, n, n), np.float32)          # [k, j, i]
+    phi[3, 3, 3] = np.float32(-1e-4)
+    nb = [3, 3, 3]
+    nb[2 - axis] += 1 if up else -1
+    phi[tuple(nb)] = np.float32(1e-4)
+    s.phi[:] = phi.reshape(-1)
+    return s
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("up", [False, True])
+def test_theta_clamp_known_answer_in_assembly(axis, up):
+    s = _isolated_cell(axis, up)
+    assert s.uw.reshape(8, 8, 9)[3, 3, 3] == 1.0   # room(): interior faces fully open
+    n, idx, sys_ = _system(s)
+    assert n == 1
+    # five clamped faces at w/0.01 = 100 and one at w/0.5 = 2
+    assert sys_.diag[0] == 502.0, sys_.diag[0]
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_theta_clamp_known_answer_in_update(axis):
+    # pressure 1 in the fluid cell, 0 in the air; u -= dt (p_hi - p_lo) / (dx theta):
+    # the -axis face (towards phi = 1, theta clamped to 0.01) moves by
+    # -0.01 * 1 / (0.125 * 0.01) = -8, the +axis face (towards phi = +1e-4,
+    # theta = 0.5) by -0.01 * (0 - 1) / (0.125 * 0.5) = +0.16
+    s = _isolated_cell(axis, True)
+    p = np.zeros(512, np.float32)
+    p[3 + 8 * (3 + 8 * 3)] = 1.0
+    grids = (s.u, s.v, s.w)
+    shapes = ((8, 8, 9), (8, 9, 8), (9, 8, 8))
+    before = [g.copy() for g in grids]
+    O.update_velocity(s, p)
+    g = (grids[axis] - before[axis]).reshape(shapes[axis])
+    lo = [3, 3, 3]
+    hi = [3, 3, 3]
+    hi[2 - axis] += 1                              # face index of the +axis face
+    assert abs(g[tuple(lo)] - (-8.0)) <= 1e-5 * 8.0, g[tuple(lo)]   # -axis face: theta clamped
+    assert abs(g[tuple(hi)] - 0.16) <= 1e-5 * 0.16, g[tuple(hi)]    # +axis face: theta = 0.5
+
+
+def test_clamp_scene_has_clamped_faces_on_every_axis():
+    s = O.scene_clamp(16)
+    assert min(clamped_faces(s)) >= 4, clamped_faces(s)
 
 
 # ----------------------------------------------------------------- preconditioner

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,9 +40,55 @@ double now_ms() {
 
 }  // namespace
 
+// ============================================================ static scratch
+// Workspace of the allocation-free projection (elq_project_device_noalloc):
+// every device buffer of such a call is carved from this array. Each device
+// holds its own copy (ELQ_NOALLOC_SCRATCH_BYTES) from the library's first
+// use on it until the process ends. In host code `scratch` names only a
+// host-side stand-in: the device address comes from hipGetSymbolAddress,
+// looked up once per device when a context is created on it.
+static_assert(ELQ_NOALLOC_SCRATCH_BYTES % 256 == 0, "scratch size");
+static __device__ __attribute__((aligned(256))) uint8_t scratch[ELQ_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint8_t* g_scratch_addr[kMaxDevices] = {};
+hipError_t g_scratch_rc[kMaxDevices] = {};
+bool g_scratch_looked_up[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, which must be current.
+hipError_t scratch_address(int device, uint8_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_looked_up[device]) {
+        void* p = nullptr;
+        g_scratch_rc[device] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        g_scratch_addr[device] = g_scratch_rc[device] == hipSuccess ? (uint8_t*)p : nullptr;
+        g_scratch_looked_up[device] = true;
+    }
+    *out = g_scratch_addr[device];
+    return g_scratch_rc[device];
+}
+
+}  // namespace
+
 // ============================================================ context basics
 hipError_t DevBuf::ensure(size_t bytes) {
     if (bytes <= cap) return hipSuccess;
+    if (arena) {
+        const size_t need = (bytes + 255) / 256 * 256;
+        if (need > arena->size - arena->off) {
+            arena->overflow = true;
+            return hipErrorOutOfMemory;
+        }
+        p = arena->base + arena->off;
+        arena->off += need;
+        cap = bytes;
+        return hipSuccess;
+    }
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -52,6 +99,11 @@ hipError_t DevBuf::ensure(size_t bytes) {
 }
 
 void DevBuf::release() {
+    if (arena) {
+        p = nullptr;
+        cap = 0;
+        return;
+    }
     if (p) (void)hipFree(p);
     p = nullptr;
     cap = 0;
@@ -128,6 +180,9 @@ Plan make_plan(int ni, int nj, int nk) {
     p.NB = (int64_t)p.nbx * p.nby * p.nbz;
     return p;
 }
+
+// Alignment padding of the scratch carve (at most 255 bytes per buffer).
+constexpr size_t kArenaSlack = 64 * 1024;
 
 // Pencil blocks of the wavefront sweeps: the block lists of every shape.
 size_t block_count(const Plan& p) {
@@ -605,6 +660,15 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
         return ctx->fail(ELQ_ERR_BOUNDARY,
                          "an unknown lies on the outer cell layer (the reference reads outside its arrays there)");
     if (W.n_unknowns == 0) return ctx->fail(ELQ_ERR_NO_FLUID, "No fluid");
+    if (W.mask.arena) {
+        // allocation-free call: the whole workspace, now that the active
+        // bricks are known, must fit the scratch array before any more work
+        const size_t need = elq_footprint_bytes(d.ni, d.nj, d.nk, o.precision, false, W.nb) + kArenaSlack;
+        if (need > W.mask.arena->size)
+            return ctx->fail(ELQ_ERR_ADMISSION, "projection workspace (" + std::to_string(need) +
+                                                    " bytes) exceeds the static scratch array (" +
+                                                    std::to_string(W.mask.arena->size) + " bytes)");
+    }
     if ((rc = setup_phase_b<T>(ctx, d, dt, levels))) return rc;
     Solver<T> S;
     S.ctx = ctx;
@@ -802,6 +866,10 @@ int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
         rate_khz = 100000;
     ctx->clock_hz = (uint64_t)rate_khz * 1000ull;
     ctx->timeout_ticks = ctx->clock_hz * 4ull;  // 4 s per wait
+    // the static scratch array of the allocation-free projection; a failed
+    // lookup only disables that entry point
+    if (scratch_address(device, &ctx->scratch) != hipSuccess) ctx->scratch = nullptr;
+    (void)hipGetLastError();
     *out = ctx;
     return ELQ_OK;
 }
@@ -846,6 +914,48 @@ int elq_project_device(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts
     (void)hipSetDevice(ctx->device);
     ctx->kt.clear();
     rc = run_project(ctx, *g, dt, o, st);
+    if (st && rc != ELQ_OK && st->status == 0) st->status = rc;
+    return rc;
+}
+
+int elq_project_device_noalloc(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) {
+    int rc = check_grid(ctx, g);
+    if (rc) return rc;
+    if (!ctx->scratch) return ctx->fail(ELQ_ERR_DEVICE, "static scratch array unavailable on this device");
+    elq_opts o;
+    if (opts)
+        o = *opts;
+    else
+        elq_default_opts(&o);
+    if (st) std::memset(st, 0, sizeof(*st));
+    // dense arrays and brick-grid metadata alone must fit (known from the dims)
+    if (valid_dims(g->ni, g->nj, g->nk) &&
+        elq_footprint_bytes(g->ni, g->nj, g->nk, o.precision, false, 0) + kArenaSlack > ELQ_NOALLOC_SCRATCH_BYTES)
+        return ctx->fail(ELQ_ERR_ADMISSION, "grid too large for the static scratch array (" +
+                                                std::to_string((size_t)ELQ_NOALLOC_SCRATCH_BYTES) + " bytes)");
+    (void)hipSetDevice(ctx->device);
+    ctx->kt.clear();
+    // a fresh carve of the scratch array: every buffer (control block,
+    // exchange and stage words included) is cleared where it must start at zero
+    ctx->arena.base = ctx->scratch;
+    ctx->arena.size = ELQ_NOALLOC_SCRATCH_BYTES;
+    ctx->arena.off = 0;
+    ctx->arena.overflow = false;
+    ctx->ws_scratch.for_each([&](DevBuf& b) {
+        b.p = nullptr;
+        b.cap = 0;
+        b.arena = &ctx->arena;
+    });
+    const bool prof = ctx->profiling;  // event timing creates events on demand: off here
+    ctx->profiling = false;
+    std::swap(ctx->ws, ctx->ws_scratch);
+    rc = run_project(ctx, *g, dt, o, st);
+    std::swap(ctx->ws, ctx->ws_scratch);
+    ctx->profiling = prof;
+    if (rc != ELQ_OK && ctx->arena.overflow) {
+        rc = ELQ_ERR_ADMISSION;
+        ctx->last_error = "projection workspace exceeds the static scratch array";
+    }
     if (st && rc != ELQ_OK && st->status == 0) st->status = rc;
     return rc;
 }

@@ -14,10 +14,22 @@
 // Bytes of the pinned host mirror a context keeps for its polls.
 constexpr size_t kHostMirrorBytes = 4096;
 
-// Device buffer that only grows (hipMalloc on growth, reused otherwise).
+// Bump allocator over a fixed device region (the allocation-free entry
+// point's static scratch array): carving never calls the HIP allocator, and
+// a request beyond the region fails with overflow set.
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t size = 0;
+    size_t off = 0;
+    bool overflow = false;
+};
+
+// Device buffer that only grows (hipMalloc on growth, reused otherwise). With
+// an arena it is carved from the arena instead and never freed.
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    Arena* arena = nullptr;
     hipError_t ensure(size_t bytes);
     void release();
 };
@@ -85,6 +97,12 @@ struct elq_ctx {
     uint64_t timeout_ticks = 0;
     uint64_t clock_hz = 0;  // device wall clock (wall_clock64) rate
     int max_groups = 0;  // cap on the grid of persistent kernels (0: whole device)
+    // allocation-free projection: this device's static scratch array (its
+    // device address, from hipGetSymbolAddress at context creation), the
+    // arena over it and the workspace carved from it for each call
+    uint8_t* scratch = nullptr;
+    Arena arena;
+    Workspace ws_scratch;
     std::string last_error;
     Workspace ws;
 

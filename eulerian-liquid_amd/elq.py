@@ -32,6 +32,7 @@ OK, NO_FLUID, NOT_CONVERGED, BREAKDOWN, DEVICE, ADMISSION, INVALID_ARG, BOUNDARY
 STATUS_NAMES = {0: "OK", 1: "NO_FLUID", 2: "NOT_CONVERGED", 3: "BREAKDOWN", 4: "DEVICE", 5: "ADMISSION",
                 6: "INVALID_ARG", 7: "BOUNDARY", 8: "BUSY"}
 FP64, MIXED = 0, 1
+NOALLOC_SCRATCH_BYTES = 64 << 20  # ELQ_NOALLOC_SCRATCH_BYTES
 IC0, MIC0 = 0, 1
 WAVEFRONT, LEVELS = 0, 1
 
@@ -96,6 +97,7 @@ def load():
     L.elq_ctx_last_error.restype = C.c_char_p
     L.elq_project.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(Opts), C.POINTER(Stats)]
     L.elq_project_device.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(Opts), C.POINTER(Stats)]
+    L.elq_project_device_noalloc.argtypes = [vp, C.POINTER(Grid), f32, C.POINTER(Opts), C.POINTER(Stats)]
     L.elq_assign_indices.argtypes = [vp, C.POINTER(Grid), vp, C.POINTER(i64)]
     L.elq_build_matrix.argtypes = [vp, C.POINTER(Grid), f32, vp, vp, vp, vp, vp]
     L.elq_update_velocity.argtypes = [vp, C.POINTER(Grid), f32]
@@ -413,6 +415,14 @@ class Context:
         st = Stats()
         rc = self.L.elq_project_device(self.h, C.byref(g), np.float32(dt), C.byref(opts) if opts else None,
                                        C.byref(st))
+        return rc, st
+
+    def project_device_noalloc(self, g: Grid, dt: float, opts: Opts | None = None) -> tuple[int, Stats]:
+        """elq_project_device_noalloc: the same projection with its whole
+        workspace in the library's static scratch array (no device allocation)."""
+        st = Stats()
+        rc = self.L.elq_project_device_noalloc(self.h, C.byref(g), np.float32(dt), C.byref(opts) if opts else None,
+                                               C.byref(st))
         return rc, st
 
     def assign_indices(self, s: Scene):

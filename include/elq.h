@@ -149,6 +149,24 @@ int elq_project(elq_ctx* ctx, const elq_grid* host_grid, float dt, const elq_opt
 int elq_project_device(elq_ctx* ctx, const elq_grid* device_grid, float dt, const elq_opts* opts,
                        elq_stats* stats);
 
+/* Allocation-free variant of elq_project_device. No reference counterpart:
+ * it is for callers that cannot allocate device memory while they run (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It calls no hipMalloc, hipFree or other allocating routine;
+ * every device buffer of the projection (mask, indices, bricks, system,
+ * factor, Krylov vectors, exchange words, control block) is carved from one
+ * static device array of ELQ_NOALLOC_SCRATCH_BYTES per device, held from the
+ * library's first use on that device until the process ends. A grid whose
+ * workspace does not fit is refused with ELQ_ERR_ADMISSION before any work
+ * (roughly up to 64^3 cells, or larger grids with fewer liquid bricks).
+ * Outputs are the same as elq_project_device's for the same inputs and
+ * context. Every call on a device shares that device's array: calls that
+ * overlap on one device (several contexts or threads) race and must be
+ * serialised by the caller; each call clears what must start at zero. */
+#define ELQ_NOALLOC_SCRATCH_BYTES (64u << 20)
+int elq_project_device_noalloc(elq_ctx* ctx, const elq_grid* device_grid, float dt, const elq_opts* opts,
+                               elq_stats* stats);
+
 /* projection.cpp:14-28 on host grids: writes indices (ELQ_UNKNOWN_INDEX for
  * non-unknowns) and *n_unknowns. */
 int elq_assign_indices(elq_ctx* ctx, const elq_grid* host_grid, int32_t* indices,

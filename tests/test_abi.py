@@ -62,3 +62,29 @@ def test_request_generator_is_anisotropic_and_seeded():
     _same(a, b)
     dims = {tuple(elq.scene_request(s).meta["workload"].split("_")[-1].split("x")) for s in range(20)}
     assert len(dims) > 5
+
+
+def test_scratch_array_is_an_exported_device_symbol(tmp_path):
+    """The allocation-free entry point finds its static scratch array with
+    hipGetSymbolAddress, which the runtime resolves by name among the symbols
+    the code object exports. A static __device__ variable is exported only if
+    the compiler externalises it (a host reference does that; marking it
+    __attribute__((used)) kept it local, and the runtime aborted in the
+    lookup), so the built code object must list it as a global dynamic symbol
+    of the full size."""
+    import shutil
+    import subprocess
+    obj = os.path.join(ROOT, "eulerian-liquid_amd", "_build", "elq_api.o")
+    tools = "/opt/rocm/llvm/bin"
+    if not os.path.exists(obj) or not os.path.exists(os.path.join(tools, "clang-offload-bundler")):
+        pytest.skip("build tree or ROCm tools absent")
+    fat, dev = str(tmp_path / "fat.bin"), str(tmp_path / "dev.co")
+    subprocess.run([os.path.join(tools, "llvm-objcopy"), f"--dump-section=.hip_fatbin={fat}", obj, str(tmp_path / "o")],
+                   check=True)
+    subprocess.run([os.path.join(tools, "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fat}",
+                    "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={dev}"], check=True)
+    out = subprocess.run([os.path.join(tools, "llvm-readelf"), "--dyn-syms", dev], check=True, capture_output=True,
+                         text=True).stdout
+    rows = [l.split() for l in out.splitlines() if "scratch" in l]
+    assert any(r[2] == str(elq.NOALLOC_SCRATCH_BYTES) and r[4] == "GLOBAL" for r in rows), out
+    shutil.rmtree(tmp_path, ignore_errors=True)

@@ -10,9 +10,11 @@ pressure scatter, velocity update) on inputs already resident in HBM.
 
 Prints one JSON line: solves/s, the dominant kernel's roofline (HIP events on
 the library's stream, algorithmic bytes per launch), the FP64 GPU drop-in on
-the same inputs, max|p - p_FP64|, the largest weighted divergence left in a
-fluid cell relative to the input's largest (both precisions), and the CPU
-oracle timed on a bounded sample.
+the same inputs, max|p - p_FP64| and max|p - p_oracle|, the largest weighted
+divergence left in a fluid cell relative to the input's largest (both
+precisions), the serving stream, the other configurations (`configs`: 64^3
+and 128^3 against the oracle's 1e-12 solve, 512^3 against the FP64 drop-in),
+and the CPU oracle timed on full projections (`cpu_baseline`).
 """
 import argparse
 import json
@@ -78,14 +80,35 @@ def weighted_divergence(s, u, v, w):
     return (((uu[:, :, 1:] - uu[:, :, :-1]) + (vv[:, 1:, :] - vv[:, :-1, :]) + (ww[1:] - ww[:-1])) / s.dx).reshape(-1)
 
 
+def max_divergence(s, u, v, w, mask=None, slab=32):
+    """Largest |weighted divergence| over the cells (where mask, if given),
+    computed in k slabs so that large grids need little extra host memory."""
+    ni, nj, nk = s.ni, s.nj, s.nk
+    uu, vv, ww = u.reshape(nk, nj, ni + 1), v.reshape(nk, nj + 1, ni), w.reshape(nk + 1, nj, ni)
+    uw, vw, wwt = s.uw.reshape(nk, nj, ni + 1), s.vw.reshape(nk, nj + 1, ni), s.ww.reshape(nk + 1, nj, ni)
+    m = mask.reshape(nk, nj, ni) if mask is not None else None
+    best = 0.0
+    for k0 in range(0, nk, slab):
+        k1 = min(nk, k0 + slab)
+        a = (uu[k0:k1] * uw[k0:k1]).astype(np.float64)
+        b = (vv[k0:k1] * vw[k0:k1]).astype(np.float64)
+        c = (ww[k0:k1 + 1] * wwt[k0:k1 + 1]).astype(np.float64)
+        d = np.abs(((a[:, :, 1:] - a[:, :, :-1]) + (b[:, 1:, :] - b[:, :-1, :]) + (c[1:] - c[:-1])) / s.dx)
+        if m is not None:
+            d = d[m[k0:k1]]
+        if d.size:
+            best = max(best, float(d.max()))
+    return best
+
+
 def fluid_divergence(ds):
     """After the last projection on ds: the largest |weighted divergence| over
     the fluid cells (those with an index), relative to the largest over all
     cells of the input velocity."""
     s = ds.s
     fluid = ds.fetch("indices", np.int32) != elq.UNKNOWN_INDEX
-    before = float(np.abs(weighted_divergence(s, s.u, s.v, s.w)).max())
-    after = float(np.abs(weighted_divergence(s, ds.fetch("u"), ds.fetch("v"), ds.fetch("w"))[fluid]).max())
+    before = max_divergence(s, s.u, s.v, s.w)
+    after = max_divergence(s, ds.fetch("u"), ds.fetch("v"), ds.fetch("w"), fluid)
     return after / before if before else 0.0
 
 
@@ -274,28 +297,172 @@ def frame_steps(ctx, ds, reps=5):
     return out
 
 
-def cpu_baseline(scene, fp64_iters):
-    """Oracle (FP64 restatement, single thread) on the same inputs: one timed
-    call with a small and one with a larger iteration cap give setup time and
-    time per PCG iteration; the projection time is extrapolated to the
-    iteration count the same algorithm needs on this workload."""
+def whole_solve_frac(cells, n, iters, outer, ms, mixed):
+    """Algorithmic bytes of a whole projection / its time, as a fraction of
+    the HBM peak: iterations x unknowns x B_iter, plus the refinement steps
+    (mixed) and the set-up (indices, assembly, factor, scatter, update)."""
+    sm = sv = 4 if mixed else 8
+    total = iters * n * B_ITER(sm, sv) + (outer * n * B_OUTER if mixed else 0) + 92 * cells + n * (9 * sm + 16)
+    return round(total / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4)
+
+
+def gpu_config(ctx, ds, opts, reps):
+    """One warm-up projection, then reps timed ones (median); the outputs of
+    the last one."""
+    project(ctx, ds, opts)
+    ts, st = [], None
+    for _ in range(reps):
+        dt_ms, st = project(ctx, ds, opts)
+        ts.append(dt_ms)
+    ms = float(np.median(ts))
+    mixed = opts.precision == elq.MIXED
+    e = {"ms_per_solve": round(ms, 2), "iterations": st.iterations, "outer_iterations": st.outer_iterations,
+         "true_rel_residual": st.true_rel_residual, "max_fluid_divergence_rel": fluid_divergence(ds),
+         "hbm_frac_whole_solve": whole_solve_frac(ds.s.cells, st.n_unknowns, st.iterations, st.outer_iterations,
+                                                  ms, mixed)}
+    return e, ds.fetch("pressure"), st
+
+
+def configs(base, seed, threads):
+    """The other configurations of the workload, on the GPU (both
+    precisions), each with its error: cfg1 (64^3 drop) and cfg2 (128^3 drop)
+    against the oracle's FP64 solve to rel.res 1e-12, cfg1 also at the
+    reference's solver defaults (tol 1e-4, 1000 iterations) against the
+    oracle's iteration count and error(); cfg5 (512^3, 128 spheres, 48.7 M
+    unknowns) against the FP64 GPU drop-in (the oracle would need about five
+    minutes of CPU there; its parity is shown at the smaller sizes)."""
     import oracle as O
-    os_ = O.Scene(scene.ni, scene.nj, scene.nk, scene.dx, scene.dt, scene.phi, scene.u.copy(), scene.v.copy(),
-                  scene.w.copy(), scene.uw, scene.vw, scene.ww)
-    m1, m2 = 8, 200  # about 15 s of CPU work at 256^3
-    t = []
-    for m in (m1, m2):
+    O.set_threads(threads, False)
+    out = {}
+    ctx = elq.Context(0)
+    for name, sc in (("cfg1_drop64", elq.scene_drop(64)), ("cfg2_drop128", elq.scene_drop(128))):
+        ref = O.solve_pressure(oracle_scene(sc), tol=1e-12, max_iter=20000)
+        scale = float(np.abs(ref.pressure).max())
+        ds = elq.DeviceScene(ctx, sc)
+        e = {"workload": f"{name[5:]}: slab y<0.4 + sphere r=0.15, one gravity step, rel.res 1e-8",
+             "unknowns": ref.n, "oracle": f"FP64 IC(0)-PCG to rel.res 1e-12 ({ref.iterations} iterations)"}
+        ps = {}
+        for key, prec in (("fp64", elq.FP64), ("mixed", elq.MIXED)):
+            e[key], ps[key], _ = gpu_config(ctx, ds, elq.default_opts(precision=prec, **base), 3)
+            e[key]["max_rel_p_minus_p_oracle"] = float(np.abs(ps[key] - ref.pressure).max()) / scale
+        e["speedup_mixed_vs_fp64"] = round(e["fp64"]["ms_per_solve"] / e["mixed"]["ms_per_solve"], 3)
+        e["max_rel_p_mixed_minus_p_fp64"] = float(np.abs(ps["mixed"] - ps["fp64"]).max()) / scale
+        if name.startswith("cfg1"):
+            _, st = project(ctx, ds, elq.default_opts())
+            ref4 = O.solve_pressure(oracle_scene(sc), tol=1e-4, max_iter=1000)
+            e["reference_defaults_tol1e-4"] = {"gpu_iterations": st.iterations, "gpu_error": st.error,
+                                               "oracle_iterations": ref4.iterations, "oracle_error": ref4.error}
+        ds.close()
+        out[name] = e
+        print(f"# {name}: fp64 {e['fp64']['ms_per_solve']} ms, mixed {e['mixed']['ms_per_solve']} ms, "
+              f"err {e['fp64']['max_rel_p_minus_p_oracle']:.2e}/{e['mixed']['max_rel_p_minus_p_oracle']:.2e}",
+              file=sys.stderr, flush=True)
+    ctx.close()
+    t0 = time.perf_counter()
+    sc = elq.scene_spheres(512, seed=seed, n_spheres=128)
+    print(f"# cfg5: 512^3 scene generated in {time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctx = elq.Context(0)
+    ds = elq.DeviceScene(ctx, sc)
+    e = {"workload": f"spheres512 seed {seed}: slab y<0.3 + 128 spheres, velocities U[-0.5,0.5], rel.res 1e-8"}
+    ps = {}
+    for key, prec in (("mixed", elq.MIXED), ("fp64", elq.FP64)):
+        e[key], ps[key], st = gpu_config(ctx, ds, elq.default_opts(precision=prec, **base), 1)
+        e["unknowns"] = st.n_unknowns
+        print(f"# cfg5 {key}: {e[key]['ms_per_solve']} ms, {e[key]['iterations']} iterations", file=sys.stderr,
+              flush=True)
+    scale = float(np.abs(ps["fp64"]).max())
+    e["speedup_mixed_vs_fp64"] = round(e["fp64"]["ms_per_solve"] / e["mixed"]["ms_per_solve"], 3)
+    e["max_rel_p_mixed_minus_p_fp64"] = float(np.abs(ps["mixed"] - ps["fp64"]).max()) / scale
+    e["error_reference"] = "FP64 GPU drop-in (oracle parity shown at cfg1/cfg2 and the golden fixtures)"
+    ds.close()
+    ctx.close()
+    out["cfg5_spheres512"] = e
+    return out
+
+
+def host_info():
+    """The host the CPU baseline runs on: CPU model, logical CPUs, the CPUs
+    this process may use, and the OpenMP threads it is given."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0))
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+    return {"cpu_model": model, "nproc": os.cpu_count(), "cpus_available": affinity, "omp_threads": threads}
+
+
+def oracle_scene(scene):
+    import oracle as O
+    return O.Scene(scene.ni, scene.nj, scene.nk, scene.dx, scene.dt, scene.phi, scene.u.copy(), scene.v.copy(),
+                   scene.w.copy(), scene.uw, scene.vw, scene.ww)
+
+
+def time_oracle(scene, reps, threads, krylov_parallel=False, tol=1e-8):
+    """Full oracle projections (indices, assembly, IC(0) factor, PCG to tol,
+    scatter, update) of scene: one warm-up, then the median of reps."""
+    import oracle as O
+    os_ = oracle_scene(scene)
+    O.set_threads(threads, krylov_parallel)
+    ts, res = [], None
+    for q in range(reps + 1):
         t0 = time.perf_counter()
-        O.solve_pressure(os_, tol=1e-8, max_iter=m)
-        t.append(time.perf_counter() - t0)
-    per_iter = (t[1] - t[0]) / (m2 - m1)
-    setup = max(t[0] - m1 * per_iter, 0.0)
-    est = setup + per_iter * fp64_iters
-    return {"value": round(1.0 / est, 5), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"oracle projection on the full workload capped at {m1} and {m2} PCG iterations "
-                      f"({t[0]:.1f}s, {t[1]:.1f}s): setup {setup:.2f}s + {per_iter * 1e3:.1f} ms/iteration "
-                      f"x {fp64_iters} iterations (FP64 IC(0) count of the GPU drop-in)",
-            "est_s_per_solve": round(est, 2)}
+        res = O.solve_pressure(os_, tol=tol, max_iter=20000)
+        if q:
+            ts.append(time.perf_counter() - t0)
+    O.set_threads(threads, False)
+    return float(np.median(ts)), res
+
+
+def cpu_baseline(scene, threads):
+    """The reference path on the host's cores: the oracle (the FP64 CPU
+    restatement of projection.cpp + Eigen's CG loop, natural-order IC(0);
+    built -O3 -DNDEBUG -fopenmp like the reference's Release build) timed on
+    full projections to rel.res 1e-8, with the reference's threading: serial
+    assembly, factorisation and PCG, OpenMP over the pressure scatter and the
+    velocity update (projection.cpp:85-134).
+    - value: one full projection of the benchmark workload (256^3), OpenMP
+      threads = the CPUs the process may use;
+    - cfg1 (64^3 drop) and cfg2 (128^3 drop): median of 5 after a warm-up,
+      with 1 thread and with all threads;
+    - companion (labelled, not the reference's threading): cfg2 with OpenMP
+      SpMV, axpys and dots as well (the IC sweeps stay serial)."""
+    info = host_info()
+    out = {"host": info}
+    cfgs = {}
+    for name, sc, reps in (("cfg1_drop64", elq.scene_drop(64), 5), ("cfg2_drop128", elq.scene_drop(128), 5)):
+        e = {}
+        for th in sorted({1, threads}):
+            t, res = time_oracle(sc, reps, th)
+            e[f"threads_{th}"] = {"s_per_solve": round(t, 4), "solves_per_s": round(1.0 / t, 3)}
+            e["iterations"] = res.iterations
+            print(f"# cpu {name} threads {th}: {t:.3f} s ({res.iterations} iterations)", file=sys.stderr, flush=True)
+        cfgs[name] = e
+    t, res = time_oracle(elq.scene_drop(128), 3, threads, krylov_parallel=True)
+    cfgs["cfg2_drop128"]["companion_openmp_krylov"] = {"threads": threads, "s_per_solve": round(t, 4),
+                                                       "note": "OpenMP SpMV/axpy/dot added; not the reference's "
+                                                               "threading"}
+    import oracle as O
+    O.set_threads(threads, False)
+    os_ = oracle_scene(scene)
+    t0 = time.perf_counter()
+    res = O.solve_pressure(os_, tol=1e-8, max_iter=20000)
+    t = time.perf_counter() - t0
+    print(f"# cpu workload: {t:.1f} s ({res.iterations} iterations)", file=sys.stderr, flush=True)
+    out["_pressure"] = res.pressure
+    out.update({"value": round(1.0 / t, 5), "unit": "solves/s", "cores": threads, "kind": "port",
+                "sample": f"one full oracle projection of the benchmark workload to rel.res 1e-8 "
+                          f"({res.iterations} IC(0)-PCG iterations, {t:.1f} s) with {threads} OpenMP threads on "
+                          f"{info['cpu_model']}; serial assembly/factor/PCG as in the reference, OpenMP scatter "
+                          f"and update",
+                "s_per_solve": round(t, 3), "iterations": res.iterations, "true_rel_residual": res.true_rel_res,
+                "configs": cfgs})
+    return out
 
 
 def cpu_baseline_frame(ctx):
@@ -443,6 +610,8 @@ def main():
     ap.add_argument("--serve", type=int, default=512, metavar="N",
                     help="serving stream of N requests after the single-solve benchmark (0: skip)")
     ap.add_argument("--slots", type=int, default=4, help="serving slots")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the other configurations (64^3, 128^3 against the oracle; 512^3)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel on MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -565,9 +734,19 @@ def main():
     ctx.close()
     if a.serve > 0:
         line["serving"] = serving(a.serve, a.slots, a.seed, base)
+    threads = host_info()["omp_threads"]
+    if not a.no_configs:
+        line["configs"] = configs(base, a.seed, threads)
+        line["configs"]["cfg3_" + wname] = "the headline: value, config, roofline, fp64_drop_in"
+        line["configs"]["cfg4_serving"] = "serving" if a.serve > 0 else "not run (--serve 0)"
     if not a.no_cpu:
-        fp64_iters = out["fp64"][1][-1]["iterations"] if "fp64" in out else iters
-        line["cpu_baseline"] = cpu_baseline(scene, fp64_iters)
+        line["cpu_baseline"] = cpu_baseline(scene, threads)
+        p_oracle = line["cpu_baseline"].pop("_pressure")
+        scale = float(np.abs(p_oracle).max())
+        if "mixed" in out:
+            line["config"]["max_rel_p_minus_p_oracle"] = float(np.abs(p_mixed - p_oracle).max()) / scale
+        if "fp64" in out and "fp64_drop_in" in line:
+            line["fp64_drop_in"]["max_rel_p_minus_p_oracle"] = float(np.abs(p_fp64 - p_oracle).max()) / scale
         if not a.no_extra:
             ctx2 = elq.Context(0)
             line["cpu_baseline"]["frame_steps"] = cpu_baseline_frame(ctx2)

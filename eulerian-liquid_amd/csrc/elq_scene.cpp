@@ -14,11 +14,32 @@
 #include <cmath>
 #include <cstdint>
 #include <limits>
+#include <thread>
 #include <vector>
 
 #include "../../include/elq.h"
 
 namespace {
+
+// Runs f(k) for k in [0, nk) on up to 16 host threads, contiguous k ranges
+// per thread; each cell's value depends only on its own inputs, so the
+// result does not depend on the split.
+template <class F>
+void parallel_k(int nk, F f) {
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int nt = std::max(1, std::min({hw > 0 ? hw : 1, 16, nk / 8 > 0 ? nk / 8 : 1}));
+    if (nt == 1) {
+        for (int k = 0; k < nk; ++k) f(k);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([=] {
+            for (int k = (int)((int64_t)nk * t / nt); k < (int)((int64_t)nk * (t + 1) / nt); ++k) f(k);
+        });
+    for (auto& x : th) x.join();
+}
+
 
 inline size_t at(int i, int j, int k, int ni, int nj) {
     return (size_t)i + (size_t)ni * ((size_t)j + (size_t)nj * (size_t)k);
@@ -104,7 +125,7 @@ double elq_uniform01(uint64_t* state) { return state ? unit(*state) : 0.0; }
 int elq_scene_level_set(int32_t ni, int32_t nj, int32_t nk, float dx, float slab_height, int32_t n_spheres,
                         const float* spheres, float* level_set) {
     if (ni < 1 || nj < 1 || nk < 1 || !level_set || (n_spheres > 0 && !spheres)) return ELQ_ERR_INVALID_ARG;
-    for (int k = 0; k < nk; ++k)
+    parallel_k(nk, [&](int k) {
         for (int j = 0; j < nj; ++j)
             for (int i = 0; i < ni; ++i) {
                 // Grid3::pos: (i + 0.5f, j + 0.5f, k + 0.5f) * dx + origin (0)
@@ -119,6 +140,7 @@ int elq_scene_level_set(int32_t ni, int32_t nj, int32_t nk, float dx, float slab
                 if (slab_height > -1e30f) phi = std::min(phi, pos[1] - slab_height);
                 level_set[at(i, j, k, ni, nj)] = phi;
             }
+    });
     return ELQ_OK;
 }
 

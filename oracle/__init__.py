@@ -84,12 +84,31 @@ def lib():
         L.orc_solve_pressure.argtypes = [C.c_int, C.c_int, C.c_int, C.c_float, C.c_float] + [_f32p] * 7 + \
             [C.c_double, C.c_int, C.c_double, C.c_double, _i32p, _f32p,
              C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
+        L.orc_max_threads.restype = C.c_int
+        L.orc_set_krylov_parallel.argtypes = [C.c_int]
+        L.orc_set_krylov_parallel.restype = None
         L.orc_splitmix64.restype = C.c_uint64
         L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
         L.orc_uniform01.restype = C.c_double
         L.orc_uniform01.argtypes = [C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
+
+
+# --------------------------------------------------------------------------- threads
+def set_threads(n: int, krylov_parallel: bool = False):
+    """OpenMP threads of the oracle: the scatter and update loops run in
+    parallel as in the reference (projection.cpp:85-134); with krylov_parallel
+    also the SpMV, axpys and dots (a timing companion; the IC sweeps and the
+    assembly stay serial)."""
+    lib().orc_set_threads(int(n))
+    lib().orc_set_krylov_parallel(1 if krylov_parallel else 0)
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
 
 
 # --------------------------------------------------------------------------- scalar helpers

@@ -12,6 +12,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -775,7 +776,21 @@ void orc_ic_apply(const orc_system* s, const double* inv_e, const double* r, dou
     }
 }
 
+/* OpenMP threads of the loops the reference runs in parallel: the pressure
+ * scatter and the velocity update (projection.cpp:85-134 through
+ * parallel_loop, array.h:452-472, collapse(3)). Assembly, the factorisation
+ * and the PCG stay serial as in the reference (serial_loop; Eigen's CG with a
+ * Lower self-adjoint view is single-threaded) unless orc_set_krylov_parallel
+ * turns on the OpenMP companion of the SpMV, axpys and dots (a timing
+ * companion, not the reference's threading; the IC sweeps stay serial). */
+static int g_krylov_parallel = 0;
+
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
+void orc_set_krylov_parallel(int on) { g_krylov_parallel = on; }
+
 void orc_spmv(const orc_system* s, const double* x, double* y) {
+#pragma omp parallel for if (g_krylov_parallel) schedule(static)
     for (int n = 0; n < s->n; ++n) {
         double acc = s->diag[n] * x[n];
         if (s->xm[n] >= 0) acc += s->ax[s->xm[n]] * x[s->xm[n]];
@@ -790,6 +805,11 @@ void orc_spmv(const orc_system* s, const double* x, double* y) {
 
 static double dot(int n, const double* a, const double* b) {
     double s = 0.0;
+    if (g_krylov_parallel) {
+#pragma omp parallel for reduction(+ : s) schedule(static)
+        for (int i = 0; i < n; ++i) s += a[i] * b[i];
+        return s;
+    }
     for (int i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
 }
@@ -840,6 +860,7 @@ int orc_pcg(const orc_system* s, const double* inv_e, double tol, int max_iter, 
         while (it < max_iter) {
             orc_spmv(s, p, q);
             const double alpha = abs_new / dot(n, p, q);
+#pragma omp parallel for if (g_krylov_parallel) schedule(static)
             for (int i = 0; i < n; ++i) {
                 x[i] += alpha * p[i];
                 r[i] -= alpha * q[i];
@@ -850,6 +871,7 @@ int orc_pcg(const orc_system* s, const double* inv_e, double tol, int max_iter, 
             const double abs_old = abs_new;
             abs_new = dot(n, r, z);
             const double beta = abs_new / abs_old;
+#pragma omp parallel for if (g_krylov_parallel) schedule(static)
             for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
             ++it;
         }
@@ -867,9 +889,13 @@ done:
 /* projection.cpp:85-91 */
 void orc_build_grid_from_vector(int ni, int nj, int nk, const double* x, const int32_t* indices,
                                 float background, float* pressure) {
-    const size_t n = (size_t)ni * nj * nk;
-    for (size_t c = 0; c < n; ++c)
-        pressure[c] = indices[c] == ORC_UNKNOWN_INDEX ? background : (float)x[indices[c]];
+#pragma omp parallel for collapse(3) schedule(static)
+    for (int k = 0; k < nk; ++k)
+        for (int j = 0; j < nj; ++j)
+            for (int i = 0; i < ni; ++i) {
+                const size_t c = IDX(i, j, k, ni, nj);
+                pressure[c] = indices[c] == ORC_UNKNOWN_INDEX ? background : (float)x[indices[c]];
+            }
 }
 
 /* projection.cpp:93-134. Ranges {1, n_face - 1} per axis of each face grid. */
@@ -877,6 +903,7 @@ void orc_update_velocity(int ni, int nj, int nk, float dx, float dt, const float
                          const float* phi, float* u, float* v, float* w) {
 #define P(i, j, k) phi[IDX(i, j, k, ni, nj)]
 #define PR(i, j, k) pressure[IDX(i, j, k, ni, nj)]
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 1; k < nk - 1; ++k)
         for (int j = 1; j < nj - 1; ++j)
             for (int i = 1; i < ni; ++i)
@@ -888,6 +915,7 @@ void orc_update_velocity(int ni, int nj, int nk, float dx, float dt, const float
                     float* f = &u[IDX(i, j, k, ni + 1, nj)];
                     *f = *f - dt * (PR(i, j, k) - PR(i - 1, j, k)) / (dx * theta);
                 }
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 1; k < nk - 1; ++k)
         for (int j = 1; j < nj; ++j)
             for (int i = 1; i < ni - 1; ++i)
@@ -899,6 +927,7 @@ void orc_update_velocity(int ni, int nj, int nk, float dx, float dt, const float
                     float* f = &v[IDX(i, j, k, ni, nj + 1)];
                     *f = *f - dt * (PR(i, j, k) - PR(i, j - 1, k)) / (dx * theta);
                 }
+#pragma omp parallel for collapse(3) schedule(static)
     for (int k = 1; k < nk; ++k)
         for (int j = 1; j < nj - 1; ++j)
             for (int i = 1; i < ni - 1; ++i)

@@ -147,6 +147,11 @@ void orc_ic_apply(const orc_system* sys, const double* inv_e, const double* r, d
 
 /* y = A x */
 void orc_spmv(const orc_system* sys, const double* x, double* y);
+/* OpenMP threads (the scatter and update loops run in parallel, as in the
+ * reference); the Krylov companion parallelises SpMV, axpys and dots. */
+void orc_set_threads(int n);
+int orc_max_threads(void);
+void orc_set_krylov_parallel(int on);
 
 /* Preconditioned CG with Eigen's ConjugateGradient semantics (x0 = 0,
  * recursive residual, stop when |r|^2 < max(tol^2 |b|^2, DBL_MIN)).

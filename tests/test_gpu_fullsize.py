@@ -10,6 +10,11 @@ that do not need a CPU solve of the same size, and at 128^3 against the oracle.
   pressure and update).
 - 128^3 drop scene: pressure within 2e-6 max|p| of the oracle's FP64 solve to
   relative residual 1e-12 (mixed: 2e-5).
+- 512^3 slab + 128 spheres (about 48.7 M unknowns, past what a test can solve
+  on the CPU): the same properties as at 256^3 on device-resident grids, and
+  both precisions converge within the reference's 1000 iterations
+  (simulator.cpp:105); cell and unknown indices stay below 2^31 (the
+  reference's int32 guard, array.h:322-325).
 """
 import numpy as np
 import pytest
@@ -66,3 +71,50 @@ def test_128_against_oracle(ctx, drop128, precision, tol):
     assert st.true_rel_residual <= 1e-8
     np.testing.assert_array_equal(idx, ref.indices)
     assert np.abs(p - ref.pressure).max() <= tol * np.abs(ref.pressure).max()
+
+
+def max_fluid_divergence(s, u, v, w, fluid=None, slab=32):
+    """Largest |weighted divergence| (over fluid cells if given), in k slabs."""
+    ni, nj, nk = s.ni, s.nj, s.nk
+    uu, vv, ww = u.reshape(nk, nj, ni + 1), v.reshape(nk, nj + 1, ni), w.reshape(nk + 1, nj, ni)
+    uw, vw, wwt = s.uw.reshape(nk, nj, ni + 1), s.vw.reshape(nk, nj + 1, ni), s.ww.reshape(nk + 1, nj, ni)
+    best = 0.0
+    for k0 in range(0, nk, slab):
+        k1 = min(nk, k0 + slab)
+        a = (uu[k0:k1] * uw[k0:k1]).astype(np.float64)
+        b = (vv[k0:k1] * vw[k0:k1]).astype(np.float64)
+        c = (ww[k0:k1 + 1] * wwt[k0:k1 + 1]).astype(np.float64)
+        d = np.abs(((a[:, :, 1:] - a[:, :, :-1]) + (b[:, 1:, :] - b[:, :-1, :]) + (c[1:] - c[:-1])) / s.dx)
+        if fluid is not None:
+            d = d[fluid.reshape(nk, nj, ni)[k0:k1]]
+        if d.size:
+            best = max(best, float(d.max()))
+    return best
+
+
+def test_512_mixed_and_fp64():
+    s = elq.scene_spheres(512, seed=1, n_spheres=128)
+    div0 = max_fluid_divergence(s, s.u, s.v, s.w)
+    c = elq.Context(0)
+    ds = elq.DeviceScene(c, s)
+    try:
+        out = {}
+        for prec in (elq.MIXED, elq.FP64):
+            print(f"512^3 precision={prec}", flush=True)
+            ds.reset_velocity()
+            rc, st = c.project_device(ds.grid(), s.dt, elq.default_opts(tol=1e-8, max_iter=1000, precision=prec,
+                                                                        check_every=32))
+            assert rc == elq.OK and st.converged, (rc, c.last_error(), st.as_dict())
+            assert st.true_rel_residual <= 1e-8, st.true_rel_residual
+            assert 40_000_000 < st.n_unknowns < 2 ** 31 and 300 < st.iterations <= 1000, st.as_dict()
+            idx = ds.fetch("indices", np.int32)
+            fluid = idx != elq.UNKNOWN_INDEX
+            assert int(fluid.sum()) == st.n_unknowns and int(idx[fluid].max()) == st.n_unknowns - 1
+            div = max_fluid_divergence(s, ds.fetch("u"), ds.fetch("v"), ds.fetch("w"), fluid)
+            assert div <= 1e-4 * div0, div / div0
+            out[prec] = ds.fetch("pressure")
+        scale = np.abs(out[elq.FP64]).max()
+        assert np.abs(out[elq.MIXED] - out[elq.FP64]).max() <= 1e-5 * scale
+    finally:
+        ds.close()
+        c.close()

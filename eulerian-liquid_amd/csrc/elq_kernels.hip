@@ -1592,7 +1592,11 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
     const int w = (int)(threadIdx.x >> 6), dy = w % BY, dz = w / BY;
     const bool gy = dy == 0, gz = dz == 0;                    // -y/-z neighbour outside the block
     const bool out_y = dy == BY - 1, out_z = dz == BZ - 1;   // +y/+z consumer outside the block
-    const bool publish = out_y || out_z;
+    // FP64: only the cells other workgroups read are published (the +y face,
+    // ly = 3, and the +z face, lz = 3, where their consumer is outside the
+    // block); FP32: the whole brick, one word per lane (measured faster at
+    // one word per value, slower at two)
+    const bool publish = sizeof(T) == 8 ? (out_y && L.ly == 3) || (out_z && L.lz == 3) : out_y || out_z;
     uint64_t* ring_y = S.ring[2 * w];                          // this wave's +y face, to wave (dy+1, dz)
     uint64_t* ring_z = S.ring[2 * w + 1];                      // this wave's +z face, to wave (dy, dz+1)
     const uint64_t* in_y = gy ? S.ring[0] : S.ring[2 * (w - 1)];
@@ -1675,7 +1679,8 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
     const int w = (int)(threadIdx.x >> 6), dy = w % BY, dz = w / BY;
     const bool gy = dy == BY - 1, gz = dz == BZ - 1;   // +y/+z neighbour outside the block
     const bool out_y = dy == 0, out_z = dz == 0;       // -y/-z consumer outside the block
-    const bool publish = out_y || out_z;
+    // the faces other workgroups read (FP64), the whole brick (FP32), as in k_fwd_block
+    const bool publish = sizeof(T) == 8 ? (out_y && L.ly == 0) || (out_z && L.lz == 0) : out_y || out_z;
     uint64_t* ring_y = S.ring[2 * w];                   // this wave's -y face, to wave (dy-1, dz)
     uint64_t* ring_z = S.ring[2 * w + 1];               // this wave's -z face, to wave (dy, dz-1)
     const uint64_t* in_y = gy ? S.ring[0] : S.ring[2 * (w + 1)];

@@ -109,6 +109,12 @@ void DevBuf::release() {
     cap = 0;
 }
 
+void elq_ctx::drop_chunk_graph() {
+    if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+    chunk_exec = nullptr;
+    chunk_key.clear();
+}
+
 int elq_ctx::fail(int code, const std::string& msg) {
     last_error = msg;
     return code;
@@ -540,8 +546,48 @@ struct Solver {
         return bwd(rz_mode);
     }
 
-    // n PCG iterations (Eigen's loop body), no host synchronisation.
+    // n PCG iterations (Eigen's loop body), no host synchronisation. With
+    // graph capture on (wavefront schedule, no event timing), the chunk of
+    // 3n launches is captured once into a HIP graph and replayed while every
+    // argument of those launches stays the same (the key below); the graph
+    // holds kernel nodes only.
     int iterations(int n) {
+        if (levels || !ctx->graphs || ctx->profiling) return iterations_direct(n);
+        Workspace& W = ctx->ws;
+        const std::vector<uintptr_t> key = {
+            (uintptr_t)n, (uintptr_t)(kiter & 1), sizeof(T), (uintptr_t)ctx->max_groups, (uintptr_t)ctx->timeout_ticks,
+            (uintptr_t)bk.nbx, (uintptr_t)bk.nby, (uintptr_t)bk.nbz, (uintptr_t)bk.nb, (uintptr_t)bk.nlev,
+            (uintptr_t)bk.map, (uintptr_t)bk.coord, (uintptr_t)bk.nbr, (uintptr_t)bk.lvl_list, (uintptr_t)pc.np,
+            (uintptr_t)pc.shape, (uintptr_t)pc.nblk, (uintptr_t)pc.blk_list, (uintptr_t)c.diag, (uintptr_t)c.ax,
+            (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
+            (uintptr_t)W.vp1.p, (uintptr_t)W.vq.p, (uintptr_t)W.vr.p, (uintptr_t)W.vx.p, (uintptr_t)prr(),
+            (uintptr_t)prz(), (uintptr_t)ctrl(), (uintptr_t)W.xch.p, (uintptr_t)stage()};
+        if (!ctx->chunk_exec || key != ctx->chunk_key) {
+            ctx->drop_chunk_graph();
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+            const int64_t k0 = kiter;
+            const int rc = iterations_direct(n);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            kiter = k0;  // the captured launches have not run
+            if (rc != ELQ_OK || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc != ELQ_OK ? rc : ctx->hip_fail(e, "hipStreamEndCapture");
+            }
+            const hipError_t ei = hipGraphInstantiate(&ctx->chunk_exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                ctx->chunk_exec = nullptr;
+                return ctx->hip_fail(ei, "hipGraphInstantiate");
+            }
+            ctx->chunk_key = key;
+        }
+        HIPCHK(hipGraphLaunch(ctx->chunk_exec, ctx->stream), "hipGraphLaunch");
+        kiter += n;
+        return ELQ_OK;
+    }
+
+    int iterations_direct(int n) {
         Workspace& W = ctx->ws;
         int rc;
         T* P[2] = {vec(W.vp0), vec(W.vp1)};
@@ -878,6 +924,7 @@ void elq_ctx_destroy(elq_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->drop_chunk_graph();
     ctx->ws.release_all();
     for (auto ev : ctx->events) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(ctx->stream);
@@ -886,6 +933,14 @@ void elq_ctx_destroy(elq_ctx* ctx) {
 }
 
 const char* elq_ctx_last_error(const elq_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int elq_set_graph_capture(elq_ctx* ctx, int enable) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    if (!enable) ctx->drop_chunk_graph();
+    ctx->graphs = enable != 0;
+    return ELQ_OK;
+}
 
 int elq_set_wait_limit(elq_ctx* ctx, double seconds) {
     if (!ctx) return ELQ_ERR_INVALID_ARG;

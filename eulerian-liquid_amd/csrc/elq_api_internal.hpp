@@ -103,6 +103,12 @@ struct elq_ctx {
     uint8_t* scratch = nullptr;
     Arena arena;
     Workspace ws_scratch;
+    // Krylov chunks replayed from a captured HIP graph while their launch
+    // arguments (chunk_key) stay the same
+    bool graphs = false;
+    hipGraphExec_t chunk_exec = nullptr;
+    std::vector<uintptr_t> chunk_key;
+    void drop_chunk_graph();
     std::string last_error;
     Workspace ws;
 

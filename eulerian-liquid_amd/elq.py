@@ -128,6 +128,7 @@ def load():
     L.elq_set_profiling.restype = None
     L.elq_set_max_groups.argtypes = [vp, C.c_int]
     L.elq_set_wait_limit.argtypes = [vp, f64]
+    L.elq_set_graph_capture.argtypes = [vp, C.c_int]
     L.elq_server_create.argtypes = [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]
     L.elq_server_destroy.argtypes = [vp]
     L.elq_server_destroy.restype = None
@@ -511,6 +512,10 @@ class Context:
     def set_max_groups(self, max_groups: int):
         """Caps the persistent kernels' grid (0: whole device)."""
         _check_rc(self.L.elq_set_max_groups(self.h, int(max_groups)), self.h)
+
+    def set_graph_capture(self, on: bool):
+        """Graph replay of the Krylov loop's poll intervals (default on)."""
+        _check_rc(self.L.elq_set_graph_capture(self.h, 1 if on else 0), self.h)
 
     def set_wait_limit(self, seconds: float):
         """Bound of every device-side wait (default 4 s); see elq_set_wait_limit."""

@@ -129,6 +129,14 @@ void elq_ctx_destroy(elq_ctx* ctx);
 /* Last error message of this context (static storage of the context). */
 const char* elq_ctx_last_error(const elq_ctx* ctx);
 
+/* Enables (1) or disables (0, the default) graph replay of the Krylov loop:
+ * the launches of one poll interval (check_every iterations: SpMV, forward
+ * and backward sweep each) are captured once into a HIP graph and replayed
+ * while their arguments stay the same, which takes the per-launch host cost
+ * off the loop. Results are identical either way. Not used with the levels
+ * schedule or while per-kernel event timing is on. No reference counterpart. */
+int elq_set_graph_capture(elq_ctx* ctx, int enable);
+
 /* Bounds every device-side wait of the persistent kernels (a workgroup waiting
  * for another's face values or partial sums) to `seconds` of device wall
  * clock; default 4. A wait that runs out raises the context's abort flag,

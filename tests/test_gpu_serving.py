@@ -7,6 +7,10 @@
   solve tolerance (the slot's grid cap changes only the reduction order):
   max|p - p_single| <= 2e-5 max|p_single| (mixed) and true FP64 relative
   residual <= 1e-8.
+- Small requests (every axis <= 48) served in a mix match the oracle's FP64
+  solve of the same scene to rel.res 1e-12: the same unknown count, pressure
+  within 2e-6 (FP64) / 2e-5 (mixed) of max|p|, velocities within that bound
+  times dt / (0.01 dx), the largest factor the update applies.
 - Admission never lets the slots together hold more device memory than the
   budget; a request that alone exceeds it is refused at submit.
 - The footprint model is an upper bound of what a projection allocates.
@@ -83,6 +87,31 @@ def test_results_match_single_context(ctx):
         assert r["n_unknowns"] == st.n_unknowns
         scale = np.abs(p_ref).max()
         assert np.abs(r["pressure"] - p_ref).max() <= 2e-5 * scale
+
+
+@pytest.mark.parametrize("precision, tol", [(elq.FP64, 2e-6), (elq.MIXED, 2e-5)])
+def test_small_requests_match_oracle(precision, tol):
+    import oracle as O
+    seeds = [418, 429, 437]
+    scenes = requests(seeds)
+    assert all(max(sc.ni, sc.nj, sc.nk) <= 48 for sc in scenes.values())
+    mix = {k: v.copy() for k, v in scenes.items()}
+    mix.update({k: v for k, v in requests([104, 105]).items()})   # larger requests in the same mix
+    with elq.Server(0, max_slots=3, opts=opts(precision)) as srv:
+        got = run(srv, mix)
+    for sd in seeds:
+        r, sc = got[sd], scenes[sd]
+        assert r["status"] == elq.OK and r["true_rel_residual"] <= TOL, r
+        ref = O.solve_pressure(O.Scene(sc.ni, sc.nj, sc.nk, sc.dx, sc.dt, sc.phi.copy(), sc.u.copy(), sc.v.copy(),
+                                       sc.w.copy(), sc.uw.copy(), sc.vw.copy(), sc.ww.copy()), tol=1e-12,
+                               max_iter=20000)
+        assert r["n_unknowns"] == ref.n
+        scale = np.abs(ref.pressure).max()
+        assert np.abs(r["pressure"] - ref.pressure).max() <= tol * scale
+        served = r["scene"]
+        for f in ("u", "v", "w"):
+            bound = tol * scale * sc.dt / (sc.dx * 0.01) + 1e-6 * np.abs(getattr(ref, f)).max()
+            assert np.abs(getattr(served, f) - getattr(ref, f)).max() <= bound, f
 
 
 def test_admission_respects_budget():

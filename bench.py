@@ -297,6 +297,53 @@ def frame_steps(ctx, ds, reps=5):
     return out
 
 
+def shim_frame(n=256, frames=4):
+    """One frame of the reference's loop through the C++ Simulator shim
+    (elq_sample3d: extrapolate_velocity, extrapolate_level_set, advect_level_set,
+    advect_velocity, reinitialize, add_gravity, find_valid_velocity,
+    set_invalid_velocity_zero, solve_pressure) on the n^3 drop scene (slab
+    y < 0.4 plus a sphere of radius 0.15), mixed precision to rel.res 1e-8,
+    grids resident in device memory for the whole run. frame_ms: host wall
+    clock of a frame (median of frames 1.., frame 0 warms up); device_sum_ms:
+    a second run with HIP-event timing of every step's kernels, the sum over
+    the frame (median of frames 1..), which is what a device-resident frame
+    costs without any host work."""
+    import subprocess
+    exe = os.path.join(ROOT, "eulerian-liquid_amd", "elq_sample3d")
+
+    def run(extra):
+        r = subprocess.run([exe, str(n), str(frames), "1e-8", "1"] + extra, capture_output=True, text=True,
+                           timeout=300)
+        if r.returncode != 0:
+            raise SystemExit(f"elq_sample3d failed: {r.stderr.strip()}")
+        return r.stdout.splitlines()
+
+    def steps(line):   # "<label>: name=ms name=ms ..."
+        return {k: float(v) for k, v in (kv.split("=") for kv in line.split(":", 1)[1].split())}
+
+    out = run([])
+    fl = [ln for ln in out if ln.startswith("frame")]
+    wall = [ln for ln in out if ln.strip().startswith("wall_ms:")]
+    frame_ms = [float(ln.split("frame_ms=")[1]) for ln in fl]
+    iters = [int(ln.split("iterations=")[1].split()[0]) for ln in fl]
+    walls = [steps(ln) for ln in wall]
+    prof = run(["--profile"])
+    dev = [ln.strip() for ln in prof if ln.strip().startswith("device_ms")]
+    dev_sum = [float(ln.split("(sum ")[1].split(")")[0]) for ln in dev]
+    dev_steps = [steps(ln.split(")", 1)[1]) for ln in dev]
+    med = lambda xs: float(np.median(xs[1:] if len(xs) > 1 else xs))
+    names = list(walls[0].keys())
+    e = {"workload": f"drop{n} (slab y<0.4 + sphere r=0.15), mixed, rel.res 1e-8",
+         "frames": len(fl), "frame_ms": round(med(frame_ms), 3),
+         "iterations": iters,
+         "wall_ms": {k: round(med([w[k] for w in walls]), 3) for k in names},
+         "device_sum_ms": round(med(dev_sum), 3),
+         "device_ms": {k: round(med([d[k] for d in dev_steps]), 3) for k in names}}
+    e["steps_outside_projection_ms"] = round(e["frame_ms"] - e["wall_ms"]["solve_pressure"], 3)
+    e["frame_over_device_sum"] = round(e["frame_ms"] / e["device_sum_ms"], 4)
+    return e
+
+
 def whole_solve_frac(cells, n, iters, outer, ms, mixed):
     """Algorithmic bytes of a whole projection / its time, as a fraction of
     the HBM peak: iterations x unknowns x B_iter, plus the refinement steps
@@ -731,6 +778,8 @@ def main():
             t_host.append((time.perf_counter() - t0) * 1e3)
         line["host_buffers"] = {"ms_per_solve": round(min(t_host), 2), "solves_per_s": round(1e3 / min(t_host), 3),
                                 "note": "elq_project on pageable host arrays, copies included; not the headline"}
+    if not a.no_extra:
+        line["shim_frame"] = shim_frame()
     ctx.close()
     if a.serve > 0:
         line["serving"] = serving(a.serve, a.slots, a.seed, base)

@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/elq.h"
@@ -123,6 +124,96 @@ int elq_ctx::fail(int code, const std::string& msg) {
 int elq_ctx::hip_fail(hipError_t e, const char* what) {
     last_error = std::string(what) + ": " + hipGetErrorString(e);
     return ELQ_ERR_DEVICE;
+}
+
+// Splits [0, n) over HostStage::kThreads threads (the caller takes one part).
+static void copy_parallel(void* dst, const void* src, size_t n) {
+    constexpr int T = HostStage::kThreads;
+    if (n < (1u << 20)) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = (n / T + 4095) & ~(size_t)4095;
+    std::thread th[T - 1];
+    int started = 0;
+    for (int t = 1; t < T && (size_t)t * per < n; ++t, ++started) {
+        const size_t a = (size_t)t * per, b = std::min(n, a + per);
+        th[t - 1] = std::thread([=] { std::memcpy((uint8_t*)dst + a, (const uint8_t*)src + a, b - a); });
+    }
+    std::memcpy(dst, src, std::min(n, per));
+    for (int t = 0; t < started; ++t) th[t].join();
+}
+
+hipError_t HostStage::ready() {
+    if (pinned) return hipSuccess;
+    hipError_t e = hipHostMalloc((void**)&pinned, kSlots * kChunk, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        pinned = nullptr;
+        return e;
+    }
+    for (int q = 0; q < kSlots; ++q) {
+        if ((e = hipEventCreateWithFlags(&ev[q], hipEventDisableTiming)) != hipSuccess) return e;
+        recorded[q] = false;
+    }
+    return hipSuccess;
+}
+
+hipError_t HostStage::h2d(void* dev, const void* host, size_t bytes, hipStream_t s) {
+    if (bytes < (1u << 20)) return hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, s);
+    hipError_t e = ready();
+    if (e != hipSuccess) return e;
+    for (size_t o = 0; o < bytes; o += kChunk) {
+        const int q = (int)(next++ % kSlots);
+        const size_t n = std::min(kChunk, bytes - o);
+        // the chunk's previous copy out of this slot must have completed
+        if (recorded[q] && (e = hipEventSynchronize(ev[q])) != hipSuccess) return e;
+        copy_parallel(pinned + q * kChunk, (const uint8_t*)host + o, n);
+        if ((e = hipMemcpyAsync((uint8_t*)dev + o, pinned + q * kChunk, n, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(ev[q], s)) != hipSuccess) return e;
+        recorded[q] = true;
+    }
+    return hipSuccess;
+}
+
+hipError_t HostStage::d2h(const std::vector<Copy>& copies, hipStream_t s) {
+    hipError_t e = ready();
+    if (e != hipSuccess) return e;
+    struct Piece {
+        uint8_t* dst;
+        const uint8_t* src;
+        size_t n;
+    };
+    std::vector<Piece> pcs;
+    for (const Copy& c : copies)
+        for (size_t o = 0; o < c.bytes; o += kChunk)
+            pcs.push_back({(uint8_t*)c.dst + o, (const uint8_t*)c.src + o, std::min(kChunk, c.bytes - o)});
+    // pieces go through slot (j % kSlots); work already queued on s (and any
+    // earlier copy through a slot) precedes them in stream order
+    auto issue = [&](size_t j) -> hipError_t {
+        const int q = (int)(j % kSlots);
+        hipError_t r = hipMemcpyAsync(pinned + q * kChunk, pcs[j].src, pcs[j].n, hipMemcpyDeviceToHost, s);
+        if (r == hipSuccess) r = hipEventRecord(ev[q], s);
+        recorded[q] = r == hipSuccess;
+        return r;
+    };
+    for (size_t j = 0; j < pcs.size() && j < (size_t)kSlots; ++j)
+        if ((e = issue(j)) != hipSuccess) return e;
+    for (size_t j = 0; j < pcs.size(); ++j) {
+        const int q = (int)(j % kSlots);
+        if ((e = hipEventSynchronize(ev[q])) != hipSuccess) return e;
+        copy_parallel(pcs[j].dst, pinned + q * kChunk, pcs[j].n);
+        if (j + kSlots < pcs.size() && (e = issue(j + kSlots)) != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(s);
+}
+
+void HostStage::release() {
+    if (!pinned) return;
+    for (int q = 0; q < kSlots; ++q)
+        if (ev[q]) (void)hipEventDestroy(ev[q]);
+    (void)hipHostFree(pinned);
+    pinned = nullptr;
 }
 
 void elq_ctx::prof_begin(const char* name) {
@@ -826,13 +917,14 @@ int stage_inputs(elq_ctx* ctx, const elq_grid& g, elq_grid& dg) {
     HIPCHK(W.s_vw.ensure(z.v * 4), "alloc");
     HIPCHK(W.s_ww.ensure(z.w * 4), "alloc");
     hipStream_t s = ctx->stream;
-    HIPCHK(hipMemcpyAsync(W.s_phi.p, g.level_set, z.c * 4, hipMemcpyHostToDevice, s), "h2d");
-    HIPCHK(hipMemcpyAsync(W.s_u.p, g.u, z.u * 4, hipMemcpyHostToDevice, s), "h2d");
-    HIPCHK(hipMemcpyAsync(W.s_v.p, g.v, z.v * 4, hipMemcpyHostToDevice, s), "h2d");
-    HIPCHK(hipMemcpyAsync(W.s_w.p, g.w, z.w * 4, hipMemcpyHostToDevice, s), "h2d");
-    HIPCHK(hipMemcpyAsync(W.s_uw.p, g.u_weight, z.u * 4, hipMemcpyHostToDevice, s), "h2d");
-    HIPCHK(hipMemcpyAsync(W.s_vw.p, g.v_weight, z.v * 4, hipMemcpyHostToDevice, s), "h2d");
-    HIPCHK(hipMemcpyAsync(W.s_ww.p, g.w_weight, z.w * 4, hipMemcpyHostToDevice, s), "h2d");
+    HostStage& H = ctx->stage;
+    HIPCHK(H.h2d(W.s_phi.p, g.level_set, z.c * 4, s), "h2d");
+    HIPCHK(H.h2d(W.s_uw.p, g.u_weight, z.u * 4, s), "h2d");
+    HIPCHK(H.h2d(W.s_vw.p, g.v_weight, z.v * 4, s), "h2d");
+    HIPCHK(H.h2d(W.s_ww.p, g.w_weight, z.w * 4, s), "h2d");
+    HIPCHK(H.h2d(W.s_u.p, g.u, z.u * 4, s), "h2d");
+    HIPCHK(H.h2d(W.s_v.p, g.v, z.v * 4, s), "h2d");
+    HIPCHK(H.h2d(W.s_w.p, g.w, z.w * 4, s), "h2d");
     dg = g;
     dg.level_set = (const float*)W.s_phi.p;
     dg.u = (float*)W.s_u.p;
@@ -924,6 +1016,7 @@ void elq_ctx_destroy(elq_ctx* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    ctx->stage.release();
     ctx->drop_chunk_graph();
     ctx->ws.release_all();
     for (auto ev : ctx->events) (void)hipEventDestroy(ev);
@@ -1050,14 +1143,11 @@ int elq_project(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts,
         return rc;
     }
     const Sizes z = sizes_of(g->ni, g->nj, g->nk);
-    hipStream_t s = ctx->stream;
-    if (hipMemcpyAsync(g->u, dg.u, z.u * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(g->v, dg.v, z.v * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(g->w, dg.w, z.w * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        (g->pressure && hipMemcpyAsync(g->pressure, dg.pressure, z.c * 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-        (g->indices && hipMemcpyAsync(g->indices, dg.indices, z.c * 4, hipMemcpyDeviceToHost, s) != hipSuccess) ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return ctx->fail(ELQ_ERR_DEVICE, "copy back");
+    std::vector<HostStage::Copy> back = {{g->u, dg.u, z.u * 4}, {g->v, dg.v, z.v * 4}, {g->w, dg.w, z.w * 4}};
+    if (g->pressure) back.push_back({g->pressure, dg.pressure, z.c * 4});
+    if (g->indices) back.push_back({g->indices, dg.indices, z.c * 4});
+    const hipError_t e = ctx->stage.d2h(back, ctx->stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "copy back");
     return rc;
 }
 
@@ -1167,13 +1257,10 @@ int elq_update_velocity(elq_ctx* ctx, const elq_grid* g, float dt) {
     const Sizes z = sizes_of(g->ni, g->nj, g->nk);
     Workspace& W = ctx->ws;
     HIPCHK(W.s_pressure.ensure(z.c * 4), "alloc");
-    HIPCHK(hipMemcpyAsync(W.s_pressure.p, g->pressure, z.c * 4, hipMemcpyHostToDevice, ctx->stream), "h2d");
+    HIPCHK(ctx->stage.h2d(W.s_pressure.p, g->pressure, z.c * 4, ctx->stream), "h2d");
     HIPCHK(launch_update(dense_of(dg), dt, (const float*)W.s_pressure.p, ctx->stream), "k_update");
-    hipStream_t s = ctx->stream;
-    HIPCHK(hipMemcpyAsync(g->u, dg.u, z.u * 4, hipMemcpyDeviceToHost, s), "d2h");
-    HIPCHK(hipMemcpyAsync(g->v, dg.v, z.v * 4, hipMemcpyDeviceToHost, s), "d2h");
-    HIPCHK(hipMemcpyAsync(g->w, dg.w, z.w * 4, hipMemcpyDeviceToHost, s), "d2h");
-    HIPCHK(hipStreamSynchronize(s), "sync");
+    HIPCHK(ctx->stage.d2h({{g->u, dg.u, z.u * 4}, {g->v, dg.v, z.v * 4}, {g->w, dg.w, z.w * 4}}, ctx->stream),
+           "d2h");
     return ELQ_OK;
 }
 

@@ -89,6 +89,31 @@ struct Workspace {
     }
 };
 
+// Copies between pageable host arrays and device memory through a ring of
+// pinned chunks: the host moves chunk q with a few threads while the DMA
+// engine moves its neighbour, so a copy runs near the pinned PCIe rate rather
+// than the runtime's pageable path. The ring is allocated on first use.
+struct HostStage {
+    static constexpr size_t kChunk = 16u << 20;
+    static constexpr int kSlots = 3;
+    static constexpr int kThreads = 4;
+    uint8_t* pinned = nullptr;  // kSlots * kChunk bytes (hipHostMalloc)
+    hipEvent_t ev[kSlots] = {};
+    bool recorded[kSlots] = {};
+    unsigned next = 0;
+    struct Copy {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    hipError_t ready();
+    // enqueues host -> device copies on s; returns once the host side is done
+    hipError_t h2d(void* dev, const void* host, size_t bytes, hipStream_t s);
+    // device -> host copies after the work already on s; returns when complete
+    hipError_t d2h(const std::vector<Copy>& copies, hipStream_t s);
+    void release();
+};
+
 struct elq_ctx {
     int device = 0;
     size_t budget = 0;
@@ -111,6 +136,7 @@ struct elq_ctx {
     void drop_chunk_graph();
     std::string last_error;
     Workspace ws;
+    HostStage stage;  // host-buffer entry points
 
     // per-kernel-class event timing
     bool profiling = false;

@@ -210,6 +210,51 @@ void reinitialize_fast_marching(Device& dev, Grid3f& grid, const Grid3c& valid, 
 
 }  // namespace fluid
 
+void DeviceArray::allocate(elq_ctx* ctx, size_t bytes) {
+    reset();
+    m_ctx = ctx;
+    check(ctx, elq_device_alloc(ctx, bytes < 4 ? 4 : bytes, &m_p));
+    m_bytes = bytes;
+}
+
+void DeviceArray::reset() {
+    if (m_p) elq_device_free(m_ctx, m_p);
+    m_p = nullptr;
+    m_bytes = 0;
+}
+
+namespace {
+
+template <class G>
+void h2d(elq_ctx* c, DeviceArray& d, const G& g) {
+    check(c, elq_memcpy_h2d(c, d.as<void>(), g.data.data(), bytes_of(g)));
+}
+
+template <class G>
+void d2h(elq_ctx* c, G& g, const DeviceArray& d) {
+    check(c, elq_memcpy_d2h(c, g.data.data(), d.as<void>(), bytes_of(g)));
+}
+
+}  // namespace
+
+elq_grid Simulator::device_grid() {
+    elq_grid g{};
+    g.ni = m_ni;
+    g.nj = m_nj;
+    g.nk = m_nk;
+    g.dx = m_dx;
+    g.level_set = d_phi.as<float>();
+    g.u = d_u.as<float>();
+    g.v = d_v.as<float>();
+    g.w = d_w.as<float>();
+    g.u_weight = d_uw.as<float>();
+    g.v_weight = d_vw.as<float>();
+    g.w_weight = d_ww.as<float>();
+    g.pressure = d_p.as<float>();
+    g.indices = d_idx.as<int32_t>();
+    return g;
+}
+
 void Simulator::create_grid(float width, int ni, int nj, int nk) {
     m_ni = ni;
     m_nj = nj;
@@ -222,6 +267,30 @@ void Simulator::create_grid(float width, int ni, int nj, int nk) {
     m_pressure.create(ni, nj, nk);
     m_velocity_valid.create(ni, nj, nk, 1);  // simulator.cpp:59,63: created with value 1
     m_level_set_valid.create(ni, nj, nk, 1);
+    elq_ctx* c = m_dev.ctx();
+    const size_t cells = (size_t)ni * nj * nk;
+    const size_t fu = (size_t)(ni + 1) * nj * nk, fv = (size_t)ni * (nj + 1) * nk, fw = (size_t)ni * nj * (nk + 1);
+    d_phi.allocate(c, cells * 4);
+    d_p.allocate(c, cells * 4);
+    d_idx.allocate(c, cells * 4);
+    d_lsv.allocate(c, cells);
+    d_u.allocate(c, fu * 4);
+    d_v.allocate(c, fv * 4);
+    d_w.allocate(c, fw * 4);
+    d_uw.allocate(c, fu * 4);
+    d_vw.allocate(c, fv * 4);
+    d_ww.allocate(c, fw * 4);
+    d_fu.allocate(c, fu);
+    d_fv.allocate(c, fv);
+    d_fw.allocate(c, fw);
+    d_nodal.allocate(c, (size_t)(ni + 1) * (nj + 1) * (nk + 1) * 4);
+    h2d(c, d_u, m_velocity.u());
+    h2d(c, d_v, m_velocity.v());
+    h2d(c, d_w, m_velocity.w());
+    h2d(c, d_fu, m_velocity_valid.c[0]);
+    h2d(c, d_fv, m_velocity_valid.c[1]);
+    h2d(c, d_fw, m_velocity_valid.c[2]);
+    h2d(c, d_lsv, m_level_set_valid);
 }
 
 void Simulator::init_scene(float pool_height, const std::vector<std::array<float, 4>>& spheres) {
@@ -230,119 +299,122 @@ void Simulator::init_scene(float pool_height, const std::vector<std::array<float
     const int rc = elq_scene_level_set(m_ni, m_nj, m_nk, m_dx, pool_height, (int32_t)spheres.size(),
                                        flat.empty() ? nullptr : flat.data(), m_level_set.data.data());
     if (rc != ELQ_OK) throw std::runtime_error("elq: invalid scene");
+    h2d(m_dev.ctx(), d_phi, m_level_set);
 }
 
 void Simulator::compute_weights() {
-    const int rc = elq_scene_weights(m_ni, m_nj, m_nk, m_velocity_weight.u().data.data(),
-                                     m_velocity_weight.v().data.data(), m_velocity_weight.w().data.data());
-    if (rc != ELQ_OK) throw std::runtime_error("elq: invalid grid");
+    // the room() solid on the nodal grid, once; the weights are computed on the
+    // device from it and stay there (mirrored to the host member once)
+    elq_ctx* c = m_dev.ctx();
+    std::vector<float> nodal((size_t)(m_ni + 1) * (m_nj + 1) * (m_nk + 1));
+    check(c, elq_scene_room_nodal(m_ni, m_nj, m_nk, nodal.data()));
+    check(c, elq_memcpy_h2d(c, d_nodal.as<void>(), nodal.data(), nodal.size() * sizeof(float)));
+    check(c, elq_compute_weights_device(c, m_ni, m_nj, m_nk, d_nodal.as<float>(), d_uw.as<float>(), d_vw.as<float>(),
+                                        d_ww.as<float>()));
+    d2h(c, m_velocity_weight.u(), d_uw);
+    d2h(c, m_velocity_weight.v(), d_vw);
+    d2h(c, m_velocity_weight.w(), d_ww);
 }
 
 void Simulator::add_gravity(float dt) {
-    const int rc = elq_scene_add_gravity(m_ni, m_nj, m_nk, m_velocity.v().data.data(), dt);
-    if (rc != ELQ_OK) throw std::runtime_error("elq: invalid grid");
+    elq_ctx* c = m_dev.ctx();
+    check(c, elq_add_gravity_device(c, m_ni, m_nj, m_nk, d_v.as<float>(), dt));
 }
 
 void Simulator::find_valid_velocity_and_zero_invalid() {
-    elq_grid g = make_grid(m_ni, m_nj, m_nk, m_dx, m_level_set, m_velocity, m_velocity_weight);
-    const int rc = elq_scene_valid_and_zero(&g);
-    if (rc != ELQ_OK) throw std::runtime_error("elq: invalid grid");
+    find_valid_velocity();
+    set_invalid_velocity_zero();
 }
 
 void Simulator::find_valid_velocity() {
     elq_ctx* c = m_dev.ctx();
-    DevArray phi(c, m_level_set.data), uw(c, m_velocity_weight.u().data), vw(c, m_velocity_weight.v().data),
-        ww(c, m_velocity_weight.w().data);
-    DevArray u(c, m_velocity.u().data), v(c, m_velocity.v().data), w(c, m_velocity.w().data);
-    m_velocity_valid.create(m_ni, m_nj, m_nk);
-    DevArray fu(c, m_velocity_valid.c[0].data.size()), fv(c, m_velocity_valid.c[1].data.size()),
-        fw(c, m_velocity_valid.c[2].data.size());
-    elq_grid g{};
-    g.ni = m_ni;
-    g.nj = m_nj;
-    g.nk = m_nk;
-    g.dx = m_dx;
-    g.level_set = phi.as<float>();
-    g.u = u.as<float>();
-    g.v = v.as<float>();
-    g.w = w.as<float>();
-    g.u_weight = uw.as<float>();
-    g.v_weight = vw.as<float>();
-    g.w_weight = ww.as<float>();
-    check(c, elq_find_valid_velocity_device(c, &g, fu.as<uint8_t>(), fv.as<uint8_t>(), fw.as<uint8_t>()));
-    fu.fetch(m_velocity_valid.c[0].data);
-    fv.fetch(m_velocity_valid.c[1].data);
-    fw.fetch(m_velocity_valid.c[2].data);
+    elq_grid g = device_grid();
+    check(c, elq_find_valid_velocity_device(c, &g, d_fu.as<uint8_t>(), d_fv.as<uint8_t>(), d_fw.as<uint8_t>()));
 }
 
 void Simulator::set_invalid_velocity_zero() {
     elq_ctx* c = m_dev.ctx();
-    DevArray u(c, m_velocity.u().data), v(c, m_velocity.v().data), w(c, m_velocity.w().data);
-    DevArray fu(c, m_velocity_valid.c[0].data), fv(c, m_velocity_valid.c[1].data), fw(c, m_velocity_valid.c[2].data);
-    elq_grid g{};
-    g.ni = m_ni;
-    g.nj = m_nj;
-    g.nk = m_nk;
-    g.u = u.as<float>();
-    g.v = v.as<float>();
-    g.w = w.as<float>();
-    check(c, elq_set_invalid_velocity_zero_device(c, &g, fu.as<uint8_t>(), fv.as<uint8_t>(), fw.as<uint8_t>()));
-    u.fetch(m_velocity.u().data);
-    v.fetch(m_velocity.v().data);
-    w.fetch(m_velocity.w().data);
+    elq_grid g = device_grid();
+    check(c, elq_set_invalid_velocity_zero_device(c, &g, d_fu.as<uint8_t>(), d_fv.as<uint8_t>(), d_fw.as<uint8_t>()));
 }
 
 void Simulator::extrapolate_velocity() {
     elq_ctx* c = m_dev.ctx();
-    DevArray u(c, m_velocity.u().data), v(c, m_velocity.v().data), w(c, m_velocity.w().data);
-    DevArray fu(c, m_velocity_valid.c[0].data), fv(c, m_velocity_valid.c[1].data), fw(c, m_velocity_valid.c[2].data);
-    elq_grid g{};
-    g.ni = m_ni;
-    g.nj = m_nj;
-    g.nk = m_nk;
-    g.u = u.as<float>();
-    g.v = v.as<float>();
-    g.w = w.as<float>();
-    check(c, elq_extrapolate_velocity_device(c, &g, fu.as<uint8_t>(), fv.as<uint8_t>(), fw.as<uint8_t>(), 10));
-    u.fetch(m_velocity.u().data);
-    v.fetch(m_velocity.v().data);
-    w.fetch(m_velocity.w().data);
-    fu.fetch(m_velocity_valid.c[0].data);
-    fv.fetch(m_velocity_valid.c[1].data);
-    fw.fetch(m_velocity_valid.c[2].data);
+    elq_grid g = device_grid();
+    check(c, elq_extrapolate_velocity_device(c, &g, d_fu.as<uint8_t>(), d_fv.as<uint8_t>(), d_fw.as<uint8_t>(), 10));
 }
 
 void Simulator::find_valid_level_set() {
     elq_ctx* c = m_dev.ctx();
-    std::vector<float> nodal((size_t)(m_ni + 1) * (m_nj + 1) * (m_nk + 1));
-    check(c, elq_scene_room_nodal(m_ni, m_nj, m_nk, nodal.data()));
-    DevArray n(c, nodal);
-    m_level_set_valid.create(m_ni, m_nj, m_nk);
-    DevArray f(c, m_level_set_valid.data.size());
-    check(c, elq_find_valid_level_set_device(c, m_ni, m_nj, m_nk, n.as<float>(), f.as<uint8_t>()));
-    f.fetch(m_level_set_valid.data);
+    check(c, elq_find_valid_level_set_device(c, m_ni, m_nj, m_nk, d_nodal.as<float>(), d_lsv.as<uint8_t>()));
 }
 
-void Simulator::extrapolate_level_set() { fluid::extrapolate_const(m_dev, m_level_set, m_level_set_valid, 10); }
+void Simulator::extrapolate_level_set() {
+    elq_ctx* c = m_dev.ctx();
+    check(c, elq_extrapolate_const_device(c, m_ni, m_nj, m_nk, d_phi.as<float>(), d_lsv.as<uint8_t>(), 10));
+}
 
 void Simulator::advect_velocity(float dt) {
-    const StaggeredGrid3f temp = m_velocity;  // m_velocity_temp = m_velocity (simulator.cpp:80)
-    fluid::advect_staggered_grid(m_dev, m_velocity, temp, temp, m_dx, dt);
+    // m_velocity_temp = m_velocity (simulator.cpp:80): the library keeps the copy
+    elq_ctx* c = m_dev.ctx();
+    elq_grid g = device_grid();
+    check(c, elq_advect_velocity_device(c, &g, dt));
 }
 
 void Simulator::advect_level_set(float dt) {
-    const Grid3f temp = m_level_set;  // m_level_set_temp = m_level_set (simulator.cpp:85)
-    fluid::advect_collocated_grid(m_dev, m_level_set, temp, m_velocity, m_dx, dt);
+    elq_ctx* c = m_dev.ctx();
+    check(c, elq_advect_level_set_device(c, m_ni, m_nj, m_nk, m_dx, dt, d_phi.as<float>(), d_u.as<float>(),
+                                         d_v.as<float>(), d_w.as<float>()));
 }
 
-void Simulator::reinitialize() { fluid::reinitialize_fast_marching(m_dev, m_level_set, m_level_set_valid, m_dx); }
+void Simulator::reinitialize() {
+    elq_ctx* c = m_dev.ctx();
+    int64_t unreached = 0;
+    check(c, elq_reinitialize_device(c, m_ni, m_nj, m_nk, m_dx, d_phi.as<float>(), d_lsv.as<uint8_t>(), &unreached,
+                                     nullptr));
+    if (unreached)
+        throw std::runtime_error("reinitialize: " + std::to_string(unreached) +
+                                 " valid cells have no path to the interface");
+}
 
 void Simulator::solve_pressure(float dt) {
-    elq_grid g = make_grid(m_ni, m_nj, m_nk, m_dx, m_level_set, m_velocity, m_velocity_weight);
-    g.pressure = m_pressure.data.data();
-    g.indices = m_indices.data.data();
-    const int rc = elq_project(m_dev.ctx(), &g, dt, &m_opts, &m_stats);
+    elq_grid g = device_grid();
+    const int rc = elq_project_device(m_dev.ctx(), &g, dt, &m_opts, &m_stats);
     if (rc != ELQ_OK && rc != ELQ_ERR_NOT_CONVERGED) raise(m_dev.ctx(), rc);
+}
+
+void Simulator::download_level_set() { d2h(m_dev.ctx(), m_level_set, d_phi); }
+
+void Simulator::download_velocity() {
+    elq_ctx* c = m_dev.ctx();
+    d2h(c, m_velocity.u(), d_u);
+    d2h(c, m_velocity.v(), d_v);
+    d2h(c, m_velocity.w(), d_w);
+}
+
+void Simulator::download_pressure() {
+    elq_ctx* c = m_dev.ctx();
+    d2h(c, m_pressure, d_p);
+    d2h(c, m_indices, d_idx);
+}
+
+void Simulator::download_valid() {
+    elq_ctx* c = m_dev.ctx();
+    d2h(c, m_velocity_valid.c[0], d_fu);
+    d2h(c, m_velocity_valid.c[1], d_fv);
+    d2h(c, m_velocity_valid.c[2], d_fw);
+    d2h(c, m_level_set_valid, d_lsv);
+}
+
+void Simulator::upload() {
+    elq_ctx* c = m_dev.ctx();
+    h2d(c, d_phi, m_level_set);
+    h2d(c, d_u, m_velocity.u());
+    h2d(c, d_v, m_velocity.v());
+    h2d(c, d_w, m_velocity.w());
+    h2d(c, d_uw, m_velocity_weight.u());
+    h2d(c, d_vw, m_velocity_weight.v());
+    h2d(c, d_ww, m_velocity_weight.w());
 }
 
 }  // namespace elq_host

@@ -139,6 +139,32 @@ void advect_staggered_grid(Device& dev, StaggeredGrid3f& grid, const StaggeredGr
 void reinitialize_fast_marching(Device& dev, Grid3f& grid, const Grid3c& valid, float dx);
 }  // namespace fluid
 
+// Device buffer owned for the lifetime of a Simulator (no per-step allocation).
+class DeviceArray {
+   public:
+    DeviceArray() = default;
+    ~DeviceArray() { reset(); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    void allocate(elq_ctx* ctx, size_t bytes);
+    void reset();
+    template <class T>
+    T* as() const { return static_cast<T*>(m_p); }
+    size_t bytes() const { return m_bytes; }
+
+   private:
+    elq_ctx* m_ctx = nullptr;
+    void* m_p = nullptr;
+    size_t m_bytes = 0;
+};
+
+// The reference's Simulator (simulator.h:16-73) with its grids resident in
+// device memory: create_grid allocates them once, every frame step runs on
+// them through the *_device entry points, and nothing is allocated or copied
+// between host and device per step. The public host members mirror the
+// reference's; they are written by the set-up steps and refreshed from the
+// device only when the caller asks (download_*), and upload() sends host
+// edits back (e.g. a velocity set by the caller).
 class Simulator {
    public:
     explicit Simulator(Device& dev) : m_dev(dev) { elq_default_opts(&m_opts); }
@@ -164,6 +190,14 @@ class Simulator {
     elq_opts& opts() { return m_opts; }
     const elq_stats& last_stats() const { return m_stats; }
 
+    // device -> host mirrors, only what the caller reads
+    void download_level_set();
+    void download_velocity();
+    void download_pressure();  // m_pressure and m_indices of the last solve_pressure
+    void download_valid();     // m_velocity_valid and m_level_set_valid
+    // host -> device: the level set, velocity and weights as the host members hold them
+    void upload();
+
     float m_dx = 0.0f;
     int m_ni = 0, m_nj = 0, m_nk = 0;
     StaggeredGrid3f m_velocity;
@@ -175,9 +209,12 @@ class Simulator {
     Grid3c m_level_set_valid;
 
    private:
+    elq_grid device_grid();
     Device& m_dev;
     elq_opts m_opts;
     elq_stats m_stats{};
+    // device-resident state
+    DeviceArray d_phi, d_u, d_v, d_w, d_uw, d_vw, d_ww, d_p, d_idx, d_fu, d_fv, d_fw, d_lsv, d_nodal;
 };
 
 }  // namespace elq_host

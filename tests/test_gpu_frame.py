@@ -10,9 +10,9 @@ the oracle, bit for bit (values as uint32 words, flags as bytes):
   (simulator.cpp:125-178) and elq_extrapolate_velocity_device
   (simulator.cpp:119-123) on scenes, in the reference's order;
 - elq_find_valid_level_set_device (simulator.cpp:160-167, 191-200);
-- the C++ shim's frame loop (elq_sample3d: extrapolate_velocity,
-  extrapolate_level_set, add_gravity, find_valid_velocity,
-  set_invalid_velocity_zero, solve_pressure) runs and converges.
+- the C++ shim's frame loop (elq_sample3d, the reference's frame order on
+  device-resident grids) against the oracle frame by frame: the level set
+  bit for bit, the projection to the solve tolerance.
 """
 import os
 import subprocess
@@ -186,13 +186,62 @@ def test_level_set_extrapolation_const(ctx):
     assert same(phi, ref) and same(flags, f0) and not same(phi, s.phi)
 
 
-def test_cpp_shim_frame_loop():
+def _oracle_steps(s, lsv, vflags, dt):
+    """The sample's frame up to the projection on the oracle, in place on the
+    scene (and the velocity flags): extrapolate_velocity, extrapolate_level_set,
+    advect_level_set, advect_velocity, reinitialize, add_gravity,
+    find_valid_velocity + set_invalid_velocity_zero."""
+    ni, nj, nk = s.ni, s.nj, s.nk
+    for a, f, d in zip((s.u, s.v, s.w), vflags, ((ni + 1, nj, nk), (ni, nj + 1, nk), (ni, nj, nk + 1))):
+        O.extrapolate(a, f, d, 10)
+    O.extrapolate_const(s.phi, lsv, (ni, nj, nk), 10)
+    O.advect_level_set(s, dt)
+    O.advect_velocity(s, dt)
+    left, _ = O.reinitialize(s.phi, lsv, (ni, nj, nk), s.dx)
+    assert left == 0
+    O.add_gravity(s, dt)
+    O.find_valid_and_zero(s)
+
+
+@pytest.mark.parametrize("mixed, ptol", [(0, 2e-6), (1, 2e-5)])
+def test_cpp_shim_frames_match_oracle(tmp_path, mixed, ptol):
+    """elq_sample3d (the Simulator shim, grids resident in device memory for
+    the whole run) against the oracle frame by frame: frame f of the oracle
+    starts from the shim's state after frame f-1 (its level set and
+    velocities, dumped), so the steps before the projection must give the
+    same level set bit for bit, and the projection must match the oracle's
+    FP64 solve (rel.res 1e-12) to the solve tolerance."""
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     exe = os.path.join(root, "eulerian-liquid_amd", "elq_sample3d")
-    r = subprocess.run([exe, "32", "3", "1e-6", "1"], capture_output=True, text=True, timeout=120)
+    n, frames, tol, dt = 32, 3, 1e-8, np.float32(0.01)
+    r = subprocess.run([exe, str(n), str(frames), str(tol), str(mixed), "--dump", str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
-    frames = [ln for ln in r.stdout.splitlines() if ln.startswith("frame")]
-    assert len(frames) == 3, r.stdout
-    for ln in frames:
-        res = float(ln.split("true_rel_res=")[1].split()[0])
-        assert res <= 1e-6, ln
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("frame")]
+    assert len(lines) == frames, r.stdout
+    for ln in lines:
+        assert float(ln.split("true_rel_res=")[1].split()[0]) <= tol, ln
+
+    def dumped(f, name):
+        return np.fromfile(tmp_path / f"frame{f}_{name}.f32", np.float32)
+
+    s = O.make_scene(n, n, n, 0.4, [(0.5, 0.7, 0.5, 0.15)], float(dt))
+    nodal = np.empty((n + 1) ** 3, np.float32)
+    O.lib().orc_room_nodal(n, n, n, nodal)
+    lsv = np.empty(n ** 3, np.uint8)
+    O.lib().orc_find_valid_level_set(n, n, n, nodal, lsv)
+    vflags = tuple(np.ones(a.size, np.uint8) for a in (s.u, s.v, s.w))   # simulator.cpp:59
+    for f in range(frames):
+        if f > 0:   # continue from the shim's own state
+            s.phi, s.u, s.v, s.w = (dumped(f - 1, k) for k in ("phi", "u", "v", "w"))
+            vflags = O.find_valid_velocity(s)
+        _oracle_steps(s, lsv, vflags, dt)
+        assert same(dumped(f, "phi"), s.phi), f"level set differs in frame {f}"
+        ref = O.solve_pressure(s, tol=1e-12, max_iter=20000)
+        scale = np.abs(ref.pressure).max()
+        assert scale > 0
+        assert np.abs(dumped(f, "pressure") - ref.pressure).max() <= ptol * scale, f
+        for k in ("u", "v", "w"):
+            got, want = dumped(f, k), getattr(ref, k)
+            bound = ptol * scale * float(dt) / (s.dx * 0.01) + 1e-6 * np.abs(want).max()
+            assert np.abs(got - want).max() <= bound, (f, k)

@@ -574,6 +574,24 @@ def copy_calibration(nbytes, reps):
                       "copy_gbs_incl_sync": round(2 * nbytes / t / 1e9, 1)}), flush=True)
 
 
+def read_calibration(nbytes, reps):
+    """Sums nbytes of float32 with torch's reduction kernel (reads nbytes,
+    writes a few bytes): the read-only companion of copy_calibration for
+    scaling per-dispatch read counters."""
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps + 1):
+        t0 = time.perf_counter()
+        v = float(x.sum())
+        ts.append(time.perf_counter() - t0)
+    assert v == nbytes // 4
+    t = min(ts[1:])
+    print(json.dumps({"read_bytes": nbytes, "dispatches": reps + 1, "best_s": round(t, 6),
+                      "read_gbs_incl_sync": round(nbytes / t / 1e9, 1)}), flush=True)
+
+
 def serve_run(scenes, arrivals, slots, opts):
     """Submits fresh copies of the scenes at the given arrival times (s from
     the start) to a server with `slots` slots; returns per-request results."""
@@ -661,12 +679,17 @@ def main():
                     help="skip the other configurations (64^3, 128^3 against the oracle; 512^3)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel on MIB MiB (HBM counter calibration)")
+    ap.add_argument("--read-calibration", type=int, default=0, metavar="MIB",
+                    help="only sum MIB MiB of float32 with torch (read-only HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "the projection runs on one GPU here; --gpus must be 1"}))
         raise SystemExit(2)
     if a.copy_calibration:
         copy_calibration(a.copy_calibration << 20, max(a.steps, 1))
+        return
+    if a.read_calibration:
+        read_calibration(a.read_calibration << 20, max(a.steps, 1))
         return
 
     t_gen = time.perf_counter()

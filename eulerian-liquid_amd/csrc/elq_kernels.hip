@@ -1274,8 +1274,22 @@ __device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, in
     d.azm_e = wz ? az_e : (T)0;
     d.poll.want[0] = wy && gy;
     d.poll.want[1] = wz && gz;
-    d.poll.slot[0] = bslot<T>(yb, ly_lane);
-    d.poll.slot[1] = bslot<T>(zb, lz_lane);
+    // every lane of the wave reads from the face it waits for (ly := 3 or
+    // lz := 3 of the neighbour brick), so a poll round touches only the
+    // face's cache lines; without a neighbour word to wait for, one word
+    if constexpr (sizeof(T) == 8) {
+        // FP64 (bricks publish their faces only): every lane polls a word of
+        // the face it waits for (ly := 3, lz := 3 of the neighbour), so a poll
+        // round touches only the face's cache lines; with no word to wait
+        // for, all lanes read one word of their own brick. (In FP32, where
+        // bricks publish all 64 words, lanes that wait for nothing reading
+        // their own brick's words measured faster.)
+        d.poll.slot[0] = nym >= 0 && gy ? bslot<T>(nym, L.lane | 12) : bslot<T>(b, 0);
+        d.poll.slot[1] = nzm >= 0 && gz ? bslot<T>(nzm, L.lane | 48) : bslot<T>(b, 0);
+    } else {
+        d.poll.slot[0] = bslot<T>(yb, ly_lane);
+        d.poll.slot[1] = bslot<T>(zb, lz_lane);
+    }
 }
 
 template <class T>
@@ -1379,8 +1393,15 @@ __device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, i
     const bool wy = L.ly == 3 && nyp >= 0, wz = L.lz == 3 && nzp >= 0;
     d.poll.want[0] = wy && gy;
     d.poll.want[1] = wz && gz;
-    d.poll.slot[0] = bslot<T>(wy ? nyp : b, wy ? L.lane - 12 : L.lane);
-    d.poll.slot[1] = bslot<T>(wz ? nzp : b, wz ? L.lane - 48 : L.lane);
+    // the faces ly := 0 and lz := 0 of the +y and +z neighbours (as in fwd_fetch)
+    if constexpr (sizeof(T) == 8) {
+        // faces ly := 0 and lz := 0 of the +y and +z neighbours (see fwd_fetch)
+        d.poll.slot[0] = nyp >= 0 && gy ? bslot<T>(nyp, L.lane & ~12) : bslot<T>(b, 0);
+        d.poll.slot[1] = nzp >= 0 && gz ? bslot<T>(nzp, L.lane & ~48) : bslot<T>(b, 0);
+    } else {
+        d.poll.slot[0] = bslot<T>(wy ? nyp : b, wy ? L.lane - 12 : L.lane);
+        d.poll.slot[1] = bslot<T>(wz ? nzp : b, wz ? L.lane - 48 : L.lane);
+    }
 }
 
 template <class T>

@@ -1393,7 +1393,6 @@ __device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, i
     const bool wy = L.ly == 3 && nyp >= 0, wz = L.lz == 3 && nzp >= 0;
     d.poll.want[0] = wy && gy;
     d.poll.want[1] = wz && gz;
-    // the faces ly := 0 and lz := 0 of the +y and +z neighbours (as in fwd_fetch)
     if constexpr (sizeof(T) == 8) {
         // faces ly := 0 and lz := 0 of the +y and +z neighbours (see fwd_fetch)
         d.poll.slot[0] = nyp >= 0 && gy ? bslot<T>(nyp, L.lane & ~12) : bslot<T>(b, 0);

@@ -22,7 +22,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle_elq.so")
+# ELQ_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of the Makefile's `sanitize` target)
+_LIB_PATH = os.environ.get("ELQ_ORACLE_LIB") or os.path.join(_HERE, "liboracle_elq.so")
 UNKNOWN_INDEX = 2147483647
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")

@@ -1004,6 +1004,7 @@ int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
         rate_khz = 100000;
     ctx->clock_hz = (uint64_t)rate_khz * 1000ull;
     ctx->timeout_ticks = ctx->clock_hz * 4ull;  // 4 s per wait
+    ctx->rei_rebase = kReiRebase;
     // the static scratch array of the allocation-free projection; a failed
     // lookup only disables that entry point
     if (scratch_address(device, &ctx->scratch) != hipSuccess) ctx->scratch = nullptr;
@@ -1032,6 +1033,13 @@ int elq_set_graph_capture(elq_ctx* ctx, int enable) {
     (void)hipSetDevice(ctx->device);
     if (!enable) ctx->drop_chunk_graph();
     ctx->graphs = enable != 0;
+    return ELQ_OK;
+}
+
+int elq_set_reinit_rebase(elq_ctx* ctx, int32_t passes) {
+    if (!ctx) return ELQ_ERR_INVALID_ARG;
+    if (passes < 1 || passes > kReiRebase) return ctx->fail(ELQ_ERR_INVALID_ARG, "rebase interval out of range");
+    ctx->rei_rebase = passes;
     return ELQ_OK;
 }
 
@@ -1643,8 +1651,8 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     uint32_t last_fixed = 1;
     while (last_fixed != 0 && l <= (int64_t)cells) {
         for (int q = 0; q < kBatch; ++q, ++l) {
-            if (l - first == kReiRebase) {
-                HIPCHK(launch_rei_rebase(r, ctx->stream), "k_rei_rebase");
+            if (l - first == ctx->rei_rebase) {
+                HIPCHK(launch_rei_rebase(r, (uint32_t)first, ctx->stream), "k_rei_rebase");
                 first = l;
             }
             ctx->prof_begin("rei_pass");
@@ -1656,12 +1664,14 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
         HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     }
     HIPCHK(launch_rei_finish(r, ctx->stream), "k_rei_finish");
-    uint32_t fin[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(fin, r.counter + 4, 8, hipMemcpyDeviceToHost, ctx->stream), "read counter");
+    uint32_t fin[5] = {0, 0, 0, 0, 0};  // counter[4..8]
+    HIPCHK(hipMemcpyAsync(fin, r.counter + 4, sizeof(fin), hipMemcpyDeviceToHost, ctx->stream), "read counter");
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     if (unreached) *unreached = fin[0];
-    if (passes) *passes = (int32_t)(first + fin[1]);
+    // the last pass that fixed cells lies in the last epoch if that epoch
+    // fixed any, else before the last rebase (the batch ran on past the end)
+    if (passes) *passes = fin[1] ? (int32_t)(first + fin[1]) : (int32_t)fin[4];
     return ELQ_OK;
 }
 

@@ -122,6 +122,7 @@ struct elq_ctx {
     uint64_t timeout_ticks = 0;
     uint64_t clock_hz = 0;  // device wall clock (wall_clock64) rate
     int max_groups = 0;  // cap on the grid of persistent kernels (0: whole device)
+    int rei_rebase = 0;  // reinitialisation tile route: passes between stamp rebases (set at creation)
     // allocation-free projection: this device's static scratch array (its
     // device address, from hipGetSymbolAddress at context creation), the
     // arena over it and the workspace carved from it for each call

@@ -639,7 +639,7 @@ __global__ __launch_bounds__(256) void k_rei_init(ReiSet r) {
         r.tfar[T.t] = far_any ? 1 : 0;
         r.tnew[T.t] = fix_any ? 0 : kReiFar;
         if (blockIdx.x == 0)
-            for (int q = 0; q < 8; ++q) r.counter[q] = 0;
+            for (int q = 0; q < 9; ++q) r.counter[q] = 0;
     }
     for (int q = blockIdx.x * 256 + threadIdx.x; q < kReiDistMax + 2; q += gridDim.x * 256) r.dcount[q] = 0;
 }
@@ -1081,13 +1081,21 @@ __global__ __launch_bounds__(256) void k_rei_layer(ReiSet r, uint32_t begin, uin
     r.phi[c] = rei_update(fxm || fxp, vx, fym || fyp, vy, fzm || fzp, vz, r.dx);
 }
 
-__global__ __launch_bounds__(256) void k_rei_rebase(ReiSet r) {
+__global__ __launch_bounds__(256) void k_rei_rebase(ReiSet r, uint32_t base) {
     const size_t n = (size_t)r.ni * r.nj * r.nk;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
+    uint32_t top = 0;
     for (size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += stride) {
         const uint16_t s = r.stamp[c];
-        if (s != kReiFar && s != kReiUnknown) r.stamp[c] = 0;
+        if (s != kReiFar && s != kReiUnknown) {
+            top = max(top, (uint32_t)s);
+            r.stamp[c] = 0;
+        }
     }
+    // passes that fixed cells up to here: a cell fixed in pass a of the
+    // epoch starting at pass base has stamp a - base + 1
+    top = __reduce_max_sync(~0ull, top);
+    if ((threadIdx.x & 63) == 0 && top) atomicMax(&r.counter[8], base + top);
     for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < (size_t)r.tiles; t += stride) {
         if (r.tnew[t] != kReiFar) r.tnew[t] = 0;  // every fixed cell now has stamp 0
     }
@@ -1204,8 +1212,8 @@ hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, hipSt
     return hipGetLastError();
 }
 
-hipError_t launch_rei_rebase(const ReiSet& r, hipStream_t s) {
-    hipLaunchKernelGGL(k_rei_rebase, dim3(2048), dim3(256), 0, s, r);
+hipError_t launch_rei_rebase(const ReiSet& r, uint32_t base, hipStream_t s) {
+    hipLaunchKernelGGL(k_rei_rebase, dim3(2048), dim3(256), 0, s, r, base);
     return hipGetLastError();
 }
 

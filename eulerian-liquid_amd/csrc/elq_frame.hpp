@@ -87,7 +87,8 @@ hipError_t launch_advect_cells(int ni, int nj, int nk, float dx, float dt, const
 // Reinitialisation (reinitialization.cpp:19-156). Cell stamps are 16-bit
 // (kReiFar: not reached yet, kReiUnknown: invalid cell, s: fixed from the
 // start of pass s on). The tile passes (32x8x8 tiles) fold the stamps back to
-// 0 every kReiRebase passes; the distance route has no stamps to fold.
+// 0 every kReiRebase passes (a context can ask for fewer); the distance route
+// has no stamps to fold.
 constexpr uint16_t kReiFar = 0xFFFF;
 constexpr uint16_t kReiUnknown = 0xFFFE;
 constexpr int kReiRebase = 60000;
@@ -108,7 +109,8 @@ struct ReiSet {
     uint8_t* fixm;         // per listed cell: its face neighbours one step closer (bits -x +x -y +y -z +z)
     uint32_t* counter;     // [0..1] list lengths, [2..3] cells fixed by alternate passes,
                            // [4] cells left unreached, [5] largest stamp,
-                           // [6] cells where dist is not the march's pass, [7] largest dist
+                           // [6] cells where dist is not the march's pass, [7] largest dist,
+                           // [8] passes that fixed cells before the last rebase
     int ni, nj, nk;
     int tx, ty, tz;
     int tiles;
@@ -132,7 +134,9 @@ hipError_t launch_rei_lists(const ReiSet& r, hipStream_t s);
 hipError_t launch_rei_layer(const ReiSet& r, uint32_t begin, uint32_t end, hipStream_t s);
 // Tile route. Pass l (0-based since the last rebase; parity = absolute pass & 1).
 hipError_t launch_rei_pass(const ReiSet& r, int l, int parity, hipStream_t s);
-hipError_t launch_rei_rebase(const ReiSet& r, hipStream_t s);
+// Stamps of fixed cells back to 0; counter[8] = max(counter[8], base + the
+// largest stamp folded), base being the first pass of the epoch folded.
+hipError_t launch_rei_rebase(const ReiSet& r, uint32_t base, hipStream_t s);
 // counter[4] = cells never reached, counter[5] = largest stamp (passes that fixed cells).
 hipError_t launch_rei_finish(const ReiSet& r, hipStream_t s);
 

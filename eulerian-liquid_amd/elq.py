@@ -128,6 +128,7 @@ def load():
     L.elq_set_profiling.restype = None
     L.elq_set_max_groups.argtypes = [vp, C.c_int]
     L.elq_set_wait_limit.argtypes = [vp, f64]
+    L.elq_set_reinit_rebase.argtypes = [vp, i32]
     L.elq_set_graph_capture.argtypes = [vp, C.c_int]
     L.elq_server_create.argtypes = [C.c_int, C.POINTER(ServerOpts), C.POINTER(vp)]
     L.elq_server_destroy.argtypes = [vp]
@@ -514,12 +515,17 @@ class Context:
         _check_rc(self.L.elq_set_max_groups(self.h, int(max_groups)), self.h)
 
     def set_graph_capture(self, on: bool):
-        """Graph replay of the Krylov loop's poll intervals (default on)."""
+        """Graph replay of the Krylov loop's poll intervals (default off)."""
         _check_rc(self.L.elq_set_graph_capture(self.h, 1 if on else 0), self.h)
 
     def set_wait_limit(self, seconds: float):
         """Bound of every device-side wait (default 4 s); see elq_set_wait_limit."""
         _check_rc(self.L.elq_set_wait_limit(self.h, float(seconds)), self.h)
+
+    def set_reinit_rebase(self, passes: int):
+        """Stamp rebase interval of the reinitialisation's tile route
+        (default 60000); see elq_set_reinit_rebase."""
+        _check_rc(self.L.elq_set_reinit_rebase(self.h, int(passes)), self.h)
 
     def set_profiling(self, on: bool, only: str = ""):
         """Per-kernel event timing; only: comma-separated kernel classes to time (all if empty)."""

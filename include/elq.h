@@ -146,6 +146,12 @@ int elq_set_graph_capture(elq_ctx* ctx, int enable);
  * waits). 0 ends every wait that is not satisfied at its first check. */
 int elq_set_wait_limit(elq_ctx* ctx, double seconds);
 
+/* Folds the 16-bit pass stamps of the reinitialisation's tile route back to
+ * 0 every `passes` passes (1..60000; default 60000, the most the stamps
+ * hold). No reference counterpart: results do not depend on it; a smaller
+ * interval exercises the rebase on short marches (tests). */
+int elq_set_reinit_rebase(elq_ctx* ctx, int32_t passes);
+
 /* Simulator::solve_pressure(dt) on host grids (simulator.cpp:96-112): copies
  * inputs to HBM, projects, copies u/v/w (and pressure/indices) back. */
 int elq_project(elq_ctx* ctx, const elq_grid* host_grid, float dt, const elq_opts* opts,

@@ -9,7 +9,8 @@ same count of passes and of unreached cells:
 - a sphere with noise on a room() grid, flags from find_valid_level_set, and
   a pocket of cells walled off by invalid cells;
 - a wall of invalid cells with a hole, so that cells behind it are reached
-  by a detour (longer than their taxicab distance);
+  by a detour (longer than their taxicab distance), also with the tile
+  route's stamp rebase forced every 1 to 80 passes;
 - the 128^3 drop scene's level set after one advection step (about 100
   passes, several host batches);
 - grids one or two cells thick on some axis.
@@ -133,6 +134,40 @@ def test_wall_with_hole_detour_bit_exact(ctx):
     left, passes = O.reinitialize(ref, valid, dims, dx)
     assert left == 0 and passes > 40   # 21 passes reach the far corner without the wall
     check(ctx, phi, valid, dims, dx, route="rei_pass")
+
+
+def wall_with_hole():
+    dims, dx = (36, 32, 40), 1.0 / 36
+    ni, nj, nk = dims
+    x, y, z = xyz(dims, dx)
+    phi = (1.3 * (y - 0.3)).astype(np.float32)   # liquid below y = 0.3
+    k, j, i = np.meshgrid(np.arange(nk), np.arange(nj), np.arange(ni), indexing="ij")
+    wall = (j == 16) & ~((np.abs(i - 6) <= 1) & (np.abs(k - 30) <= 1))
+    valid = np.ones(phi.size, np.uint8)
+    valid[wall.reshape(-1)] = 0
+    return phi, valid, dims, dx
+
+
+@pytest.mark.parametrize("every", [1, 2, 5, 38, 76, 77, 78, 80])
+def test_tile_route_stamp_rebase_bit_exact(ctx, every):
+    # The tile route folds its 16-bit stamps back to 0 every `every` passes
+    # (60000 by default, more than any test reaches). The wall scene's march
+    # takes 77 passes in host batches of 32: rebases before, at and after
+    # its last pass (also inside the batch that runs on past the end) must
+    # leave values, pass count and unreached count unchanged.
+    phi, valid, dims, dx = wall_with_hole()
+    ctx.set_reinit_rebase(every)
+    try:
+        assert check(ctx, phi, valid, dims, dx, route="rei_pass") == 77
+    finally:
+        ctx.set_reinit_rebase(60000)
+
+
+def test_reinit_rebase_interval_range(ctx):
+    for bad in (0, -1, 60001):
+        with pytest.raises(elq.ElqError) as e:
+            ctx.set_reinit_rebase(bad)
+        assert e.value.code == elq.INVALID_ARG
 
 
 def test_drop128_after_advection_bit_exact(ctx):

@@ -12,6 +12,8 @@ oracle's formulas:
   checked are those whose upwind cone stays off the domain walls. A wrong
   formula, a neighbour taken from the wrong side, a wrong tetrahedron or a
   missing sign flip misses by a sizeable fraction of dx;
+- the three-neighbour update is 0.999999 times the exact root of its
+  upwind quadratic (the reference's 0.333333 for 1/3);
 - signs are kept; invalid cells are untouched; cells already at 0 stay 0;
 - a pocket of cells walled off by invalid cells is reported (the reference
   would loop forever there) and left unchanged.
@@ -48,6 +50,45 @@ def test_planar_fronts_become_signed_distances(normal, scale, tol):
     err = np.abs(phi[inner] - dist.reshape(-1)[inner]).max()
     assert err < tol, err
     assert np.array_equal(np.sign(phi), np.sign(dist.reshape(-1)))
+
+
+def test_three_neighbour_update_uses_the_references_third():
+    # reinitialization.cpp writes 1/3 as the literal 0.333333, so its
+    # three-neighbour update is 0.999999 times the exact root of the upwind
+    # quadratic (sum_a (d - p_a)^2 = dx^2) of the FIX values it reads. On the
+    # positive side of a (1,1,1) plane, the cells of the first marched layer
+    # whose -x, -y and -z neighbours are interface cells take that update;
+    # the ratio of their value to the exact root of their neighbours' values
+    # is 0.999999 up to float rounding (an exact third would give 1).
+    n, dx = 40, 1.0 / 32
+    x, y, z, i, j, k = grid_xyz(n, dx)
+    nv = np.ones(3) / np.sqrt(3.0)
+    c = nv @ np.array([0.6, 0.6, 0.6]) + 0.3 * dx
+    phi = (nv[0] * x + nv[1] * y + nv[2] * z - c).astype(np.float32).reshape(-1)
+    sgn = np.sign(phi).reshape(n, n, n)
+    iface = np.zeros((n, n, n), bool)
+    for ax in range(3):
+        lo = [slice(None)] * 3
+        hi = [slice(None)] * 3
+        lo[ax], hi[ax] = slice(0, -1), slice(1, None)
+        flip = sgn[tuple(lo)] != sgn[tuple(hi)]
+        iface[tuple(lo)] |= flip
+        iface[tuple(hi)] |= flip
+    valid = np.ones(phi.size, np.uint8)
+    O.reinitialize(phi, valid, (n, n, n), dx)
+    v = phi.reshape(n, n, n).astype(np.float64)
+    sel = (sgn > 0) & ~iface
+    sel[0, :, :] = sel[:, 0, :] = sel[:, :, 0] = False
+    sel[1:, 1:, 1:] &= iface[:-1, 1:, 1:] & iface[1:, :-1, 1:] & iface[1:, 1:, :-1]
+    kk, jj, ii = np.nonzero(sel)
+    assert kk.size > 200
+    p = np.stack([v[kk, jj, ii - 1], v[kk, jj - 1, ii], v[kk - 1, jj, ii]])
+    s_ = p.sum(0)
+    root = (s_ + np.sqrt(s_ * s_ - 3.0 * ((p * p).sum(0) - dx * dx))) / 3.0
+    assert np.all(root > p.max(0))        # the three-neighbour case applies
+    ratio = v[kk, jj, ii] / root
+    assert abs(np.median(ratio) - 0.999999) < 2e-7, np.median(ratio)
+    assert np.abs(ratio - 0.999999).max() < 8e-7
 
 
 def test_invalid_cells_zero_cells_and_unreachable_pockets():

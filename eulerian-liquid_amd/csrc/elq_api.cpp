@@ -90,11 +90,11 @@ hipError_t DevBuf::ensure(size_t bytes) {
         cap = bytes;
         return hipSuccess;
     }
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, stream);
     p = nullptr;
     cap = 0;
     if (bytes == 0) return hipSuccess;
-    hipError_t e = hipMalloc(&p, bytes);
+    hipError_t e = hipMallocAsync(&p, bytes, stream);
     if (e == hipSuccess) cap = bytes;
     return e;
 }
@@ -105,7 +105,7 @@ void DevBuf::release() {
         cap = 0;
         return;
     }
-    if (p) (void)hipFree(p);
+    if (p) (void)hipFreeAsync(p, stream);
     p = nullptr;
     cap = 0;
 }
@@ -1005,6 +1005,8 @@ int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
     ctx->clock_hz = (uint64_t)rate_khz * 1000ull;
     ctx->timeout_ticks = ctx->clock_hz * 4ull;  // 4 s per wait
     ctx->rei_rebase = kReiRebase;
+    ctx->ws.for_each([&](DevBuf& b) { b.stream = ctx->stream; });
+    ctx->ws_scratch.for_each([&](DevBuf& b) { b.stream = ctx->stream; });
     // the static scratch array of the allocation-free projection; a failed
     // lookup only disables that entry point
     if (scratch_address(device, &ctx->scratch) != hipSuccess) ctx->scratch = nullptr;
@@ -1020,6 +1022,7 @@ void elq_ctx_destroy(elq_ctx* ctx) {
     ctx->stage.release();
     ctx->drop_chunk_graph();
     ctx->ws.release_all();
+    (void)hipStreamSynchronize(ctx->stream);  // the stream-ordered frees
     for (auto ev : ctx->events) (void)hipEventDestroy(ev);
     (void)hipStreamDestroy(ctx->stream);
     if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
@@ -1145,7 +1148,8 @@ int elq_project(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts,
         // indices are meaningful even on NO_FLUID (all INT_MAX); copy them back
         if (rc == ELQ_ERR_NO_FLUID && g->indices) {
             const Sizes z = sizes_of(g->ni, g->nj, g->nk);
-            if (hipMemcpy(g->indices, dg.indices, z.c * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            if (hipMemcpyAsync(g->indices, dg.indices, z.c * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
                 return ctx->fail(ELQ_ERR_DEVICE, "d2h");
         }
         return rc;
@@ -1168,7 +1172,8 @@ int elq_assign_indices(elq_ctx* ctx, const elq_grid* g, int32_t* indices, int64_
     if ((rc = stage_inputs(ctx, *g, dg))) return rc;
     if ((rc = setup_phase_a(ctx, dense_of(dg), nullptr))) return rc;
     const Sizes z = sizes_of(g->ni, g->nj, g->nk);
-    if (hipMemcpy(indices, ctx->ws.cur_indices, z.c * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpyAsync(indices, ctx->ws.cur_indices, z.c * 4, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
         return ctx->fail(ELQ_ERR_DEVICE, "d2h");
     if (n_unknowns) *n_unknowns = ctx->ws.n_unknowns;
     if (ctx->ws.n_unknowns == 0) return ctx->fail(ELQ_ERR_NO_FLUID, "No fluid");
@@ -1200,9 +1205,10 @@ int elq_build_matrix(elq_ctx* ctx, const elq_grid* g, float dt, double* diag, do
                                  (const double*)W.b64.p, (const double*)W.cax.p, (const double*)W.cay.p,
                                  (const double*)W.caz.p, e, e + n, e + 2 * n, e + 3 * n, e + 4 * n, ctx->stream),
            "k_export");
-    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     double* outs[5] = {diag, ax, ay, az, b};
-    for (int q = 0; q < 5; ++q) HIPCHK(hipMemcpy(outs[q], e + q * n, n * 8, hipMemcpyDeviceToHost), "d2h");
+    for (int q = 0; q < 5; ++q)
+        HIPCHK(hipMemcpyAsync(outs[q], e + q * n, n * 8, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
 }
 
@@ -1251,7 +1257,8 @@ int elq_ic_factor(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opt
            "k_export_cells");
     if ((rc = read_ctrl(ctx, hc))) return rc;
     if (hc.abort) return ctx->fail(ELQ_ERR_DEVICE, "wavefront wait exceeded its time limit");
-    HIPCHK(hipMemcpy(inv_e, W.export_buf.p, n * 8, hipMemcpyDeviceToHost), "d2h");
+    HIPCHK(hipMemcpyAsync(inv_e, W.export_buf.p, n * 8, hipMemcpyDeviceToHost, ctx->stream), "d2h");
+    HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
 }
 

@@ -26,10 +26,15 @@ struct Arena {
 
 // Device buffer that only grows (hipMalloc on growth, reused otherwise). With
 // an arena it is carved from the arena instead and never freed.
+// A device buffer that grows on demand. Outside an arena it is allocated and
+// freed in the order of `stream` (hipMallocAsync / hipFreeAsync): growing or
+// releasing it never waits for work on other streams, as hipFree would (it
+// synchronises the device, which stalled every other serving slot).
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
     Arena* arena = nullptr;
+    hipStream_t stream = nullptr;
     hipError_t ensure(size_t bytes);
     void release();
 };

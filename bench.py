@@ -628,7 +628,7 @@ def serve_summary(got):
             "mean_iterations": round(float(np.mean([r["iterations"] for r in got])), 1)}
 
 
-def serving(n_req, slots, seed, base):
+def serving(n_req, slots, seed, base, fp64=True):
     """Serving stream: n_req independent requests, dims per axis from
     {32,48,64,96,128}, 1-8 spheres (radii 0.05-0.2), slab, face velocities
     U[-1,1] (request seeds seed*100000 + i). Runs: saturation (all submitted
@@ -647,6 +647,11 @@ def serving(n_req, slots, seed, base):
     arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req)).tolist()
     poisson = serve_summary(serve_run(scenes, arrivals, slots, o_mixed))
     print(f"# serving: saturation {sat}, poisson {poisson}", file=sys.stderr, flush=True)
+    out = {"workload": f"{n_req} requests, dims per axis from (32,48,64,96,128), {cells} cells in all, "
+                       f"mixed precision to rel.res 1e-8, {slots} slots",
+           "poisson_rate_per_s": round(lam, 2), "poisson": poisson, "saturation": sat}
+    if not fp64:
+        return out
     seq64 = serve_summary(serve_run(scenes, [0.0] * n_req, 1, o_fp64))
     print(f"# serving: fp64 one slot {seq64}", file=sys.stderr, flush=True)
     return {"workload": f"{n_req} requests, dims per axis from (32,48,64,96,128), {cells} cells in all, "
@@ -654,6 +659,24 @@ def serving(n_req, slots, seed, base):
             "poisson_rate_per_s": round(lam, 2), "poisson": poisson, "saturation": sat,
             "fp64_drop_in_one_slot_saturation": seq64,
             "speedup_vs_fp64_one_slot": round(sat["solves_per_s"] / seq64["solves_per_s"], 2)}
+
+
+def serving_child(a):
+    """The same serving stream with more slots than hardware queues allow
+    by default: a child process with GPU_MAX_HW_QUEUES=16 (the runtime reads
+    it at start-up, so it cannot change inside this process) and
+    --throughput-slots slots. More slots raise the saturated rate; the
+    Poisson run at 0.8x that rate then has longer queues."""
+    import subprocess
+    env = dict(os.environ, GPU_MAX_HW_QUEUES="16")
+    cmd = [sys.executable, os.path.abspath(__file__), "--serve-only", "--serve", str(a.serve), "--slots",
+           str(a.throughput_slots), "--seed", str(a.seed), "--precond", a.precond, "--schedule", a.schedule]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=600)
+    if r.returncode != 0:
+        raise SystemExit(f"serving child failed: {r.stderr.strip()[-500:]}")
+    out = json.loads(r.stdout.strip().splitlines()[-1])["serving"]
+    out["hw_queues"] = 16
+    return out
 
 
 def main():
@@ -675,6 +698,11 @@ def main():
     ap.add_argument("--serve", type=int, default=512, metavar="N",
                     help="serving stream of N requests after the single-solve benchmark (0: skip)")
     ap.add_argument("--slots", type=int, default=4, help="serving slots")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only the serving stream (mixed precision), as one JSON object (child runs)")
+    ap.add_argument("--throughput-slots", type=int, default=12, metavar="N",
+                    help="slots of the throughput-oriented serving run in a child process with "
+                         "GPU_MAX_HW_QUEUES=16 (0: skip)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (64^3, 128^3 against the oracle; 512^3)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -690,6 +718,11 @@ def main():
         return
     if a.read_calibration:
         read_calibration(a.read_calibration << 20, max(a.steps, 1))
+        return
+    if a.serve_only:
+        base = dict(tol=1e-8, max_iter=20000, precond=elq.IC0 if a.precond == "ic0" else elq.MIC0,
+                    schedule=elq.WAVEFRONT if a.schedule == "wavefront" else elq.LEVELS, check_every=32)
+        print(json.dumps({"serving": serving(a.serve, a.slots, a.seed, base, fp64=False)}), flush=True)
         return
 
     t_gen = time.perf_counter()
@@ -806,6 +839,8 @@ def main():
     ctx.close()
     if a.serve > 0:
         line["serving"] = serving(a.serve, a.slots, a.seed, base)
+        if a.throughput_slots > 0:
+            line["serving"]["throughput_config"] = serving_child(a)
     threads = host_info()["omp_threads"]
     if not a.no_configs:
         line["configs"] = configs(base, a.seed, threads)

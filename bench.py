@@ -370,7 +370,7 @@ def gpu_config(ctx, ds, opts, reps):
     return e, ds.fetch("pressure"), st
 
 
-def configs(base, seed, threads):
+def configs(base, seed, threads, cfg5_oracle=False):
     """The other configurations of the workload, on the GPU (both
     precisions), each with its error: cfg1 (64^3 drop) and cfg2 (128^3 drop)
     against the oracle's FP64 solve to rel.res 1e-12, cfg1 also at the
@@ -423,6 +423,25 @@ def configs(base, seed, threads):
     e["error_reference"] = "FP64 GPU drop-in (oracle parity shown at cfg1/cfg2 and the golden fixtures)"
     ds.close()
     ctx.close()
+    if cfg5_oracle:
+        # the checker at this size: the oracle's FP64 solve to 1e-10 with
+        # OpenMP over SpMV, axpys and dots (the IC sweeps stay serial);
+        # minutes of host time, so off by default
+        import oracle as O
+        O.set_threads(threads, True)
+        O.set_progress(50)
+        t0 = time.perf_counter()
+        ref = O.solve_pressure(oracle_scene(sc), tol=1e-10, max_iter=20000)
+        O.set_progress(0)
+        O.set_threads(threads, False)
+        sref = float(np.abs(ref.pressure).max())
+        e["oracle"] = {"solve": "FP64 IC(0)-PCG to rel.res 1e-10", "iterations": ref.iterations,
+                       "host_s": round(time.perf_counter() - t0, 1), "same_unknowns": ref.n == e["unknowns"]}
+        for key in ("mixed", "fp64"):
+            e[key]["max_rel_p_minus_p_oracle"] = float(np.abs(ps[key] - ref.pressure).max()) / sref
+        e["error_reference"] = "the oracle (and the FP64 GPU drop-in)"
+        print(f"# cfg5 vs oracle: {e['oracle']}, mixed {e['mixed']['max_rel_p_minus_p_oracle']:.2e}, "
+              f"fp64 {e['fp64']['max_rel_p_minus_p_oracle']:.2e}", file=sys.stderr, flush=True)
     out["cfg5_spheres512"] = e
     return out
 
@@ -592,19 +611,21 @@ def read_calibration(nbytes, reps):
                       "read_gbs_incl_sync": round(nbytes / t / 1e9, 1)}), flush=True)
 
 
-def serve_run(scenes, arrivals, slots, opts):
+def serve_run(scenes, arrivals, slots, opts, keep_pressure=False):
     """Submits fresh copies of the scenes at the given arrival times (s from
-    the start) to a server with `slots` slots; returns per-request results."""
+    the start) to a server with `slots` slots; returns per-request results
+    (with keep_pressure, each keeps its pressure and the index of its scene)."""
     work = [sc.copy() for sc in scenes]
     n = len(work)
     got = []
+    index = {}
     with elq.Server(0, max_slots=slots, max_queue=n, opts=opts) as srv:
         t0 = time.perf_counter()
         i = 0
         while len(got) < n:
             now = time.perf_counter() - t0
             while i < n and arrivals[i] <= now:
-                srv.submit(work[i])
+                index[srv.submit(work[i])] = i
                 i += 1
             nxt = (arrivals[i] - now) if i < n else 1.0
             if srv.in_flight() == 0 and i < n:
@@ -614,8 +635,33 @@ def serve_run(scenes, arrivals, slots, opts):
             got += srv.poll()
     for r in got:
         r.pop("scene")
-        r.pop("pressure")
+        r["index"] = index[r["id"]]
+        if not keep_pressure:
+            r.pop("pressure")
     return got
+
+
+def serving_oracle_errors(scenes, got, threads):
+    """Every served request against the oracle's FP64 solve of its scene to
+    rel.res 1e-12 (the checker; solves in parallel host threads, one OpenMP
+    thread each): max|p - p_oracle| / max|p_oracle| per request."""
+    import concurrent.futures as cf
+    import oracle as O
+    O.set_threads(1, False)
+
+    def err(r):
+        ref = O.solve_pressure(oracle_scene(scenes[r["index"]]), tol=1e-12, max_iter=20000)
+        scale = float(np.abs(ref.pressure).max())
+        return float(np.abs(r["pressure"] - ref.pressure).max()) / scale if scale else 0.0, ref.n == r["n_unknowns"]
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(err, got))
+    O.set_threads(threads, False)
+    e = np.array([x[0] for x in res])
+    return {"requests": len(res), "max_rel_p_minus_p_oracle": float(e.max()),
+            "median_rel_p_minus_p_oracle": float(np.median(e)), "same_unknowns": all(x[1] for x in res),
+            "oracle": "FP64 IC(0)-PCG to rel.res 1e-12", "host_s": round(time.perf_counter() - t0, 1)}
 
 
 def serve_summary(got):
@@ -628,7 +674,7 @@ def serve_summary(got):
             "mean_iterations": round(float(np.mean([r["iterations"] for r in got])), 1)}
 
 
-def serving(n_req, slots, seed, base, fp64=True):
+def serving(n_req, slots, seed, base, fp64=True, check_threads=0):
     """Serving stream: n_req independent requests, dims per axis from
     {32,48,64,96,128}, 1-8 spheres (radii 0.05-0.2), slab, face velocities
     U[-1,1] (request seeds seed*100000 + i). Runs: saturation (all submitted
@@ -645,19 +691,21 @@ def serving(n_req, slots, seed, base, fp64=True):
     lam = 0.8 * sat["solves_per_s"]
     rng = np.random.default_rng(seed)
     arrivals = np.cumsum(rng.exponential(1.0 / lam, n_req)).tolist()
-    poisson = serve_summary(serve_run(scenes, arrivals, slots, o_mixed))
+    got = serve_run(scenes, arrivals, slots, o_mixed, keep_pressure=check_threads > 0)
+    poisson = serve_summary(got)
     print(f"# serving: saturation {sat}, poisson {poisson}", file=sys.stderr, flush=True)
     out = {"workload": f"{n_req} requests, dims per axis from (32,48,64,96,128), {cells} cells in all, "
                        f"mixed precision to rel.res 1e-8, {slots} slots",
            "poisson_rate_per_s": round(lam, 2), "poisson": poisson, "saturation": sat}
+    if check_threads > 0:
+        out["poisson_vs_oracle"] = serving_oracle_errors(scenes, got, check_threads)
+        print(f"# serving vs oracle: {out['poisson_vs_oracle']}", file=sys.stderr, flush=True)
+    del got
     if not fp64:
         return out
     seq64 = serve_summary(serve_run(scenes, [0.0] * n_req, 1, o_fp64))
     print(f"# serving: fp64 one slot {seq64}", file=sys.stderr, flush=True)
-    return {"workload": f"{n_req} requests, dims per axis from (32,48,64,96,128), {cells} cells in all, "
-                        f"mixed precision to rel.res 1e-8, {slots} slots",
-            "poisson_rate_per_s": round(lam, 2), "poisson": poisson, "saturation": sat,
-            "fp64_drop_in_one_slot_saturation": seq64,
+    return {**out, "fp64_drop_in_one_slot_saturation": seq64,
             "speedup_vs_fp64_one_slot": round(sat["solves_per_s"] / seq64["solves_per_s"], 2)}
 
 
@@ -705,6 +753,8 @@ def main():
                          "GPU_MAX_HW_QUEUES=16 (0: skip)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the other configurations (64^3, 128^3 against the oracle; 512^3)")
+    ap.add_argument("--cfg5-oracle", action="store_true",
+                    help="also check the 512^3 configuration against the oracle (minutes of host time)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run the library's plain copy kernel on MIB MiB (HBM counter calibration)")
     ap.add_argument("--read-calibration", type=int, default=0, metavar="MIB",
@@ -838,12 +888,13 @@ def main():
         line["shim_frame"] = shim_frame()
     ctx.close()
     if a.serve > 0:
-        line["serving"] = serving(a.serve, a.slots, a.seed, base)
+        line["serving"] = serving(a.serve, a.slots, a.seed, base,
+                                  check_threads=0 if a.no_cpu else host_info()["omp_threads"])
         if a.throughput_slots > 0:
             line["serving"]["throughput_config"] = serving_child(a)
     threads = host_info()["omp_threads"]
     if not a.no_configs:
-        line["configs"] = configs(base, a.seed, threads)
+        line["configs"] = configs(base, a.seed, threads, a.cfg5_oracle)
         line["configs"]["cfg3_" + wname] = "the headline: value, config, roofline, fp64_drop_in"
         line["configs"]["cfg4_serving"] = "serving" if a.serve > 0 else "not run (--serve 0)"
     if not a.no_cpu:

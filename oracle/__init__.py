@@ -91,6 +91,8 @@ def lib():
         L.orc_max_threads.restype = C.c_int
         L.orc_set_krylov_parallel.argtypes = [C.c_int]
         L.orc_set_krylov_parallel.restype = None
+        L.orc_set_progress.argtypes = [C.c_int]
+        L.orc_set_progress.restype = None
         L.orc_splitmix64.restype = C.c_uint64
         L.orc_splitmix64.argtypes = [C.POINTER(C.c_uint64)]
         L.orc_uniform01.restype = C.c_double
@@ -107,6 +109,12 @@ def set_threads(n: int, krylov_parallel: bool = False):
     assembly stay serial)."""
     lib().orc_set_threads(int(n))
     lib().orc_set_krylov_parallel(1 if krylov_parallel else 0)
+
+
+def set_progress(every: int):
+    """orc_pcg reports its iteration count to stderr every `every`
+    iterations (0: silent)."""
+    lib().orc_set_progress(int(every))
 
 
 def max_threads() -> int:

@@ -13,6 +13,7 @@
 #include <float.h>
 #include <math.h>
 #include <omp.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -789,6 +790,9 @@ void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 int orc_max_threads(void) { return omp_get_max_threads(); }
 void orc_set_krylov_parallel(int on) { g_krylov_parallel = on; }
 
+static int g_progress = 0;
+void orc_set_progress(int every) { g_progress = every > 0 ? every : 0; }
+
 void orc_spmv(const orc_system* s, const double* x, double* y) {
 #pragma omp parallel for if (g_krylov_parallel) schedule(static)
     for (int n = 0; n < s->n; ++n) {
@@ -874,6 +878,10 @@ int orc_pcg(const orc_system* s, const double* inv_e, double tol, int max_iter, 
 #pragma omp parallel for if (g_krylov_parallel) schedule(static)
             for (int i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
             ++it;
+            if (g_progress > 0 && it % g_progress == 0) {
+                fprintf(stderr, "# oracle pcg: iteration %d, |r|^2/|b|^2 %.3e\n", it, res2 / rhs2);
+                fflush(stderr);
+            }
         }
         *iters = it;
         *err = sqrt(res2 / rhs2);

@@ -152,6 +152,9 @@ void orc_spmv(const orc_system* sys, const double* x, double* y);
 void orc_set_threads(int n);
 int orc_max_threads(void);
 void orc_set_krylov_parallel(int on);
+/* every > 0: orc_pcg prints its iteration count and |r|^2/|b|^2 to stderr
+ * every `every` iterations (progress of long solves); 0: silent. */
+void orc_set_progress(int every);
 
 /* Preconditioned CG with Eigen's ConjugateGradient semantics (x0 = 0,
  * recursive residual, stop when |r|^2 < max(tol^2 |b|^2, DBL_MIN)).

@@ -1,7 +1,9 @@
 """GPU parity of the projection against the CPU oracle, through the C ABI.
 
 Bit-exact: indices (projection.cpp:14-28), A and b (projection.cpp:30-83),
-the velocity update (projection.cpp:93-134) for a given pressure.
+the velocity update (projection.cpp:93-134) for a given pressure; also on
+grids with one axis of 4092 cells (1023 bricks, the limit of the packed
+brick coordinates).
 Tolerance (solve): both sides converge to relative residual 1e-8 with the
 same preconditioner algebra; the pressure agrees to within
 max|p - p_ref| <= 2e-6 * max|p_ref| (FP64) and 2e-5 (mixed), the true FP64
@@ -31,7 +33,32 @@ SCENES = {
     # interface 1e-5 from cell centres: ghost-fluid faces with theta < 0.01 on
     # every axis, so A, b and the update go through the clamp (projection.cpp:59-63,100-131)
     "clamp16": lambda: elq.scene_clamp(16),
+    # one axis of 4092 cells, 1023 bricks: the largest brick coordinate the
+    # packed coordinates and block lists hold
+    "long_x": lambda: _long(0),
+    "long_y": lambda: _long(1),
+    "long_z": lambda: _long(2),
 }
+
+
+def _long(axis):
+    """10 x 10 cells across, 4092 along `axis`; face velocities U[-0.5, 0.5].
+    Along x or z a slab fills 55% of the height over the whole length; along
+    y (gravity) a slab 5% deep and a drop just under the top wall, so that
+    unknowns reach brick row 1021."""
+    dims = [10, 10, 10]
+    dims[axis] = 4092
+    ni, nj, nk = dims
+    dx = 1.0 / ni
+    lx, ly, lz = ni * dx, nj * dx, nk * dx
+    if axis == 1:
+        slab, sph = 0.05 * ly, [(0.5 * lx, ly - 5.5 * dx, 0.5 * lz, 2.5 * dx)]
+    else:
+        slab, sph = 0.55 * ly, []
+    s = elq.make_scene(ni, nj, nk, slab, sph, 0.01, 1.0)
+    elq.Rng(3).velocity(s, 0.5)
+    elq.valid_and_zero(s)
+    return s
 
 
 def _odd():
@@ -99,6 +126,20 @@ def test_projection_fp64(ctx, name, schedule):
     umax = max(np.abs(ref.u).max(), np.abs(ref.v).max(), np.abs(ref.w).max())
     for f in ("u", "v", "w"):
         assert np.abs(getattr(g, f) - getattr(ref, f)).max() <= 2e-6 * umax, f
+
+
+@pytest.mark.parametrize("precision, tol", [(elq.FP64, 2e-6), (elq.MIXED, 2e-5)])
+@pytest.mark.parametrize("name", ["long_x", "long_y", "long_z"])
+def test_projection_long_axis(ctx, name, precision, tol):
+    s = SCENES[name]()
+    g = s.copy()
+    o = elq.default_opts(tol=1e-8, max_iter=5000, precision=precision)
+    p, idx, st = ctx.solve_pressure(g, o)
+    assert st.status == elq.OK and st.converged, st.as_dict()
+    _check_solution(s, p, idx, st, tol)
+    if name == "long_y":   # unknowns up to the top brick rows
+        idx_rows = (idx.reshape(s.nk, s.nj, s.ni) != O.UNKNOWN_INDEX).any(axis=(0, 2))
+        assert np.nonzero(idx_rows)[0].max() // 4 >= 1020
 
 
 @pytest.mark.parametrize("name", ["drop32", "spheres24", "aniso", "clamp16"])

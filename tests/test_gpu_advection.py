@@ -50,7 +50,10 @@ def random_fields(ni, nj, nk, seed, amp):
 CASES = [((40, 40, 40), 1 / 40, 0.01, 1.0, 1), ((23, 17, 29), 1 / 29, 0.02, 0.8, 2),
          ((16, 16, 16), 1 / 16, 0.5, 1.0, 3),    # leaves the grid
          # an axis of one cell (every index on it clamps to 0), large non-dyadic steps
-         ((1, 7, 5), 1 / 7, 0.05, 1.0, 4), ((33, 1, 2), 0.37, 0.9, 2.0, 5), ((3, 2, 1), 0.3, 0.2, 1.5, 6)]
+         ((1, 7, 5), 1 / 7, 0.05, 1.0, 4), ((33, 1, 2), 0.37, 0.9, 2.0, 5), ((3, 2, 1), 0.3, 0.2, 1.5, 6),
+         # axes of 4092 cells
+         ((4092, 3, 2), 1 / 4092, 0.01, 1.0, 7), ((2, 4092, 3), 1 / 64, 0.05, 1.0, 8),
+         ((3, 2, 4092), 1 / 64, 0.05, 1.0, 9)]
 
 
 @pytest.mark.parametrize("dims,dx,dt,amp,seed", CASES)

@@ -57,7 +57,9 @@ CASES = [((37, 21, 19), 0.02, 1, 10), ((37, 21, 19), 0.3, 2, 3), ((64, 16, 16), 
          ((16, 16, 16), 0.1, 7, 0), ((16, 16, 16), 0.1, 8, -3),
          # one-cell axes; a row of exactly one 64-cell word and one of 65 cells
          ((1, 30, 20), 0.02, 9, 10), ((25, 1, 1), 0.08, 11, 10), ((64, 5, 1), 0.01, 11, 10),
-         ((65, 1, 7), 0.01, 12, 10)]
+         ((65, 1, 7), 0.01, 12, 10),
+         # axes of 4092 cells (rows of 64 words; 4092 rows; 4092 planes)
+         ((4092, 3, 5), 0.01, 13, 10), ((5, 4092, 3), 0.01, 14, 10), ((3, 5, 4092), 0.01, 15, 10)]
 
 
 @pytest.mark.parametrize("dims,density,seed,layers", CASES)

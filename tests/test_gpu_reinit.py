@@ -13,7 +13,8 @@ same count of passes and of unreached cells:
   route's stamp rebase forced every 1 to 80 passes;
 - the 128^3 drop scene's level set after one advection step (about 100
   passes, several host batches);
-- grids one or two cells thick on some axis.
+- grids one or two cells thick on some axis, and fronts along an axis of
+  4092 cells.
 
 The library takes one of two routes, both checked here: passes ordered by
 the taxicab distance to the interface where that distance is the pass count
@@ -134,6 +135,20 @@ def test_wall_with_hole_detour_bit_exact(ctx):
     left, passes = O.reinitialize(ref, valid, dims, dx)
     assert left == 0 and passes > 40   # 21 passes reach the far corner without the wall
     check(ctx, phi, valid, dims, dx, route="rei_pass")
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_front_along_a_4092_cell_axis(ctx, axis):
+    # a plane across the middle of a 4092-cell axis: about 2046 passes, the
+    # distance route's lists and 16-bit distances near their longest
+    dims = [6, 6, 6]
+    dims[axis] = 4092
+    dx = 1.0 / 4092
+    x, y, z = xyz(tuple(dims), dx)
+    coord = (x, y, z)[axis]
+    phi = (1.7 * (coord - 0.5 - 0.3 * dx)).astype(np.float32)
+    valid = np.ones(phi.size, np.uint8)
+    assert check(ctx, phi, valid, tuple(dims), dx) > 2000
 
 
 def wall_with_hole():

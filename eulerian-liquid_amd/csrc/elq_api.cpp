@@ -135,13 +135,18 @@ static void copy_parallel(void* dst, const void* src, size_t n) {
     }
     const size_t per = (n / T + 4095) & ~(size_t)4095;
     std::thread th[T - 1];
-    int started = 0;
-    for (int t = 1; t < T && (size_t)t * per < n; ++t, ++started) {
+    for (int t = 1; t < T && (size_t)t * per < n; ++t) {
         const size_t a = (size_t)t * per, b = std::min(n, a + per);
-        th[t - 1] = std::thread([=] { std::memcpy((uint8_t*)dst + a, (const uint8_t*)src + a, b - a); });
+        auto part = [=] { std::memcpy((uint8_t*)dst + a, (const uint8_t*)src + a, b - a); };
+        try {
+            th[t - 1] = std::thread(part);
+        } catch (...) {  // no thread to be had: this part on the calling thread (nothing throws past the C ABI)
+            part();
+        }
     }
     std::memcpy(dst, src, std::min(n, per));
-    for (int t = 0; t < started; ++t) th[t].join();
+    for (auto& x : th)
+        if (x.joinable()) x.join();
 }
 
 hipError_t HostStage::ready() {

@@ -32,12 +32,19 @@ void parallel_k(int nk, F f) {
         for (int k = 0; k < nk; ++k) f(k);
         return;
     }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t)
-        th.emplace_back([=] {
+    std::vector<std::thread> th(nt);
+    for (int t = 0; t < nt; ++t) {
+        auto part = [=] {
             for (int k = (int)((int64_t)nk * t / nt); k < (int)((int64_t)nk * (t + 1) / nt); ++k) f(k);
-        });
-    for (auto& x : th) x.join();
+        };
+        try {
+            th[t] = std::thread(part);
+        } catch (...) {  // no thread to be had: this range on the calling thread
+            part();
+        }
+    }
+    for (auto& x : th)
+        if (x.joinable()) x.join();
 }
 
 

@@ -204,7 +204,14 @@ int elq_server_create(int device, const elq_server_opts* opts, elq_server** out)
         sl.ctx->max_groups = groups;
     }
     S->t0 = std::chrono::steady_clock::now();
-    for (int k = 0; k < opts->max_slots; ++k) S->slots[(size_t)k].th = std::thread(&elq_server::worker, S, k);
+    for (int k = 0; k < opts->max_slots; ++k) {
+        try {
+            S->slots[(size_t)k].th = std::thread(&elq_server::worker, S, k);
+        } catch (...) {  // no host thread for a slot: undo and report, nothing throws past the C ABI
+            elq_server_destroy(S);
+            return ELQ_ERR_DEVICE;
+        }
+    }
     *out = S;
     return ELQ_OK;
 }

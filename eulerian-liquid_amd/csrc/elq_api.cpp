@@ -982,7 +982,7 @@ void elq_default_opts(elq_opts* o) {
     o->check_every = 8;
 }
 
-int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
+int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) try {
     if (!out) return ELQ_ERR_INVALID_ARG;
     *out = nullptr;
     int n = 0;
@@ -1018,9 +1018,11 @@ int elq_ctx_create(int device, size_t hbm_budget_bytes, elq_ctx** out) {
     (void)hipGetLastError();
     *out = ctx;
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-void elq_ctx_destroy(elq_ctx* ctx) {
+void elq_ctx_destroy(elq_ctx* ctx) try {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
@@ -1032,41 +1034,50 @@ void elq_ctx_destroy(elq_ctx* ctx) {
     (void)hipStreamDestroy(ctx->stream);
     if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
     delete ctx;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
 }
 
 const char* elq_ctx_last_error(const elq_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
 
-int elq_set_graph_capture(elq_ctx* ctx, int enable) {
+int elq_set_graph_capture(elq_ctx* ctx, int enable) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
     if (!enable) ctx->drop_chunk_graph();
     ctx->graphs = enable != 0;
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_set_reinit_rebase(elq_ctx* ctx, int32_t passes) {
+int elq_set_reinit_rebase(elq_ctx* ctx, int32_t passes) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (passes < 1 || passes > kReiRebase) return ctx->fail(ELQ_ERR_INVALID_ARG, "rebase interval out of range");
     ctx->rei_rebase = passes;
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_set_wait_limit(elq_ctx* ctx, double seconds) {
+int elq_set_wait_limit(elq_ctx* ctx, double seconds) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!(seconds >= 0.0) || seconds > 3600.0) return ctx->fail(ELQ_ERR_INVALID_ARG, "wait limit out of range");
     ctx->timeout_ticks = (uint64_t)(seconds * (double)ctx->clock_hz);
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 size_t elq_ctx_device_bytes(const elq_ctx* ctx) { return ctx ? ctx->ws.bytes() : 0; }
 
-size_t elq_projection_footprint(int32_t ni, int32_t nj, int32_t nk, int32_t precision, const float* level_set) {
+size_t elq_projection_footprint(int32_t ni, int32_t nj, int32_t nk, int32_t precision, const float* level_set) try {
     if (!valid_dims(ni, nj, nk)) return 0;
     const int64_t nb = level_set ? liquid_bricks(level_set, ni, nj, nk) : -1;
     return elq_footprint_bytes(ni, nj, nk, precision, true, nb);
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return 0;
 }
 
-int elq_project_device(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) {
+int elq_project_device(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) try {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     elq_opts o;
@@ -1080,9 +1091,11 @@ int elq_project_device(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts
     rc = run_project(ctx, *g, dt, o, st);
     if (st && rc != ELQ_OK && st->status == 0) st->status = rc;
     return rc;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_project_device_noalloc(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) {
+int elq_project_device_noalloc(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) try {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     if (!ctx->scratch) return ctx->fail(ELQ_ERR_DEVICE, "static scratch array unavailable on this device");
@@ -1122,9 +1135,11 @@ int elq_project_device_noalloc(elq_ctx* ctx, const elq_grid* g, float dt, const 
     }
     if (st && rc != ELQ_OK && st->status == 0) st->status = rc;
     return rc;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_project(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) {
+int elq_project(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, elq_stats* st) try {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     elq_opts o;
@@ -1166,9 +1181,11 @@ int elq_project(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts,
     const hipError_t e = ctx->stage.d2h(back, ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "copy back");
     return rc;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_assign_indices(elq_ctx* ctx, const elq_grid* g, int32_t* indices, int64_t* n_unknowns) {
+int elq_assign_indices(elq_ctx* ctx, const elq_grid* g, int32_t* indices, int64_t* n_unknowns) try {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     if (!indices) return ctx->fail(ELQ_ERR_INVALID_ARG, "null indices");
@@ -1184,10 +1201,12 @@ int elq_assign_indices(elq_ctx* ctx, const elq_grid* g, int32_t* indices, int64_
     if (ctx->ws.n_unknowns == 0) return ctx->fail(ELQ_ERR_NO_FLUID, "No fluid");
     if (ctx->ws.setup_err & kErrBoundary) return ctx->fail(ELQ_ERR_BOUNDARY, "unknown on the outer cell layer");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_build_matrix(elq_ctx* ctx, const elq_grid* g, float dt, double* diag, double* ax, double* ay, double* az,
-                     double* b) {
+                     double* b) try {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     if (!diag || !ax || !ay || !az || !b) return ctx->fail(ELQ_ERR_INVALID_ARG, "null output");
@@ -1215,9 +1234,11 @@ int elq_build_matrix(elq_ctx* ctx, const elq_grid* g, float dt, double* diag, do
         HIPCHK(hipMemcpyAsync(outs[q], e + q * n, n * 8, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_ic_factor(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, double* inv_e) {
+int elq_ic_factor(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opts, double* inv_e) try {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     if (!inv_e) return ctx->fail(ELQ_ERR_INVALID_ARG, "null output");
@@ -1265,9 +1286,11 @@ int elq_ic_factor(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opt
     HIPCHK(hipMemcpyAsync(inv_e, W.export_buf.p, n * 8, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_update_velocity(elq_ctx* ctx, const elq_grid* g, float dt) {
+int elq_update_velocity(elq_ctx* ctx, const elq_grid* g, float dt) try {
     int rc = check_grid(ctx, g);
     if (rc) return rc;
     if (!g->pressure) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pressure");
@@ -1282,10 +1305,12 @@ int elq_update_velocity(elq_ctx* ctx, const elq_grid* g, float dt) {
     HIPCHK(ctx->stage.d2h({{g->u, dg.u, z.u * 4}, {g->v, dg.v, z.v * 4}, {g->w, dg.w, z.w * 4}}, ctx->stream),
            "d2h");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_compute_weights_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, const float* solid_nodal,
-                               float* u_weight, float* v_weight, float* w_weight) {
+                               float* u_weight, float* v_weight, float* w_weight) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!solid_nodal || !u_weight || !v_weight || !w_weight) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
@@ -1297,9 +1322,11 @@ int elq_compute_weights_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk,
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_add_gravity_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* v, float dt) {
+int elq_add_gravity_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* v, float dt) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!v) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
@@ -1311,19 +1338,23 @@ int elq_add_gravity_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, flo
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_valid_and_zero_device(elq_ctx* ctx, const elq_grid* g) {
+int elq_valid_and_zero_device(elq_ctx* ctx, const elq_grid* g) try {
     const int rc = check_grid(ctx, g);
     if (rc) return rc;
     (void)hipSetDevice(ctx->device);
     HIPCHK(launch_valid_zero(dense_of(*g), ctx->stream), "k_valid_zero");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_find_valid_velocity_device(elq_ctx* ctx, const elq_grid* g, uint8_t* u_valid, uint8_t* v_valid,
-                                   uint8_t* w_valid) {
+                                   uint8_t* w_valid) try {
     const int rc = check_grid(ctx, g);
     if (rc) return rc;
     if (!u_valid || !v_valid || !w_valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
@@ -1335,10 +1366,12 @@ int elq_find_valid_velocity_device(elq_ctx* ctx, const elq_grid* g, uint8_t* u_v
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_set_invalid_velocity_zero_device(elq_ctx* ctx, const elq_grid* g, const uint8_t* u_valid,
-                                         const uint8_t* v_valid, const uint8_t* w_valid) {
+                                         const uint8_t* v_valid, const uint8_t* w_valid) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!g || !g->u || !g->v || !g->w || !u_valid || !v_valid || !w_valid)
         return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
@@ -1354,10 +1387,12 @@ int elq_set_invalid_velocity_zero_device(elq_ctx* ctx, const elq_grid* g, const 
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_find_valid_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, const float* solid_nodal,
-                                    uint8_t* valid) {
+                                    uint8_t* valid) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!solid_nodal || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
@@ -1369,6 +1404,8 @@ int elq_find_valid_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 namespace {
@@ -1447,27 +1484,31 @@ int run_extrapolate(elq_ctx* ctx, const ExtArgs* a, int count, int layers) {
 }  // namespace
 
 int elq_extrapolate_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* grid, uint8_t* valid,
-                           int32_t layers) {
+                           int32_t layers) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!grid || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
     (void)hipSetDevice(ctx->device);
     const ExtArgs a{ni, nj, nk, grid, valid, valid};
     return run_extrapolate(ctx, &a, 1, layers);
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_extrapolate_const_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float* grid,
-                                 const uint8_t* valid, int32_t layers) {
+                                 const uint8_t* valid, int32_t layers) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!grid || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
     (void)hipSetDevice(ctx->device);
     const ExtArgs a{ni, nj, nk, grid, valid, nullptr};
     return run_extrapolate(ctx, &a, 1, layers);
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_extrapolate_velocity_device(elq_ctx* ctx, const elq_grid* g, uint8_t* u_valid, uint8_t* v_valid,
-                                    uint8_t* w_valid, int32_t layers) {
+                                    uint8_t* w_valid, int32_t layers) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!g || !g->u || !g->v || !g->w || !u_valid || !v_valid || !w_valid)
         return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
@@ -1478,11 +1519,13 @@ int elq_extrapolate_velocity_device(elq_ctx* ctx, const elq_grid* g, uint8_t* u_
                           {ni, nj + 1, nk, g->v, v_valid, v_valid},
                           {ni, nj, nk + 1, g->w, w_valid, w_valid}};
     return run_extrapolate(ctx, a, 3, layers);
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_advect_staggered_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt,
                                 float* out_u, float* out_v, float* out_w, const float* val_u, const float* val_v,
-                                const float* val_w, const float* vel_u, const float* vel_v, const float* vel_w) {
+                                const float* val_w, const float* vel_u, const float* vel_v, const float* vel_w) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!out_u || !out_v || !out_w || !val_u || !val_v || !val_w || !vel_u || !vel_v || !vel_w)
         return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
@@ -1501,10 +1544,12 @@ int elq_advect_staggered_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_advect_collocated_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt, float* out,
-                                 const float* val, const float* vel_u, const float* vel_v, const float* vel_w) {
+                                 const float* val, const float* vel_u, const float* vel_v, const float* vel_w) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!out || !val || !vel_u || !vel_v || !vel_w) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk) || !(dx > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
@@ -1518,9 +1563,11 @@ int elq_advect_collocated_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t n
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_advect_velocity_device(elq_ctx* ctx, const elq_grid* g, float dt) {
+int elq_advect_velocity_device(elq_ctx* ctx, const elq_grid* g, float dt) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!g || !g->u || !g->v || !g->w) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(g->ni, g->nj, g->nk) || !(g->dx > 0.0f))
@@ -1542,10 +1589,12 @@ int elq_advect_velocity_device(elq_ctx* ctx, const elq_grid* g, float dt) {
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_advect_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float dt,
-                                float* level_set, const float* u, const float* v, const float* w) {
+                                float* level_set, const float* u, const float* v, const float* w) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!level_set || !u || !v || !w) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk) || !(dx > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
@@ -1562,10 +1611,12 @@ int elq_advect_level_set_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk
     ctx->prof_collect();
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, float dx, float* level_set,
-                            const uint8_t* valid, int64_t* unreached, int32_t* passes) {
+                            const uint8_t* valid, int64_t* unreached, int32_t* passes) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (!level_set || !valid) return ctx->fail(ELQ_ERR_INVALID_ARG, "null pointer");
     if (!valid_dims(ni, nj, nk) || !(dx > 0.0f)) return ctx->fail(ELQ_ERR_INVALID_ARG, "grid dimensions out of range");
@@ -1685,66 +1736,82 @@ int elq_reinitialize_device(elq_ctx* ctx, int32_t ni, int32_t nj, int32_t nk, fl
     // fixed any, else before the last rebase (the batch ran on past the end)
     if (passes) *passes = fin[1] ? (int32_t)(first + fin[1]) : (int32_t)fin[4];
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out) {
+int elq_device_alloc(elq_ctx* ctx, size_t bytes, void** out) try {
     if (!ctx || !out) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
     HIPCHK(hipMalloc(out, bytes), "hipMalloc");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_device_free(elq_ctx* ctx, void* ptr) {
+int elq_device_free(elq_ctx* ctx, void* ptr) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     HIPCHK(hipFree(ptr), "hipFree");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_memcpy_h2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
+int elq_memcpy_h2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "h2d");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_memcpy_d2h(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
+int elq_memcpy_d2h(elq_ctx* ctx, void* dst, const void* src, size_t bytes) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_memcpy_d2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes) {
+int elq_memcpy_d2d(elq_ctx* ctx, void* dst, const void* src, size_t bytes) try {
     if (!ctx || (bytes & 3) || (((uintptr_t)dst | (uintptr_t)src) & 3)) return ELQ_ERR_INVALID_ARG;
     if (bytes == 0) return ELQ_OK;
     (void)hipSetDevice(ctx->device);
     HIPCHK(launch_copy(dst, src, bytes, ctx->stream), "k_copy");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-void elq_set_profiling(elq_ctx* ctx, int enable) {
+void elq_set_profiling(elq_ctx* ctx, int enable) try {
     if (ctx) ctx->profiling = enable != 0;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
 }
 
-void elq_set_profiling_filter(elq_ctx* ctx, const char* classes) {
+void elq_set_profiling_filter(elq_ctx* ctx, const char* classes) try {
     if (!ctx) return;
     ctx->prof_filter.clear();
     if (classes && *classes) ctx->prof_filter = "," + std::string(classes) + ",";
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
 }
 
-int elq_set_max_groups(elq_ctx* ctx, int max_groups) {
+int elq_set_max_groups(elq_ctx* ctx, int max_groups) try {
     if (!ctx) return ELQ_ERR_INVALID_ARG;
     if (max_groups < 0) return ctx->fail(ELQ_ERR_INVALID_ARG, "max_groups must be >= 0");
     ctx->max_groups = max_groups;
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, double* ms, int64_t* launches, int cap) {
+int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, double* ms, int64_t* launches, int cap) try {
     if (!ctx || !names || names_cap == 0) return 0;
     std::string all;
     int n = 0;
@@ -1759,6 +1826,8 @@ int elq_last_kernel_times(const elq_ctx* ctx, char* names, size_t names_cap, dou
     std::strncpy(names, all.c_str(), names_cap - 1);
     names[names_cap - 1] = '\0';
     return n;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return 0;
 }
 
 }  // extern "C"

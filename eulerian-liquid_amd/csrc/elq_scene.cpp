@@ -130,7 +130,7 @@ extern "C" {
 double elq_uniform01(uint64_t* state) { return state ? unit(*state) : 0.0; }
 
 int elq_scene_level_set(int32_t ni, int32_t nj, int32_t nk, float dx, float slab_height, int32_t n_spheres,
-                        const float* spheres, float* level_set) {
+                        const float* spheres, float* level_set) try {
     if (ni < 1 || nj < 1 || nk < 1 || !level_set || (n_spheres > 0 && !spheres)) return ELQ_ERR_INVALID_ARG;
     parallel_k(nk, [&](int k) {
         for (int j = 0; j < nj; ++j)
@@ -149,9 +149,11 @@ int elq_scene_level_set(int32_t ni, int32_t nj, int32_t nk, float dx, float slab
             }
     });
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_scene_room_nodal(int32_t ni, int32_t nj, int32_t nk, float* nodal) {
+int elq_scene_room_nodal(int32_t ni, int32_t nj, int32_t nk, float* nodal) try {
     if (ni < 1 || nj < 1 || nk < 1 || !nodal) return ELQ_ERR_INVALID_ARG;
     // room() on the nodal grid of (ni+1)(nj+1)(nk+1) nodes, origin -dx/2
     const int a = ni + 1, b = nj + 1, c = nk + 1;
@@ -168,9 +170,11 @@ int elq_scene_room_nodal(int32_t ni, int32_t nj, int32_t nk, float* nodal) {
                 nodal[at(i, j, k, a, b)] = std::min(std::numeric_limits<float>::max(), v);
             }
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* uw, float* vw, float* ww) {
+int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* uw, float* vw, float* ww) try {
     if (ni < 1 || nj < 1 || nk < 1 || !uw || !vw || !ww) return ELQ_ERR_INVALID_ARG;
     const int a = ni + 1, b = nj + 1, c = nk + 1;
     std::vector<float> nodal((size_t)a * b * c);
@@ -195,17 +199,21 @@ int elq_scene_weights(int32_t ni, int32_t nj, int32_t nk, float* uw, float* vw, 
                     std::clamp(1.0f - inside4(N(i, j, k), N(i, j + 1, k), N(i + 1, j, k), N(i + 1, j + 1, k)), 0.0f,
                                1.0f);
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_scene_add_gravity(int32_t ni, int32_t nj, int32_t nk, float* v, float dt) {
+int elq_scene_add_gravity(int32_t ni, int32_t nj, int32_t nk, float* v, float dt) try {
     if (ni < 1 || nj < 1 || nk < 1 || !v) return ELQ_ERR_INVALID_ARG;
     const float g = 9.80665f * dt;  // constants::gravity<float> * dt
     const size_t n = (size_t)ni * (nj + 1) * nk;
     for (size_t c = 0; c < n; ++c) v[c] -= g;
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_scene_valid_and_zero(const elq_grid* g) {
+int elq_scene_valid_and_zero(const elq_grid* g) try {
     if (!g || !g->level_set || !g->u || !g->v || !g->w || !g->u_weight || !g->v_weight || !g->w_weight)
         return ELQ_ERR_INVALID_ARG;
     const int ni = g->ni, nj = g->nj, nk = g->nk;
@@ -234,10 +242,12 @@ int elq_scene_valid_and_zero(const elq_grid* g) {
     for (size_t c = 0; c < nw; ++c)
         if (!wv[c]) g->w[c] = 0.0f;
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_scene_random_spheres(uint64_t* state, int32_t count, double c_lo, double c_hi, double r_lo, double r_hi,
-                             double lx, double ly, double lz, float* spheres) {
+                             double lx, double ly, double lz, float* spheres) try {
     if (!state || count < 0 || (count > 0 && !spheres)) return ELQ_ERR_INVALID_ARG;
     for (int s = 0; s < count; ++s) {
         spheres[4 * s + 0] = draw(*state, c_lo * lx, c_hi * lx);
@@ -246,16 +256,20 @@ int elq_scene_random_spheres(uint64_t* state, int32_t count, double c_lo, double
         spheres[4 * s + 3] = draw(*state, r_lo, r_hi);
     }
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 int elq_scene_random_velocity(uint64_t* state, int32_t ni, int32_t nj, int32_t nk, double amp, float* u, float* v,
-                              float* w) {
+                              float* w) try {
     if (!state || !u || !v || !w || ni < 1 || nj < 1 || nk < 1) return ELQ_ERR_INVALID_ARG;
     const size_t nu = (size_t)(ni + 1) * nj * nk, nv = (size_t)ni * (nj + 1) * nk, nw = (size_t)ni * nj * (nk + 1);
     for (size_t c = 0; c < nu; ++c) u[c] = draw(*state, -amp, amp);
     for (size_t c = 0; c < nv; ++c) v[c] = draw(*state, -amp, amp);
     for (size_t c = 0; c < nw; ++c) w[c] = draw(*state, -amp, amp);
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 }  // extern "C"

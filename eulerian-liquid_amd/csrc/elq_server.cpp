@@ -169,7 +169,7 @@ struct elq_server {
 
 extern "C" {
 
-int elq_server_create(int device, const elq_server_opts* opts, elq_server** out) {
+int elq_server_create(int device, const elq_server_opts* opts, elq_server** out) try {
     if (!out) return ELQ_ERR_INVALID_ARG;
     *out = nullptr;
     if (!opts || opts->max_slots < 1 || opts->max_slots > kMaxSlots || opts->max_queue < 1)
@@ -214,9 +214,11 @@ int elq_server_create(int device, const elq_server_opts* opts, elq_server** out)
     }
     *out = S;
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-void elq_server_destroy(elq_server* S) {
+void elq_server_destroy(elq_server* S) try {
     if (!S) return;
     {
         std::lock_guard<std::mutex> lk(S->mu);
@@ -228,9 +230,10 @@ void elq_server_destroy(elq_server* S) {
         if (sl.th.joinable()) sl.th.join();
     for (auto& sl : S->slots) elq_ctx_destroy(sl.ctx);
     delete S;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
 }
 
-int elq_server_submit(elq_server* S, const elq_grid* g, float dt, int64_t* id) {
+int elq_server_submit(elq_server* S, const elq_grid* g, float dt, int64_t* id) try {
     if (!S || !g || !g->level_set || !g->u || !g->v || !g->w || !g->u_weight || !g->v_weight || !g->w_weight)
         return ELQ_ERR_INVALID_ARG;
     const size_t need = elq_projection_footprint(g->ni, g->nj, g->nk, S->o.opts.precision, g->level_set);
@@ -249,9 +252,11 @@ int elq_server_submit(elq_server* S, const elq_grid* g, float dt, int64_t* id) {
     if (id) *id = r.id;
     S->admit();
     return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
-int elq_server_wait(elq_server* S, int timeout_ms) {
+int elq_server_wait(elq_server* S, int timeout_ms) try {
     if (!S) return 0;
     std::unique_lock<std::mutex> lk(S->mu);
     auto ready = [&] { return !S->done.empty() || (S->queue.empty() && S->running == 0); };
@@ -260,9 +265,11 @@ int elq_server_wait(elq_server* S, int timeout_ms) {
     else
         S->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
     return (int)S->done.size();
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return 0;
 }
 
-int elq_server_poll(elq_server* S, elq_result* out, int cap) {
+int elq_server_poll(elq_server* S, elq_result* out, int cap) try {
     if (!S || !out || cap <= 0) return 0;
     std::lock_guard<std::mutex> lk(S->mu);
     int n = 0;
@@ -271,20 +278,26 @@ int elq_server_poll(elq_server* S, elq_result* out, int cap) {
         S->done.pop_front();
     }
     return n;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return 0;
 }
 
-int elq_server_in_flight(const elq_server* S) {
+int elq_server_in_flight(const elq_server* S) try {
     if (!S) return 0;
     auto* s = const_cast<elq_server*>(S);
     std::lock_guard<std::mutex> lk(s->mu);
     return (int)s->queue.size() + s->running;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return 0;
 }
 
-size_t elq_server_device_bytes(const elq_server* S) {
+size_t elq_server_device_bytes(const elq_server* S) try {
     if (!S) return 0;
     auto* s = const_cast<elq_server*>(S);
     std::lock_guard<std::mutex> lk(s->mu);
     return s->held_total();
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return 0;
 }
 
 }  // extern "C"

@@ -40,7 +40,8 @@ typedef enum {
     ELQ_ERR_NOT_CONVERGED = 2,  /* max_iter reached; the reference ignores this
                                    (simulator.cpp:105-109), the C++ shim does too */
     ELQ_ERR_BREAKDOWN = 3,      /* non-finite scalar inside the Krylov loop */
-    ELQ_ERR_DEVICE = 4,         /* HIP error or no gfx950 device */
+    ELQ_ERR_DEVICE = 4,         /* HIP error, no gfx950 device, or host memory
+                                   or threads exhausted */
     ELQ_ERR_ADMISSION = 5,      /* request exceeds the context's HBM budget */
     ELQ_ERR_INVALID_ARG = 6,
     ELQ_ERR_BOUNDARY = 7,       /* an unknown lies on the outer cell layer: the

@@ -324,10 +324,7 @@ size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, i
     b += (size_t)nb * (8 * 4 + 4 + (size_t)kXchWords * 8 + 2 * 8);  // nbr, lvl_list, xch, partials
     b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
     b += cells * (3 * 8 + 10 * tsz + 4);                         // diag64, b64, inv_e64, coefficients, vectors, inv_e32
-    if (precision == ELQ_PRECISION_MIXED) {
-        b += cells * 8;                                           // x64
-        b += cells * 12 + stream_face_words(p.nbx, p.nby, p.nbz) * 8;  // stream sweeps: couplings, face words
-    }
+    if (precision == ELQ_PRECISION_MIXED) b += cells * 8;        // x64
     const size_t faces = (size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + (size_t)ni * nj * (nk + 1);
     if (staged) b += 4 * (C + 2 * faces);
     return b;
@@ -552,38 +549,8 @@ struct Solver {
     Coef<T> c;
     const elq_opts* o;
     bool levels;
-    bool stream = false;  // stream sweeps (FP32 storage only)
-    Pencils pc4;          // blocks of the stream sweeps (4x4 shape)
-    const uint8_t* bflag4 = nullptr;
 
     Ctrl* ctrl() { return (Ctrl*)ctx->ws.ctrl.p; }
-
-    // one stream sweep (mode: StreamMode) ending in a grid tail that applies tail_mode
-    int stream_sweep(int mode, int tail_mode, T* p_cur) {
-        if constexpr (std::is_same<T, float>::value) {
-            Workspace& W = ctx->ws;
-            StreamArgs a;
-            a.r = vec(W.vr);
-            a.x = vec(W.vx);
-            a.p = p_cur;
-            a.q = vec(W.vq);
-            a.y = vec(W.vy);
-            a.ie = c.inv_e;
-            const bool fwd = mode != kStreamBwd;
-            a.c1 = fwd ? (const float*)W.lxm.p : c.ax;
-            a.c2 = fwd ? (const float*)W.lym.p : c.ay;
-            a.c3 = fwd ? (const float*)W.lzm.p : c.az;
-            HIPCHK(launch_stream(bk, pc4, a, mode, ctrl(), (uint64_t*)W.face.p, bflag4, stage(), tail_mode,
-                                 ctx->timeout_ticks, ctx->max_groups, ctx->stream),
-                   "k_stream");
-            return ELQ_OK;
-        } else {
-            (void)mode;
-            (void)tail_mode;
-            (void)p_cur;
-            return ctx->fail(ELQ_ERR_INVALID_ARG, "stream sweeps need FP32 storage");
-        }
-    }
     double* prr() { return (double*)ctx->ws.part_rr.p; }
     double* prz() { return (double*)ctx->ws.part_rz.p; }
     uint64_t* stage() { return (uint64_t*)ctx->ws.stage.p; }
@@ -601,11 +568,6 @@ struct Solver {
     int fwd(int update, T* p_cur) {
         Workspace& W = ctx->ws;
         ctx->prof_begin(update ? "fwd_sweep_update" : "fwd_sweep");
-        if (stream) {
-            const int rc = stream_sweep(update ? kStreamFwdUpdate : kStreamFwd, update ? SC_RR : SC_NOP, p_cur);
-            ctx->prof_end();
-            return rc;
-        }
         if (!levels) {
             HIPCHK(launch_fwd<T>(bk, pc, c, vec(W.vr), vec(W.vx), p_cur, vec(W.vq), vec(W.vy), prr(), ctrl(),
                                  (uint64_t*)W.xch.p, stage(), update, -1, 0, 0, ctx->timeout_ticks,
@@ -630,11 +592,6 @@ struct Solver {
     int bwd(int mode) {
         Workspace& W = ctx->ws;
         ctx->prof_begin("bwd_sweep");
-        if (stream) {
-            const int rc = stream_sweep(kStreamBwd, mode, nullptr);
-            ctx->prof_end();
-            return rc;
-        }
         if (!levels) {
             HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(), mode,
                                  -1, 0, 0, ctx->timeout_ticks, ctx->max_groups, ctx->stream),
@@ -691,7 +648,7 @@ struct Solver {
     // argument of those launches stays the same (the key below); the graph
     // holds kernel nodes only.
     int iterations(int n) {
-        if (levels || stream || !ctx->graphs || ctx->profiling) return iterations_direct(n);
+        if (levels || !ctx->graphs || ctx->profiling) return iterations_direct(n);
         Workspace& W = ctx->ws;
         const std::vector<uintptr_t> key = {
             (uintptr_t)n, (uintptr_t)(kiter & 1), sizeof(T), (uintptr_t)ctx->max_groups, (uintptr_t)ctx->timeout_ticks,
@@ -867,35 +824,11 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     S.c.ay = (const T*)W.cay.p;
     S.c.az = (const T*)W.caz.p;
     S.c.inv_e = mixed ? (const T*)W.inv_e32.p : (const T*)W.inv_e64.p;
-    S.stream = mixed && o.schedule == ELQ_SCHED_STREAM;
-    if (S.stream) {
-        S.pc4.np = W.np;
-        S.pc4.shape = kShape4x4;
-        S.pc4.nblk = W.nblk[kShape4x4];
-        S.pc4.blk_list = (const int32_t*)W.blist.p + W.blk_base[kShape4x4];
-        S.bflag4 = (const uint8_t*)W.bkflag.p + W.blk_base[kShape4x4];
-        const size_t cells = (size_t)W.nb * kBrickCells;
-        for (DevBuf* b : {&W.lxm, &W.lym, &W.lzm}) HIPCHK(b->ensure(cells * 4), "alloc");
-        // tagged face words: zeroed on (re)allocation; tags (epochs) never 0
-        const size_t fbytes = stream_face_words(p.nbx, p.nby, p.nbz) * 8;
-        if (fbytes > W.face.cap) {
-            HIPCHK(W.face.ensure(fbytes), "alloc face words");
-            HIPCHK(hipMemsetAsync(W.face.p, 0, W.face.cap, ctx->stream), "memset face words");
-        }
-    }
     Ctrl hc;
     ctrl_host_init(hc, o, mixed);
     if ((rc = upload_ctrl(ctx, hc))) return rc;
     const double tau = o.precond == ELQ_PRECOND_MIC0 ? o.tau : 0.0;
     if ((rc = S.factor(tau, o.sigma, mixed ? (float*)W.inv_e32.p : nullptr))) return rc;
-    if (S.stream) {
-        ctx->prof_begin("factor");
-        HIPCHK(launch_lower(S.bk, (const float*)W.cax.p, (const float*)W.cay.p, (const float*)W.caz.p,
-                            (const float*)W.inv_e32.p, (float*)W.lxm.p, (float*)W.lym.p, (float*)W.lzm.p,
-                            ctx->stream),
-               "k_lower");
-        ctx->prof_end();
-    }
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     const double t1 = now_ms();
     if constexpr (std::is_same<T, float>::value) {
@@ -950,7 +883,7 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
 
 bool check_opts(const elq_opts& o) {
     return o.tol >= 0 && o.max_iter >= 0 && (o.precision == 0 || o.precision == 1) &&
-           (o.precond == 0 || o.precond == 1) && (o.schedule >= 0 && o.schedule <= 2) && o.sigma >= 0 &&
+           (o.precond == 0 || o.precond == 1) && (o.schedule == 0 || o.schedule == 1) && o.sigma >= 0 &&
            o.inner_tol > 0 && o.inner_tol < 1;
 }
 

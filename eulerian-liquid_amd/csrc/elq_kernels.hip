@@ -20,7 +20,6 @@
 #include <cfloat>
 #include <climits>
 #include <tuple>
-#include <type_traits>
 #include <utility>
 
 #include "elq_kernels.hpp"
@@ -1765,392 +1764,6 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
     grid_tail_n<NW>(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout);
 }
 
-// ---------------------------------------------------------------- stream sweeps
-// Hyperplane-streaming triangular sweeps (FP32 storage; schedule
-// ELQ_SCHED_STREAM). A workgroup of four waves sweeps a block of 4 x 4
-// x-pencils (16 x 16 cells across). Wave w takes the pencil row 4Y + w and the
-// four pencils 4Z .. 4Z+3 in z: lane (jj, kk) = (lane >> 4, lane & 15) owns
-// the x-row of the cells at y = 4 (4Y + w) + jj, z = 16 Z + kk. A lane advances
-// one cell along its row per step: at step h it computes cell i = h - jj - kk.
-// Its -x neighbour is the lane's own previous cell, its -y neighbour
-// (i, jj - 1, kk) was computed one step earlier by lane - 16 and its -z
-// neighbour (i, jj, kk - 1) one step earlier by lane - 1, so every lane does
-// useful work at every step (two lane moves and three fmas), where the
-// pencil-block sweeps above finish a brick (ten dependent hyperplane steps)
-// before its faces move on.
-//
-// All coordinates below are sweep coordinates: the backward sweep mirrors x,
-// y, z, the wave order and the block order, and runs the same code.
-//
-// Slab t of a wave = its four bricks at x-brick t. Lane l loads one x-row (4
-// cells) of brick l >> 4 of each input array with one 16-byte load (1 KB per
-// array and slab, coalesced), combines them (x and r updates, right-hand side
-// b0 and scaled couplings) and writes them into the wave's LDS ring of 8
-// slabs; the skewed steps read b0 and three couplings from the ring and
-// overwrite b0 with the result; a slab's results leave (one 16-byte store per
-// lane) 6 groups after it entered, once the last lane (jj + kk = 18) has
-// passed it. The ring's x-rows are swizzled (row jj ^ (kk >> 2) of brick
-// kk >> 2), which makes the 32-bit accesses of a step bank-conflict free.
-//
-// Steps run in groups of 4 with a workgroup barrier after each group; wave w
-// runs kStreamLag groups behind wave w - 1 and takes its -y face from wave
-// w - 1's LDS ring, written at least one barrier earlier. Faces between blocks
-// go through tagged words in the face buffer: per block and group, 64 words of
-// its +y face (wave 3's jj = 3 lanes over the group's four steps) and 4 x 16 of
-// its +z face (each wave's kk = 15 lanes), indexed by the step at which the
-// producer computed them, so a consumer reads one quad per group for all its
-// lanes. A consumer finishes a quad (tags checked, re-polled while stale, with
-// the bounded wait of the other sweeps) one group before its first use, two
-// groups after loading it.
-constexpr int kStreamSlots = 8;   // slabs in a wave's LDS ring (power of two)
-constexpr int kStreamLag = 2;     // groups by which wave w + 1 follows wave w
-constexpr int kStreamDelay = 6;   // groups from a slab's entry to its store
-
-struct StreamLds {
-    alignas(16) float slab[4][4][kStreamSlots * 256];  // [wave][b0 (then result), c1, c2, c3][slot * 256 + cell]
-    alignas(16) float ring_y[4][16 * 16];              // [wave][(i & 15) * 16 + kk]: results of the jj = 3 lanes
-    alignas(16) float ring_z[4][16 * 4];               // [wave][(i & 15) * 4 + jj]: results of the kk = 15 lanes
-    alignas(16) float halo_y[4][4 * 64];               // [wave (0 only)][quad & 3][u * 16 + kk]
-    alignas(16) float halo_z[4][4 * 16];               // [wave][quad & 3][u * 4 + jj]
-    int32_t ids[4][kStreamSlots][4];                   // [wave][slot][brick]: ids of the slabs in the ring
-    float junk[4][64];                                 // stores of lanes that publish nothing
-};
-
-template <int MODE>
-struct StreamIn {
-    static constexpr int N = MODE == kStreamFwdUpdate ? 8 : 5;
-    float4 v[N];
-    int id;
-};
-
-__device__ __forceinline__ float4 f4_rev(float4 a) { return make_float4(a.w, a.z, a.y, a.x); }
-__device__ __forceinline__ float4 f4_keep(bool keep, float4 a) {
-    return make_float4(keep ? a.x : 0.0f, keep ? a.y : 0.0f, keep ? a.z : 0.0f, keep ? a.w : 0.0f);
-}
-__device__ __forceinline__ float4 f4_mul(float4 a, float4 b) {
-    return make_float4(a.x * b.x, a.y * b.y, a.z * b.z, a.w * b.w);
-}
-
-// Workgroup barrier that orders LDS accesses only: global loads in flight
-// stay in flight across it.
-__device__ __forceinline__ void lds_barrier() {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
-    __builtin_amdgcn_s_barrier();
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-}
-
-// Completes a face word loaded earlier: re-polls (bounded, as Polls2) until
-// its tag is e in every lane that wants it. Returns the value (0 unwanted).
-__device__ __forceinline__ float face_finish(uint64_t& v, const uint64_t* p, bool want, uint32_t e, Ctrl* c,
-                                             uint64_t timeout) {
-    if (!__all(!want || (uint32_t)(v >> 32) == e)) {
-        const uint64_t t0 = wall_clock64();
-        for (int round = 1;; ++round) {
-            __builtin_amdgcn_s_sleep(kPollSleep);
-            v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (__all(!want || (uint32_t)(v >> 32) == e)) break;
-            if ((round & 63) == 0) {
-                if (__hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
-                if (wall_clock64() - t0 > timeout) {
-                    __hip_atomic_store(&c->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-        }
-    }
-    return want ? __uint_as_float((uint32_t)v) : 0.0f;
-}
-
-// Per-block values of one wave of a stream sweep (uniform).
-struct StreamWave {
-    int w, nbx, ngroups, last_quad;
-    bool has_y, has_z, row_in;
-    const uint64_t* in_y;  // face words of the -y producer block
-    const uint64_t* in_z;  // face words of the -z producer block
-    uint64_t* out;         // this block's face words
-    uint64_t* junk;        // 1 KB of this wave's: stores of lanes with nothing to store
-    const int32_t* mrow;   // map row of the slab-phase lane's pencil
-    uint32_t epoch;
-    float alpha;
-    Ctrl* ctrl;
-    uint64_t timeout;
-};
-
-// One wave's part of a stream block. ROLE 0: wave 0 (its -y face comes from
-// the -y block's face words), 1: waves 1 and 2, 2: wave 3 (also publishes the
-// block's +y face). Every memory access of a group is unconditional (lanes
-// and groups without data use clamped addresses and the junk words), so the
-// compiler's memory counters wait exactly for the loads issued two groups
-// earlier and never for the ones in flight.
-template <int MODE, int ROLE>
-__device__ __forceinline__ void stream_wave(StreamLds& S, const StreamArgs& a, const StreamWave& X, double& acc) {
-    constexpr bool REV = MODE == kStreamBwd;
-    constexpr bool UPD = MODE == kStreamFwdUpdate;
-    constexpr int SL = kStreamSlots * 256;  // floats per array of a wave's ring
-    const int w = X.w, nbx = X.nbx;
-    const int lane = (int)(threadIdx.x & 63);
-    // skewed phase: lane (jj, kk); LDS base of its x-row
-    const int jj = lane >> 4, kk = lane & 15, sk = jj + kk;
-    const int lbase = 16 * kk + 4 * (jj ^ (kk >> 2));
-    // slab phase: x-row (ly, lz) of brick qs; its LDS base; its offset in the real brick
-    const int qs = lane >> 4, lys = lane & 3, lzs = (lane >> 2) & 3;
-    const int sbase = 64 * qs + 16 * lzs + 4 * (lys ^ qs);
-    const int row_off = 4 * (REV ? 3 - lys : lys) + 16 * (REV ? 3 - lzs : lzs);
-    float* sl = &S.slab[w][0][0];
-    float* junk4 = (float*)X.junk + 4 * lane;
-    const float alpha = X.alpha;
-    auto valid_slab = [&](int t) { return t >= 0 && t < nbx; };
-    auto clamp_slab = [&](int t) { return t < 0 ? 0 : (t >= nbx ? nbx - 1 : t); };
-    auto fetch = [&](StreamIn<MODE>& d, int id) {
-        d.id = id;
-        const size_t o = (size_t)(id < 0 ? 0 : id) * kBrickCells + row_off;
-        if constexpr (UPD) {
-            d.v[0] = *(const float4*)(a.r + o);
-            d.v[1] = *(const float4*)(a.x + o);
-            d.v[2] = *(const float4*)(a.p + o);
-            d.v[3] = *(const float4*)(a.q + o);
-            d.v[4] = *(const float4*)(a.ie + o);
-            d.v[5] = *(const float4*)(a.c1 + o);
-            d.v[6] = *(const float4*)(a.c2 + o);
-            d.v[7] = *(const float4*)(a.c3 + o);
-        } else {
-            d.v[0] = *(const float4*)((REV ? a.y : a.r) + o);
-            d.v[1] = *(const float4*)(a.ie + o);
-            d.v[2] = *(const float4*)(a.c1 + o);
-            d.v[3] = *(const float4*)(a.c2 + o);
-            d.v[4] = *(const float4*)(a.c3 + o);
-        }
-    };
-    // slab t enters the ring (zeros outside the rows)
-    auto enter = [&](const StreamIn<MODE>& d, int t) {
-        const int at = (t & (kStreamSlots - 1)) * 256 + sbase;
-        const bool on = valid_slab(t) && d.id >= 0;
-        float4 b0, c1, c2, c3;
-        if constexpr (UPD) {
-            const float4 rv = d.v[0], xv = d.v[1], pv = d.v[2], qv = d.v[3];
-            const float4 xn = make_float4(fmaf(alpha, pv.x, xv.x), fmaf(alpha, pv.y, xv.y), fmaf(alpha, pv.z, xv.z),
-                                          fmaf(alpha, pv.w, xv.w));
-            float4 rn = make_float4(fmaf(-alpha, qv.x, rv.x), fmaf(-alpha, qv.y, rv.y), fmaf(-alpha, qv.z, rv.z),
-                                    fmaf(-alpha, qv.w, rv.w));
-            const size_t o = (size_t)d.id * kBrickCells + row_off;
-            *(float4*)(on ? a.x + o : junk4) = xn;
-            *(float4*)(on ? a.r + o : junk4) = rn;
-            rn = f4_keep(on, rn);
-            acc += (double)rn.x * (double)rn.x;
-            acc += (double)rn.y * (double)rn.y;
-            acc += (double)rn.z * (double)rn.z;
-            acc += (double)rn.w * (double)rn.w;
-            b0 = f4_mul(rn, d.v[4]);
-            c1 = f4_keep(on, d.v[5]);
-            c2 = f4_keep(on, d.v[6]);
-            c3 = f4_keep(on, d.v[7]);
-        } else if constexpr (MODE == kStreamFwd) {
-            b0 = f4_mul(f4_keep(on, d.v[0]), d.v[1]);
-            c1 = f4_keep(on, d.v[2]);
-            c2 = f4_keep(on, d.v[3]);
-            c3 = f4_keep(on, d.v[4]);
-        } else {
-            b0 = f4_keep(on, d.v[0]);
-            c1 = f4_keep(on, f4_mul(d.v[1], d.v[2]));
-            c2 = f4_keep(on, f4_mul(d.v[1], d.v[3]));
-            c3 = f4_keep(on, f4_mul(d.v[1], d.v[4]));
-        }
-        if (REV) {
-            b0 = f4_rev(b0);
-            c1 = f4_rev(c1);
-            c2 = f4_rev(c2);
-            c3 = f4_rev(c3);
-        }
-        *(float4*)(sl + at) = b0;
-        *(float4*)(sl + SL + at) = c1;
-        *(float4*)(sl + 2 * SL + at) = c2;
-        *(float4*)(sl + 3 * SL + at) = c3;
-        S.ids[w][t & (kStreamSlots - 1)][qs] = on ? d.id : -1;
-    };
-    // r of slab t (backward sweep: r.z when the slab leaves)
-    auto load_r = [&](int t) -> float4 {
-        const int id = valid_slab(t) ? S.ids[w][t & (kStreamSlots - 1)][qs] : -1;
-        const float4 v = *(const float4*)(a.r + (size_t)(id < 0 ? 0 : id) * kBrickCells + row_off);
-        return f4_keep(id >= 0, v);
-    };
-    // slab t leaves: results to y (backward: z, and r.z)
-    auto leave = [&](int t, float4 rv) {
-        float4 v = *(const float4*)(sl + (t & (kStreamSlots - 1)) * 256 + sbase);
-        if (REV) v = f4_rev(v);
-        const int id = valid_slab(t) ? S.ids[w][t & (kStreamSlots - 1)][qs] : -1;
-        *(float4*)(id >= 0 ? a.y + (size_t)id * kBrickCells + row_off : junk4) = v;
-        if constexpr (REV) {
-            acc += (double)rv.x * (double)v.x;
-            acc += (double)rv.y * (double)v.y;
-            acc += (double)rv.z * (double)v.z;
-            acc += (double)rv.w * (double)v.w;
-        }
-    };
-    // face quads of the -y producer (wave 0) and the -z producer
-    auto cq = [&](int q) { return (size_t)(q < 0 ? 0 : (q > nbx + 5 ? nbx + 5 : q)) * kStreamFace; };
-    auto yq = [&](int q) { return X.in_y + cq(q) + lane; };
-    auto zq = [&](int q) { return X.in_z + cq(q) + 64 + 16 * w + (lane & 15); };
-    auto issue = [&](uint64_t& v, const uint64_t* p) {
-        v = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    };
-    float yp = 0.0f;  // the lane's previous result (its -x neighbour)
-    auto steps = [&](int g) {
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int h = 4 * g + u, i = h - sk;
-            const int idx = ((i >> 2) & (kStreamSlots - 1)) * 256 + lbase + (i & 3);
-            const float b0 = sl[idx], c1 = sl[SL + idx], c2 = sl[2 * SL + idx], c3 = sl[3 * SL + idx];
-            const int sy = h + 3, sz = h + 15;
-            const float ey = ROLE == 0 ? S.halo_y[w][((sy >> 2) & 3) * 64 + (sy & 3) * 16 + kk]
-                                       : S.ring_y[ROLE == 0 ? 0 : w - 1][(i & 15) * 16 + kk];
-            const float ez = S.halo_z[w][((sz >> 2) & 3) * 16 + (sz & 3) * 4 + jj];
-            const float my = move<Mv::zm>(yp), mz = move<Mv::xm>(yp);
-            const float vy = jj == 0 ? ey : my, vz = kk == 0 ? ez : mz;
-            float t = fmaf(-c1, yp, b0);
-            t = fmaf(-c2, vy, t);
-            const float yv = fmaf(-c3, vz, t);
-            sl[idx] = yv;
-            *(jj == 3 ? &S.ring_y[w][(i & 15) * 16 + kk] : &S.junk[w][lane]) = yv;
-            *(kk == 15 ? &S.ring_z[w][(i & 15) * 4 + jj] : &S.junk[w][lane]) = yv;
-            yp = yv;
-        }
-    };
-    // the block's +y face (wave 3) and +z face of the group's steps; the
-    // groups before the rows publish into spare quads nbx + 6 .. nbx + 9
-    auto publish = [&](int g) {
-        const size_t pq = (size_t)(g >= 0 ? g : nbx + 10 + g) * kStreamFace;
-        if constexpr (ROLE == 2) {
-            const int u = lane >> 4, k2 = lane & 15, i2 = 4 * g + u - 3 - k2;
-            xput(X.out, pq + lane, S.ring_y[3][(i2 & 15) * 16 + k2], X.epoch);
-        }
-        const int l16 = lane & 15, u = l16 >> 2, j2 = l16 & 3, i2 = 4 * g + u - j2 - 15;
-        xput(X.out, pq + 64 + 16 * w + l16, S.ring_z[w][(i2 & 15) * 4 + j2], X.epoch);
-    };
-
-    StreamIn<MODE> sb[2];
-    uint64_t hy[2] = {0ull, 0ull}, hz[2] = {0ull, 0ull};
-    float4 rb[2] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
-    int idb[2] = {0, 0};
-    // Group g uses the buffers [g & 1], all loaded at group g - 2: sb holds
-    // slab g, idb the map entry of slab g + 2, hy face quad g + 1 of the -y
-    // producer, hz quad g + 4 of the -z producer, rb r of the slab leaving
-    // (backward sweep). Everything a group waits for was thus issued before
-    // everything of the previous group (the memory counters count in order).
-    // Groups -4 .. -1 fill the pipeline (their cells lie before the rows).
-    auto body = [&](auto P, int g) {
-        constexpr int p = decltype(P)::value;
-        if constexpr (ROLE == 0) {
-            S.halo_y[w][((g + 1) & 3) * 64 + lane] =
-                face_finish(hy[p], yq(g + 1), X.has_y && g + 1 >= 0 && g + 1 <= X.last_quad, X.epoch, X.ctrl,
-                            X.timeout);
-            issue(hy[p], yq(g + 3));
-        }
-        S.halo_z[w][((g + 4) & 3) * 16 + (lane & 15)] =
-            face_finish(hz[p], zq(g + 4), X.has_z && g + 4 >= 3 && g + 4 <= X.last_quad, X.epoch, X.ctrl, X.timeout);
-        issue(hz[p], zq(g + 6));
-        enter(sb[p], g);
-        fetch(sb[p], X.row_in && valid_slab(g + 2) ? idb[p] : -1);
-        idb[p] = X.mrow[REV ? nbx - 1 - clamp_slab(g + 4) : clamp_slab(g + 4)];
-        leave(g - kStreamDelay, rb[p]);
-        if constexpr (REV) rb[p] = load_r(g - kStreamDelay + 2);
-        steps(g);
-        publish(g);
-        lds_barrier();
-    };
-    for (int G = 0; G < kStreamLag * w; ++G) lds_barrier();
-    for (int g = -4;; g += 2) {
-        body(std::integral_constant<int, 0>{}, g);
-        if (g + 1 >= X.ngroups) break;
-        body(std::integral_constant<int, 1>{}, g + 1);
-        if (g + 2 >= X.ngroups) break;
-    }
-    for (int G = 0; G < kStreamLag * (3 - w); ++G) lds_barrier();
-}
-
-template <int MODE>
-__global__ __launch_bounds__(256, 1) void k_stream(Bricks bk, Pencils pc, StreamArgs a, Ctrl* ctrl, uint64_t* face,
-                                                   const uint8_t* bflag, uint64_t* stage, int tail_mode,
-                                                   uint64_t timeout) {
-    constexpr bool REV = MODE == kStreamBwd;
-    __shared__ StreamLds S;
-    if (ctrl->done) return;
-    StreamWave X;
-    X.w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int lane = (int)(threadIdx.x & 63);
-    X.epoch = ctrl->epoch;
-    const uint32_t tag = next_tag(ctrl->red_epoch);
-    X.alpha = MODE == kStreamFwdUpdate ? (float)ctrl->alpha : 0.0f;
-    X.ctrl = ctrl;
-    X.timeout = timeout;
-    X.nbx = bk.nbx;
-    X.ngroups = bk.nbx + kStreamDelay;
-    X.last_quad = bk.nbx + 4;  // later quads only meet cells past the end of the rows
-    const int NBY = (bk.nby + 3) >> 2, NBZ = (bk.nbz + 3) >> 2;
-    const size_t fstride = (size_t)(bk.nbx + 10) * kStreamFace;
-    X.junk = face + (size_t)NBY * NBZ * fstride + (size_t)(((int)blockIdx.x & 63) * 4 + X.w) * kStreamFace;
-    const int qr = REV ? 3 - (lane >> 4) : (lane >> 4);  // the slab-phase lane's brick in z
-    double acc = 0.0;
-    int round = 0;
-    for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
-        const int bc = pc.blk_list[REV ? pc.nblk - 1 - pos : pos];
-        sweep_priority(pos, round);
-        const int Y = bc & 1023, Z = (bc >> 10) & 1023;
-        // blocks that produce this block's -y and -z faces (sweep coordinates)
-        const int Yp = REV ? Y + 1 : Y - 1, Zp = REV ? Z + 1 : Z - 1;
-        X.has_y = Yp >= 0 && Yp < NBY && bflag[(size_t)Yp + (size_t)NBY * Z] != 0;
-        X.has_z = Zp >= 0 && Zp < NBZ && bflag[(size_t)Y + (size_t)NBY * Zp] != 0;
-        X.in_y = face + (X.has_y ? (size_t)Yp + (size_t)NBY * Z : (size_t)0) * fstride;
-        X.in_z = face + (X.has_z ? (size_t)Y + (size_t)NBY * Zp : (size_t)0) * fstride;
-        X.out = face + ((size_t)Y + (size_t)NBY * Z) * fstride;
-        const int by = 4 * Y + (REV ? 3 - X.w : X.w), bz = 4 * Z + qr;
-        X.row_in = by < bk.nby && bz < bk.nbz;
-        X.mrow = bk.map + (X.row_in ? (size_t)bk.nbx * ((size_t)by + (size_t)bk.nby * bz) : (size_t)0);
-        // negative slabs (cells before the rows) read as zero
-        float4* ring = (float4*)&S.slab[X.w][0][0];
-        for (int k = lane; k < kStreamSlots * 256; k += 64) ring[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        __syncthreads();
-        if (X.w == 0)
-            stream_wave<MODE, 0>(S, a, X, acc);
-        else if (X.w == 3)
-            stream_wave<MODE, 2>(S, a, X, acc);
-        else
-            stream_wave<MODE, 1>(S, a, X, acc);
-        __syncthreads();
-    }
-    grid_tail_n<4>(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout);
-}
-
-// Scaled lower couplings of the forward stream sweep (one wave per brick):
-// lxm = (coupling of the -x neighbour to this cell) x (this cell's 1/e), the
-// product the pencil-block sweeps form on the fly (fwd_pre, fwd_compute);
-// likewise y and z; 0 without the neighbour.
-__global__ __launch_bounds__(256) void k_lower(Bricks bk, const float* ax, const float* ay, const float* az,
-                                               const float* ie, float* lxm, float* lym, float* lzm) {
-    const int b = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (b >= bk.nb) return;
-    const Lane L;
-    const size_t o = (size_t)b * kBrickCells + L.lane;
-    const int nx = nbr_id(bk, 0, b), ny = nbr_id(bk, 2, b), nz = nbr_id(bk, 4, b);
-    float axm = 0.0f, aym = 0.0f, azm = 0.0f;
-    if (L.lx > 0)
-        axm = ax[o - 1];
-    else if (nx >= 0)
-        axm = ax[(size_t)nx * kBrickCells + L.lane + 3];
-    if (L.ly > 0)
-        aym = ay[o - 4];
-    else if (ny >= 0)
-        aym = ay[(size_t)ny * kBrickCells + L.lane + 12];
-    if (L.lz > 0)
-        azm = az[o - 16];
-    else if (nz >= 0)
-        azm = az[(size_t)nz * kBrickCells + L.lane + 48];
-    const float e = ie[o];
-    lxm[o] = axm * e;
-    lym[o] = aym * e;
-    lzm[o] = azm * e;
-}
-
 // Lists of pencil blocks, one workgroup per block shape: the blocks (BY x BZ
 // pencils) holding at least one active pencil, sorted by block level Y + Z,
 // then Y (deterministic). Workgroup s writes flag + base[s], list + base[s]
@@ -2604,32 +2217,6 @@ hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, con
         hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
                            level_base, timeout_ticks);
     return hipGetLastError();
-}
-
-hipError_t launch_lower(const Bricks& bk, const float* ax, const float* ay, const float* az, const float* ie,
-                        float* lxm, float* lym, float* lzm, hipStream_t s) {
-    if (bk.nb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lower, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, ax, ay, az, ie, lxm, lym, lzm);
-    return hipGetLastError();
-}
-
-// One workgroup per CU: the LDS rings of a stream block take most of a CU.
-hipError_t launch_stream(const Bricks& bk, const Pencils& pc, const StreamArgs& a, int mode, Ctrl* ctrl,
-                         uint64_t* face, const uint8_t* bflag, uint64_t* stage, int tail_mode,
-                         uint64_t timeout_ticks, int max_groups, hipStream_t s) {
-    switch (mode) {
-    case kStreamFwdUpdate:
-        return persistent_wg(k_stream<kStreamFwdUpdate>, pc.nblk, 256, 1, max_groups, s, bk, pc, a, ctrl, face,
-                             bflag, stage, tail_mode, timeout_ticks);
-    case kStreamFwd:
-        return persistent_wg(k_stream<kStreamFwd>, pc.nblk, 256, 1, max_groups, s, bk, pc, a, ctrl, face, bflag,
-                             stage, tail_mode, timeout_ticks);
-    case kStreamBwd:
-        return persistent_wg(k_stream<kStreamBwd>, pc.nblk, 256, 1, max_groups, s, bk, pc, a, ctrl, face, bflag,
-                             stage, tail_mode, timeout_ticks);
-    default:
-        return hipErrorInvalidValue;
-    }
 }
 
 int sweep_round_groups(int max_groups) {

@@ -49,11 +49,9 @@ struct Pencils {
 // one wave each. Faces between the waves of a block go through LDS; faces
 // between blocks through the exchange words in HBM. BY * BZ is a power of
 // two (the grid tail's reduction tree over the waves of a workgroup).
-// kShape4x4 is the block of the stream sweeps (k_stream: four waves, one
-// pencil row in y each, four pencils in z).
-enum BlockShape : int { kShape2x2 = 0, kShape1x4 = 1, kShape2x4 = 2, kShape4x4 = 3, kShapeCount = 4 };
-constexpr int kShapeY[kShapeCount] = {2, 1, 2, 4};
-constexpr int kShapeZ[kShapeCount] = {2, 4, 4, 4};
+enum BlockShape : int { kShape2x2 = 0, kShape1x4 = 1, kShape2x4 = 2, kShapeCount = 3 };
+constexpr int kShapeY[kShapeCount] = {2, 1, 2};
+constexpr int kShapeZ[kShapeCount] = {2, 4, 4};
 
 // Pencil blocks of one shape over an nby x nbz pencil grid.
 inline size_t shape_blocks(int nby, int nbz, int shape) {
@@ -78,33 +76,6 @@ struct Coef {
     const T* az;
     const T* inv_e;  // IC factor pivots 1/e
 };
-
-// Stream sweeps (FP32 storage, schedule ELQ_SCHED_STREAM): forward sweep
-// with the x/r updates and |r|^2 (SC_RR tail), forward sweep alone (SC_NOP
-// tail), backward sweep with r.z (tail_mode).
-enum StreamMode : int { kStreamFwdUpdate = 0, kStreamFwd = 1, kStreamBwd = 2 };
-
-struct StreamArgs {
-    float* r;
-    float* x;
-    const float* p;
-    const float* q;
-    float* y;         // result (backward sweep: y in, z out, in place)
-    const float* ie;  // pivots 1/e
-    // forward: the scaled lower couplings (k_lower); backward: ax, ay, az
-    const float* c1;
-    const float* c2;
-    const float* c3;
-};
-
-// Tagged face words of the stream sweeps: per block of the 4x4 shape and
-// per group of four steps, 64 words of its +y face and 4 x 16 of its +z face
-// (nbx + 10 groups per block), then 64 x 4 x 128 junk words (stores of lanes
-// with nothing to store).
-constexpr int kStreamFace = 128;
-inline size_t stream_face_words(int nbx, int nby, int nbz) {
-    return shape_blocks(nby, nbz, kShape4x4) * (size_t)(nbx + 10) * kStreamFace + (size_t)64 * 4 * kStreamFace;
-}
 
 // ---------------------------------------------------------------- setup
 hipError_t launch_mask(const Dense& d, uint8_t* mask, int32_t* err, hipStream_t s);
@@ -175,16 +146,6 @@ template <class T>
 hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, const T* r, T* y, double* part, Ctrl* ctrl,
                       uint64_t* xch, uint64_t* stage, int tail_mode, int level, int level_base, int level_count,
                       uint64_t timeout_ticks, int max_groups, hipStream_t s);
-// lxm/lym/lzm = the -x/-y/-z neighbour's +x/+y/+z coupling times the
-// cell's pivot inverse (0 without that neighbour): the forward stream sweep's
-// couplings.
-hipError_t launch_lower(const Bricks& bk, const float* ax, const float* ay, const float* az, const float* ie,
-                        float* lxm, float* lym, float* lzm, hipStream_t s);
-// One stream sweep over the blocks of pc (the 4x4 shape); bflag: that
-// shape's block flags; face: stream_face_words() words, zeroed once.
-hipError_t launch_stream(const Bricks& bk, const Pencils& pc, const StreamArgs& a, int mode, Ctrl* ctrl,
-                         uint64_t* face, const uint8_t* bflag, uint64_t* stage, int tail_mode,
-                         uint64_t timeout_ticks, int max_groups, hipStream_t s);
 // r64 = b64 - A x64 in FP64 (couplings TC read exactly); partials |r64|^2 and,
 // when r32 != nullptr, |float(r64)|^2 with r32 = float(r64); x32 and p0/p1 are
 // zeroed when non-null.

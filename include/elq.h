@@ -64,11 +64,7 @@ typedef enum {
 typedef enum {
     ELQ_SCHED_WAVEFRONT = 0, /* one launch per sweep; bricks taken in level order,
                                 predecessor faces arrive as epoch-tagged words */
-    ELQ_SCHED_LEVELS = 1,    /* one launch per brick level (no device waits) */
-    ELQ_SCHED_STREAM = 2     /* mixed precision: FP32 sweeps in which every lane
-                                advances one cell along its x-row per step, over
-                                blocks of 4 x 4 pencils; the FP64 drop-in runs
-                                the wavefront schedule */
+    ELQ_SCHED_LEVELS = 1     /* one launch per brick level (no device waits) */
 } elq_schedule;
 
 /* Inputs/outputs of one projection. Host or device pointers depending on the

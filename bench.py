@@ -611,15 +611,17 @@ def read_calibration(nbytes, reps):
                       "read_gbs_incl_sync": round(nbytes / t / 1e9, 1)}), flush=True)
 
 
-def serve_run(scenes, arrivals, slots, opts, keep_pressure=False):
+def serve_run(scenes, arrivals, slots, opts, keep_pressure=False, batched=0, stats=None):
     """Submits fresh copies of the scenes at the given arrival times (s from
-    the start) to a server with `slots` slots; returns per-request results
-    (with keep_pressure, each keeps its pressure and the index of its scene)."""
+    the start) to a server with `slots` slots (batched > 0: Krylov chunks of the
+    running requests advance in shared launches); returns per-request results
+    (with keep_pressure, each keeps its pressure and the index of its scene).
+    stats (a dict) receives the server's batch counts."""
     work = [sc.copy() for sc in scenes]
     n = len(work)
     got = []
     index = {}
-    with elq.Server(0, max_slots=slots, max_queue=n, opts=opts) as srv:
+    with elq.Server(0, max_slots=slots, max_queue=n, opts=opts, batched=batched) as srv:
         t0 = time.perf_counter()
         i = 0
         while len(got) < n:
@@ -633,6 +635,8 @@ def serve_run(scenes, arrivals, slots, opts, keep_pressure=False):
             else:
                 srv.wait(int(max(nxt, 0.0) * 1e3) + 1)
             got += srv.poll()
+        if stats is not None:
+            stats["batches"], stats["batched_requests"] = srv.batch_stats()
     for r in got:
         r.pop("scene")
         r["index"] = index[r["id"]]
@@ -674,7 +678,27 @@ def serve_summary(got):
             "mean_iterations": round(float(np.mean([r["iterations"] for r in got])), 1)}
 
 
-def serving(n_req, slots, seed, base, fp64=True, check_threads=0):
+def serving_batched(scenes, slots, o_mixed, groups, seed):
+    """The same stream on a batched server (elq_server_opts.batched = groups):
+    the Krylov chunks of the running requests advance in lockstep, one launch
+    per kernel class for a whole group of slots, replayed from one captured
+    HIP graph. Saturation and Poisson arrivals at 0.8x its own saturation
+    rate; batches launched and requests per batch."""
+    n = len(scenes)
+    serve_run(scenes[:min(8, n)], [0.0] * min(8, n), slots, o_mixed, batched=groups)  # warm-up, graph capture
+    st = {}
+    sat = serve_summary(serve_run(scenes, [0.0] * n, slots, o_mixed, batched=groups, stats=st))
+    lam = 0.8 * sat["solves_per_s"]
+    arrivals = np.cumsum(np.random.default_rng(seed).exponential(1.0 / lam, n)).tolist()
+    poisson = serve_summary(serve_run(scenes, arrivals, slots, o_mixed, batched=groups))
+    out = {"groups": groups, "slots": slots, "saturation": sat, "poisson_rate_per_s": round(lam, 2),
+           "poisson": poisson, "batches": st["batches"],
+           "requests_per_batch": round(st["batched_requests"] / max(st["batches"], 1), 2)}
+    print(f"# serving, batched lockstep: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def serving(n_req, slots, seed, base, fp64=True, check_threads=0, batch_groups=0):
     """Serving stream: n_req independent requests, dims per axis from
     {32,48,64,96,128}, 1-8 spheres (radii 0.05-0.2), slab, face velocities
     U[-1,1] (request seeds seed*100000 + i). Runs: saturation (all submitted
@@ -701,6 +725,8 @@ def serving(n_req, slots, seed, base, fp64=True, check_threads=0):
         out["poisson_vs_oracle"] = serving_oracle_errors(scenes, got, check_threads)
         print(f"# serving vs oracle: {out['poisson_vs_oracle']}", file=sys.stderr, flush=True)
     del got
+    if batch_groups > 0:
+        out["batched_lockstep"] = serving_batched(scenes, slots, o_mixed, batch_groups, seed)
     if not fp64:
         return out
     seq64 = serve_summary(serve_run(scenes, [0.0] * n_req, 1, o_fp64))
@@ -746,6 +772,8 @@ def main():
     ap.add_argument("--serve", type=int, default=512, metavar="N",
                     help="serving stream of N requests after the single-solve benchmark (0: skip)")
     ap.add_argument("--slots", type=int, default=4, help="serving slots")
+    ap.add_argument("--batch-groups", type=int, default=2, metavar="G",
+                    help="also serve the stream batched in G lockstep groups of slots (0: skip)")
     ap.add_argument("--serve-only", action="store_true",
                     help="only the serving stream (mixed precision), as one JSON object (child runs)")
     ap.add_argument("--throughput-slots", type=int, default=12, metavar="N",
@@ -889,7 +917,8 @@ def main():
     ctx.close()
     if a.serve > 0:
         line["serving"] = serving(a.serve, a.slots, a.seed, base,
-                                  check_threads=0 if a.no_cpu else host_info()["omp_threads"])
+                                  check_threads=0 if a.no_cpu else host_info()["omp_threads"],
+                                  batch_groups=a.batch_groups)
         if a.throughput_slots > 0:
             line["serving"]["throughput_config"] = serving_child(a)
     threads = host_info()["omp_threads"]

@@ -22,6 +22,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <thread>
 #include <vector>
 
@@ -648,6 +649,33 @@ struct Solver {
     // argument of those launches stays the same (the key below); the graph
     // holds kernel nodes only.
     int iterations(int n) {
+        if constexpr (std::is_same<T, float>::value) {
+            if (ctx->batcher && !levels && !ctx->profiling &&
+                (pc.shape == kShape1x4 || pc.shape == kShape2x2)) {
+                Workspace& W = ctx->ws;
+                KrylovJob j{};
+                j.bk = bk;
+                j.pc = pc;
+                j.c = c;
+                j.r = vec(W.vr);
+                j.x = vec(W.vx);
+                j.p0 = vec(W.vp0);
+                j.p1 = vec(W.vp1);
+                j.q = vec(W.vq);
+                j.y = vec(W.vy);
+                j.ctrl = ctrl();
+                j.xch = (uint64_t*)W.xch.p;
+                j.stage = stage();
+                j.timeout = ctx->timeout_ticks;
+                krylov_groups(bk, pc, ctx->max_groups, &j.g_spmv, &j.g_fwd, &j.g_bwd);
+                j.par = (int32_t)(kiter & 1);
+                const int rc = ctx->batcher->run(ctx, ctx->batch_slot, j, n);
+                if (rc != KrylovBatcher::kNotBatched) {
+                    if (rc == ELQ_OK) kiter += n;
+                    return rc;
+                }
+            }
+        }
         if (levels || !ctx->graphs || ctx->profiling) return iterations_direct(n);
         Workspace& W = ctx->ws;
         const std::vector<uintptr_t> key = {
@@ -719,6 +747,7 @@ struct Solver {
             if ((rc = poll(hc))) return rc;
             if (hc.done) break;
         }
+        if (ctx->batcher) ctx->batcher->leave(ctx->batch_slot);
         return ELQ_OK;
     }
 };
@@ -781,6 +810,7 @@ int solve_mixed(elq_ctx* ctx, Solver<float>& S, Ctrl& hc) {
             if ((rc = S.precondition(SC_RZ_REPLACE))) return rc;
         }
     }
+    if (ctx->batcher) ctx->batcher->leave(ctx->batch_slot);
     return ELQ_OK;
 }
 

@@ -119,6 +119,22 @@ struct HostStage {
     void release();
 };
 
+namespace elq {
+// Serving hook: runs a context's Krylov chunks in launches shared with other
+// requests (set by the server on its slot contexts; see elq_server.cpp).
+struct KrylovBatcher {
+    virtual ~KrylovBatcher() = default;
+    // n iterations of the request of slot `slot` (its solver state in job),
+    // enqueued after everything on ctx->stream and before anything enqueued
+    // there later; returns an elq_status, or kNotBatched when this chunk
+    // must run as direct launches.
+    virtual int run(elq_ctx* ctx, int slot, const KrylovJob& job, int n) = 0;
+    // the request of slot `slot` has left its Krylov loop (no more chunks)
+    virtual void leave(int slot) = 0;
+    static constexpr int kNotBatched = -1;
+};
+}  // namespace elq
+
 struct elq_ctx {
     int device = 0;
     size_t budget = 0;
@@ -140,6 +156,9 @@ struct elq_ctx {
     hipGraphExec_t chunk_exec = nullptr;
     std::vector<uintptr_t> chunk_key;
     void drop_chunk_graph();
+    // serving: batched Krylov chunks (mixed precision, wavefront sweeps)
+    elq::KrylovBatcher* batcher = nullptr;
+    int batch_slot = -1;
     std::string last_error;
     Workspace ws;
     HostStage stage;  // host-buffer entry points

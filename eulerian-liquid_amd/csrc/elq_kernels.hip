@@ -830,12 +830,16 @@ __global__ __launch_bounds__(kReduceThreads) void k_scalar(Ctrl* c, int mode, co
 // a fixed tree) and applies the scalar recurrence. The result depends only on
 // the grid size, not on timing. Every thread of every workgroup must call
 // this; ctrl must not be read after it.
+// g, G: this workgroup's index among the G workgroups of the reduction
+// (the launch's blockIdx.x and gridDim.x, or a request's share of a batched
+// launch).
 template <int NW>
-__device__ void grid_tail_n(Ctrl* c, int mode, double wave_total, uint32_t tag, uint64_t* stage, uint64_t timeout) {
+__device__ void grid_tail_n(Ctrl* c, int mode, double wave_total, uint32_t tag, uint64_t* stage, uint64_t timeout,
+                            int g, int G) {
     constexpr int NT = 64 * NW;
     __shared__ double sw[NW];
     __shared__ double st[NT];
-    const int t = threadIdx.x, g = blockIdx.x, G = gridDim.x;
+    const int t = threadIdx.x;
     if ((t & 63) == 0) sw[t >> 6] = wave_total;
     __syncthreads();
     double own = 0.0;
@@ -871,8 +875,8 @@ __device__ void grid_tail_n(Ctrl* c, int mode, double wave_total, uint32_t tag, 
 }
 
 __device__ __forceinline__ void grid_tail(Ctrl* c, int mode, double wave_total, uint32_t tag, uint64_t* stage,
-                                          uint64_t timeout) {
-    grid_tail_n<4>(c, mode, wave_total, tag, stage, timeout);
+                                          uint64_t timeout, int g, int G) {
+    grid_tail_n<4>(c, mode, wave_total, tag, stage, timeout, g, G);
 }
 
 // The six neighbour brick ids of brick b (wave-uniform): lanes 0..5 load
@@ -971,20 +975,28 @@ __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef
 // that the second brick's loads overlap the first's arithmetic (the
 // restrict-qualified arrays let the compiler move them); its p.q terms
 // accumulate per lane in brick order; the grid tail computes alpha.
+// Workgroup g of G (see grid_tail_n).
 template <class T>
-__global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
-                                              Ctrl* ctrl, uint64_t* stage, uint64_t timeout) {
+__device__ __forceinline__ void spmv_body(int g, int G, const Bricks& bk, const Coef<T>& c, const T* z,
+                                          const T* p_in, T* p_out, T* q, Ctrl* ctrl, uint64_t* stage,
+                                          uint64_t timeout) {
     if (ctrl->done) return;
     const uint32_t tag = next_tag(ctrl->red_epoch);
     const T beta = (T)ctrl->beta;
     const Lane L;
-    const int W = (int)gridDim.x * 4;
+    const int W = G * 4;
     double acc = 0.0;
-    for (int b = __builtin_amdgcn_readfirstlane((int)blockIdx.x * 4 + (int)(threadIdx.x >> 6)); b < bk.nb; b += 2 * W) {
+    for (int b = __builtin_amdgcn_readfirstlane(g * 4 + (int)(threadIdx.x >> 6)); b < bk.nb; b += 2 * W) {
         acc += spmv_brick<T>(b, bk, c, z, p_in, p_out, q, beta, L);
         if (b + W < bk.nb) acc += spmv_brick<T>(b + W, bk, c, z, p_in, p_out, q, beta, L);
     }
-    grid_tail(ctrl, SC_ALPHA, wave_sum(acc), tag, stage, timeout);
+    grid_tail(ctrl, SC_ALPHA, wave_sum(acc), tag, stage, timeout, g, G);
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_spmv(Bricks bk, Coef<T> c, const T* z, const T* p_in, T* p_out, T* q,
+                                              Ctrl* ctrl, uint64_t* stage, uint64_t timeout) {
+    spmv_body<T>((int)blockIdx.x, (int)gridDim.x, bk, c, z, p_in, p_out, q, ctrl, stage, timeout);
 }
 
 // Forward sweep of M^{-1} = (E+L^T)^{-1} E (E+L)^{-1}: y = (r - L y) / e, one
@@ -1590,19 +1602,19 @@ __device__ __forceinline__ void block_lds_init(BlockLds<T, NW>& s) {
 // At most 128 VGPRs in FP32 (4 waves per SIMD: 4 workgroups of 4 waves per CU).
 // Issue priority of a sweep wave from its block's place in the current
 // round of the block list: blocks early in the list feed the ones after them.
-__device__ __forceinline__ void sweep_priority(int pos, int round) {
-    const int q = (4 * (pos - round * (int)gridDim.x)) / (int)gridDim.x;
+__device__ __forceinline__ void sweep_priority(int pos, int round, int G) {
+    const int q = (4 * (pos - round * G)) / G;
     if (q <= 0) __builtin_amdgcn_s_setprio(3);
     else if (q == 1) __builtin_amdgcn_s_setprio(2);
     else if (q == 2) __builtin_amdgcn_s_setprio(1);
     else __builtin_amdgcn_s_setprio(0);
 }
 
+// Workgroup g of G (see grid_tail_n).
 template <class T, bool UPDATE, int BY, int BZ>
-__global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_block(Bricks bk, Pencils pc, Coef<T> c,
-                                                                                   T* r, T* x,
-                                                            const T* p, const T* q, T* y, Ctrl* ctrl, uint64_t* xch,
-                                                            uint64_t* stage, uint64_t timeout) {
+__device__ __forceinline__ void fwd_block_body(int g, int G, const Bricks& bk, const Pencils& pc, const Coef<T>& c,
+                                               T* r, T* x, const T* p, const T* q, T* y, Ctrl* ctrl, uint64_t* xch,
+                                               uint64_t* stage, uint64_t timeout) {
     constexpr int NW = BY * BZ;
     static_assert((NW & (NW - 1)) == 0, "grid_tail_n reduces a power-of-two number of waves");
     __shared__ BlockLds<T, NW> S;
@@ -1630,9 +1642,9 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
     const T alpha = UPDATE ? (T)ctrl->alpha : (T)0;
     double acc = 0.0;
     int round = 0;
-    for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
+    for (int pos = g; pos < pc.nblk; pos += G, ++round) {
         const int bc = pc.blk_list[pos];
-        sweep_priority(pos, round);
+        sweep_priority(pos, round, G);
         const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
         if (by < bk.nby && bz < bk.nbz) {
             PencilIter<false> it(map_row(bk, by, bz), map_row(bk, by - 1, bz), map_row(bk, by, bz - 1), bk.nbx);
@@ -1680,16 +1692,24 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
         }
         if (L.lane == 0) prog_store(&S.prog[w], ring_end(round));
     }
-    grid_tail_n<NW>(ctrl, UPDATE ? SC_RR : SC_NOP, UPDATE ? wave_sum(acc) : 0.0, tag, stage, timeout);
+    grid_tail_n<NW>(ctrl, UPDATE ? SC_RR : SC_NOP, UPDATE ? wave_sum(acc) : 0.0, tag, stage, timeout, g, G);
+}
+
+template <class T, bool UPDATE, int BY, int BZ>
+__global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_block(Bricks bk, Pencils pc, Coef<T> c,
+                                                                                   T* r, T* x,
+                                                            const T* p, const T* q, T* y, Ctrl* ctrl, uint64_t* xch,
+                                                            uint64_t* stage, uint64_t timeout) {
+    fwd_block_body<T, UPDATE, BY, BZ>((int)blockIdx.x, (int)gridDim.x, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
+                                      timeout);
 }
 
 // Backward sweep over pencil blocks: blocks in reverse list order, bricks in
 // descending x, +y/+z faces from LDS inside the block.
 template <class T, int BY, int BZ>
-__global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_block(Bricks bk, Pencils pc, Coef<T> c,
-                                                                                   const T* r, T* y,
-                                                            Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode,
-                                                            uint64_t timeout) {
+__device__ __forceinline__ void bwd_block_body(int g, int G, const Bricks& bk, const Pencils& pc, const Coef<T>& c,
+                                               const T* r, T* y, Ctrl* ctrl, uint64_t* xch, uint64_t* stage,
+                                               int tail_mode, uint64_t timeout) {
     constexpr int NW = BY * BZ;
     static_assert((NW & (NW - 1)) == 0, "grid_tail_n reduces a power-of-two number of waves");
     __shared__ BlockLds<T, NW> S;
@@ -1713,9 +1733,9 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
     const uint32_t tag = next_tag(ctrl->red_epoch);
     double acc = 0.0;
     int round = 0;
-    for (int pos = (int)blockIdx.x; pos < pc.nblk; pos += (int)gridDim.x, ++round) {
+    for (int pos = g; pos < pc.nblk; pos += G, ++round) {
         const int bc = pc.blk_list[pc.nblk - 1 - pos];
-        sweep_priority(pos, round);
+        sweep_priority(pos, round, G);
         const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
         if (by < bk.nby && bz < bk.nbz) {
             PencilIter<true> it(map_row(bk, by, bz), map_row(bk, by + 1, bz), map_row(bk, by, bz + 1), bk.nbx);
@@ -1761,7 +1781,15 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
         }
         if (L.lane == 0) prog_store(&S.prog[w], ring_end(round));
     }
-    grid_tail_n<NW>(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout);
+    grid_tail_n<NW>(ctrl, tail_mode, wave_sum(acc), tag, stage, timeout, g, G);
+}
+
+template <class T, int BY, int BZ>
+__global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_block(Bricks bk, Pencils pc, Coef<T> c,
+                                                                                   const T* r, T* y,
+                                                            Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode,
+                                                            uint64_t timeout) {
+    bwd_block_body<T, BY, BZ>((int)blockIdx.x, (int)gridDim.x, bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout);
 }
 
 // Lists of pencil blocks, one workgroup per block shape: the blocks (BY x BZ
@@ -1816,6 +1844,43 @@ __global__ __launch_bounds__(1024) void k_block_list(const int32_t* pmap, int nb
         for (int Y2 = max(0, l - NBZ + 1); Y2 < Y; ++Y2) rank += flag[(size_t)Y2 + (size_t)NBY * (l - Y2)];
         list[off[l] + rank] = Y | (Z << 10);
     }
+}
+
+// ---------------------------------------------------------------- batched Krylov chunks
+// Serving: one launch per kernel class advances every request of a batch
+// (KrylovJob). A workgroup finds its request from its index and runs the
+// same code, with the same workgroup index and count, as that request's
+// direct launch; each request reduces over its own workgroups only, so its
+// result does not depend on the batch it ran in.
+__global__ __launch_bounds__(256) void k_spmv_batch(const KrylovJob* __restrict__ jobs, int per_job, int it) {
+    const int k = (int)blockIdx.x / per_job, g = (int)blockIdx.x % per_job;
+    const KrylovJob J = jobs[k];
+    if (J.ctrl == nullptr || g >= J.g_spmv) return;
+    const bool odd = ((J.par + it) & 1) != 0;  // p_in = P[kiter & 1], p_out = P[(kiter + 1) & 1]
+    spmv_body<float>(g, J.g_spmv, J.bk, J.c, J.y, odd ? J.p1 : J.p0, odd ? J.p0 : J.p1, J.q, J.ctrl, J.stage,
+                     J.timeout);
+}
+
+// One kernel per sweep shape (1x4, 2x2: both 256 threads); workgroups of
+// requests of the other shape exit at once.
+template <int BY, int BZ>
+__global__ __launch_bounds__(256, 4) void k_fwd_batch(const KrylovJob* __restrict__ jobs, int per_job, int it) {
+    constexpr int kShape = BY == 1 ? kShape1x4 : kShape2x2;
+    const int k = (int)blockIdx.x / per_job, g = (int)blockIdx.x % per_job;
+    const KrylovJob J = jobs[k];
+    if (J.ctrl == nullptr || J.pc.shape != kShape || g >= J.g_fwd) return;
+    const float* p = ((J.par + it) & 1) != 0 ? J.p0 : J.p1;  // P[(kiter + 1) & 1]
+    fwd_block_body<float, true, BY, BZ>(g, J.g_fwd, J.bk, J.pc, J.c, J.r, J.x, p, J.q, J.y, J.ctrl, J.xch, J.stage,
+                                        J.timeout);
+}
+
+template <int BY, int BZ>
+__global__ __launch_bounds__(256, 4) void k_bwd_batch(const KrylovJob* __restrict__ jobs, int per_job) {
+    constexpr int kShape = BY == 1 ? kShape1x4 : kShape2x2;
+    const int k = (int)blockIdx.x / per_job, g = (int)blockIdx.x % per_job;
+    const KrylovJob J = jobs[k];
+    if (J.ctrl == nullptr || J.pc.shape != kShape || g >= J.g_bwd) return;
+    bwd_block_body<float, BY, BZ>(g, J.g_bwd, J.bk, J.pc, J.c, J.r, J.y, J.ctrl, J.xch, J.stage, SC_BETA, J.timeout);
 }
 
 // ---------------------------------------------------------------- pencil setup
@@ -1983,18 +2048,23 @@ int occupancy(const void* kern, int threads) {
 // residency but costs about 20 us of extra dispatch latency per kernel here.
 // If another workload keeps some workgroups from starting, the bounded
 // waits end at the time limit and the launch reports the abort.
-template <typename... KArgs, typename... Args, size_t... I>
-hipError_t persistent_impl(void (*kern)(KArgs...), int items, int per_group, int threads, int blocks_per_cu,
-                           int max_groups, hipStream_t s, std::index_sequence<I...>, Args&&... args) {
-    std::tuple<KArgs...> t(std::forward<Args>(args)...);
-    void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
-    const int occ = occupancy((const void*)kern, threads);
+int persistent_grid(const void* kern, int items, int per_group, int threads, int blocks_per_cu, int max_groups) {
+    const int occ = occupancy(kern, threads);
     const int per_cu = occ < blocks_per_cu ? occ : blocks_per_cu;
     int g = (items + per_group - 1) / per_group;
     if (g > per_cu * device_cus()) g = per_cu * device_cus();
     if (g > kTailMaxGroups) g = kTailMaxGroups;
     if (max_groups > 0 && g > max_groups) g = max_groups;
     if (g < 1) g = 1;
+    return g;
+}
+
+template <typename... KArgs, typename... Args, size_t... I>
+hipError_t persistent_impl(void (*kern)(KArgs...), int items, int per_group, int threads, int blocks_per_cu,
+                           int max_groups, hipStream_t s, std::index_sequence<I...>, Args&&... args) {
+    std::tuple<KArgs...> t(std::forward<Args>(args)...);
+    void* ptrs[sizeof...(KArgs)] = {(void*)&std::get<I>(t)...};
+    const int g = persistent_grid((const void*)kern, items, per_group, threads, blocks_per_cu, max_groups);
     return hipLaunchKernel((const void*)kern, dim3(g), dim3(threads), ptrs, 0, s);
 }
 
@@ -2216,6 +2286,35 @@ hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, con
     if (level_count > 0)
         hipLaunchKernelGGL((k_bwd_level<T>), dim3(level_count), dim3(64), 0, s, bk, c, r, y, part, ctrl, xch,
                            level_base, timeout_ticks);
+    return hipGetLastError();
+}
+
+template <int BY, int BZ>
+void krylov_groups_shape(const Bricks& bk, const Pencils& pc, int max_groups, int* g_spmv, int* g_fwd, int* g_bwd) {
+    *g_spmv = persistent_grid((const void*)k_spmv<float>, bk.nb, 4, 256, kSpmvBlocksPerCU, max_groups);
+    *g_fwd = persistent_grid((const void*)k_fwd_block<float, true, BY, BZ>, pc.nblk, 1, 64 * BY * BZ,
+                             kBlockGroupsPerCU, max_groups);
+    *g_bwd = persistent_grid((const void*)k_bwd_block<float, BY, BZ>, pc.nblk, 1, 64 * BY * BZ, kBlockGroupsPerCU,
+                             max_groups);
+}
+
+void krylov_groups(const Bricks& bk, const Pencils& pc, int max_groups, int* g_spmv, int* g_fwd, int* g_bwd) {
+    if (pc.shape == kShape1x4)
+        krylov_groups_shape<1, 4>(bk, pc, max_groups, g_spmv, g_fwd, g_bwd);
+    else
+        krylov_groups_shape<2, 2>(bk, pc, max_groups, g_spmv, g_fwd, g_bwd);
+}
+
+hipError_t launch_krylov_batch(const KrylovJob* jobs, int njobs, int per_job, int n, int shapes, hipStream_t s) {
+    const dim3 grid((unsigned)(njobs * per_job)), block(256);
+    const bool s14 = (shapes & (1 << kShape1x4)) != 0, s22 = (shapes & (1 << kShape2x2)) != 0;
+    for (int it = 0; it < n; ++it) {
+        hipLaunchKernelGGL(k_spmv_batch, grid, block, 0, s, jobs, per_job, it);
+        if (s14) hipLaunchKernelGGL((k_fwd_batch<1, 4>), grid, block, 0, s, jobs, per_job, it);
+        if (s22) hipLaunchKernelGGL((k_fwd_batch<2, 2>), grid, block, 0, s, jobs, per_job, it);
+        if (s14) hipLaunchKernelGGL((k_bwd_batch<1, 4>), grid, block, 0, s, jobs, per_job);
+        if (s22) hipLaunchKernelGGL((k_bwd_batch<2, 2>), grid, block, 0, s, jobs, per_job);
+    }
     return hipGetLastError();
 }
 

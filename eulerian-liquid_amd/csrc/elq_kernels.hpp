@@ -139,6 +139,39 @@ template <class T>
 hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* r, T* x, const T* p, const T* q, T* y,
                       double* part, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int update, int level,
                       int level_base, int level_count, uint64_t timeout_ticks, int max_groups, hipStream_t s);
+// One request of a batched Krylov chunk (serving, mixed precision, wavefront
+// sweeps of the 1x4 or 2x2 shape): its solver state and the workgroups each
+// kernel class of its chunk uses (those its direct launches would use, so the
+// arithmetic, reductions included, is the same). Workgroup g of a batched
+// launch belongs to request g / per_job and is its workgroup g % per_job;
+// requests with ctrl == nullptr, and workgroups past a request's count, exit
+// at once.
+struct KrylovJob {
+    Bricks bk;
+    Pencils pc;
+    Coef<float> c;
+    float* r;
+    float* x;
+    float* p0;
+    float* p1;
+    float* q;
+    float* y;
+    Ctrl* ctrl;
+    uint64_t* xch;
+    uint64_t* stage;
+    uint64_t timeout;
+    int32_t g_spmv, g_fwd, g_bwd;
+    int32_t par;  // iterations done before the chunk, mod 2 (selects the p buffer)
+};
+// Workgroups of the direct launches of one Krylov iteration (SpMV, forward
+// sweep with updates, backward sweep) under the cap max_groups.
+void krylov_groups(const Bricks& bk, const Pencils& pc, int max_groups, int* g_spmv, int* g_fwd, int* g_bwd);
+// n iterations (SpMV, forward sweep with updates, backward sweep; Eigen's
+// loop body) of every request of jobs[0 .. njobs), each launch with
+// njobs * per_job workgroups; shapes: bit mask (1 << BlockShape) of the sweep
+// shapes the requests use (the sweeps launch once per shape).
+hipError_t launch_krylov_batch(const KrylovJob* jobs, int njobs, int per_job, int n, int shapes, hipStream_t s);
+
 // Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
 // level < 0: persistent launch over the pencil blocks ending in a grid tail
 // that applies tail_mode; else one brick level with per-brick partials in part.

@@ -11,6 +11,23 @@
 // configuration, which keeps a request's arithmetic (and its reduction
 // order) the same whatever else is running.
 //
+// Batched Krylov chunks (elq_server_opts.batched, mixed precision): each
+// slot thread still runs its projection (set-up, polls, residual
+// replacements, scatter and update on its own stream), but hands every Krylov
+// chunk (check_every iterations) to the Batcher, and the chunks of all slots
+// inside their Krylov loops advance together: one launch per kernel class
+// and iteration for all of them (launch_krylov_batch), recorded once into a
+// HIP graph per job table and set of sweep shapes and replayed for every
+// batch after. The launch arguments never change (the grid is max_slots
+// shares of the per-slot cap, the job table sits at a fixed device address
+// and is uploaded before each batch), so the graph serves every request
+// that ever runs. The first slot to arrive leads the batch: it waits until
+// every slot inside its Krylov loop has joined (at most batch_wait_us), then
+// enqueues on the batcher's stream, after each member's earlier work, the
+// table upload and the graph, and makes each member's stream wait for the
+// batch. A member's workgroups are those its direct launches would use, so
+// its result is the same batched or not, alone or in any mix.
+//
 // Admission: FIFO queue; the front request is admitted when a slot is free
 // and the device bytes held by all slots, with the admitted slot bounded by
 // what it holds plus the request's footprint (elq_projection_footprint), fit
@@ -24,8 +41,10 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/elq.h"
@@ -58,6 +77,179 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+constexpr int kDefaultBatchWaitUs = 20000;
+
+struct Batcher final : elq::KrylovBatcher {
+    int nslots = 0, per_job = 0, wait_us = kDefaultBatchWaitUs;
+    hipStream_t stream = nullptr;
+    elq::KrylovJob* dev[2] = {nullptr, nullptr};   // job tables, one per graph family
+    elq::KrylovJob* host[2] = {nullptr, nullptr};  // pinned staging of each table
+    hipEvent_t uploaded[2] = {nullptr, nullptr};   // the table's last upload has run
+    hipEvent_t done = nullptr;                     // the last batch has run
+    std::vector<hipEvent_t> ready;                 // per slot: its work before the chunk
+    std::vector<elq_ctx*> ctxs;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;  // (table, shapes, n)
+    std::mutex mu;
+    std::condition_variable cv;
+    uint64_t round = 0;
+    bool leading = false;     // a leader exists for the coming batch
+    bool collecting = false;  // ... and still takes members
+    std::vector<char> joined, in_krylov;
+    std::vector<int> rc_slot;
+    std::vector<elq::KrylovJob> pending;
+    int64_t batches = 0, jobs = 0;
+
+    int init(int slots, int groups, int wait) {
+        nslots = slots;
+        per_job = groups;
+        if (wait > 0) wait_us = wait;
+        ready.assign((size_t)slots, nullptr);
+        ctxs.assign((size_t)slots, nullptr);
+        joined.assign((size_t)slots, 0);
+        in_krylov.assign((size_t)slots, 0);
+        rc_slot.assign((size_t)slots, ELQ_OK);
+        pending.assign((size_t)slots, elq::KrylovJob{});
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) return ELQ_ERR_DEVICE;
+        for (int b = 0; b < 2; ++b) {
+            if (hipMalloc((void**)&dev[b], sizeof(elq::KrylovJob) * (size_t)slots) != hipSuccess)
+                return ELQ_ERR_DEVICE;
+            if (hipHostMalloc((void**)&host[b], sizeof(elq::KrylovJob) * (size_t)slots, hipHostMallocDefault) !=
+                hipSuccess)
+                return ELQ_ERR_DEVICE;
+            if (hipEventCreateWithFlags(&uploaded[b], hipEventDisableTiming) != hipSuccess) return ELQ_ERR_DEVICE;
+        }
+        if (hipEventCreateWithFlags(&done, hipEventDisableTiming) != hipSuccess) return ELQ_ERR_DEVICE;
+        for (auto& e : ready)
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return ELQ_ERR_DEVICE;
+        return ELQ_OK;
+    }
+
+    // Releases everything init() created (after the slot threads have ended).
+    void release() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& kv : graphs)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+        for (auto& e : ready)
+            if (e) (void)hipEventDestroy(e);
+        ready.clear();
+        if (done) (void)hipEventDestroy(done);
+        done = nullptr;
+        for (int b = 0; b < 2; ++b) {
+            if (uploaded[b]) (void)hipEventDestroy(uploaded[b]);
+            if (dev[b]) (void)hipFree(dev[b]);
+            if (host[b]) (void)hipHostFree(host[b]);
+            uploaded[b] = nullptr;
+            dev[b] = nullptr;
+            host[b] = nullptr;
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+
+    // Slot k's request has left its Krylov loop (for good, or for a residual
+    // replacement, after which it rejoins): no batch waits for it meanwhile.
+    void leave(int k) override {
+        std::lock_guard<std::mutex> lk(mu);
+        in_krylov[(size_t)k] = 0;
+        cv.notify_all();
+    }
+
+    bool all_joined() const {
+        for (int j = 0; j < nslots; ++j)
+            if (in_krylov[(size_t)j] && !joined[(size_t)j]) return false;
+        return true;
+    }
+
+    // Enqueues one batch (mu not held): after each member's earlier work, the
+    // table upload and the chunk's graph; each member's stream then waits.
+    int launch(int buf, int n, int shapes, const std::vector<int>& members) {
+        for (int j : members)
+            if (hipStreamWaitEvent(stream, ready[(size_t)j], 0) != hipSuccess) return ELQ_ERR_DEVICE;
+        if (hipMemcpyAsync(dev[buf], host[buf], sizeof(elq::KrylovJob) * (size_t)nslots, hipMemcpyHostToDevice,
+                           stream) != hipSuccess)
+            return ELQ_ERR_DEVICE;
+        if (hipEventRecord(uploaded[buf], stream) != hipSuccess) return ELQ_ERR_DEVICE;
+        hipGraphExec_t& g = graphs[std::make_tuple(buf, shapes, n)];
+        if (!g) {
+            hipGraph_t gr = nullptr;
+            if (hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed) != hipSuccess) return ELQ_ERR_DEVICE;
+            const hipError_t e1 = elq::launch_krylov_batch(dev[buf], nslots, per_job, n, shapes, stream);
+            const hipError_t e2 = hipStreamEndCapture(stream, &gr);
+            if (e1 != hipSuccess || e2 != hipSuccess || !gr) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return ELQ_ERR_DEVICE;
+            }
+            const hipError_t e3 = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            if (e3 != hipSuccess) {
+                g = nullptr;
+                return ELQ_ERR_DEVICE;
+            }
+        }
+        if (hipGraphLaunch(g, stream) != hipSuccess) return ELQ_ERR_DEVICE;
+        if (hipEventRecord(done, stream) != hipSuccess) return ELQ_ERR_DEVICE;
+        for (int j : members)
+            if (hipStreamWaitEvent(ctxs[(size_t)j]->stream, done, 0) != hipSuccess) return ELQ_ERR_DEVICE;
+        return ELQ_OK;
+    }
+
+    int run(elq_ctx* ctx, int k, const elq::KrylovJob& job, int n) override {
+        if (hipEventRecord(ready[(size_t)k], ctx->stream) != hipSuccess)
+            return ctx->fail(ELQ_ERR_DEVICE, "batched Krylov chunk: hipEventRecord failed");
+        std::unique_lock<std::mutex> lk(mu);
+        in_krylov[(size_t)k] = 1;
+        for (;;) {
+            if (!leading) break;  // lead the next batch
+            if (collecting) {     // join the batch being collected
+                pending[(size_t)k] = job;
+                joined[(size_t)k] = 1;
+                const uint64_t my = round;
+                cv.notify_all();  // the leader re-checks whether everyone has joined
+                cv.wait(lk, [&] { return round != my; });
+                const int rc = rc_slot[(size_t)k];
+                lk.unlock();
+                return rc == ELQ_OK ? rc : ctx->fail(rc, "batched Krylov chunk failed to launch");
+            }
+            cv.wait(lk, [&] { return !leading; });  // a batch is launching without this slot
+        }
+        leading = true;
+        collecting = true;
+        pending[(size_t)k] = job;
+        joined[(size_t)k] = 1;
+        const auto until = std::chrono::steady_clock::now() + std::chrono::microseconds(wait_us);
+        cv.wait_until(lk, until, [&] { return all_joined(); });
+        collecting = false;
+        const int buf = (int)(round & 1);
+        // the previous upload of this table (two batches ago) has run before
+        // the table is rewritten
+        (void)hipEventSynchronize(uploaded[buf]);
+        std::vector<int> members;
+        int shapes = 0;
+        for (int j = 0; j < nslots; ++j) {
+            if (joined[(size_t)j]) {
+                members.push_back(j);
+                host[buf][j] = pending[(size_t)j];
+                shapes |= 1 << pending[(size_t)j].pc.shape;
+                joined[(size_t)j] = 0;
+            } else {
+                host[buf][j] = elq::KrylovJob{};
+            }
+        }
+        lk.unlock();
+        const int rc = launch(buf, n, shapes, members);
+        lk.lock();
+        for (int j : members) rc_slot[(size_t)j] = rc;
+        ++round;
+        ++batches;
+        jobs += (int64_t)members.size();
+        leading = false;
+        cv.notify_all();
+        lk.unlock();
+        return rc == ELQ_OK ? rc : ctx->fail(rc, "batched Krylov chunk failed to launch");
+    }
+};
+
 }  // namespace
 
 struct elq_server {
@@ -72,6 +264,9 @@ struct elq_server {
     int64_t next_id = 1;
     int running = 0;
     bool stop = false;
+    // batched Krylov chunks (opts.batched = number of lockstep groups):
+    // slot k belongs to group k % groups as its member k / groups
+    std::vector<Batcher*> batch;
 
     size_t held_total() const {
         size_t n = 0;
@@ -142,6 +337,7 @@ struct elq_server {
             lk.unlock();
             elq_stats st{};
             const int rc = elq_project(ctx, &r.g, r.dt, &o.opts, &st);
+            if (!batch.empty()) batch[(size_t)k % batch.size()]->leave(k / (int)batch.size());
             const double t_done = ms_since(t0);
             const size_t bytes = ctx->ws.bytes();
             lk.lock();
@@ -203,6 +399,31 @@ int elq_server_create(int device, const elq_server_opts* opts, elq_server** out)
         }
         sl.ctx->max_groups = groups;
     }
+    if (opts->batched > 0 && opts->opts.precision == ELQ_PRECISION_MIXED &&
+        opts->opts.schedule == ELQ_SCHED_WAVEFRONT) {
+        const int ng = std::min(opts->batched, opts->max_slots);
+        for (int gi = 0; gi < ng; ++gi) {
+            Batcher* B = new Batcher();
+            S->batch.push_back(B);
+            const int members = (opts->max_slots - gi + ng - 1) / ng;
+            const int rc = B->init(members, groups, opts->batch_wait_us);
+            if (rc != ELQ_OK) {
+                for (Batcher* b : S->batch) {
+                    b->release();
+                    delete b;
+                }
+                for (auto& t : S->slots) elq_ctx_destroy(t.ctx);
+                delete S;
+                return rc;
+            }
+        }
+        for (int k = 0; k < opts->max_slots; ++k) {
+            Batcher* B = S->batch[(size_t)k % (size_t)ng];
+            B->ctxs[(size_t)(k / ng)] = S->slots[(size_t)k].ctx;
+            S->slots[(size_t)k].ctx->batcher = B;
+            S->slots[(size_t)k].ctx->batch_slot = k / ng;
+        }
+    }
     S->t0 = std::chrono::steady_clock::now();
     for (int k = 0; k < opts->max_slots; ++k) {
         try {
@@ -228,6 +449,13 @@ void elq_server_destroy(elq_server* S) try {
     }
     for (auto& sl : S->slots)
         if (sl.th.joinable()) sl.th.join();
+    for (Batcher* b : S->batch) {
+        b->release();
+        delete b;
+    }
+    for (auto& sl : S->slots)
+        if (sl.ctx) sl.ctx->batcher = nullptr;
+    S->batch.clear();
     for (auto& sl : S->slots) elq_ctx_destroy(sl.ctx);
     delete S;
 } catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
@@ -289,6 +517,20 @@ int elq_server_in_flight(const elq_server* S) try {
     return (int)s->queue.size() + s->running;
 } catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
     return 0;
+}
+
+int elq_server_batch_stats(const elq_server* S, int64_t* batches, int64_t* jobs) try {
+    if (!S || !batches || !jobs) return ELQ_ERR_INVALID_ARG;
+    *batches = 0;
+    *jobs = 0;
+    for (Batcher* b : S->batch) {
+        std::lock_guard<std::mutex> lk(b->mu);
+        *batches += b->batches;
+        *jobs += b->jobs;
+    }
+    return ELQ_OK;
+} catch (...) {  // host resources exhausted: reported, never thrown past the C ABI
+    return ELQ_ERR_DEVICE;
 }
 
 size_t elq_server_device_bytes(const elq_server* S) try {

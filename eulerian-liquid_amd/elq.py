@@ -62,7 +62,7 @@ class Stats(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_slots", C.c_int32), ("max_queue", C.c_int32),
-                ("opts", Opts)]
+                ("opts", Opts), ("batched", C.c_int32), ("batch_wait_us", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -139,6 +139,7 @@ def load():
     L.elq_server_in_flight.argtypes = [vp]
     L.elq_server_device_bytes.argtypes = [vp]
     L.elq_server_device_bytes.restype = sz
+    L.elq_server_batch_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     L.elq_projection_footprint.argtypes = [i32, i32, i32, i32, vp]
     L.elq_projection_footprint.restype = sz
     L.elq_ctx_device_bytes.argtypes = [vp]
@@ -594,13 +595,15 @@ class Server:
     scene must not be touched until then."""
 
     def __init__(self, device: int = 0, max_slots: int = 4, max_queue: int = 4096, hbm_budget: int = 0,
-                 opts: Opts | None = None):
+                 opts: Opts | None = None, batched: int = 0, batch_wait_us: int = 0):
         self.L = load()
         so = ServerOpts()
         so.hbm_budget_bytes = hbm_budget
         so.max_slots = max_slots
         so.max_queue = max_queue
         so.opts = opts if opts is not None else default_opts()
+        so.batched = int(batched)
+        so.batch_wait_us = int(batch_wait_us)
         h = C.c_void_p()
         rc = self.L.elq_server_create(device, C.byref(so), C.byref(h))
         if rc != OK:
@@ -639,6 +642,12 @@ class Server:
 
     def device_bytes(self) -> int:
         return int(self.L.elq_server_device_bytes(self.h))
+
+    def batch_stats(self) -> tuple[int, int]:
+        """(batched Krylov chunks launched, requests they advanced)."""
+        b, j = C.c_int64(), C.c_int64()
+        _check_rc(self.L.elq_server_batch_stats(self.h, C.byref(b), C.byref(j)), None)
+        return b.value, j.value
 
     def close(self):
         if getattr(self, "h", None) is not None:

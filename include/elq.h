@@ -350,6 +350,17 @@ typedef struct {
     int32_t max_slots;       /* concurrently running requests, 1..16 */
     int32_t max_queue;       /* waiting requests; submit returns ELQ_ERR_BUSY beyond */
     elq_opts opts;           /* solver options for every request */
+    int32_t batched;         /* > 0: the Krylov iterations of the running
+                                requests advance together in lockstep, one
+                                launch per kernel class for all of them,
+                                replayed from a HIP graph captured once (mixed
+                                precision, wavefront schedule; ignored
+                                otherwise), in `batched` groups of slots (slot
+                                k in group k % batched) on streams of their
+                                own; 0: each slot launches its own */
+    int32_t batch_wait_us;   /* batched: at most how long a chunk waits for the
+                                other requests inside their Krylov loops to join
+                                it (0: 20 ms) */
 } elq_server_opts;
 
 typedef struct {
@@ -383,6 +394,10 @@ int elq_server_poll(elq_server* srv, elq_result* out, int cap);
 int elq_server_in_flight(const elq_server* srv);
 /* Device bytes the slots hold now. */
 size_t elq_server_device_bytes(const elq_server* srv);
+/* Batched servers: batched Krylov chunks launched so far and the requests
+ * they advanced (jobs / batches = mean batch size); 0 and 0 otherwise. No
+ * reference counterpart (serving only). */
+int elq_server_batch_stats(const elq_server* srv, int64_t* batches, int64_t* jobs);
 
 /* Device bytes a projection requests from the allocator on a fresh context,
  * host inputs staged (an upper bound; level_set NULL counts every brick,

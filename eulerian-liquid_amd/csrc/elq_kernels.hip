@@ -75,6 +75,25 @@ __device__ __forceinline__ T move(T v) {
     }
 }
 
+// The z move of a hyperplane step (lane - 16, or lane + 16 with UP): in FP64
+// through the LDS crossbar (ds_bpermute, two per value), which takes no VALU
+// issue slots; the two permlane swaps and select of move<zm/zp> per 32-bit
+// half cost the FP64 sweeps more (measured 164 against 172 ms per 256^3
+// solve), while in FP32, where the step waits on latency rather than issue,
+// the crossbar's latency costs more than the swaps (100 against 96 ms).
+template <bool UP, class T>
+__device__ __forceinline__ T step_zmove(T v, int addr) {
+    if constexpr (sizeof(T) == 4) {
+        (void)addr;
+        return move<UP ? Mv::zp : Mv::zm>(v);
+    } else {
+        const uint64_t u = (uint64_t)__double_as_longlong(v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)u);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_bpermute(addr, (int)(uint32_t)(u >> 32));
+        return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -1366,9 +1385,10 @@ __device__ __forceinline__ T fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdP
     const T cyy = iy ? pr.aym * cur.ie : (T)0;
     const T cz = iz ? pr.azm * cur.ie : (T)0;
     T yv = (T)0;
+    const int zaddr = ((L.lane - 16) & 63) << 2;  // ds_bpermute byte address of lane - 16
 #pragma unroll
     for (int h = 0; h < 10; ++h) {
-        const T vx = move<Mv::xm>(yv), vy = move<Mv::ym>(yv), vz = move<Mv::zm>(yv);
+        const T vx = move<Mv::xm>(yv), vy = move<Mv::ym>(yv), vz = step_zmove<false>(yv, zaddr);
         T t = fma(-cx, vx, b0);
         t = fma(-cyy, vy, t);
         yv = fma(-cz, vz, t);
@@ -1435,9 +1455,10 @@ __device__ __forceinline__ T bwd_compute(const BwdIn<T>& cur, T ex, const T (&go
     rhs = iz ? rhs : fma(-cz, got[1], rhs);
     const T kx = ix ? cx : (T)0, ky = iy ? cyy : (T)0, kz = iz ? cz : (T)0;
     T zv = (T)0;
+    const int zaddr = ((L.lane + 16) & 63) << 2;  // ds_bpermute byte address of lane + 16
 #pragma unroll
     for (int h = 9; h >= 0; --h) {
-        const T vx = move<Mv::xp>(zv), vy = move<Mv::yp>(zv), vz = move<Mv::zp>(zv);
+        const T vx = move<Mv::xp>(zv), vy = move<Mv::yp>(zv), vz = step_zmove<true>(zv, zaddr);
         T t = fma(-kx, vx, rhs);
         t = fma(-ky, vy, t);
         zv = fma(-kz, vz, t);

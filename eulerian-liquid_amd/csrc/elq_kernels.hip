@@ -94,6 +94,46 @@ __device__ __forceinline__ T step_zmove(T v, int addr) {
     }
 }
 
+// One hyperplane step of a brick sweep: b0 - cx v(x) - cy v(y) - cz v(z),
+// v(.) the -x/-y/-z neighbour's value (UP: +x/+y/+z). A wave issues one
+// instruction per visit of its SIMD, so the step's instruction count sets a
+// sweep's pace. FP32: the z neighbour takes two permlane swaps and a select;
+// the second swap works in the register of the x neighbour's value, dead once
+// its fma has run, so the only copy is the one the first swap needs (its two
+// operands are both written). FP64: step_zmove.
+template <bool UP, class T>
+__device__ __forceinline__ T hyper_step(T v, T b0, T cx, T cy, T cz, int zaddr) {
+    if constexpr (sizeof(T) == 4) {
+        (void)zaddr;
+        const uint32_t u = __float_as_uint(v);
+        const T vx = move<UP ? Mv::xp : Mv::xm>(v), vy = move<UP ? Mv::yp : Mv::ym>(v);
+        // a = ([r0 r0 r2 r2], [r1 r1 r3 r3]) in rows r0..r3 of v
+        const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+        T t = fma(-cx, vx, b0);
+        t = fma(-cy, vy, t);
+        const int row = (int)(threadIdx.x & 63) >> 4;
+        uint32_t z;
+        if constexpr (!UP) {
+            // [- r0 r1 r2]: rows 1, 3 of a[0]; row 2 = row 0 of a[1], moved by
+            // the swap of rows 2, 3 of (vx's register) with rows 0, 1 of a[1]
+            const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(vx), a[1], false, false);
+            z = row == 2 ? b[0] : a[0];
+        } else {
+            // [r1 r2 r3 -]: rows 0, 2 of a[1]; row 1 = row 3 of a[0], moved by
+            // the swap of rows 2, 3 of a[0] with rows 0, 1 of (vx's register)
+            const auto b = __builtin_amdgcn_permlane32_swap(a[0], __float_as_uint(vx), false, false);
+            z = row == 1 ? b[1] : a[1];
+        }
+        return fma(-cz, __uint_as_float(z), t);
+    } else {
+        const T vx = move<UP ? Mv::xp : Mv::xm>(v), vy = move<UP ? Mv::yp : Mv::ym>(v);
+        const T vz = step_zmove<UP>(v, zaddr);
+        T t = fma(-cx, vx, b0);
+        t = fma(-cy, vy, t);
+        return fma(-cz, vz, t);
+    }
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
@@ -1387,12 +1427,7 @@ __device__ __forceinline__ T fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdP
     T yv = (T)0;
     const int zaddr = ((L.lane - 16) & 63) << 2;  // ds_bpermute byte address of lane - 16
 #pragma unroll
-    for (int h = 0; h < 10; ++h) {
-        const T vx = move<Mv::xm>(yv), vy = move<Mv::ym>(yv), vz = step_zmove<false>(yv, zaddr);
-        T t = fma(-cx, vx, b0);
-        t = fma(-cyy, vy, t);
-        yv = fma(-cz, vz, t);
-    }
+    for (int h = 0; h < 10; ++h) yv = hyper_step<false>(yv, b0, cx, cyy, cz, zaddr);
     if (publish) bput<T>(xch, cur.b, yv, epoch, L);
     y[(size_t)cur.b * kBrickCells + L.lane] = yv;
     cy.prev_y = yv;
@@ -1457,12 +1492,7 @@ __device__ __forceinline__ T bwd_compute(const BwdIn<T>& cur, T ex, const T (&go
     T zv = (T)0;
     const int zaddr = ((L.lane + 16) & 63) << 2;  // ds_bpermute byte address of lane + 16
 #pragma unroll
-    for (int h = 9; h >= 0; --h) {
-        const T vx = move<Mv::xp>(zv), vy = move<Mv::yp>(zv), vz = step_zmove<true>(zv, zaddr);
-        T t = fma(-kx, vx, rhs);
-        t = fma(-ky, vy, t);
-        zv = fma(-kz, vz, t);
-    }
+    for (int h = 9; h >= 0; --h) zv = hyper_step<true>(zv, rhs, kx, ky, kz, zaddr);
     if (publish) bput<T>(xch, cur.b, zv, epoch, L);
     y[(size_t)cur.b * kBrickCells + L.lane] = zv;
     cy.prev_z = zv;

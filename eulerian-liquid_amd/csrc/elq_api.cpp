@@ -325,6 +325,7 @@ size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, i
     b += (size_t)nb * (8 * 4 + 4 + (size_t)kXchWords * 8 + 2 * 8);  // nbr, lvl_list, xch, partials
     b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
     b += cells * (3 * 8 + 10 * tsz + 4);                         // diag64, b64, inv_e64, coefficients, vectors, inv_e32
+    if (precision == ELQ_PRECISION_FP64) b += cells * 3 * 8;     // scaled lower couplings
     if (precision == ELQ_PRECISION_MIXED) b += cells * 8;        // x64
     const size_t faces = (size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + (size_t)ni * nj * (nk + 1);
     if (staged) b += 4 * (C + 2 * faces);
@@ -489,6 +490,7 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     for (DevBuf* b : {&W.cdiag, &W.cax, &W.cay, &W.caz, &W.vx, &W.vr, &W.vp0, &W.vp1, &W.vq, &W.vy})
         HIPCHK(b->ensure(cells * sizeof(T)), "alloc");
     HIPCHK(W.inv_e32.ensure(cells * 4), "alloc");
+    if (sizeof(T) == 8) HIPCHK(W.cscaled.ensure(3 * cells * 8), "alloc");
     HIPCHK(W.part_rr.ensure((size_t)nb * 8), "alloc");
     HIPCHK(W.part_rz.ensure((size_t)nb * 8), "alloc");
     if (!W.ctrl.p) {
@@ -683,7 +685,7 @@ struct Solver {
             (uintptr_t)bk.nbx, (uintptr_t)bk.nby, (uintptr_t)bk.nbz, (uintptr_t)bk.nb, (uintptr_t)bk.nlev,
             (uintptr_t)bk.map, (uintptr_t)bk.coord, (uintptr_t)bk.nbr, (uintptr_t)bk.lvl_list, (uintptr_t)pc.np,
             (uintptr_t)pc.shape, (uintptr_t)pc.nblk, (uintptr_t)pc.blk_list, (uintptr_t)c.diag, (uintptr_t)c.ax,
-            (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
+            (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)c.lx, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
             (uintptr_t)W.vp1.p, (uintptr_t)W.vq.p, (uintptr_t)W.vr.p, (uintptr_t)W.vx.p, (uintptr_t)prr(),
             (uintptr_t)prz(), (uintptr_t)ctrl(), (uintptr_t)W.xch.p, (uintptr_t)stage()};
         if (!ctx->chunk_exec || key != ctx->chunk_key) {
@@ -854,11 +856,21 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     S.c.ay = (const T*)W.cay.p;
     S.c.az = (const T*)W.caz.p;
     S.c.inv_e = mixed ? (const T*)W.inv_e32.p : (const T*)W.inv_e64.p;
+    if constexpr (sizeof(T) == 8) {
+        const size_t nc = (size_t)S.bk.nb * kBrickCells;
+        S.c.lx = (const T*)W.cscaled.p;
+        S.c.ly = S.c.lx + nc;
+        S.c.lz = S.c.lx + 2 * nc;
+    }
     Ctrl hc;
     ctrl_host_init(hc, o, mixed);
     if ((rc = upload_ctrl(ctx, hc))) return rc;
     const double tau = o.precond == ELQ_PRECOND_MIC0 ? o.tau : 0.0;
     if ((rc = S.factor(tau, o.sigma, mixed ? (float*)W.inv_e32.p : nullptr))) return rc;
+    if constexpr (sizeof(T) == 8)
+        HIPCHK(launch_scale_lower(S.bk, S.c.ax, S.c.ay, S.c.az, S.c.inv_e, (double*)S.c.lx, (double*)S.c.ly,
+                                  (double*)S.c.lz, ctx->stream),
+               "k_scale_lower");
     HIPCHK(hipStreamSynchronize(ctx->stream), "sync");
     const double t1 = now_ms();
     if constexpr (std::is_same<T, float>::value) {

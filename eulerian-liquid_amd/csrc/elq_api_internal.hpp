@@ -49,6 +49,7 @@ struct Workspace {
     DevBuf pflag, pmap, bkflag, blist;
     // system
     DevBuf diag64, b64, inv_e64, inv_e32, cdiag, cax, cay, caz;
+    DevBuf cscaled;  // FP64: pivot-scaled lower couplings lx, ly, lz of the forward sweep
     // Krylov vectors (storage type of the solve)
     DevBuf vx, vr, vp0, vp1, vq, vy, x64;
     DevBuf part_rr, part_rz, ctrl, nfixed;
@@ -77,7 +78,7 @@ struct Workspace {
         for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
                           &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &bkflag, &blist,
                           &diag64, &b64, &inv_e64, &inv_e32,
-                          &cdiag, &cax, &cay, &caz, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
+                          &cdiag, &cax, &cay, &caz, &cscaled, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
                           &ctrl, &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
                           &export_buf, &ext, &adv, &rei})
             f(*b);

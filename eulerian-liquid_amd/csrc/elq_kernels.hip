@@ -1312,6 +1312,7 @@ template <class T, bool UPDATE>
 struct FwdIn {
     int b, bx, ny, nz;
     T rv, xv, pv, qv, ie, axo, ayo, azo, aym_e, azm_e;
+    T lx, ly, lz;  // FP64: pivot-scaled lower couplings (Coef::lx, ly, lz)
     Polls2<T> poll;
 };
 
@@ -1333,21 +1334,24 @@ __device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, in
         d.qv = q[o];
     }
     d.ie = c.inv_e[o];
-    d.axo = c.ax[o];
-    d.ayo = c.ay[o];
-    d.azo = c.az[o];
     const bool wy = L.ly == 0 && nym >= 0, wz = L.lz == 0 && nzm >= 0;
     const int ly_lane = wy ? L.lane + 12 : L.lane, lz_lane = wz ? L.lane + 48 : L.lane;
     const int yb = wy ? nym : b, zb = wz ? nzm : b;
-    const T ay_e = c.ay[(size_t)yb * kBrickCells + ly_lane];
-    const T az_e = c.az[(size_t)zb * kBrickCells + lz_lane];
-    d.aym_e = wy ? ay_e : (T)0;
-    d.azm_e = wz ? az_e : (T)0;
+    if constexpr (sizeof(T) == 8) {
+        d.lx = c.lx[o];
+        d.ly = c.ly[o];
+        d.lz = c.lz[o];
+    } else {
+        d.axo = c.ax[o];
+        d.ayo = c.ay[o];
+        d.azo = c.az[o];
+        const T ay_e = c.ay[(size_t)yb * kBrickCells + ly_lane];
+        const T az_e = c.az[(size_t)zb * kBrickCells + lz_lane];
+        d.aym_e = wy ? ay_e : (T)0;
+        d.azm_e = wz ? az_e : (T)0;
+    }
     d.poll.want[0] = wy && gy;
     d.poll.want[1] = wz && gz;
-    // every lane of the wave reads from the face it waits for (ly := 3 or
-    // lz := 3 of the neighbour brick), so a poll round touches only the
-    // face's cache lines; without a neighbour word to wait for, one word
     if constexpr (sizeof(T) == 8) {
         // FP64 (bricks publish their faces only): every lane polls a word of
         // the face it waits for (ly := 3, lz := 3 of the neighbour), so a poll
@@ -1394,12 +1398,14 @@ __device__ __forceinline__ FwdPre<T, UPDATE> fwd_pre(const FwdIn<T, UPDATE>& cur
     }
     pr.rv = rv;
     pr.adj = cy.prev_bx == cur.bx - 1;
-    const T sx = move<Mv::xm>(cur.axo), sxp = move<Mv::q3>(cy.prev_ax);
-    const T sy = move<Mv::ym>(cur.ayo), sz = move<Mv::zm>(cur.azo);
+    if constexpr (sizeof(T) == 4) {
+        const T sx = move<Mv::xm>(cur.axo), sxp = move<Mv::q3>(cy.prev_ax);
+        const T sy = move<Mv::ym>(cur.ayo), sz = move<Mv::zm>(cur.azo);
+        pr.axm = ix ? sx : (pr.adj ? sxp : (T)0);
+        pr.aym = iy ? sy : cur.aym_e;
+        pr.azm = iz ? sz : cur.azm_e;
+    }
     const T pyx = move<Mv::q3>(cy.prev_y);
-    pr.axm = ix ? sx : (pr.adj ? sxp : (T)0);
-    pr.aym = iy ? sy : cur.aym_e;
-    pr.azm = iz ? sz : cur.azm_e;
     pr.ex = pr.adj ? pyx : (T)0;
     return pr;
 }
@@ -1416,14 +1422,29 @@ __device__ __forceinline__ T fwd_compute(const FwdIn<T, UPDATE>& cur, const FwdP
     // three dependent fmas. Every lane evaluates every step; a lane's value is
     // final from its own hyperplane lx+ly+lz on (its inputs are final by then
     // and it keeps recomputing the same value), so no per-step select is needed.
-    T rhs = pr.rv;
-    rhs = ix ? rhs : fma(-pr.axm, pr.ex, rhs);
-    rhs = iy ? rhs : fma(-pr.aym, got[0], rhs);
-    rhs = iz ? rhs : fma(-pr.azm, got[1], rhs);
-    const T b0 = rhs * cur.ie;
-    const T cx = ix ? pr.axm * cur.ie : (T)0;
-    const T cyy = iy ? pr.aym * cur.ie : (T)0;
-    const T cz = iz ? pr.azm * cur.ie : (T)0;
+    // FP32 builds the scaled couplings here from ax, ay, az, which the SpMV
+    // reads too: three more arrays per solve would push the working set of an
+    // iteration out of the 256 MB last-level cache at 256^3. FP64, far beyond
+    // that cache anyway, reads them precomputed (Coef::lx, ly, lz).
+    T b0, cx, cyy, cz;
+    if constexpr (sizeof(T) == 8) {
+        b0 = pr.rv * cur.ie;
+        b0 = ix ? b0 : fma(-cur.lx, pr.ex, b0);
+        b0 = iy ? b0 : fma(-cur.ly, got[0], b0);
+        b0 = iz ? b0 : fma(-cur.lz, got[1], b0);
+        cx = ix ? cur.lx : (T)0;
+        cyy = iy ? cur.ly : (T)0;
+        cz = iz ? cur.lz : (T)0;
+    } else {
+        T rhs = pr.rv;
+        rhs = ix ? rhs : fma(-pr.axm, pr.ex, rhs);
+        rhs = iy ? rhs : fma(-pr.aym, got[0], rhs);
+        rhs = iz ? rhs : fma(-pr.azm, got[1], rhs);
+        b0 = rhs * cur.ie;
+        cx = ix ? pr.axm * cur.ie : (T)0;
+        cyy = iy ? pr.aym * cur.ie : (T)0;
+        cz = iz ? pr.azm * cur.ie : (T)0;
+    }
     T yv = (T)0;
     const int zaddr = ((L.lane - 16) & 63) << 2;  // ds_bpermute byte address of lane - 16
 #pragma unroll
@@ -1841,6 +1862,37 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
                                                             Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode,
                                                             uint64_t timeout) {
     bwd_block_body<T, BY, BZ>((int)blockIdx.x, (int)gridDim.x, bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout);
+}
+
+// Pivot-scaled lower couplings of the FP64 forward sweep (one wave per
+// brick): lx = (coupling of the -x neighbour to this cell) * (this cell's
+// 1/e), the neighbour's +x coupling read across the brick face where it lies
+// in the -x brick, 0 without a -x neighbour; likewise ly, lz.
+__global__ __launch_bounds__(256) void k_scale_lower(Bricks bk, const double* ax, const double* ay,
+                                                     const double* az, const double* ie, double* lx, double* ly,
+                                                     double* lz) {
+    const int b = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (b >= bk.nb) return;
+    const Lane L;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    const int nx = nbr_id(bk, 0, b), ny = nbr_id(bk, 2, b), nz = nbr_id(bk, 4, b);
+    double axm = 0.0, aym = 0.0, azm = 0.0;
+    if (L.lx > 0)
+        axm = ax[o - 1];
+    else if (nx >= 0)
+        axm = ax[(size_t)nx * kBrickCells + L.lane + 3];
+    if (L.ly > 0)
+        aym = ay[o - 4];
+    else if (ny >= 0)
+        aym = ay[(size_t)ny * kBrickCells + L.lane + 12];
+    if (L.lz > 0)
+        azm = az[o - 16];
+    else if (nz >= 0)
+        azm = az[(size_t)nz * kBrickCells + L.lane + 48];
+    const double e = ie[o];
+    lx[o] = axm * e;
+    ly[o] = aym * e;
+    lz[o] = azm * e;
 }
 
 // Lists of pencil blocks, one workgroup per block shape: the blocks (BY x BZ
@@ -2354,6 +2406,13 @@ void krylov_groups(const Bricks& bk, const Pencils& pc, int max_groups, int* g_s
         krylov_groups_shape<1, 4>(bk, pc, max_groups, g_spmv, g_fwd, g_bwd);
     else
         krylov_groups_shape<2, 2>(bk, pc, max_groups, g_spmv, g_fwd, g_bwd);
+}
+
+hipError_t launch_scale_lower(const Bricks& bk, const double* ax, const double* ay, const double* az,
+                              const double* ie, double* lx, double* ly, double* lz, hipStream_t s) {
+    if (bk.nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_lower, dim3(cdiv(bk.nb, 4)), dim3(256), 0, s, bk, ax, ay, az, ie, lx, ly, lz);
+    return hipGetLastError();
 }
 
 hipError_t launch_krylov_batch(const KrylovJob* jobs, int njobs, int per_job, int n, int shapes, hipStream_t s) {

@@ -70,11 +70,17 @@ int sweep_round_groups(int max_groups);
 // Coefficients of the stored system (device pointers), T = storage type.
 template <class T>
 struct Coef {
-    const T* diag;
-    const T* ax;  // coupling of a cell to its +x neighbour (0 if none)
-    const T* ay;
-    const T* az;
-    const T* inv_e;  // IC factor pivots 1/e
+    const T* diag = nullptr;
+    const T* ax = nullptr;  // coupling of a cell to its +x neighbour (0 if none)
+    const T* ay = nullptr;
+    const T* az = nullptr;
+    const T* inv_e = nullptr;  // IC factor pivots 1/e
+    // FP64 forward sweep: lower couplings scaled by the cell's pivot inverse
+    // (launch_scale_lower), lx = (coupling of the -x neighbour to the cell) / e,
+    // 0 without that neighbour; likewise y, z. Unused (null) for FP32.
+    const T* lx = nullptr;
+    const T* ly = nullptr;
+    const T* lz = nullptr;
 };
 
 // ---------------------------------------------------------------- setup
@@ -171,6 +177,10 @@ void krylov_groups(const Bricks& bk, const Pencils& pc, int max_groups, int* g_s
 // njobs * per_job workgroups; shapes: bit mask (1 << BlockShape) of the sweep
 // shapes the requests use (the sweeps launch once per shape).
 hipError_t launch_krylov_batch(const KrylovJob* jobs, int njobs, int per_job, int n, int shapes, hipStream_t s);
+
+// Pivot-scaled lower couplings of the FP64 forward sweep (Coef::lx, ly, lz).
+hipError_t launch_scale_lower(const Bricks& bk, const double* ax, const double* ay, const double* az,
+                              const double* ie, double* lx, double* ly, double* lz, hipStream_t s);
 
 // Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
 // level < 0: persistent launch over the pencil blocks ending in a grid tail

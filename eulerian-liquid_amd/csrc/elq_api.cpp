@@ -326,6 +326,7 @@ size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, i
     b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
     b += cells * (3 * 8 + 10 * tsz + 4);                         // diag64, b64, inv_e64, coefficients, vectors, inv_e32
     if (precision == ELQ_PRECISION_FP64) b += cells * 3 * 8;     // scaled lower couplings
+    b += (size_t)std::max<int64_t>(nb, 1) * 4 + NB * 4 + 2 * kBrickCells * tsz;  // uniform-brick flags, constant rows
     if (precision == ELQ_PRECISION_MIXED) b += cells * 8;        // x64
     const size_t faces = (size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + (size_t)ni * nj * (nk + 1);
     if (staged) b += 4 * (C + 2 * faces);
@@ -491,6 +492,9 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
         HIPCHK(b->ensure(cells * sizeof(T)), "alloc");
     HIPCHK(W.inv_e32.ensure(cells * 4), "alloc");
     if (sizeof(T) == 8) HIPCHK(W.cscaled.ensure(3 * cells * 8), "alloc");
+    HIPCHK(W.uflag.ensure((size_t)std::max(nb, 1) * 4), "alloc");
+    HIPCHK(W.umap.ensure((size_t)p.NB * 4), "alloc");
+    HIPCHK(W.urow.ensure(2 * kBrickCells * sizeof(T)), "alloc");
     HIPCHK(W.part_rr.ensure((size_t)nb * 8), "alloc");
     HIPCHK(W.part_rz.ensure((size_t)nb * 8), "alloc");
     if (!W.ctrl.p) {
@@ -685,7 +689,7 @@ struct Solver {
             (uintptr_t)bk.nbx, (uintptr_t)bk.nby, (uintptr_t)bk.nbz, (uintptr_t)bk.nb, (uintptr_t)bk.nlev,
             (uintptr_t)bk.map, (uintptr_t)bk.coord, (uintptr_t)bk.nbr, (uintptr_t)bk.lvl_list, (uintptr_t)pc.np,
             (uintptr_t)pc.shape, (uintptr_t)pc.nblk, (uintptr_t)pc.blk_list, (uintptr_t)c.diag, (uintptr_t)c.ax,
-            (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)c.lx, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
+            (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)c.lx, (uintptr_t)c.uflag, (uintptr_t)c.umap, (uintptr_t)c.urow, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
             (uintptr_t)W.vp1.p, (uintptr_t)W.vq.p, (uintptr_t)W.vr.p, (uintptr_t)W.vx.p, (uintptr_t)prr(),
             (uintptr_t)prz(), (uintptr_t)ctrl(), (uintptr_t)W.xch.p, (uintptr_t)stage()};
         if (!ctx->chunk_exec || key != ctx->chunk_key) {
@@ -856,6 +860,9 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     S.c.ay = (const T*)W.cay.p;
     S.c.az = (const T*)W.caz.p;
     S.c.inv_e = mixed ? (const T*)W.inv_e32.p : (const T*)W.inv_e64.p;
+    S.c.uflag = (const int32_t*)W.uflag.p;
+    S.c.umap = (const int32_t*)W.umap.p;
+    S.c.urow = (const T*)W.urow.p;
     if constexpr (sizeof(T) == 8) {
         const size_t nc = (size_t)S.bk.nb * kBrickCells;
         S.c.lx = (const T*)W.cscaled.p;
@@ -867,6 +874,11 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     if ((rc = upload_ctrl(ctx, hc))) return rc;
     const double tau = o.precond == ELQ_PRECOND_MIC0 ? o.tau : 0.0;
     if ((rc = S.factor(tau, o.sigma, mixed ? (float*)W.inv_e32.p : nullptr))) return rc;
+    // interior coefficients: couplings -1 (full fluid faces), diagonal 6 (FP32
+    // keeps the diagonal's ghost part, 0 there)
+    HIPCHK(launch_brick_uniform<T>(S.bk, S.c.diag, S.c.ax, S.c.ay, S.c.az, (T)-1, sizeof(T) == 4 ? (T)0 : (T)6,
+                                   (int32_t*)W.uflag.p, (int32_t*)W.umap.p, (T*)W.urow.p, ctx->stream),
+           "k_brick_uniform");
     if constexpr (sizeof(T) == 8)
         HIPCHK(launch_scale_lower(S.bk, S.c.ax, S.c.ay, S.c.az, S.c.inv_e, (double*)S.c.lx, (double*)S.c.ly,
                                   (double*)S.c.lz, ctx->stream),

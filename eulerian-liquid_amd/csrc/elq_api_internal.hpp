@@ -50,6 +50,7 @@ struct Workspace {
     // system
     DevBuf diag64, b64, inv_e64, inv_e32, cdiag, cax, cay, caz;
     DevBuf cscaled;  // FP64: pivot-scaled lower couplings lx, ly, lz of the forward sweep
+    DevBuf uflag, umap, urow;  // uniform-brick flags (per brick, per position) and constant coefficient rows
     // Krylov vectors (storage type of the solve)
     DevBuf vx, vr, vp0, vp1, vq, vy, x64;
     DevBuf part_rr, part_rz, ctrl, nfixed;
@@ -78,7 +79,7 @@ struct Workspace {
         for (DevBuf* b : {&mask, &indices, &pressure, &scan_cnt, &scan_off, &bflag, &map, &coord, &lvl_cnt, &lvl_off,
                           &lvl_fill, &info, &nbr, &lvl_list, &xch, &stage, &pflag, &pmap, &bkflag, &blist,
                           &diag64, &b64, &inv_e64, &inv_e32,
-                          &cdiag, &cax, &cay, &caz, &cscaled, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
+                          &cdiag, &cax, &cay, &caz, &cscaled, &uflag, &umap, &urow, &vx, &vr, &vp0, &vp1, &vq, &vy, &x64, &part_rr, &part_rz,
                           &ctrl, &nfixed, &s_phi, &s_u, &s_v, &s_w, &s_uw, &s_vw, &s_ww, &s_pressure, &s_indices,
                           &export_buf, &ext, &adv, &rei})
             f(*b);

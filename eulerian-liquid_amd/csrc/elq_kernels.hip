@@ -29,6 +29,8 @@ namespace {
 
 // ------------------------------------------------------------------ helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ uint32_t bits_of(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ uint64_t bits_of(double v) { return (uint64_t)__double_as_longlong(v); }
 
 template <class T>
 __device__ __forceinline__ T shfl(T v, int src) {
@@ -978,14 +980,20 @@ __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef
     size_t e[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) e[d] = halo_offset(nb, d, o, lc, L.lane, has[d]);
-    const T zc = z[o], pi = p_in[o], dg = c.diag[o], axo = c.ax[o], ayo = c.ay[o], azo = c.az[o];
+    // a uniform brick reads its coefficients (and the -x/-y/-z face
+    // couplings, which equal them) from the 64-entry constant rows
+    const bool uni = c.uflag[b] != 0;
+    const size_t oc = uni ? (size_t)L.lane : o;
+    const T *cd = uni ? c.urow + kBrickCells : c.diag, *cx = uni ? c.urow : c.ax, *cy = uni ? c.urow : c.ay,
+            *cz = uni ? c.urow : c.az;
+    const T zc = z[o], pi = p_in[o], dg = cd[oc], axo = cx[oc], ayo = cy[oc], azo = cz[oc];
     T ze[6], pe[6];
 #pragma unroll
     for (int d = 0; d < 6; ++d) {
         ze[d] = z[e[d]];
         pe[d] = p_in[e[d]];
     }
-    const T axe = c.ax[e[0]], aye = c.ay[e[2]], aze = c.az[e[4]];
+    const T axe = cx[uni ? oc : e[0]], aye = cy[uni ? oc : e[2]], aze = cz[uni ? oc : e[4]];
     const T pc = fma(beta, pi, zc);
     p_out[o] = pc;
     T pn[6];
@@ -1178,14 +1186,20 @@ __global__ __launch_bounds__(64) void k_bwd_level(Bricks bk, Coef<T> c, const T*
 // Walks the active bricks of one pencil in x order (REVERSE: descending),
 // 64 map entries per chunk held one per lane, together with the map rows of
 // the two neighbouring pencils the sweep waits on (-y/-z forward, +y/+z
-// backward), so neighbour brick ids cost no memory round trip per brick.
+// backward) and the pencil's row of uniform-brick flags, so neighbour brick
+// ids and flags cost no memory round trip per brick. (The flags come by
+// position, loaded beside the map row: looking them up by brick id would put
+// a second dependent round trip at the start of every pencil, which measured
+// 10-13% slower sweeps.)
 template <bool REVERSE>
 struct PencilIter {
     const int32_t* row;
     const int32_t* row_y;  // nullptr when the neighbour pencil is outside the grid
     const int32_t* row_z;
+    const int32_t* urow_;  // uniform flags by brick position (Coef::umap), this pencil's row
     int nbx, base;
-    int mv, my, mz;
+    int mv, my, mz, mu;
+    int u = 0;  // uniform flag of the brick last returned by next()
     unsigned long long mask;
     __device__ void load_chunk() {
         const int bx = base + lane_id();
@@ -1193,10 +1207,11 @@ struct PencilIter {
         mv = in ? row[bx] : -1;
         my = (in && row_y) ? row_y[bx] : -1;
         mz = (in && row_z) ? row_z[bx] : -1;
+        mu = in ? urow_[bx] : 0;
         mask = __ballot(mv >= 0);
     }
-    __device__ PencilIter(const int32_t* r, const int32_t* ry, const int32_t* rz, int n)
-        : row(r), row_y(ry), row_z(rz), nbx(n) {
+    __device__ PencilIter(const int32_t* r, const int32_t* ry, const int32_t* rz, int n, const int32_t* ur)
+        : row(r), row_y(ry), row_z(rz), urow_(ur), nbx(n) {
         base = REVERSE ? ((n - 1) / 64) * 64 : 0;
         load_chunk();
     }
@@ -1220,6 +1235,7 @@ struct PencilIter {
         b = __builtin_amdgcn_readlane(mv, l);
         ny = __builtin_amdgcn_readlane(my, l);
         nz = __builtin_amdgcn_readlane(mz, l);
+        u = __builtin_amdgcn_readlane(mu, l);
         return true;
     }
 };
@@ -1319,9 +1335,9 @@ struct FwdIn {
 // Issues the loads of one brick (nothing is consumed here). Every load is
 // unconditional; requests a lane does not need read a valid dummy address.
 template <class T, bool UPDATE>
-__device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, int nym, int nzm, const Coef<T>& c,
-                                          const T* r, const T* x, const T* p, const T* q, const Lane& L,
-                                          bool gy = true, bool gz = true) {
+__device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, int nym, int nzm, int u,
+                                          const Coef<T>& c, const T* r, const T* x, const T* p, const T* q,
+                                          const Lane& L, bool gy = true, bool gz = true) {
     d.b = b;
     d.bx = bx;
     d.ny = nym;
@@ -1342,11 +1358,15 @@ __device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, in
         d.ly = c.ly[o];
         d.lz = c.lz[o];
     } else {
-        d.axo = c.ax[o];
-        d.ayo = c.ay[o];
-        d.azo = c.az[o];
-        const T ay_e = c.ay[(size_t)yb * kBrickCells + ly_lane];
-        const T az_e = c.az[(size_t)zb * kBrickCells + lz_lane];
+        // uniform brick: couplings, -y/-z face ones included, from the constant row
+        const bool uni = u != 0;
+        const T *cx = uni ? c.urow : c.ax, *cy = uni ? c.urow : c.ay, *cz = uni ? c.urow : c.az;
+        const size_t oc = uni ? (size_t)L.lane : o;
+        d.axo = cx[oc];
+        d.ayo = cy[oc];
+        d.azo = cz[oc];
+        const T ay_e = cy[uni ? oc : (size_t)yb * kBrickCells + ly_lane];
+        const T az_e = cz[uni ? oc : (size_t)zb * kBrickCells + lz_lane];
         d.aym_e = wy ? ay_e : (T)0;
         d.azm_e = wz ? az_e : (T)0;
     }
@@ -1465,7 +1485,7 @@ struct BwdIn {
 };
 
 template <class T>
-__device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, int nzp, const Coef<T>& c,
+__device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, int nzp, int u, const Coef<T>& c,
                                           const T* r, const T* y, const Lane& L, bool gy = true, bool gz = true) {
     d.b = b;
     d.bx = bx;
@@ -1474,9 +1494,11 @@ __device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, i
     const size_t o = (size_t)b * kBrickCells + L.lane;
     d.yv = y[o];
     d.ie = c.inv_e[o];
-    d.axo = c.ax[o];
-    d.ayo = c.ay[o];
-    d.azo = c.az[o];
+    const bool uni = u != 0;
+    const size_t oc = uni ? (size_t)L.lane : o;
+    d.axo = (uni ? c.urow : c.ax)[oc];
+    d.ayo = (uni ? c.urow : c.ay)[oc];
+    d.azo = (uni ? c.urow : c.az)[oc];
     d.rv = r[o];
     const bool wy = L.ly == 3 && nyp >= 0, wz = L.lz == 3 && nzp >= 0;
     d.poll.want[0] = wy && gy;
@@ -1719,12 +1741,13 @@ __device__ __forceinline__ void fwd_block_body(int g, int G, const Bricks& bk, c
         sweep_priority(pos, round, G);
         const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
         if (by < bk.nby && bz < bk.nbz) {
-            PencilIter<false> it(map_row(bk, by, bz), map_row(bk, by - 1, bz), map_row(bk, by, bz - 1), bk.nbx);
+            PencilIter<false> it(map_row(bk, by, bz), map_row(bk, by - 1, bz), map_row(bk, by, bz - 1), bk.nbx,
+                                 c.umap + (map_row(bk, by, bz) - bk.map));
             int bx, b, ny, nz;
             if (it.next(bx, b, ny, nz)) {
                 FwdIn<T, UPDATE> A, B, C;
                 FwdCarry<T> cy{(T)0, (T)0, -2, acc};
-                fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, c, r, x, p, q, L, gy, gz);
+                fwd_fetch<T, UPDATE>(A, b, bx, ny, nz, it.u, c, r, x, p, q, L, gy, gz);
                 A.poll.issue(xch);
                 T got[2];
                 auto step = [&](FwdIn<T, UPDATE>& cur, FwdIn<T, UPDATE>& nxt) {
@@ -1744,7 +1767,7 @@ __device__ __forceinline__ void fwd_block_body(int g, int G, const Bricks& bk, c
                 // ahead (earlier polls would mostly find stale tags)
                 auto ahead = [&](FwdIn<T, UPDATE>& buf, bool prev) {
                     const bool has = prev && it.next(bx, b, ny, nz);
-                    fwd_fetch<T, UPDATE>(buf, b, bx, ny, nz, c, r, x, p, q, L, gy, gz);
+                    fwd_fetch<T, UPDATE>(buf, b, bx, ny, nz, it.u, c, r, x, p, q, L, gy, gz);
                     return has;
                 };
                 bool hB = ahead(B, true);
@@ -1810,12 +1833,13 @@ __device__ __forceinline__ void bwd_block_body(int g, int G, const Bricks& bk, c
         sweep_priority(pos, round, G);
         const int by = (bc & 1023) * BY + dy, bz = ((bc >> 10) & 1023) * BZ + dz;
         if (by < bk.nby && bz < bk.nbz) {
-            PencilIter<true> it(map_row(bk, by, bz), map_row(bk, by + 1, bz), map_row(bk, by, bz + 1), bk.nbx);
+            PencilIter<true> it(map_row(bk, by, bz), map_row(bk, by + 1, bz), map_row(bk, by, bz + 1), bk.nbx,
+                                c.umap + (map_row(bk, by, bz) - bk.map));
             int bx, b, ny, nz;
             if (it.next(bx, b, ny, nz)) {
                 BwdIn<T> A, B, C;
                 BwdCarry<T> cy{(T)0, -2, acc};
-                bwd_fetch<T>(A, b, bx, ny, nz, c, r, y, L, gy, gz);
+                bwd_fetch<T>(A, b, bx, ny, nz, it.u, c, r, y, L, gy, gz);
                 A.poll.issue(xch);
                 T got[2];
                 auto step = [&](BwdIn<T>& cur, BwdIn<T>& nxt) {
@@ -1833,7 +1857,7 @@ __device__ __forceinline__ void bwd_block_body(int g, int G, const Bricks& bk, c
                 };
                 auto ahead = [&](BwdIn<T>& buf, bool prev) {
                     const bool has = prev && it.next(bx, b, ny, nz);
-                    bwd_fetch<T>(buf, b, bx, ny, nz, c, r, y, L, gy, gz);
+                    bwd_fetch<T>(buf, b, bx, ny, nz, it.u, c, r, y, L, gy, gz);
                     return has;
                 };
                 bool hB = ahead(B, true);
@@ -1862,6 +1886,49 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
                                                             Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode,
                                                             uint64_t timeout) {
     bwd_block_body<T, BY, BZ>((int)blockIdx.x, (int)gridDim.x, bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout);
+}
+
+// Uniform bricks (one wave per brick): every cell is an unknown whose
+// diagonal entry (FP32: its ghost part) and six couplings, the -x/-y/-z ones
+// read across the brick faces, equal the constants cref and dref bit for bit,
+// as in the interior of the fluid (full faces: coupling -1, diagonal 6, ghost
+// part 0). uflag[b] = 1 for those; workgroup 0 also writes the constant rows
+// urow[0, 64) = cref, urow[64, 128) = dref, from which the SpMV and the sweeps
+// then read a uniform brick's coefficients (same values, a few cache lines
+// instead of four arrays).
+template <class T>
+__global__ __launch_bounds__(256) void k_brick_uniform(Bricks bk, const T* diag, const T* ax, const T* ay,
+                                                       const T* az, T cref, T dref, int32_t* uflag, int32_t* umap,
+                                                       T* urow) {
+    if (blockIdx.x == 0 && threadIdx.x < 2 * kBrickCells) urow[threadIdx.x] = threadIdx.x < kBrickCells ? cref : dref;
+    const int b = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (b >= bk.nb) return;
+    const Lane L;
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    const int nx = nbr_id(bk, 0, b), ny = nbr_id(bk, 2, b), nz = nbr_id(bk, 4, b);
+    T axm = (T)0, aym = (T)0, azm = (T)0;
+    if (L.lx > 0)
+        axm = ax[o - 1];
+    else if (nx >= 0)
+        axm = ax[(size_t)nx * kBrickCells + L.lane + 3];
+    if (L.ly > 0)
+        aym = ay[o - 4];
+    else if (ny >= 0)
+        aym = ay[(size_t)ny * kBrickCells + L.lane + 12];
+    if (L.lz > 0)
+        azm = az[o - 16];
+    else if (nz >= 0)
+        azm = az[(size_t)nz * kBrickCells + L.lane + 48];
+    const auto same = [](T a, T v) { return bits_of(a) == bits_of(v); };
+    const bool ok = same(diag[o], dref) && same(ax[o], cref) && same(ay[o], cref) && same(az[o], cref) &&
+                    same(axm, cref) && same(aym, cref) && same(azm, cref);
+    const bool all = __all(ok);
+    if (L.lane == 0) {
+        const int cc = bk.coord[b];
+        const size_t pos = (size_t)(cc & 1023) + (size_t)bk.nbx * (((cc >> 10) & 1023) + (size_t)bk.nby * ((cc >> 20) & 1023));
+        uflag[b] = all ? 1 : 0;
+        umap[pos] = all ? 1 : 0;
+    }
 }
 
 // Pivot-scaled lower couplings of the FP64 forward sweep (one wave per
@@ -2408,6 +2475,14 @@ void krylov_groups(const Bricks& bk, const Pencils& pc, int max_groups, int* g_s
         krylov_groups_shape<2, 2>(bk, pc, max_groups, g_spmv, g_fwd, g_bwd);
 }
 
+template <class T>
+hipError_t launch_brick_uniform(const Bricks& bk, const T* diag, const T* ax, const T* ay, const T* az, T cref,
+                                T dref, int32_t* uflag, int32_t* umap, T* urow, hipStream_t s) {
+    hipLaunchKernelGGL((k_brick_uniform<T>), dim3(std::max(1u, cdiv(bk.nb, 4))), dim3(256), 0, s, bk, diag, ax, ay,
+                       az, cref, dref, uflag, umap, urow);
+    return hipGetLastError();
+}
+
 hipError_t launch_scale_lower(const Bricks& bk, const double* ax, const double* ay, const double* az,
                               const double* ie, double* lx, double* ly, double* lz, hipStream_t s) {
     if (bk.nb == 0) return hipSuccess;
@@ -2656,6 +2731,8 @@ hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStr
                                       hipStream_t);                                                                 \
     template hipError_t launch_bwd<T>(const Bricks&, const Pencils&, const Coef<T>&, const T*, T*, double*, Ctrl*,   \
                                       uint64_t*, uint64_t*, int, int, int, int, uint64_t, int, hipStream_t);        \
+    template hipError_t launch_brick_uniform<T>(const Bricks&, const T*, const T*, const T*, const T*, T, T, int32_t*, \
+                                                int32_t*, T*, hipStream_t);                                         \
     template hipError_t launch_resid64<T>(const Bricks&, const double*, const T*, const T*, const T*, const double*, \
                                           const double*, double*, double*, float*, float*, float*, float*, Ctrl*,   \
                                           hipStream_t);

@@ -81,6 +81,12 @@ struct Coef {
     const T* lx = nullptr;
     const T* ly = nullptr;
     const T* lz = nullptr;
+    // per active brick: 1 if its coefficients are the uniform interior ones
+    // (launch_brick_uniform); the SpMV and the wavefront sweeps then read them
+    // from urow: [0, 64) the coupling, [64, 128) the diagonal (FP32: its ghost part)
+    const int32_t* uflag = nullptr;
+    const int32_t* umap = nullptr;  // the same flags by brick position (as Bricks::map); inactive: stale
+    const T* urow = nullptr;
 };
 
 // ---------------------------------------------------------------- setup
@@ -178,6 +184,12 @@ void krylov_groups(const Bricks& bk, const Pencils& pc, int max_groups, int* g_s
 // shapes the requests use (the sweeps launch once per shape).
 hipError_t launch_krylov_batch(const KrylovJob* jobs, int njobs, int per_job, int n, int shapes, hipStream_t s);
 
+// Uniform-brick flags and constant rows (Coef::uflag, urow): a brick is
+// uniform when every cell's diagonal entry equals dref and its six couplings
+// equal cref, bit for bit.
+template <class T>
+hipError_t launch_brick_uniform(const Bricks& bk, const T* diag, const T* ax, const T* ay, const T* az, T cref,
+                                T dref, int32_t* uflag, int32_t* umap, T* urow, hipStream_t s);
 // Pivot-scaled lower couplings of the FP64 forward sweep (Coef::lx, ly, lz).
 hipError_t launch_scale_lower(const Bricks& bk, const double* ax, const double* ay, const double* az,
                               const double* ie, double* lx, double* ly, double* lz, hipStream_t s);

@@ -969,10 +969,93 @@ __device__ __forceinline__ size_t halo_offset(const Nbr6& nb, int d, size_t own,
 // neighbour values of p are rebuilt from z and p_old; in-brick ones come
 // through lane shuffles. All loads are issued unconditionally and at once.
 // Accumulation order: diag (FP32: its ghost part), -x, +x, -y, +y, -z, +z.
+// The FP32 SpMV issues two bricks' loads before either brick's arithmetic
+// (spmv_load, spmv_compute); FP64 runs one brick after the other in one
+// function (spmv_brick), whose code measured 4% faster than the same steps
+// through the split form. Both take the same loads in the same order.
+template <class T>
+struct SpmvIn {
+    size_t o;
+    bool has[6];
+    T zc, pi, dg, axo, ayo, azo, axe, aye, aze;
+    T ze[6], pe[6];
+};
+
+// Issues every load of brick b (nothing is consumed here).
+template <class T>
+__device__ __forceinline__ void spmv_load(SpmvIn<T>& d, int b, const Bricks& bk, const Coef<T>& c,
+                                          const T* __restrict__ z, const T* __restrict__ p_in, const Lane& L) {
+    const Nbr6 nb = load_nbr6(bk, b);
+    const size_t o = (size_t)b * kBrickCells + L.lane;
+    d.o = o;
+    const int lc[3] = {L.lx, L.ly, L.lz};
+    size_t e[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) e[k] = halo_offset(nb, k, o, lc, L.lane, d.has[k]);
+    // a uniform brick reads its coefficients (and the -x/-y/-z face
+    // couplings, which equal them) from the 64-entry constant rows
+    const bool uni = c.uflag[b] != 0;
+    const size_t oc = uni ? (size_t)L.lane : o;
+    const T *cd = uni ? c.urow + kBrickCells : c.diag, *cx = uni ? c.urow : c.ax, *cy = uni ? c.urow : c.ay,
+            *cz = uni ? c.urow : c.az;
+    d.zc = z[o];
+    d.pi = p_in[o];
+    d.dg = cd[oc];
+    d.axo = cx[oc];
+    d.ayo = cy[oc];
+    d.azo = cz[oc];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        d.ze[k] = z[e[k]];
+        d.pe[k] = p_in[e[k]];
+    }
+    d.axe = cx[uni ? oc : e[0]];
+    d.aye = cy[uni ? oc : e[2]];
+    d.aze = cz[uni ? oc : e[4]];
+}
+
+template <class T>
+__device__ __forceinline__ double spmv_compute(const SpmvIn<T>& d, T* __restrict__ p_out, T* __restrict__ q,
+                                               T beta, const Lane& L) {
+    static_assert(sizeof(T) == 4, "the split SpMV brick is the FP32 one");
+    const T pc = fma(beta, d.pi, d.zc);
+    p_out[d.o] = pc;
+    T pn[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) pn[k] = d.has[k] ? fma(beta, d.pe[k], d.ze[k]) : (T)0;
+    const T sxm = move<Mv::xm>(pc), sxp = move<Mv::xp>(pc);
+    const T sym = move<Mv::ym>(pc), syp = move<Mv::yp>(pc);
+    const T szm = move<Mv::zm>(pc), szp = move<Mv::zp>(pc);
+    const T cxm = move<Mv::xm>(d.axo), cym = move<Mv::ym>(d.ayo), czm = move<Mv::zm>(d.azo);
+    const T p_xm = L.lx > 0 ? sxm : pn[0];
+    const T p_xp = L.lx < 3 ? sxp : pn[1];
+    const T p_ym = L.ly > 0 ? sym : pn[2];
+    const T p_yp = L.ly < 3 ? syp : pn[3];
+    const T p_zm = L.lz > 0 ? szm : pn[4];
+    const T p_zp = L.lz < 3 ? szp : pn[5];
+    const T axm = L.lx > 0 ? cxm : (d.has[0] ? d.axe : (T)0);
+    const T aym = L.ly > 0 ? cym : (d.has[2] ? d.aye : (T)0);
+    const T azm = L.lz > 0 ? czm : (d.has[4] ? d.aze : (T)0);
+    // difference form g p_i + sum a_ij (p_j - p_i), g the ghost part of the
+    // diagonal: for a smooth p the sum of diag p_i and the off-diagonal terms
+    // cancels to a small value that FP32 would round away, while the
+    // differences are exact for close values
+    T acc = d.dg * pc;
+    acc = fma(axm, p_xm - pc, acc);
+    acc = fma(d.axo, p_xp - pc, acc);
+    acc = fma(aym, p_ym - pc, acc);
+    acc = fma(d.ayo, p_yp - pc, acc);
+    acc = fma(azm, p_zm - pc, acc);
+    acc = fma(d.azo, p_zp - pc, acc);
+    q[d.o] = acc;
+    return (double)pc * (double)acc;
+}
+
 template <class T>
 __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef<T>& c, const T* __restrict__ z,
                                              const T* __restrict__ p_in, T* __restrict__ p_out, T* __restrict__ q,
                                              T beta, const Lane& L) {
+    static_assert(sizeof(T) == 8, "the one-function SpMV brick is the FP64 one");
     const Nbr6 nb = load_nbr6(bk, b);
     const size_t o = (size_t)b * kBrickCells + L.lane;
     const int lc[3] = {L.lx, L.ly, L.lz};
@@ -1012,36 +1095,19 @@ __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef
     const T axm = L.lx > 0 ? cxm : (has[0] ? axe : (T)0);
     const T aym = L.ly > 0 ? cym : (has[2] ? aye : (T)0);
     const T azm = L.lz > 0 ? czm : (has[4] ? aze : (T)0);
-    T acc;
-    if constexpr (sizeof(T) == 4) {
-        // difference form g p_i + sum a_ij (p_j - p_i), g the ghost part of
-        // the diagonal: for a smooth p the sum of diag p_i and the
-        // off-diagonal terms cancels to a small value that FP32 would round
-        // away, while the differences are exact for close values
-        acc = dg * pc;
-        acc = fma(axm, p_xm - pc, acc);
-        acc = fma(axo, p_xp - pc, acc);
-        acc = fma(aym, p_ym - pc, acc);
-        acc = fma(ayo, p_yp - pc, acc);
-        acc = fma(azm, p_zm - pc, acc);
-        acc = fma(azo, p_zp - pc, acc);
-    } else {
-        acc = dg * pc;
-        acc = fma(axm, p_xm, acc);
-        acc = fma(axo, p_xp, acc);
-        acc = fma(aym, p_ym, acc);
-        acc = fma(ayo, p_yp, acc);
-        acc = fma(azm, p_zm, acc);
-        acc = fma(azo, p_zp, acc);
-    }
+    T acc = dg * pc;
+    acc = fma(axm, p_xm, acc);
+    acc = fma(axo, p_xp, acc);
+    acc = fma(aym, p_ym, acc);
+    acc = fma(ayo, p_yp, acc);
+    acc = fma(azm, p_zm, acc);
+    acc = fma(azo, p_zp, acc);
     q[o] = acc;
     return (double)pc * (double)acc;
 }
 
-// Persistent: wave w of W takes bricks w, w+W, ..., two per loop step so
-// that the second brick's loads overlap the first's arithmetic (the
-// restrict-qualified arrays let the compiler move them); its p.q terms
-// accumulate per lane in brick order; the grid tail computes alpha.
+// Persistent: wave w of W takes bricks w, w+W, ..., two per loop step; its
+// p.q terms accumulate per lane in brick order; the grid tail computes alpha.
 // Workgroup g of G (see grid_tail_n).
 template <class T>
 __device__ __forceinline__ void spmv_body(int g, int G, const Bricks& bk, const Coef<T>& c, const T* z,
@@ -1053,9 +1119,26 @@ __device__ __forceinline__ void spmv_body(int g, int G, const Bricks& bk, const 
     const Lane L;
     const int W = G * 4;
     double acc = 0.0;
-    for (int b = __builtin_amdgcn_readfirstlane(g * 4 + (int)(threadIdx.x >> 6)); b < bk.nb; b += 2 * W) {
-        acc += spmv_brick<T>(b, bk, c, z, p_in, p_out, q, beta, L);
-        if (b + W < bk.nb) acc += spmv_brick<T>(b + W, bk, c, z, p_in, p_out, q, beta, L);
+    const int b0 = __builtin_amdgcn_readfirstlane(g * 4 + (int)(threadIdx.x >> 6));
+    if constexpr (sizeof(T) == 4) {
+        // both bricks' loads go out before either brick's arithmetic (without
+        // a second brick the first is loaded twice and the copy is dropped):
+        // 77 VGPRs, six workgroups per CU
+        for (int b = b0; b < bk.nb; b += 2 * W) {
+            const bool two = b + W < bk.nb;
+            SpmvIn<T> A, B;
+            spmv_load<T>(A, b, bk, c, z, p_in, L);
+            spmv_load<T>(B, two ? b + W : b, bk, c, z, p_in, L);
+            acc += spmv_compute<T>(A, p_out, q, beta, L);
+            if (two) acc += spmv_compute<T>(B, p_out, q, beta, L);
+        }
+    } else {
+        // FP64 would need 85 VGPRs for that and measured slower with the five
+        // workgroups per CU they leave: one brick after the other
+        for (int b = b0; b < bk.nb; b += 2 * W) {
+            acc += spmv_brick<T>(b, bk, c, z, p_in, p_out, q, beta, L);
+            if (b + W < bk.nb) acc += spmv_brick<T>(b + W, bk, c, z, p_in, p_out, q, beta, L);
+        }
     }
     grid_tail(ctrl, SC_ALPHA, wave_sum(acc), tag, stage, timeout, g, G);
 }
@@ -2184,7 +2267,10 @@ __global__ __launch_bounds__(256) void k_update(Dense d, float dt, const float* 
 inline unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 constexpr int kWaveBlocksPerCU = 2;    // persistent factor sweep: 2 workgroups (8 waves) per CU
-constexpr int kSpmvBlocksPerCU = 4;    // persistent SpMV: up to 4 workgroups (16 waves) per CU
+// persistent SpMV: up to 8 workgroups (32 waves) per CU in FP32 (the
+// occupancy of its two-brick loop allows six), 4 in FP64 (8 measured 2% slower)
+template <class T>
+constexpr int kSpmvBlocksPerCU = sizeof(T) == 4 ? 8 : 4;
 constexpr int kBlockGroupsPerCU = 4;   // pencil-block sweeps: up to 4 workgroups per CU
 
 int device_cus() {
@@ -2378,7 +2464,7 @@ hipError_t launch_scalar(Ctrl* ctrl, int mode, const double* parts, int n, const
 template <class T>
 hipError_t launch_spmv(const Bricks& bk, const Coef<T>& c, const T* z, const T* p_in, T* p_out, T* q, Ctrl* ctrl,
                        uint64_t* stage, uint64_t timeout_ticks, int max_groups, hipStream_t s) {
-    return persistent(k_spmv<T>, bk.nb, kSpmvBlocksPerCU, max_groups, s, bk, c, z, p_in, p_out, q, ctrl, stage,
+    return persistent(k_spmv<T>, bk.nb, kSpmvBlocksPerCU<T>, max_groups, s, bk, c, z, p_in, p_out, q, ctrl, stage,
                       timeout_ticks);
 }
 
@@ -2461,7 +2547,7 @@ hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, con
 
 template <int BY, int BZ>
 void krylov_groups_shape(const Bricks& bk, const Pencils& pc, int max_groups, int* g_spmv, int* g_fwd, int* g_bwd) {
-    *g_spmv = persistent_grid((const void*)k_spmv<float>, bk.nb, 4, 256, kSpmvBlocksPerCU, max_groups);
+    *g_spmv = persistent_grid((const void*)k_spmv<float>, bk.nb, 4, 256, kSpmvBlocksPerCU<float>, max_groups);
     *g_fwd = persistent_grid((const void*)k_fwd_block<float, true, BY, BZ>, pc.nblk, 1, 64 * BY * BZ,
                              kBlockGroupsPerCU, max_groups);
     *g_bwd = persistent_grid((const void*)k_bwd_block<float, BY, BZ>, pc.nblk, 1, 64 * BY * BZ, kBlockGroupsPerCU,

@@ -326,7 +326,7 @@ size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, i
     b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
     b += cells * (3 * 8 + 10 * tsz + 4);                         // diag64, b64, inv_e64, coefficients, vectors, inv_e32
     if (precision == ELQ_PRECISION_FP64) b += cells * 3 * 8;     // scaled lower couplings
-    b += (size_t)std::max<int64_t>(nb, 1) * 4 + NB * 4 + 2 * kBrickCells * tsz;  // uniform-brick flags, constant rows
+    b += (size_t)std::max<int64_t>(nb, 1) * 4 + NB * 4 + 4 * kBrickCells * tsz;  // uniform-brick flags, constant rows
     if (precision == ELQ_PRECISION_MIXED) b += cells * 8;        // x64
     const size_t faces = (size_t)(ni + 1) * nj * nk + (size_t)ni * (nj + 1) * nk + (size_t)ni * nj * (nk + 1);
     if (staged) b += 4 * (C + 2 * faces);
@@ -494,7 +494,7 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
     if (sizeof(T) == 8) HIPCHK(W.cscaled.ensure(3 * cells * 8), "alloc");
     HIPCHK(W.uflag.ensure((size_t)std::max(nb, 1) * 4), "alloc");
     HIPCHK(W.umap.ensure((size_t)p.NB * 4), "alloc");
-    HIPCHK(W.urow.ensure(2 * kBrickCells * sizeof(T)), "alloc");
+    HIPCHK(W.urow.ensure(4 * kBrickCells * sizeof(T)), "alloc");
     HIPCHK(W.part_rr.ensure((size_t)nb * 8), "alloc");
     HIPCHK(W.part_rz.ensure((size_t)nb * 8), "alloc");
     if (!W.ctrl.p) {
@@ -875,9 +875,10 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     const double tau = o.precond == ELQ_PRECOND_MIC0 ? o.tau : 0.0;
     if ((rc = S.factor(tau, o.sigma, mixed ? (float*)W.inv_e32.p : nullptr))) return rc;
     // interior coefficients: couplings -1 (full fluid faces), diagonal 6 (FP32
-    // keeps the diagonal's ghost part, 0 there)
+    // keeps the diagonal's ghost part, 0 there); FP64 also flags interior pivots
     HIPCHK(launch_brick_uniform<T>(S.bk, S.c.diag, S.c.ax, S.c.ay, S.c.az, (T)-1, sizeof(T) == 4 ? (T)0 : (T)6,
-                                   (int32_t*)W.uflag.p, (int32_t*)W.umap.p, (T*)W.urow.p, ctx->stream),
+                                   (const double*)W.inv_e64.p, tau, o.sigma, (int32_t*)W.uflag.p,
+                                   (int32_t*)W.umap.p, (T*)W.urow.p, ctx->stream),
            "k_brick_uniform");
     if constexpr (sizeof(T) == 8)
         HIPCHK(launch_scale_lower(S.bk, S.c.ax, S.c.ay, S.c.az, S.c.inv_e, (double*)S.c.lx, (double*)S.c.ly,

@@ -515,6 +515,46 @@ __global__ __launch_bounds__(256) void k_assemble(Dense d, float scale, const ui
     az[o] = (T)cpl[2];
 }
 
+// One cell's pivot inverse from its lower couplings and the lower
+// neighbours' pivot inverses, in the oracle's operation order (see below).
+__device__ __forceinline__ double pivot_inverse(double dg, double axm, double aym, double azm, double ay_xm,
+                                                double az_xm, double ax_ym, double az_ym, double ax_zm, double ay_zm,
+                                                double exm, double eym, double ezm, double tau, double sigma,
+                                                int& fixed) {
+    double t = dg;
+    t = t - axm * axm * exm;
+    t = t - aym * aym * eym;
+    t = t - azm * azm * ezm;
+    if (tau != 0.0) {
+        double m = 0.0;
+        m = m + axm * (ay_xm + az_xm) * exm;
+        m = m + aym * (ax_ym + az_ym) * eym;
+        m = m + azm * (ax_zm + ay_zm) * ezm;
+        t = t - tau * m;
+    }
+    if (!(t > sigma * dg)) {
+        t = dg;
+        fixed = 1;
+    }
+    return 1.0 / t;
+}
+
+// Pivot inverse of an interior cell whose lower neighbours hold the same
+// value (couplings -1, diagonal 6): the fixed point of pivot_inverse reached
+// from 0, which the factorisation reaches away from the -x/-y/-z boundaries
+// (0.18350341907227394 for IC(0)). Every caller computes the same value.
+__device__ __forceinline__ double interior_pivot(double tau, double sigma) {
+    double e = 0.0;
+    int fixed = 0;
+    for (int k = 0; k < 128; ++k) {
+        const double en = pivot_inverse(6.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, e, e, e, tau,
+                                        sigma, fixed);
+        if (__double_as_longlong(en) == __double_as_longlong(e)) break;
+        e = en;
+    }
+    return e;
+}
+
 // Natural-order IC(0)/MIC(0) pivots, one brick per wave, FP64 arithmetic in
 // the oracle's operation order (bit-identical pivots):
 //   t = d - a_xm^2/e_xm - a_ym^2/e_ym - a_zm^2/e_zm - tau*(...);
@@ -572,22 +612,8 @@ __device__ __forceinline__ void factor_brick(int b, uint32_t epoch, const Bricks
         const double vx = shfl(ie, L.lane - 1), vy = shfl(ie, L.lane - 4), vz = shfl(ie, L.lane - 16);
         if (h == h_own && dg > 0.0) {
             const double exm = ix ? vx : ex, eym = iy ? vy : ey, ezm = iz ? vz : ez;
-            double t = dg;
-            t = t - axm * axm * exm;
-            t = t - aym * aym * eym;
-            t = t - azm * azm * ezm;
-            if (tau != 0.0) {
-                double m = 0.0;
-                m = m + axm * (ay_xm + az_xm) * exm;
-                m = m + aym * (ax_ym + az_ym) * eym;
-                m = m + azm * (ax_zm + ay_zm) * ezm;
-                t = t - tau * m;
-            }
-            if (!(t > sigma * dg)) {
-                t = dg;
-                fixed = 1;
-            }
-            ie = 1.0 / t;
+            ie = pivot_inverse(dg, axm, aym, azm, ay_xm, az_xm, ax_ym, az_ym, ax_zm, ay_zm, exm, eym, ezm, tau,
+                               sigma, fixed);
         }
     }
     if (!LEVELS) {
@@ -994,7 +1020,7 @@ __device__ __forceinline__ void spmv_load(SpmvIn<T>& d, int b, const Bricks& bk,
     for (int k = 0; k < 6; ++k) e[k] = halo_offset(nb, k, o, lc, L.lane, d.has[k]);
     // a uniform brick reads its coefficients (and the -x/-y/-z face
     // couplings, which equal them) from the 64-entry constant rows
-    const bool uni = c.uflag[b] != 0;
+    const bool uni = (c.uflag[b] & 1) != 0;
     const size_t oc = uni ? (size_t)L.lane : o;
     const T *cd = uni ? c.urow + kBrickCells : c.diag, *cx = uni ? c.urow : c.ax, *cy = uni ? c.urow : c.ay,
             *cz = uni ? c.urow : c.az;
@@ -1065,7 +1091,7 @@ __device__ __forceinline__ double spmv_brick(int b, const Bricks& bk, const Coef
     for (int d = 0; d < 6; ++d) e[d] = halo_offset(nb, d, o, lc, L.lane, has[d]);
     // a uniform brick reads its coefficients (and the -x/-y/-z face
     // couplings, which equal them) from the 64-entry constant rows
-    const bool uni = c.uflag[b] != 0;
+    const bool uni = (c.uflag[b] & 1) != 0;
     const size_t oc = uni ? (size_t)L.lane : o;
     const T *cd = uni ? c.urow + kBrickCells : c.diag, *cx = uni ? c.urow : c.ax, *cy = uni ? c.urow : c.ay,
             *cz = uni ? c.urow : c.az;
@@ -1435,17 +1461,22 @@ __device__ __forceinline__ void fwd_fetch(FwdIn<T, UPDATE>& d, int b, int bx, in
         d.pv = p[o];
         d.qv = q[o];
     }
-    d.ie = c.inv_e[o];
     const bool wy = L.ly == 0 && nym >= 0, wz = L.lz == 0 && nzm >= 0;
     const int ly_lane = wy ? L.lane + 12 : L.lane, lz_lane = wz ? L.lane + 48 : L.lane;
     const int yb = wy ? nym : b, zb = wz ? nzm : b;
     if constexpr (sizeof(T) == 8) {
-        d.lx = c.lx[o];
-        d.ly = c.ly[o];
-        d.lz = c.lz[o];
+        // bricks with interior coefficients and pivots: 1/e and the scaled
+        // lower couplings from the constant rows E and -E
+        const bool up = (u & 2) != 0;
+        const size_t oc = up ? (size_t)L.lane : o;
+        d.ie = (up ? c.urow + 2 * kBrickCells : c.inv_e)[oc];
+        d.lx = (up ? c.urow + 3 * kBrickCells : c.lx)[oc];
+        d.ly = (up ? c.urow + 3 * kBrickCells : c.ly)[oc];
+        d.lz = (up ? c.urow + 3 * kBrickCells : c.lz)[oc];
     } else {
+        d.ie = c.inv_e[o];
         // uniform brick: couplings, -y/-z face ones included, from the constant row
-        const bool uni = UNI && u != 0;
+        const bool uni = UNI && (u & 1) != 0;
         const T *cx = uni ? c.urow : c.ax, *cy = uni ? c.urow : c.ay, *cz = uni ? c.urow : c.az;
         const size_t oc = uni ? (size_t)L.lane : o;
         d.axo = cx[oc];
@@ -1579,8 +1610,9 @@ __device__ __forceinline__ void bwd_fetch(BwdIn<T>& d, int b, int bx, int nyp, i
     d.nz = nzp;
     const size_t o = (size_t)b * kBrickCells + L.lane;
     d.yv = y[o];
-    d.ie = c.inv_e[o];
-    const bool uni = u != 0;
+    const bool up = sizeof(T) == 8 && (u & 2) != 0;
+    d.ie = (up ? c.urow + 2 * kBrickCells : c.inv_e)[up ? (size_t)L.lane : o];
+    const bool uni = (u & 1) != 0;
     const size_t oc = uni ? (size_t)L.lane : o;
     d.axo = (uni ? c.urow : c.ax)[oc];
     d.ayo = (uni ? c.urow : c.ay)[oc];
@@ -1978,15 +2010,22 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
 // diagonal entry (FP32: its ghost part) and six couplings, the -x/-y/-z ones
 // read across the brick faces, equal the constants cref and dref bit for bit,
 // as in the interior of the fluid (full faces: coupling -1, diagonal 6, ghost
-// part 0). uflag[b] = 1 for those; workgroup 0 also writes the constant rows
-// urow[0, 64) = cref, urow[64, 128) = dref, from which the SpMV and the sweeps
-// then read a uniform brick's coefficients (same values, a few cache lines
-// instead of four arrays).
+// part 0). Flag bit 1 marks those; in FP64 bit 2 marks the ones whose pivot
+// inverses also all equal the interior value E (interior_pivot, the same
+// arithmetic as the factorisation). Workgroup 0 writes the constant rows
+// urow[0, 64) = cref, [64, 128) = dref, [128, 192) = E, [192, 256) = -E (the
+// pre-scaled lower coupling -1 * E), from which the SpMV and the sweeps read
+// a flagged brick's coefficients (same values, a few cache lines instead of
+// the arrays).
 template <class T>
 __global__ __launch_bounds__(256) void k_brick_uniform(Bricks bk, const T* diag, const T* ax, const T* ay,
-                                                       const T* az, T cref, T dref, int32_t* uflag, int32_t* umap,
-                                                       T* urow) {
-    if (blockIdx.x == 0 && threadIdx.x < 2 * kBrickCells) urow[threadIdx.x] = threadIdx.x < kBrickCells ? cref : dref;
+                                                       const T* az, T cref, T dref, const double* ie64, double tau,
+                                                       double sigma, int32_t* uflag, int32_t* umap, T* urow) {
+    const double e_int = sizeof(T) == 8 ? interior_pivot(tau, sigma) : 0.0;
+    if (blockIdx.x == 0) {
+        const int t = (int)threadIdx.x, row = t >> 6;
+        urow[t] = row == 0 ? cref : row == 1 ? dref : row == 2 ? (T)e_int : (T)(-e_int);
+    }
     const int b = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     if (b >= bk.nb) return;
     const Lane L;
@@ -2008,12 +2047,15 @@ __global__ __launch_bounds__(256) void k_brick_uniform(Bricks bk, const T* diag,
     const auto same = [](T a, T v) { return bits_of(a) == bits_of(v); };
     const bool ok = same(diag[o], dref) && same(ax[o], cref) && same(ay[o], cref) && same(az[o], cref) &&
                     same(axm, cref) && same(aym, cref) && same(azm, cref);
-    const bool all = __all(ok);
+    int flag = __all(ok) ? 1 : 0;
+    if constexpr (sizeof(T) == 8) {
+        if (flag && __all(bits_of(ie64[o]) == bits_of(e_int))) flag |= 2;
+    }
     if (L.lane == 0) {
         const int cc = bk.coord[b];
         const size_t pos = (size_t)(cc & 1023) + (size_t)bk.nbx * (((cc >> 10) & 1023) + (size_t)bk.nby * ((cc >> 20) & 1023));
-        uflag[b] = all ? 1 : 0;
-        umap[pos] = all ? 1 : 0;
+        uflag[b] = flag;
+        umap[pos] = flag;
     }
 }
 
@@ -2566,9 +2608,10 @@ void krylov_groups(const Bricks& bk, const Pencils& pc, int max_groups, int* g_s
 
 template <class T>
 hipError_t launch_brick_uniform(const Bricks& bk, const T* diag, const T* ax, const T* ay, const T* az, T cref,
-                                T dref, int32_t* uflag, int32_t* umap, T* urow, hipStream_t s) {
+                                T dref, const double* ie64, double tau, double sigma, int32_t* uflag, int32_t* umap,
+                                T* urow, hipStream_t s) {
     hipLaunchKernelGGL((k_brick_uniform<T>), dim3(std::max(1u, cdiv(bk.nb, 4))), dim3(256), 0, s, bk, diag, ax, ay,
-                       az, cref, dref, uflag, umap, urow);
+                       az, cref, dref, ie64, tau, sigma, uflag, umap, urow);
     return hipGetLastError();
 }
 
@@ -2820,8 +2863,8 @@ hipError_t launch_update(const Dense& d, float dt, const float* pressure, hipStr
                                       hipStream_t);                                                                 \
     template hipError_t launch_bwd<T>(const Bricks&, const Pencils&, const Coef<T>&, const T*, T*, double*, Ctrl*,   \
                                       uint64_t*, uint64_t*, int, int, int, int, uint64_t, int, hipStream_t);        \
-    template hipError_t launch_brick_uniform<T>(const Bricks&, const T*, const T*, const T*, const T*, T, T, int32_t*, \
-                                                int32_t*, T*, hipStream_t);                                         \
+    template hipError_t launch_brick_uniform<T>(const Bricks&, const T*, const T*, const T*, const T*, T, T,         \
+                                                const double*, double, double, int32_t*, int32_t*, T*, hipStream_t); \
     template hipError_t launch_resid64<T>(const Bricks&, const double*, const T*, const T*, const T*, const double*, \
                                           const double*, double*, double*, float*, float*, float*, float*, Ctrl*,   \
                                           hipStream_t);

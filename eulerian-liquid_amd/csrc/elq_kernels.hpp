@@ -81,9 +81,11 @@ struct Coef {
     const T* lx = nullptr;
     const T* ly = nullptr;
     const T* lz = nullptr;
-    // per active brick: 1 if its coefficients are the uniform interior ones
+    // per active brick: bit 1 if its coefficients are the uniform interior
+    // ones, bit 2 (FP64) if its pivot inverses are also the interior value E
     // (launch_brick_uniform); the SpMV and the wavefront sweeps then read them
-    // from urow: [0, 64) the coupling, [64, 128) the diagonal (FP32: its ghost part)
+    // from urow: [0, 64) the coupling, [64, 128) the diagonal (FP32: its ghost
+    // part), [128, 192) E, [192, 256) -E (the pre-scaled lower couplings)
     const int32_t* uflag = nullptr;
     const int32_t* umap = nullptr;  // the same flags by brick position (as Bricks::map); inactive: stale
     const T* urow = nullptr;
@@ -189,7 +191,8 @@ hipError_t launch_krylov_batch(const KrylovJob* jobs, int njobs, int per_job, in
 // equal cref, bit for bit.
 template <class T>
 hipError_t launch_brick_uniform(const Bricks& bk, const T* diag, const T* ax, const T* ay, const T* az, T cref,
-                                T dref, int32_t* uflag, int32_t* umap, T* urow, hipStream_t s);
+                                T dref, const double* ie64, double tau, double sigma, int32_t* uflag, int32_t* umap,
+                                T* urow, hipStream_t s);
 // Pivot-scaled lower couplings of the FP64 forward sweep (Coef::lx, ly, lz).
 hipError_t launch_scale_lower(const Bricks& bk, const double* ax, const double* ay, const double* az,
                               const double* ie, double* lx, double* ly, double* lz, hipStream_t s);

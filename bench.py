@@ -138,6 +138,23 @@ def run_steps(ctx, ds, opts, steps, warmup):
     return times, stats, breakdown, (dominant, dom[0], dom[1])
 
 
+def class_rates(breakdown, n_unknowns, sm, sv):
+    """Per kernel class of the profiled (breakdown) step: launches, average
+    launch time and the algorithmic bytes per launch over it, in GB/s and as
+    a fraction of peak. Averages cover every launch of the class, as the
+    roofline's do, including the few queued past convergence that exit at
+    once."""
+    out = {}
+    for k, (ms, launches) in sorted(breakdown.items()):
+        if k not in BYTES_PER_UNKNOWN or launches <= 0 or ms <= 0:
+            continue
+        avg_s = ms / launches / 1e3
+        gbs = BYTES_PER_UNKNOWN[k](sm, sv) * n_unknowns / avg_s / 1e9
+        out[k] = {"launches": int(launches), "avg_us": round(avg_s * 1e6, 2), "achieved_gbs": round(gbs, 1),
+                  "frac": round(gbs / PEAK_HBM_GBS, 4)}
+    return out
+
+
 def roofline(dom, n_unknowns, sm, sv):
     kernel, ms, launches = dom
     avg_s = ms / launches / 1e3
@@ -874,15 +891,19 @@ def main():
         "hbm_frac_whole_solve": round(total_bytes / (ms_step / 1e3) / 1e9 / PEAK_HBM_GBS, 4),
         "roofline": roof,
         "kernel_ms_per_step": {k: round(v[0], 3) for k, v in sorted(kt.items())},
+        "pcg_ms_per_solve": round(float(np.median([x["t_solve_ms"] for x in stats])), 2),
+        "kernel_classes": class_rates(kt, n, sm, sv),
     }
     if "fp64" in out and key == "mixed":
         t64, s64 = out["fp64"][0], out["fp64"][1]
         ms64 = float(np.mean(t64))
         scale = float(np.abs(p_fp64).max())
         line["fp64_drop_in"] = {"solves_per_s": round(1000.0 / ms64, 4), "ms_per_solve": round(ms64, 2),
+                                "pcg_ms_per_solve": round(float(np.median([x["t_solve_ms"] for x in s64])), 2),
                                 "iterations": s64[-1]["iterations"],
                                 "true_rel_residual": s64[-1]["true_rel_residual"],
-                                "max_fluid_divergence_rel": div_fp64}
+                                "max_fluid_divergence_rel": div_fp64,
+                                "kernel_classes": class_rates(out["fp64"][2], n, 8, 8)}
         line["speedup_mixed_vs_fp64"] = round(ms64 / ms_step, 3)
         line["max_abs_p_minus_p_fp64"] = float(np.abs(p_mixed - p_fp64).max())
         line["max_rel_p_minus_p_fp64"] = float(np.abs(p_mixed - p_fp64).max()) / scale if scale else 0.0

@@ -689,7 +689,8 @@ struct Solver {
             (uintptr_t)bk.nbx, (uintptr_t)bk.nby, (uintptr_t)bk.nbz, (uintptr_t)bk.nb, (uintptr_t)bk.nlev,
             (uintptr_t)bk.map, (uintptr_t)bk.coord, (uintptr_t)bk.nbr, (uintptr_t)bk.lvl_list, (uintptr_t)pc.np,
             (uintptr_t)pc.shape, (uintptr_t)pc.nblk, (uintptr_t)pc.blk_list, (uintptr_t)c.diag, (uintptr_t)c.ax,
-            (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)c.lx, (uintptr_t)c.uflag, (uintptr_t)c.umap, (uintptr_t)c.urow, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
+            (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)c.lx, (uintptr_t)c.uflag,
+            (uintptr_t)c.umap, (uintptr_t)c.urow, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
             (uintptr_t)W.vp1.p, (uintptr_t)W.vq.p, (uintptr_t)W.vr.p, (uintptr_t)W.vx.p, (uintptr_t)prr(),
             (uintptr_t)prz(), (uintptr_t)ctrl(), (uintptr_t)W.xch.p, (uintptr_t)stage()};
         if (!ctx->chunk_exec || key != ctx->chunk_key) {

@@ -2053,7 +2053,8 @@ __global__ __launch_bounds__(256) void k_brick_uniform(Bricks bk, const T* diag,
     }
     if (L.lane == 0) {
         const int cc = bk.coord[b];
-        const size_t pos = (size_t)(cc & 1023) + (size_t)bk.nbx * (((cc >> 10) & 1023) + (size_t)bk.nby * ((cc >> 20) & 1023));
+        const size_t bx = (size_t)(cc & 1023), by = (size_t)((cc >> 10) & 1023), bz = (size_t)((cc >> 20) & 1023);
+        const size_t pos = bx + (size_t)bk.nbx * (by + (size_t)bk.nby * bz);
         uflag[b] = flag;
         umap[pos] = flag;
     }

@@ -323,7 +323,7 @@ size_t elq_footprint_bytes(int ni, int nj, int nk, int precision, bool staged, i
     b += NP * (1 + 4);                                           // pencil flags, map
     b += block_count(p) * 5;                                     // pencil blocks
     b += (size_t)nb * (8 * 4 + 4 + (size_t)kXchWords * 8 + 2 * 8);  // nbr, lvl_list, xch, partials
-    b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
+    b += (size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 4) * 8 + sizeof(Ctrl) + 4;  // stage, ctrl, nfixed
     b += cells * (3 * 8 + 10 * tsz + 4);                         // diag64, b64, inv_e64, coefficients, vectors, inv_e32
     if (precision == ELQ_PRECISION_FP64) b += cells * 3 * 8;     // scaled lower couplings
     b += (size_t)std::max<int64_t>(nb, 1) * 4 + NB * 4 + 4 * kBrickCells * tsz;  // uniform-brick flags, constant rows
@@ -482,7 +482,7 @@ int setup_phase_b(elq_ctx* ctx, const Dense& d, float dt, bool need_levels) {
         HIPCHK(hipMemsetAsync(W.xch.p, 0, W.xch.cap, s), "memset xch");
     }
     if (!W.stage.p) {
-        HIPCHK(W.stage.ensure((size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 2) * 8), "alloc stage");
+        HIPCHK(W.stage.ensure((size_t)std::max(kRedBlocks * 4, kTailMaxGroups * 4) * 8), "alloc stage");
         HIPCHK(hipMemsetAsync(W.stage.p, 0, W.stage.cap, s), "memset stage");
     }
     HIPCHK(W.diag64.ensure(cells * 8), "alloc");
@@ -556,6 +556,7 @@ struct Solver {
     Coef<T> c;
     const elq_opts* o;
     bool levels;
+    bool fused = false;  // forward and backward sweep of an iteration in one launch (launch_sweeps)
 
     Ctrl* ctrl() { return (Ctrl*)ctx->ws.ctrl.p; }
     double* prr() { return (double*)ctx->ws.part_rr.p; }
@@ -692,7 +693,7 @@ struct Solver {
             (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)c.lx, (uintptr_t)c.uflag,
             (uintptr_t)c.umap, (uintptr_t)c.urow, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
             (uintptr_t)W.vp1.p, (uintptr_t)W.vq.p, (uintptr_t)W.vr.p, (uintptr_t)W.vx.p, (uintptr_t)prr(),
-            (uintptr_t)prz(), (uintptr_t)ctrl(), (uintptr_t)W.xch.p, (uintptr_t)stage()};
+            (uintptr_t)prz(), (uintptr_t)ctrl(), (uintptr_t)W.xch.p, (uintptr_t)stage(), (uintptr_t)fused};
         if (!ctx->chunk_exec || key != ctx->chunk_key) {
             ctx->drop_chunk_graph();
             HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -728,6 +729,17 @@ struct Solver {
                                   ctx->timeout_ticks, ctx->max_groups, ctx->stream),
                    "k_spmv");
             ctx->prof_end();
+            if constexpr (std::is_same<T, float>::value) {
+                if (fused) {
+                    ctx->prof_begin("sweeps");
+                    HIPCHK(launch_sweeps(bk, pc, c, vec(W.vr), vec(W.vx), P[(kiter + 1) & 1], vec(W.vq), vec(W.vy),
+                                         ctrl(), (uint64_t*)W.xch.p, stage(), ctx->timeout_ticks, ctx->max_groups,
+                                         ctx->stream),
+                           "k_sweeps");
+                    ctx->prof_end();
+                    continue;
+                }
+            }
             if ((rc = fwd(1, P[(kiter + 1) & 1]))) return rc;
             if ((rc = bwd(SC_BETA))) return rc;
         }
@@ -856,6 +868,13 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     S.pc = make_pencils(ctx, p, sizeof(T) == 8);
     S.o = &o;
     S.levels = levels;
+    // one launch for both sweeps of an iteration: mixed precision, 1x4 blocks,
+    // on grids of up to kFuseMaxBricks active bricks (where the launch and the
+    // forward grid tail between the sweeps are a larger share of a sweep), or
+    // whenever asked for
+    if constexpr (std::is_same<T, float>::value)
+        S.fused = !levels && o.schedule != ELQ_SCHED_WAVEFRONT_SPLIT && sweeps_fusable(S.pc) &&
+                  (o.schedule == ELQ_SCHED_WAVEFRONT_FUSED || S.bk.nb <= kFuseMaxBricks);
     S.c.diag = (const T*)W.cdiag.p;
     S.c.ax = (const T*)W.cax.p;
     S.c.ay = (const T*)W.cay.p;
@@ -939,7 +958,7 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
 
 bool check_opts(const elq_opts& o) {
     return o.tol >= 0 && o.max_iter >= 0 && (o.precision == 0 || o.precision == 1) &&
-           (o.precond == 0 || o.precond == 1) && (o.schedule == 0 || o.schedule == 1) && o.sigma >= 0 &&
+           (o.precond == 0 || o.precond == 1) && (o.schedule >= 0 && o.schedule <= 3) && o.sigma >= 0 &&
            o.inner_tol > 0 && o.inner_tol < 1;
 }
 

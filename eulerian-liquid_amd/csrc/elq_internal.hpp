@@ -27,6 +27,7 @@ constexpr int kScanChunk = kScanBlock * kScanItems;
 constexpr int kReduceThreads = 256;           // scalar kernel workgroup width
 constexpr int kRedBlocks = 128;               // max workgroups of the scalar kernel
 constexpr int kTailMaxGroups = 4096;          // max workgroups of a kernel ending in a grid tail
+constexpr int kFuseMaxBricks = 32768;         // mixed precision fuses the two sweeps up to this many bricks
 constexpr int kRedPerBlock = 2048;            // partials per scalar workgroup (target)
 constexpr int kXchWords = 192;                // face-exchange words per brick (6 x 16 x 2)
 

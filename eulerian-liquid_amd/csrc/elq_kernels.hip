@@ -1823,14 +1823,16 @@ __device__ __forceinline__ void sweep_priority(int pos, int round, int G) {
 }
 
 // Workgroup g of G (see grid_tail_n).
-template <class T, bool UPDATE, int BY, int BZ>
+// epoch: the exchange-word tag of this sweep; tag: its reduction tag (the
+// caller reads both, and ctrl->done, before the sweep starts). UNI: uniform
+// bricks read their couplings from the constant row (see fwd_fetch).
+template <class T, bool UPDATE, int BY, int BZ, bool UNI = BY == 1>
 __device__ __forceinline__ void fwd_block_body(int g, int G, const Bricks& bk, const Pencils& pc, const Coef<T>& c,
                                                T* r, T* x, const T* p, const T* q, T* y, Ctrl* ctrl, uint64_t* xch,
-                                               uint64_t* stage, uint64_t timeout) {
+                                               uint64_t* stage, uint64_t timeout, uint32_t epoch, uint32_t tag) {
     constexpr int NW = BY * BZ;
     static_assert((NW & (NW - 1)) == 0, "grid_tail_n reduces a power-of-two number of waves");
     __shared__ BlockLds<T, NW> S;
-    if (ctrl->done) return;
     block_lds_init<T, NW>(S);
     const Lane L;
     const int w = (int)(threadIdx.x >> 6), dy = w % BY, dz = w / BY;
@@ -1849,8 +1851,6 @@ __device__ __forceinline__ void fwd_block_body(int g, int G, const Bricks& bk, c
     const uint32_t* prog_z = &S.prog[out_z ? w : w + BY];
     uint32_t seen_y = 0, seen_z = 0;
     const int f_y = L.lx + 4 * L.lz, f_z = L.lx + 4 * L.ly;   // face index (producer and consumer side)
-    const uint32_t epoch = ctrl->epoch;
-    const uint32_t tag = next_tag(ctrl->red_epoch);
     const T alpha = UPDATE ? (T)ctrl->alpha : (T)0;
     double acc = 0.0;
     int round = 0;
@@ -1865,7 +1865,7 @@ __device__ __forceinline__ void fwd_block_body(int g, int G, const Bricks& bk, c
             if (it.next(bx, b, ny, nz)) {
                 FwdIn<T, UPDATE> A, B, C;
                 FwdCarry<T> cy{(T)0, (T)0, -2, acc};
-                fwd_fetch<T, UPDATE, BY == 1>(A, b, bx, ny, nz, it.u, c, r, x, p, q, L, gy, gz);
+                fwd_fetch<T, UPDATE, UNI>(A, b, bx, ny, nz, it.u, c, r, x, p, q, L, gy, gz);
                 A.poll.issue(xch);
                 T got[2];
                 auto step = [&](FwdIn<T, UPDATE>& cur, FwdIn<T, UPDATE>& nxt) {
@@ -1885,7 +1885,7 @@ __device__ __forceinline__ void fwd_block_body(int g, int G, const Bricks& bk, c
                 // ahead (earlier polls would mostly find stale tags)
                 auto ahead = [&](FwdIn<T, UPDATE>& buf, bool prev) {
                     const bool has = prev && it.next(bx, b, ny, nz);
-                    fwd_fetch<T, UPDATE, BY == 1>(buf, b, bx, ny, nz, it.u, c, r, x, p, q, L, gy, gz);
+                    fwd_fetch<T, UPDATE, UNI>(buf, b, bx, ny, nz, it.u, c, r, x, p, q, L, gy, gz);
                     return has;
                 };
                 bool hB = ahead(B, true);
@@ -1913,8 +1913,9 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
                                                                                    T* r, T* x,
                                                             const T* p, const T* q, T* y, Ctrl* ctrl, uint64_t* xch,
                                                             uint64_t* stage, uint64_t timeout) {
+    if (ctrl->done) return;
     fwd_block_body<T, UPDATE, BY, BZ>((int)blockIdx.x, (int)gridDim.x, bk, pc, c, r, x, p, q, y, ctrl, xch, stage,
-                                      timeout);
+                                      timeout, ctrl->epoch, next_tag(ctrl->red_epoch));
 }
 
 // Backward sweep over pencil blocks: blocks in reverse list order, bricks in
@@ -1922,11 +1923,10 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_fwd_bl
 template <class T, int BY, int BZ>
 __device__ __forceinline__ void bwd_block_body(int g, int G, const Bricks& bk, const Pencils& pc, const Coef<T>& c,
                                                const T* r, T* y, Ctrl* ctrl, uint64_t* xch, uint64_t* stage,
-                                               int tail_mode, uint64_t timeout) {
+                                               int tail_mode, uint64_t timeout, uint32_t epoch, uint32_t tag) {
     constexpr int NW = BY * BZ;
     static_assert((NW & (NW - 1)) == 0, "grid_tail_n reduces a power-of-two number of waves");
     __shared__ BlockLds<T, NW> S;
-    if (ctrl->done) return;
     block_lds_init<T, NW>(S);
     const Lane L;
     const int w = (int)(threadIdx.x >> 6), dy = w % BY, dz = w / BY;
@@ -1942,8 +1942,6 @@ __device__ __forceinline__ void bwd_block_body(int g, int G, const Bricks& bk, c
     const uint32_t* prog_z = &S.prog[out_z ? w : w - BY];
     uint32_t seen_y = 0, seen_z = 0;
     const int f_y = L.lx + 4 * L.lz, f_z = L.lx + 4 * L.ly;
-    const uint32_t epoch = ctrl->epoch;
-    const uint32_t tag = next_tag(ctrl->red_epoch);
     double acc = 0.0;
     int round = 0;
     for (int pos = g; pos < pc.nblk; pos += G, ++round) {
@@ -2003,7 +2001,51 @@ __global__ __launch_bounds__(64 * BY * BZ, sizeof(T) == 4 ? 4 : 2) void k_bwd_bl
                                                                                    const T* r, T* y,
                                                             Ctrl* ctrl, uint64_t* xch, uint64_t* stage, int tail_mode,
                                                             uint64_t timeout) {
-    bwd_block_body<T, BY, BZ>((int)blockIdx.x, (int)gridDim.x, bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout);
+    if (ctrl->done) return;
+    bwd_block_body<T, BY, BZ>((int)blockIdx.x, (int)gridDim.x, bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout,
+                              ctrl->epoch, next_tag(ctrl->red_epoch));
+}
+
+// Forward sweep with the x, r updates, then the backward sweep, in one
+// persistent launch (FP32, small grids): a workgroup starts its backward
+// blocks as soon as its forward blocks are done, without the kernel boundary
+// and the forward grid tail in between.
+// - The backward phase of workgroup g takes its own forward blocks in reverse
+//   order: it is the backward kernel's workgroup gb = (nblk - 1 - g) mod G,
+//   whose list positions are exactly those. So a brick's backward step runs
+//   in the wave that did its forward step (y and r are that wave's own
+//   stores), and after every brick that consumed its forward face has
+//   finished its forward step (the backward step waits for their backward
+//   faces, which follow it), so its backward exchange words never replace
+//   forward words still waited for. (Taking other workgroups' blocks breaks
+//   both: a backward step could read a y not yet written, and a late
+//   forward store could replace a backward word a consumer waits for.)
+// - Tags: the backward ones follow the forward ones (next_tag), as the two
+//   launches would read them, so nothing is read from ctrl after the forward
+//   tail (which updates it).
+// - The backward tail uses the second half of the stage buffer (a workgroup
+//   may publish its backward partial before the forward ones are gathered)
+//   and slot gb, with the blocks of the backward kernel's workgroup gb in
+//   the same order: the sums are the two launches' bit for bit.
+// - Waits end: the forward phase waits only on forward words; a workgroup
+//   publishes its forward partial before its backward blocks, which wait
+//   only on backward words of blocks of higher level.
+// - After a convergence found by the forward tail the backward sweep still
+//   runs; it only rewrites y, and its tail leaves the state alone.
+template <int BY, int BZ>
+__global__ __launch_bounds__(64 * BY * BZ, 4) void k_sweeps_block(Bricks bk, Pencils pc, Coef<float> c, float* r,
+                                                                  float* x, const float* p, const float* q, float* y,
+                                                                  Ctrl* ctrl, uint64_t* xch, uint64_t* stage,
+                                                                  uint64_t timeout) {
+    if (ctrl->done) return;
+    const uint32_t epoch = ctrl->epoch, tag = next_tag(ctrl->red_epoch);
+    const int g = (int)blockIdx.x, G = (int)gridDim.x;
+    // the forward phase without the constant-row selects: with them the
+    // combined kernel needs more than 128 VGPRs
+    fwd_block_body<float, true, BY, BZ, false>(g, G, bk, pc, c, r, x, p, q, y, ctrl, xch, stage, timeout, epoch, tag);
+    const int gb = ((pc.nblk - 1 - g) % G + G) % G;
+    bwd_block_body<float, BY, BZ>(gb, G, bk, pc, c, r, y, ctrl, xch, stage + 2 * kTailMaxGroups, SC_BETA, timeout,
+                                  next_tag(epoch), next_tag(tag));
 }
 
 // Uniform bricks (one wave per brick): every cell is an unknown whose
@@ -2169,8 +2211,9 @@ __global__ __launch_bounds__(256, 4) void k_fwd_batch(const KrylovJob* __restric
     const KrylovJob J = jobs[k];
     if (J.ctrl == nullptr || J.pc.shape != kShape || g >= J.g_fwd) return;
     const float* p = ((J.par + it) & 1) != 0 ? J.p0 : J.p1;  // P[(kiter + 1) & 1]
+    if (J.ctrl->done) return;
     fwd_block_body<float, true, BY, BZ>(g, J.g_fwd, J.bk, J.pc, J.c, J.r, J.x, p, J.q, J.y, J.ctrl, J.xch, J.stage,
-                                        J.timeout);
+                                        J.timeout, J.ctrl->epoch, next_tag(J.ctrl->red_epoch));
 }
 
 template <int BY, int BZ>
@@ -2179,7 +2222,9 @@ __global__ __launch_bounds__(256, 4) void k_bwd_batch(const KrylovJob* __restric
     const int k = (int)blockIdx.x / per_job, g = (int)blockIdx.x % per_job;
     const KrylovJob J = jobs[k];
     if (J.ctrl == nullptr || J.pc.shape != kShape || g >= J.g_bwd) return;
-    bwd_block_body<float, BY, BZ>(g, J.g_bwd, J.bk, J.pc, J.c, J.r, J.y, J.ctrl, J.xch, J.stage, SC_BETA, J.timeout);
+    if (J.ctrl->done) return;
+    bwd_block_body<float, BY, BZ>(g, J.g_bwd, J.bk, J.pc, J.c, J.r, J.y, J.ctrl, J.xch, J.stage, SC_BETA, J.timeout,
+                                  J.ctrl->epoch, next_tag(J.ctrl->red_epoch));
 }
 
 // ---------------------------------------------------------------- pencil setup
@@ -2534,6 +2579,16 @@ hipError_t launch_bwd_shape(const Bricks& bk, const Pencils& pc, const Coef<T>& 
     constexpr int BY = kShapeY[S], BZ = kShapeZ[S];
     return persistent_wg(k_bwd_block<T, BY, BZ>, pc.nblk, 64 * BY * BZ, kBlockGroupsPerCU, max_groups, s, bk, pc, c,
                          r, y, ctrl, xch, stage, tail_mode, timeout_ticks);
+}
+
+bool sweeps_fusable(const Pencils& pc) { return pc.shape == kShape1x4; }
+
+hipError_t launch_sweeps(const Bricks& bk, const Pencils& pc, const Coef<float>& c, float* r, float* x, const float* p,
+                         const float* q, float* y, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, uint64_t timeout_ticks,
+                         int max_groups, hipStream_t s) {
+    if (!sweeps_fusable(pc)) return hipErrorInvalidValue;
+    return persistent_wg(k_sweeps_block<1, 4>, pc.nblk, 256, kBlockGroupsPerCU, max_groups, s, bk, pc, c, r, x, p, q,
+                         y, ctrl, xch, stage, timeout_ticks);
 }
 
 template <class T>

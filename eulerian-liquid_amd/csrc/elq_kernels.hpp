@@ -197,6 +197,15 @@ hipError_t launch_brick_uniform(const Bricks& bk, const T* diag, const T* ax, co
 hipError_t launch_scale_lower(const Bricks& bk, const double* ax, const double* ay, const double* az,
                               const double* ie, double* lx, double* ly, double* lz, hipStream_t s);
 
+// Forward sweep with the x, r updates and the backward sweep in one persistent
+// launch (FP32, 1x4 pencil blocks): the same operations and reductions as
+// launch_fwd(update) followed by launch_bwd(SC_BETA), bit for bit. The stage
+// buffer holds 4 * kTailMaxGroups words (the backward tail uses the second half).
+bool sweeps_fusable(const Pencils& pc);
+hipError_t launch_sweeps(const Bricks& bk, const Pencils& pc, const Coef<float>& c, float* r, float* x, const float* p,
+                         const float* q, float* y, Ctrl* ctrl, uint64_t* xch, uint64_t* stage, uint64_t timeout_ticks,
+                         int max_groups, hipStream_t s);
+
 // Backward sweep z = y - E^{-1} L^T z (in place over y), fused with r.z partials.
 // level < 0: persistent launch over the pencil blocks ending in a grid tail
 // that applies tail_mode; else one brick level with per-brick partials in part.

@@ -400,7 +400,7 @@ int elq_server_create(int device, const elq_server_opts* opts, elq_server** out)
         sl.ctx->max_groups = groups;
     }
     if (opts->batched > 0 && opts->opts.precision == ELQ_PRECISION_MIXED &&
-        opts->opts.schedule == ELQ_SCHED_WAVEFRONT) {
+        opts->opts.schedule != ELQ_SCHED_LEVELS) {
         const int ng = std::min(opts->batched, opts->max_slots);
         for (int gi = 0; gi < ng; ++gi) {
             Batcher* B = new Batcher();

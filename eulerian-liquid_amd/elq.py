@@ -34,7 +34,7 @@ STATUS_NAMES = {0: "OK", 1: "NO_FLUID", 2: "NOT_CONVERGED", 3: "BREAKDOWN", 4: "
 FP64, MIXED = 0, 1
 NOALLOC_SCRATCH_BYTES = 64 << 20  # ELQ_NOALLOC_SCRATCH_BYTES
 IC0, MIC0 = 0, 1
-WAVEFRONT, LEVELS = 0, 1
+WAVEFRONT, LEVELS, WAVEFRONT_SPLIT, WAVEFRONT_FUSED = 0, 1, 2, 3
 
 
 class Grid(C.Structure):

@@ -449,21 +449,34 @@ Bricks make_bricks(elq_ctx* ctx, const Plan& p) {
     return bk;
 }
 
-// Pencil-block shape of the sweeps: the first choice while its blocks fit in
-// one round of four workgroups per CU, else the second (FP32: 1x4, else 2x2;
-// FP64: 2x2, else 2x4). FP64 sweeps hold two workgroups per CU, so there the
-// first choice covers up to two rounds. The choice depends only on the
-// active pencils, the precision and the context's workgroup cap.
-Pencils make_pencils(elq_ctx* ctx, const Plan& p, bool dbl) {
+Pencils pencils_of(elq_ctx* ctx, int shape) {
     Workspace& W = ctx->ws;
-    const int a = dbl ? kShape2x2 : kShape1x4, b = dbl ? kShape2x4 : kShape2x2;
-    const int shape = W.nblk[a] <= sweep_round_groups(ctx->max_groups) ? a : b;
     Pencils pc;
     pc.np = W.np;
     pc.shape = shape;
     pc.nblk = W.nblk[shape];
     pc.blk_list = (const int32_t*)W.blist.p + W.blk_base[shape];
     return pc;
+}
+
+// Pencil-block shape of a sweep: the first choice while its blocks fit in
+// one round of four workgroups per CU, else the second (FP32: 1x4, else 2x2;
+// FP64: 2x2, else 2x4). FP64 sweeps hold two workgroups per CU, so there the
+// first choice covers up to two rounds. The FP32 forward sweep takes 1x4 at
+// every size (fwd_shape): only its 1x4 kernel reads uniform bricks from the
+// constant rows, which over several rounds (512^3) makes it faster than 2x2,
+// while the backward sweep stays faster with 2x2 there. The choice depends
+// only on the active pencils, the precision, the sweep and the context's
+// workgroup cap.
+Pencils make_pencils(elq_ctx* ctx, bool dbl) {
+    const int a = dbl ? kShape2x2 : kShape1x4, b = dbl ? kShape2x4 : kShape2x2;
+    return pencils_of(ctx, ctx->ws.nblk[a] <= sweep_round_groups(ctx->max_groups) ? a : b);
+}
+
+// Forward-sweep blocks given the backward sweep's (bwd): FP32 1x4, except in
+// the batched lockstep mode, whose kernels run both sweeps on one shape.
+Pencils fwd_shape(elq_ctx* ctx, bool dbl, const Pencils& bwd) {
+    return dbl || ctx->batcher ? bwd : pencils_of(ctx, kShape1x4);
 }
 
 // Phase B: lists, assembly (T storage). FP64 arithmetic for diag/b always.
@@ -552,7 +565,8 @@ struct Solver {
     elq_ctx* ctx;
     Plan p;
     Bricks bk;
-    Pencils pc;
+    Pencils pc;   // forward sweep blocks (both sweeps when fused or batched)
+    Pencils pcb;  // backward sweep blocks
     Coef<T> c;
     const elq_opts* o;
     bool levels;
@@ -601,13 +615,13 @@ struct Solver {
         Workspace& W = ctx->ws;
         ctx->prof_begin("bwd_sweep");
         if (!levels) {
-            HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(), mode,
-                                 -1, 0, 0, ctx->timeout_ticks, ctx->max_groups, ctx->stream),
+            HIPCHK(launch_bwd<T>(bk, pcb, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(),
+                                 mode, -1, 0, 0, ctx->timeout_ticks, ctx->max_groups, ctx->stream),
                    "k_bwd");
         } else {
             for (int l = p.nlev - 1; l >= 0; --l) {
                 const int base = W.lvl_host[l], cnt = W.lvl_host[l + 1] - base;
-                HIPCHK(launch_bwd<T>(bk, pc, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(),
+                HIPCHK(launch_bwd<T>(bk, pcb, c, vec(W.vr), vec(W.vy), prz(), ctrl(), (uint64_t*)W.xch.p, stage(),
                                      mode, l, base, cnt, ctx->timeout_ticks, ctx->max_groups, ctx->stream),
                        "k_bwd");
             }
@@ -689,7 +703,8 @@ struct Solver {
             (uintptr_t)n, (uintptr_t)(kiter & 1), sizeof(T), (uintptr_t)ctx->max_groups, (uintptr_t)ctx->timeout_ticks,
             (uintptr_t)bk.nbx, (uintptr_t)bk.nby, (uintptr_t)bk.nbz, (uintptr_t)bk.nb, (uintptr_t)bk.nlev,
             (uintptr_t)bk.map, (uintptr_t)bk.coord, (uintptr_t)bk.nbr, (uintptr_t)bk.lvl_list, (uintptr_t)pc.np,
-            (uintptr_t)pc.shape, (uintptr_t)pc.nblk, (uintptr_t)pc.blk_list, (uintptr_t)c.diag, (uintptr_t)c.ax,
+            (uintptr_t)pc.shape, (uintptr_t)pc.nblk, (uintptr_t)pc.blk_list, (uintptr_t)pcb.shape,
+            (uintptr_t)pcb.nblk, (uintptr_t)pcb.blk_list, (uintptr_t)c.diag, (uintptr_t)c.ax,
             (uintptr_t)c.ay, (uintptr_t)c.az, (uintptr_t)c.inv_e, (uintptr_t)c.lx, (uintptr_t)c.uflag,
             (uintptr_t)c.umap, (uintptr_t)c.urow, (uintptr_t)W.vy.p, (uintptr_t)W.vp0.p,
             (uintptr_t)W.vp1.p, (uintptr_t)W.vq.p, (uintptr_t)W.vr.p, (uintptr_t)W.vx.p, (uintptr_t)prr(),
@@ -865,7 +880,8 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     S.ctx = ctx;
     S.p = p;
     S.bk = make_bricks(ctx, p);
-    S.pc = make_pencils(ctx, p, sizeof(T) == 8);
+    S.pcb = make_pencils(ctx, sizeof(T) == 8);
+    S.pc = fwd_shape(ctx, sizeof(T) == 8, S.pcb);
     S.o = &o;
     S.levels = levels;
     // one launch for both sweeps of an iteration: mixed precision, 1x4 blocks,
@@ -874,7 +890,7 @@ int project_impl(elq_ctx* ctx, const Dense& d, float dt, const elq_opts& o, elq_
     // whenever asked for
     if constexpr (std::is_same<T, float>::value)
         S.fused = !levels && o.schedule != ELQ_SCHED_WAVEFRONT_SPLIT && sweeps_fusable(S.pc) &&
-                  (o.schedule == ELQ_SCHED_WAVEFRONT_FUSED || S.bk.nb <= kFuseMaxBricks);
+                  sweeps_fusable(S.pcb) && (o.schedule == ELQ_SCHED_WAVEFRONT_FUSED || S.bk.nb <= kFuseMaxBricks);
     S.c.diag = (const T*)W.cdiag.p;
     S.c.ax = (const T*)W.cax.p;
     S.c.ay = (const T*)W.cay.p;
@@ -1339,7 +1355,7 @@ int elq_ic_factor(elq_ctx* ctx, const elq_grid* g, float dt, const elq_opts* opt
     S.ctx = ctx;
     S.p = p;
     S.bk = make_bricks(ctx, p);
-    S.pc = make_pencils(ctx, p, true);
+    S.pc = S.pcb = make_pencils(ctx, true);
     S.o = &o;
     S.levels = levels;
     S.c.diag = (const double*)W.cdiag.p;

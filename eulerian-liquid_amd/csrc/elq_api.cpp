@@ -459,18 +459,18 @@ Pencils pencils_of(elq_ctx* ctx, int shape) {
     return pc;
 }
 
-// Pencil-block shape of a sweep: the first choice while its blocks fit in
-// one round of four workgroups per CU, else the second (FP32: 1x4, else 2x2;
-// FP64: 2x2, else 2x4). FP64 sweeps hold two workgroups per CU, so there the
-// first choice covers up to two rounds. The FP32 forward sweep takes 1x4 at
-// every size (fwd_shape): only its 1x4 kernel reads uniform bricks from the
-// constant rows, which over several rounds (512^3) makes it faster than 2x2,
-// while the backward sweep stays faster with 2x2 there. The choice depends
-// only on the active pencils, the precision, the sweep and the context's
-// workgroup cap.
+// Pencil-block shape of a sweep. FP64: 2x2 at every size (against 2x4 blocks
+// for either sweep, from 64^3 to 512^3: the fastest or within 1%, except 2%
+// slower at 256^3; 2x4 took 3.5% longer at 384^3, 8% at 512^3). FP32 backward
+// sweep: 1x4 while its blocks fit in one round of four workgroups per CU,
+// else 2x2. The FP32 forward sweep takes 1x4 at every size (fwd_shape): only
+// its 1x4 kernel reads uniform bricks from the constant rows, which over
+// several rounds (512^3) makes it faster than 2x2, while the backward sweep
+// stays faster with 2x2 there. The choice depends only on the active
+// pencils, the precision, the sweep and the context's workgroup cap.
 Pencils make_pencils(elq_ctx* ctx, bool dbl) {
-    const int a = dbl ? kShape2x2 : kShape1x4, b = dbl ? kShape2x4 : kShape2x2;
-    return pencils_of(ctx, ctx->ws.nblk[a] <= sweep_round_groups(ctx->max_groups) ? a : b);
+    if (dbl) return pencils_of(ctx, kShape2x2);
+    return pencils_of(ctx, ctx->ws.nblk[kShape1x4] <= sweep_round_groups(ctx->max_groups) ? kShape1x4 : kShape2x2);
 }
 
 // Forward-sweep blocks given the backward sweep's (bwd): FP32 1x4, except in

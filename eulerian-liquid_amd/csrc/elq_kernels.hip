@@ -2603,9 +2603,6 @@ hipError_t launch_fwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, T* 
         case kShape1x4:
             return launch_fwd_shape<T, kShape1x4>(bk, pc, c, r, x, p, q, y, ctrl, xch, stage, update, timeout_ticks,
                                                   max_groups, s);
-        case kShape2x4:
-            return launch_fwd_shape<T, kShape2x4>(bk, pc, c, r, x, p, q, y, ctrl, xch, stage, update, timeout_ticks,
-                                                  max_groups, s);
         default:
             return hipErrorInvalidValue;
         }
@@ -2632,9 +2629,6 @@ hipError_t launch_bwd(const Bricks& bk, const Pencils& pc, const Coef<T>& c, con
                                                   max_groups, s);
         case kShape1x4:
             return launch_bwd_shape<T, kShape1x4>(bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout_ticks,
-                                                  max_groups, s);
-        case kShape2x4:
-            return launch_bwd_shape<T, kShape2x4>(bk, pc, c, r, y, ctrl, xch, stage, tail_mode, timeout_ticks,
                                                   max_groups, s);
         default:
             return hipErrorInvalidValue;

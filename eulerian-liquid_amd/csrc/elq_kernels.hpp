@@ -49,9 +49,9 @@ struct Pencils {
 // one wave each. Faces between the waves of a block go through LDS; faces
 // between blocks through the exchange words in HBM. BY * BZ is a power of
 // two (the grid tail's reduction tree over the waves of a workgroup).
-enum BlockShape : int { kShape2x2 = 0, kShape1x4 = 1, kShape2x4 = 2, kShapeCount = 3 };
-constexpr int kShapeY[kShapeCount] = {2, 1, 2};
-constexpr int kShapeZ[kShapeCount] = {2, 4, 4};
+enum BlockShape : int { kShape2x2 = 0, kShape1x4 = 1, kShapeCount = 2 };
+constexpr int kShapeY[kShapeCount] = {2, 1};
+constexpr int kShapeZ[kShapeCount] = {2, 4};
 
 // Pencil blocks of one shape over an nby x nbz pencil grid.
 inline size_t shape_blocks(int nby, int nbz, int shape) {

@@ -188,10 +188,10 @@ def test_mixed_replacement_far_above_recursion_restarts(ctx, inner_tol):
 @pytest.mark.parametrize("cap", [4, 8])
 @pytest.mark.parametrize("precision, tol", [(elq.MIXED, 2e-5), (elq.FP64, 2e-6)])
 def test_projection_capped_grid(precision, tol, cap):
-    """A workgroup cap below the pencil-block count: the sweeps switch to
-    their second block shape (FP32 2x2, FP64 2x4) and walk the block list in
-    several rounds, the SpMV and factor in more bricks per wave; the solution
-    meets the same bounds as the uncapped solve. A negative cap is refused."""
+    """A workgroup cap below the pencil-block count: the sweeps walk the
+    block list in several rounds (FP32: 1x4 forward, 2x2 backward; FP64: 2x2),
+    the SpMV and factor in more bricks per wave; the solution meets the same
+    bounds as the uncapped solve. A negative cap is refused."""
     c = elq.Context(0)
     try:
         with pytest.raises(elq.ElqError):

@@ -65,11 +65,14 @@ typedef enum {
     ELQ_SCHED_WAVEFRONT = 0,       /* one launch per sweep; bricks taken in level order,
                                       predecessor faces arrive as epoch-tagged words. Mixed
                                       precision on grids of up to 32768 active bricks (128^3)
-                                      runs the forward and backward sweeps in one launch */
+                                      whose pencil blocks fit one round of the persistent
+                                      grid runs the forward and backward sweeps in one launch */
     ELQ_SCHED_LEVELS = 1,          /* one launch per brick level (no device waits) */
     ELQ_SCHED_WAVEFRONT_SPLIT = 2, /* wavefront, always one launch per sweep */
     ELQ_SCHED_WAVEFRONT_FUSED = 3  /* wavefront, mixed precision: both sweeps in one launch
-                                      whatever the grid size (same results as split) */
+                                      whenever the pencil blocks fit one round of the
+                                      persistent grid, at any brick count (same results as
+                                      split); otherwise as ELQ_SCHED_WAVEFRONT_SPLIT */
 } elq_schedule;
 
 /* Inputs/outputs of one projection. Host or device pointers depending on the
